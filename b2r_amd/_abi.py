@@ -1,0 +1,304 @@
+"""ctypes binding of the C ABI declared in include/b2r_hip.h.
+
+Loads ``libb2r_hip.so`` built in-tree next to this file.  There is no CPU
+fallback: if the library is missing, importing this module raises ImportError
+with the build command, and every call that fails on the device raises
+:class:`B2RError` carrying the library's return code and message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libb2r_hip.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build the HIP library first "
+        "(python -c 'import __graft_entry__ as g; g.build()' or make -C b2r_amd/csrc)")
+
+_lib = C.CDLL(LIB_PATH)
+
+B2R_OK = 0
+B2R_E_ARG = -1
+B2R_E_GEOM = -2
+B2R_E_HIP = -3
+B2R_E_NOMEM = -4
+B2R_E_UNSUPPORTED = -5
+
+BOUNDARY_FROZEN = 0
+BOUNDARY_PERIODIC = 1
+HEAT_F32 = 0
+HEAT_REF = 1
+
+
+class B2RError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"b2r error {code}: {msg}")
+        self.code = code
+
+
+class Geom(C.Structure):
+    _fields_ = [("env_x", C.c_int), ("env_y", C.c_int), ("env_z", C.c_int),
+                ("block_dim_x", C.c_int), ("block_dim_y", C.c_int), ("block_dim_z", C.c_int),
+                ("R", C.c_int), ("radius", C.c_int), ("boundary", C.c_int),
+                ("elem_size", C.c_int)]
+
+
+class Layout(C.Structure):
+    _fields_ = [("B", C.c_int * 3), ("S", C.c_int * 3), ("halo", C.c_int),
+                ("nblocks", C.c_int), ("x_offset", C.c_int), ("pitch", C.c_int),
+                ("slice", C.c_longlong), ("block_stride", C.c_longlong),
+                ("field_elems", C.c_longlong), ("elem_size", C.c_int)]
+
+
+class HeatParams(C.Structure):
+    _fields_ = [("D", C.c_int), ("mode", C.c_int), ("k", C.c_float * 3),
+                ("adt", C.c_double), ("h2", C.c_double * 3)]
+
+
+_vp = C.c_void_p
+_i = C.c_int
+_pi = C.POINTER(C.c_int)
+_pf = C.POINTER(C.c_float)
+
+_SIGS = {
+    "b2r_last_error": (C.c_char_p, []),
+    "b2r_version": (_i, []),
+    "b2r_create": (_i, [C.POINTER(Geom), _i, C.POINTER(_vp)]),
+    "b2r_destroy": (_i, [_vp]),
+    "b2r_get_layout": (_i, [_vp, C.POINTER(Layout)]),
+    "b2r_synchronize": (_i, [_vp]),
+    "b2r_sync_flag": (_i, [_vp, _i, C.POINTER(C.c_byte)]),
+    "b2r_set_frozen_faces": (_i, [_vp, _i]),
+    "b2r_field_alloc": (_i, [_vp, C.POINTER(_vp)]),
+    "b2r_field_free": (_i, [_vp, _vp]),
+    "b2r_field_fill": (_i, [_vp, _vp, _vp]),
+    "b2r_field_upload": (_i, [_vp, _vp, _vp]),
+    "b2r_field_download": (_i, [_vp, _vp, _vp]),
+    "b2r_block_upload": (_i, [_vp, _vp, _i, _vp]),
+    "b2r_block_download": (_i, [_vp, _vp, _i, _vp]),
+    "b2r_field_copy": (_i, [_vp, _vp, _vp]),
+    "b2r_field_init_uniform": (_i, [_vp, _vp, C.c_uint64]),
+    "b2r_field_init_bernoulli": (_i, [_vp, _vp, C.c_uint64, C.c_uint32]),
+    "b2r_field_init_blobs": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "b2r_boundary_updating": (_i, [_vp, _i, _vp]),
+    "b2r_boundary_updating_direct": (_i, [_vp, _i, C.POINTER(_vp), _i]),
+    "b2r_msg_elems": (C.c_longlong, [_vp, _i]),
+    "b2r_pack": (_i, [_vp, _i, _vp, _vp]),
+    "b2r_unpack": (_i, [_vp, _i, _vp, _vp]),
+    "b2r_heat_step": (_i, [_vp, _vp, _vp, C.POINTER(HeatParams), _i]),
+    "b2r_heat_run": (_i, [_vp, _vp, _vp, C.POINTER(HeatParams), _i, _i, _pi, _pf]),
+    "b2r_gol_step": (_i, [_vp, _vp, _vp, _i]),
+    "b2r_gol_run": (_i, [_vp, _vp, _vp, _i, _i, _pi, _pf]),
+    "b2r_stream_copy": (_i, [_vp, _vp, _vp, C.c_size_t, _i, _pf]),
+    "b2r_time_steps": (_i, [_vp, _i, _vp, _vp, C.POINTER(HeatParams), _i, _pf]),
+    "b2r_dev_alloc": (_i, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "b2r_dev_free": (_i, [_vp, _vp]),
+    "b2r_memcpy_h2d": (_i, [_vp, _vp, _vp, C.c_size_t]),
+    "b2r_memcpy_d2h": (_i, [_vp, _vp, _vp, C.c_size_t]),
+}
+
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+EXPORTED = tuple(_SIGS)
+
+
+def lib():
+    return _lib
+
+
+def last_error() -> str:
+    return _lib.b2r_last_error().decode()
+
+
+def check(rc: int) -> int:
+    if rc < 0:
+        raise B2RError(rc, last_error())
+    return rc
+
+
+_DT = {1: np.uint8, 4: np.float32, 8: np.float64}
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_vp)
+
+
+class Context:
+    """One B2R geometry on one GPU: Bx*By*Bz padded blocks per field."""
+
+    def __init__(self, env, blocks, R, radius, boundary=BOUNDARY_FROZEN, elem_size=4,
+                 device=0):
+        g = Geom(env[0], env[1], env[2], blocks[0], blocks[1], blocks[2], R, radius,
+                 boundary, elem_size)
+        self.geom = g
+        h = _vp()
+        check(_lib.b2r_create(C.byref(g), device, C.byref(h)))
+        self.h = h
+        lay = Layout()
+        check(_lib.b2r_get_layout(h, C.byref(lay)))
+        self.layout = lay
+        self.env = tuple(env)
+        self.dtype = _DT[elem_size]
+
+    # -- lifetime ------------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.b2r_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_frozen_faces(self, mask):
+        check(_lib.b2r_set_frozen_faces(self.h, mask))
+
+    def sync_flag(self, rank):
+        f = (C.c_byte * 6)()
+        check(_lib.b2r_sync_flag(self.h, rank, f))
+        return list(f)
+
+    # -- fields ----------------------------------------------------------------
+    def field(self):
+        p = _vp()
+        check(_lib.b2r_field_alloc(self.h, C.byref(p)))
+        return p
+
+    def free(self, f):
+        check(_lib.b2r_field_free(self.h, f))
+
+    def fill(self, f, value):
+        v = np.array([value], dtype=self.dtype)
+        check(_lib.b2r_field_fill(self.h, f, _ptr(v)))
+
+    def upload(self, f, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        gx, gy, gz = self.env
+        assert a.shape == (gz, gy, gx), (a.shape, (gz, gy, gx))
+        check(_lib.b2r_field_upload(self.h, f, _ptr(a)))
+
+    def download(self, f) -> np.ndarray:
+        gx, gy, gz = self.env
+        a = np.empty((gz, gy, gx), dtype=self.dtype)
+        check(_lib.b2r_field_download(self.h, f, _ptr(a)))
+        return a
+
+    def raw(self, f) -> np.ndarray:
+        """The whole field allocation (layout order), copied to the host."""
+        n = self.layout.field_elems
+        a = np.empty(n, dtype=self.dtype)
+        check(_lib.b2r_memcpy_d2h(self.h, _ptr(a), f, a.nbytes))
+        return a
+
+    def raw_upload(self, f, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.layout.field_elems
+        check(_lib.b2r_memcpy_h2d(self.h, f, _ptr(a), a.nbytes))
+
+    def init_uniform(self, f, seed):
+        check(_lib.b2r_field_init_uniform(self.h, f, seed))
+
+    def init_bernoulli(self, f, seed, p_q24):
+        check(_lib.b2r_field_init_bernoulli(self.h, f, seed, p_q24))
+
+    def init_blobs(self, f, amp, ex, ey, ez):
+        amp = np.ascontiguousarray(amp, np.float64)
+        ex, ey, ez = (np.ascontiguousarray(t, np.float64) for t in (ex, ey, ez))
+        check(_lib.b2r_field_init_blobs(self.h, f, len(amp), _ptr(amp), _ptr(ex), _ptr(ey),
+                                        _ptr(ez)))
+
+    def sync(self):
+        check(_lib.b2r_synchronize(self.h))
+
+    # -- exchange ---------------------------------------------------------------
+    def boundary_updating(self, f, ts=0):
+        check(_lib.b2r_boundary_updating(self.h, ts, f))
+
+    def boundary_updating_direct(self, fields, ts=0):
+        arr = (_vp * len(fields))(*fields)
+        check(_lib.b2r_boundary_updating_direct(self.h, ts, arr, len(fields)))
+
+    def msg_elems(self, dim):
+        return check(_lib.b2r_msg_elems(self.h, dim))
+
+    def dev_alloc(self, nbytes):
+        p = _vp()
+        check(_lib.b2r_dev_alloc(self.h, nbytes, C.byref(p)))
+        return p
+
+    def dev_free(self, p):
+        check(_lib.b2r_dev_free(self.h, p))
+
+    def pack(self, dim, f, msgs):
+        check(_lib.b2r_pack(self.h, dim, f, msgs))
+
+    def unpack(self, dim, f, msgs):
+        check(_lib.b2r_unpack(self.h, dim, f, msgs))
+
+    def d2h(self, p, nbytes, dtype):
+        a = np.empty(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        check(_lib.b2r_memcpy_d2h(self.h, _ptr(a), p, nbytes))
+        return a
+
+    def h2d(self, p, a):
+        a = np.ascontiguousarray(a)
+        check(_lib.b2r_memcpy_h2d(self.h, p, _ptr(a), a.nbytes))
+
+    # -- models ---------------------------------------------------------------
+    def heat_step(self, fin, fout, params: HeatParams, b2r_i):
+        check(_lib.b2r_heat_step(self.h, fin, fout, C.byref(params), b2r_i))
+
+    def heat_run(self, fa, fb, params: HeatParams, nsteps, fused=0, timed=False):
+        rib = C.c_int(0)
+        ms = C.c_float(0.0)
+        check(_lib.b2r_heat_run(self.h, fa, fb, C.byref(params), nsteps, fused, C.byref(rib),
+                                C.byref(ms) if timed else None))
+        return (fb if rib.value else fa), (ms.value if timed else None)
+
+    def gol_step(self, fin, fout, b2r_i):
+        check(_lib.b2r_gol_step(self.h, fin, fout, b2r_i))
+
+    def gol_run(self, fa, fb, nsteps, fused=0, timed=False):
+        rib = C.c_int(0)
+        ms = C.c_float(0.0)
+        check(_lib.b2r_gol_run(self.h, fa, fb, nsteps, fused, C.byref(rib),
+                               C.byref(ms) if timed else None))
+        return (fb if rib.value else fa), (ms.value if timed else None)
+
+    def time_steps(self, kind, fa, fb, iters, params=None):
+        ms = C.c_float(0.0)
+        check(_lib.b2r_time_steps(self.h, kind, fa, fb, C.byref(params) if params else None,
+                                  iters, C.byref(ms)))
+        return ms.value
+
+    def stream_copy(self, dst, src, nbytes, iters):
+        ms = C.c_float(0.0)
+        check(_lib.b2r_stream_copy(self.h, dst, src, nbytes, iters, C.byref(ms)))
+        return ms.value
+
+
+def heat_params(D, mode=HEAT_F32, k=(0.1, 0.1, 0.1), adt=0.1 * 0.1, h2=(1.0, 1.0, 0.25)):
+    p = HeatParams()
+    p.D = D
+    p.mode = mode
+    for a in range(3):
+        p.k[a] = k[a]
+        p.h2[a] = h2[a]
+    p.adt = adt
+    return p

@@ -1,0 +1,139 @@
+// b2r_common.h -- internal definitions shared by the gfx950 translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <set>
+#include <string>
+
+#include "../../include/b2r_hip.h"
+
+// ---------------------------------------------------------------------------
+// errors: every ABI call returns a code and leaves a message (never exits)
+// ---------------------------------------------------------------------------
+void b2r_set_error(const char* fmt, ...);
+
+#define B2R_HIP(call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            b2r_set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),  \
+                          __FILE__, __LINE__);                                     \
+            return B2R_E_HIP;                                                      \
+        }                                                                          \
+    } while (0)
+
+#define B2R_CHECK(cond, code, ...)                                                 \
+    do {                                                                           \
+        if (!(cond)) {                                                             \
+            b2r_set_error(__VA_ARGS__);                                            \
+            return (code);                                                         \
+        }                                                                          \
+    } while (0)
+
+#define B2R_LAUNCH_CHECK()                                                         \
+    do {                                                                           \
+        hipError_t e_ = hipGetLastError();                                         \
+        if (e_ != hipSuccess) {                                                    \
+            b2r_set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(e_),\
+                          __FILE__, __LINE__);                                     \
+            return B2R_E_HIP;                                                      \
+        }                                                                          \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// geometry passed to kernels by value
+// ---------------------------------------------------------------------------
+struct KGeo {
+    int B[3];           // owned cells per block
+    int S[3];           // padded block size
+    int nb[3];          // blocks per axis
+    int h;              // halo depth
+    int periodic;       // 1: global faces wrap
+    int fmask;          // bit 2*axis+side: that global face is frozen (clip to owned)
+    int pitch;          // elements per row
+    int xoff;           // elements before x = 0 in a row
+    long long slice;    // elements per plane
+    long long bstride;  // elements per block
+    int nblk;
+};
+
+struct b2r_ctx {
+    b2r_geom g;
+    b2r_layout L;
+    KGeo k;
+    int device;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    std::set<void*> fields;   // allocations made by b2r_field_alloc
+    std::set<void*> scratch;  // allocations made by b2r_dev_alloc
+    void* msgbuf;             // boundary_updating message buffers
+    size_t msgbuf_bytes;
+};
+
+int b2r_activate(b2r_ctx* ctx);  // hipSetDevice(ctx->device)
+int b2r_check_field(const b2r_ctx* ctx, const void* f);
+int b2r_direct_exchange(b2r_ctx* ctx, void* const* fields, int nfields);
+int b2r_time_begin(b2r_ctx* ctx, float* ms);
+int b2r_time_end(b2r_ctx* ctx, float* ms);
+int b2r_heat_launch(b2r_ctx* ctx, const void* in, void* out, const b2r_heat_params* p, int i);
+int b2r_gol_launch(b2r_ctx* ctx, const void* in, void* out, int i);
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+// Inner-step update region of block b for a per-step radius r at inner step i:
+// [r*(i+1), S-1-r*(i+1)] per axis (heat-3D.cu:191, 268), clipped to the owned
+// range at frozen global faces (heat-3D.cu:81-114).  An open global face (its
+// fmask bit clear) is updated like an interior one, as the reference's general
+// kernels do (heat-3D.cu:173-235), and no exchange ever refreshes it.
+struct Box {
+    int lo[3], hi[3];
+};
+
+__host__ __device__ inline void block_coords(const KGeo& g, int b, int c[3])
+{
+    c[0] = b % g.nb[0];
+    c[1] = (b / g.nb[0]) % g.nb[1];
+    c[2] = b / (g.nb[0] * g.nb[1]);
+}
+
+__host__ __device__ inline Box update_box(const KGeo& g, int b, int r, int i)
+{
+    int c[3];
+    block_coords(g, b, c);
+    Box bx;
+    for (int a = 0; a < 3; ++a) {
+        bool lo_edge = !g.periodic && c[a] == 0 && ((g.fmask >> (2 * a)) & 1);
+        bool hi_edge = !g.periodic && c[a] == g.nb[a] - 1 && ((g.fmask >> (2 * a + 1)) & 1);
+        int lo = r * (i + 1), hi = g.S[a] - 1 - r * (i + 1);
+        bx.lo[a] = (lo_edge && g.h > lo) ? g.h : lo;
+        bx.hi[a] = (hi_edge && g.h + g.B[a] - 1 < hi) ? g.h + g.B[a] - 1 : hi;
+    }
+    return bx;
+}
+
+static inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// Neighbour block of b along dim on side 0 (minus) or 1 (plus), -1 if the
+// block sits on a frozen global face (b2r_sync_flag, b2r_halo_sync.cu:140-183;
+// rank map b2r_comm.hpp:25-28; periodic wrap added).
+__host__ __device__ inline int neighbour_block(const KGeo& g, int b, int dim, int side)
+{
+    int c[3];
+    block_coords(g, b, c);
+    c[dim] += side ? 1 : -1;
+    if (c[dim] < 0 || c[dim] >= g.nb[dim]) {
+        if (!g.periodic) return -1;
+        c[dim] = (c[dim] + g.nb[dim]) % g.nb[dim];
+    }
+    return (c[2] * g.nb[1] + c[1]) * g.nb[0] + c[0];
+}
+
+// Element offset of local cell (x,y,z) of block b.
+__host__ __device__ inline long long cell_index(const KGeo& g, int b, int x, int y, int z)
+{
+    return (long long)b * g.bstride + g.xoff + (long long)z * g.slice + (long long)y * g.pitch + x;
+}

@@ -1,0 +1,828 @@
+// b2r_core.hip -- context, field layout, uploads, seeded generators, the
+// device-local halo exchange (boundary_updating) and measurement helpers.
+//
+// The reference keeps its geometry in globals (b2r.h:72-87), checks it in
+// b2r.cpp:154-167 and simu_env_init (b2r.cpp:47-67), allocates message buffers in
+// b2r_sync_init (b2r_halo_sync.cu:74-111) and exchanges one field per call with
+// pack -> D2H -> MPI -> H2D -> unpack per dimension (b2r_halo_sync.cu:442-508).
+// Here every block of a field is resident on one GPU: one launch per dimension
+// packs the faces of all blocks into device message buffers and one launch
+// unpacks them, or (the direct path) one launch per dimension copies each halo
+// cell straight from its source block for several fields at once.
+#include "b2r_common.h"
+
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+static thread_local char g_err[512] = "";
+
+void b2r_set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char* b2r_last_error(void) { return g_err; }
+extern "C" int b2r_version(void) { return 100; }
+
+int b2r_activate(b2r_ctx* ctx)
+{
+    B2R_HIP(hipSetDevice(ctx->device));
+    return B2R_OK;
+}
+
+int b2r_check_field(const b2r_ctx* ctx, const void* f)
+{
+    B2R_CHECK(f != nullptr, B2R_E_ARG, "null field pointer");
+    B2R_CHECK(ctx->fields.count(const_cast<void*>(f)) != 0, B2R_E_ARG,
+              "pointer %p is not a field of this context", f);
+    return B2R_OK;
+}
+
+int b2r_time_begin(b2r_ctx* ctx, float* ms)
+{
+    if (ms) B2R_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    return B2R_OK;
+}
+
+int b2r_time_end(b2r_ctx* ctx, float* ms)
+{
+    if (ms) {
+        B2R_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        B2R_HIP(hipEventSynchronize(ctx->ev1));
+        B2R_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    }
+    return B2R_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+static long long round_up(long long a, long long m) { return (a + m - 1) / m * m; }
+
+extern "C" int b2r_create(const b2r_geom* geom, int device, b2r_ctx** out)
+{
+    B2R_CHECK(geom && out, B2R_E_ARG, "null argument");
+    *out = nullptr;
+    const b2r_geom& g = *geom;
+    int env[3] = {g.env_x, g.env_y, g.env_z};
+    int bd[3] = {g.block_dim_x, g.block_dim_y, g.block_dim_z};
+    B2R_CHECK(g.elem_size == 1 || g.elem_size == 4 || g.elem_size == 8, B2R_E_ARG,
+              "elem_size must be 1, 4 or 8 (got %d)", g.elem_size);
+    B2R_CHECK(g.boundary == B2R_BOUNDARY_FROZEN || g.boundary == B2R_BOUNDARY_PERIODIC,
+              B2R_E_ARG, "unknown boundary kind %d", g.boundary);
+    B2R_CHECK(g.R >= 1, B2R_E_GEOM, "R must be >= 1 (got %d)", g.R);
+    B2R_CHECK(g.radius >= 1, B2R_E_GEOM, "radius must be >= 1 (got %d)", g.radius);
+    for (int a = 0; a < 3; ++a) {
+        B2R_CHECK(env[a] >= 1 && bd[a] >= 1, B2R_E_GEOM, "extent and block counts must be >= 1");
+        // b2r.cpp:154-159: the environment must divide evenly into blocks
+        B2R_CHECK(env[a] % bd[a] == 0, B2R_E_GEOM,
+                  "ENV_DIM (%d) not divisible by BLOCK_DIM (%d) on axis %d", env[a], bd[a], a);
+    }
+    int B[3] = {env[0] / bd[0], env[1] / bd[1], env[2] / bd[2]};
+    int h = g.radius * g.R;
+    int bmin = B[0] < B[1] ? B[0] : B[1];
+    bmin = bmin < B[2] ? bmin : B[2];
+    // b2r.cpp:54-59: 1 <= R*D <= min(B)/2
+    B2R_CHECK(h <= bmin / 2, B2R_E_GEOM,
+              "should meet 1 <= R*radius <= min(B)/2 (R*radius=%d, min(B)=%d)", h, bmin);
+
+    int ndev = 0;
+    B2R_HIP(hipGetDeviceCount(&ndev));
+    B2R_CHECK(device >= 0 && device < ndev, B2R_E_ARG, "device %d of %d", device, ndev);
+    B2R_HIP(hipSetDevice(device));
+
+    b2r_ctx* c = new b2r_ctx();
+    c->g = g;
+    c->device = device;
+    c->msgbuf = nullptr;
+    c->msgbuf_bytes = 0;
+    b2r_layout& L = c->L;
+    memset(&L, 0, sizeof(L));
+    for (int a = 0; a < 3; ++a) {
+        L.B[a] = B[a];
+        L.S[a] = B[a] + 2 * h;
+    }
+    L.halo = h;
+    L.nblocks = bd[0] * bd[1] * bd[2];
+    L.elem_size = g.elem_size;
+    if (g.elem_size == 1) {
+        // byte cells: owned x = h lands on a 16-byte boundary, rows padded to 16
+        // bytes with 3 spare bytes so any dword touching [0,S_x) is in the row.
+        L.x_offset = (16 - h % 16) % 16;
+        L.pitch = (int)round_up(L.x_offset + L.S[0] + 3, 16);
+    } else {
+        L.x_offset = 0;
+        L.pitch = L.S[0];
+    }
+    L.slice = (long long)L.pitch * L.S[1];
+    L.block_stride = round_up(L.slice * L.S[2], 256 / g.elem_size);
+    L.field_elems = L.block_stride * L.nblocks;
+
+    KGeo& k = c->k;
+    for (int a = 0; a < 3; ++a) {
+        k.B[a] = L.B[a];
+        k.S[a] = L.S[a];
+        k.nb[a] = bd[a];
+    }
+    k.h = h;
+    k.periodic = g.boundary == B2R_BOUNDARY_PERIODIC;
+    k.fmask = k.periodic ? 0 : 0x3F;
+    k.pitch = L.pitch;
+    k.xoff = L.x_offset;
+    k.slice = L.slice;
+    k.bstride = L.block_stride;
+    k.nblk = L.nblocks;
+
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+        b2r_set_error("stream/event creation failed: %s", hipGetErrorString(e));
+        delete c;
+        return B2R_E_HIP;
+    }
+    *out = c;
+    return B2R_OK;
+}
+
+extern "C" int b2r_destroy(b2r_ctx* ctx)
+{
+    if (!ctx) return B2R_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (void* p : ctx->fields) (void)hipFree(p);
+    for (void* p : ctx->scratch) (void)hipFree(p);
+    if (ctx->msgbuf) (void)hipFree(ctx->msgbuf);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return B2R_OK;
+}
+
+extern "C" int b2r_get_layout(const b2r_ctx* ctx, b2r_layout* out)
+{
+    B2R_CHECK(ctx && out, B2R_E_ARG, "null argument");
+    *out = ctx->L;
+    return B2R_OK;
+}
+
+extern "C" int b2r_synchronize(b2r_ctx* ctx)
+{
+    B2R_CHECK(ctx, B2R_E_ARG, "null context");
+    B2R_HIP(hipStreamSynchronize(ctx->stream));
+    return B2R_OK;
+}
+
+extern "C" int b2r_set_frozen_faces(b2r_ctx* ctx, int mask)
+{
+    B2R_CHECK(ctx, B2R_E_ARG, "null context");
+    B2R_CHECK(mask >= 0 && mask <= 0x3F, B2R_E_ARG, "mask 0x%x outside 6 bits", mask);
+    B2R_CHECK(!ctx->k.periodic || mask == 0, B2R_E_ARG, "a periodic grid has no global faces");
+    ctx->k.fmask = mask;
+    return B2R_OK;
+}
+
+extern "C" int b2r_sync_flag(const b2r_ctx* ctx, int rank, signed char flags[6])
+{
+    B2R_CHECK(ctx && flags, B2R_E_ARG, "null argument");
+    B2R_CHECK(rank >= 0 && rank < ctx->k.nblk, B2R_E_ARG, "rank %d out of range", rank);
+    int c[3];
+    block_coords(ctx->k, rank, c);
+    for (int dim = 0; dim < 3; ++dim)
+        for (int side = 0; side < 2; ++side) {
+            int n = neighbour_block(ctx->k, rank, dim, side);
+            bool wraps = side ? c[dim] == ctx->k.nb[dim] - 1 : c[dim] == 0;
+            flags[2 * dim + side] = n < 0 ? 0 : (wraps ? -1 : 1);
+        }
+    return B2R_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fields
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void fill_kernel(T* __restrict__ f, long long n, T v)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        f[i] = v;
+}
+
+static int launch_blocks(long long n, int per = 256, int cap = 65536)
+{
+    long long b = (n + per - 1) / per;
+    if (b < 1) b = 1;
+    return (int)(b > cap ? cap : b);
+}
+
+extern "C" int b2r_field_alloc(b2r_ctx* ctx, void** d_field)
+{
+    B2R_CHECK(ctx && d_field, B2R_E_ARG, "null argument");
+    if (int rc = b2r_activate(ctx)) return rc;
+    size_t bytes = (size_t)ctx->L.field_elems * ctx->g.elem_size;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        b2r_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        return B2R_E_NOMEM;
+    }
+    B2R_HIP(hipMemsetAsync(p, 0, bytes, ctx->stream));
+    ctx->fields.insert(p);
+    *d_field = p;
+    return B2R_OK;
+}
+
+extern "C" int b2r_field_free(b2r_ctx* ctx, void* d_field)
+{
+    B2R_CHECK(ctx, B2R_E_ARG, "null context");
+    if (int rc = b2r_check_field(ctx, d_field)) return rc;
+    B2R_HIP(hipStreamSynchronize(ctx->stream));
+    B2R_HIP(hipFree(d_field));
+    ctx->fields.erase(d_field);
+    return B2R_OK;
+}
+
+extern "C" int b2r_field_fill(b2r_ctx* ctx, void* d_field, const void* value)
+{
+    B2R_CHECK(ctx && value, B2R_E_ARG, "null argument");
+    if (int rc = b2r_check_field(ctx, d_field)) return rc;
+    long long n = ctx->L.field_elems;
+    int nb = launch_blocks(n);
+    switch (ctx->g.elem_size) {
+    case 1: fill_kernel<<<nb, 256, 0, ctx->stream>>>((uint8_t*)d_field, n, *(const uint8_t*)value); break;
+    case 4: fill_kernel<<<nb, 256, 0, ctx->stream>>>((uint32_t*)d_field, n, *(const uint32_t*)value); break;
+    default: fill_kernel<<<nb, 256, 0, ctx->stream>>>((uint64_t*)d_field, n, *(const uint64_t*)value); break;
+    }
+    B2R_LAUNCH_CHECK();
+    return B2R_OK;
+}
+
+// dense global grid <-> owned cells of all blocks
+template <typename T, bool TO_FIELD>
+__global__ void owned_copy_kernel(T* __restrict__ f, T* __restrict__ dense, KGeo g)
+{
+    long long gx = (long long)g.B[0] * g.nb[0], gy = (long long)g.B[1] * g.nb[1];
+    long long n = gx * gy * (long long)g.B[2] * g.nb[2];
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        int x = (int)(i % gx);
+        int y = (int)((i / gx) % gy);
+        int z = (int)(i / (gx * gy));
+        int b = ((z / g.B[2]) * g.nb[1] + y / g.B[1]) * g.nb[0] + x / g.B[0];
+        long long c = cell_index(g, b, g.h + x % g.B[0], g.h + y % g.B[1], g.h + z % g.B[2]);
+        if (TO_FIELD) f[c] = dense[i];
+        else dense[i] = f[c];
+    }
+}
+
+template <bool TO_FIELD>
+static int owned_copy(b2r_ctx* ctx, void* f, void* dense)
+{
+    long long n = (long long)ctx->g.env_x * ctx->g.env_y * ctx->g.env_z;
+    int nb = launch_blocks(n);
+    switch (ctx->g.elem_size) {
+    case 1: owned_copy_kernel<uint8_t, TO_FIELD><<<nb, 256, 0, ctx->stream>>>((uint8_t*)f, (uint8_t*)dense, ctx->k); break;
+    case 4: owned_copy_kernel<uint32_t, TO_FIELD><<<nb, 256, 0, ctx->stream>>>((uint32_t*)f, (uint32_t*)dense, ctx->k); break;
+    default: owned_copy_kernel<uint64_t, TO_FIELD><<<nb, 256, 0, ctx->stream>>>((uint64_t*)f, (uint64_t*)dense, ctx->k); break;
+    }
+    B2R_LAUNCH_CHECK();
+    return B2R_OK;
+}
+
+extern "C" int b2r_field_upload(b2r_ctx* ctx, void* d_field, const void* h_global)
+{
+    B2R_CHECK(ctx && h_global, B2R_E_ARG, "null argument");
+    if (int rc = b2r_check_field(ctx, d_field)) return rc;
+    if (int rc = b2r_activate(ctx)) return rc;
+    size_t bytes = (size_t)ctx->g.env_x * ctx->g.env_y * ctx->g.env_z * ctx->g.elem_size;
+    void* tmp = nullptr;
+    B2R_HIP(hipMalloc(&tmp, bytes));
+    int rc = B2R_OK;
+    if (hipMemcpyAsync(tmp, h_global, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+        b2r_set_error("upload copy failed");
+        rc = B2R_E_HIP;
+    }
+    if (!rc) rc = owned_copy<true>(ctx, d_field, tmp);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(tmp);
+    return rc;
+}
+
+extern "C" int b2r_field_download(b2r_ctx* ctx, const void* d_field, void* h_global)
+{
+    B2R_CHECK(ctx && h_global, B2R_E_ARG, "null argument");
+    if (int rc = b2r_check_field(ctx, d_field)) return rc;
+    if (int rc = b2r_activate(ctx)) return rc;
+    size_t bytes = (size_t)ctx->g.env_x * ctx->g.env_y * ctx->g.env_z * ctx->g.elem_size;
+    void* tmp = nullptr;
+    B2R_HIP(hipMalloc(&tmp, bytes));
+    int rc = owned_copy<false>(ctx, const_cast<void*>(d_field), tmp);
+    if (!rc && hipMemcpyAsync(h_global, tmp, bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) {
+        b2r_set_error("download copy failed");
+        rc = B2R_E_HIP;
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(tmp);
+    return rc;
+}
+
+static int block_copy(b2r_ctx* ctx, void* d_field, int block, void* h_block, bool up)
+{
+    B2R_CHECK(ctx && h_block, B2R_E_ARG, "null argument");
+    if (int rc = b2r_check_field(ctx, d_field)) return rc;
+    B2R_CHECK(block >= 0 && block < ctx->k.nblk, B2R_E_ARG, "block %d out of range", block);
+    if (int rc = b2r_activate(ctx)) return rc;
+    size_t es = ctx->g.elem_size;
+    char* dev = (char*)d_field + (size_t)cell_index(ctx->k, block, 0, 0, 0) * es;
+    size_t w = (size_t)ctx->L.S[0] * es;
+    size_t rows = (size_t)ctx->L.S[1] * ctx->L.S[2];
+    if (up)
+        B2R_HIP(hipMemcpy2DAsync(dev, (size_t)ctx->L.pitch * es, h_block, w, w, rows,
+                                 hipMemcpyHostToDevice, ctx->stream));
+    else
+        B2R_HIP(hipMemcpy2DAsync(h_block, w, dev, (size_t)ctx->L.pitch * es, w, rows,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    B2R_HIP(hipStreamSynchronize(ctx->stream));
+    return B2R_OK;
+}
+
+extern "C" int b2r_block_upload(b2r_ctx* ctx, void* d_field, int block, const void* h_block)
+{
+    return block_copy(ctx, d_field, block, const_cast<void*>(h_block), true);
+}
+
+extern "C" int b2r_block_download(b2r_ctx* ctx, const void* d_field, int block, void* h_block)
+{
+    return block_copy(ctx, const_cast<void*>(d_field), block, h_block, false);
+}
+
+extern "C" int b2r_field_copy(b2r_ctx* ctx, void* d_dst, const void* d_src)
+{
+    B2R_CHECK(ctx, B2R_E_ARG, "null context");
+    if (int rc = b2r_check_field(ctx, d_dst)) return rc;
+    if (int rc = b2r_check_field(ctx, d_src)) return rc;
+    B2R_HIP(hipMemcpyAsync(d_dst, d_src, (size_t)ctx->L.field_elems * ctx->g.elem_size,
+                           hipMemcpyDeviceToDevice, ctx->stream));
+    return B2R_OK;
+}
+
+// ---------------------------------------------------------------------------
+// seeded generators (owned cells, keyed by the global cell index)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t sm64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// kind 0: uniform f32, kind 1: bernoulli u8
+template <int KIND>
+__global__ void gen_kernel(void* __restrict__ f, KGeo g, uint64_t key, uint32_t p_q24)
+{
+    long long gx = (long long)g.B[0] * g.nb[0], gy = (long long)g.B[1] * g.nb[1];
+    long long n = gx * gy * (long long)g.B[2] * g.nb[2];
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        int x = (int)(i % gx);
+        int y = (int)((i / gx) % gy);
+        int z = (int)(i / (gx * gy));
+        int b = ((z / g.B[2]) * g.nb[1] + y / g.B[1]) * g.nb[0] + x / g.B[0];
+        long long c = cell_index(g, b, g.h + x % g.B[0], g.h + y % g.B[1], g.h + z % g.B[2]);
+        uint32_t d = (uint32_t)(sm64(key + (uint64_t)i) >> 40);
+        if (KIND == 0) ((float*)f)[c] = (float)d * (1.0f / 16777216.0f);
+        else ((uint8_t*)f)[c] = d < p_q24 ? 1 : 0;
+    }
+}
+
+extern "C" int b2r_field_init_uniform(b2r_ctx* ctx, void* d_field, uint64_t seed)
+{
+    B2R_CHECK(ctx, B2R_E_ARG, "null context");
+    if (int rc = b2r_check_field(ctx, d_field)) return rc;
+    B2R_CHECK(ctx->g.elem_size == 4, B2R_E_UNSUPPORTED, "uniform init needs f32 cells");
+    uint64_t key = seed;  // host-side splitmix of the seed
+    key += 0x9E3779B97F4A7C15ull;
+    key = (key ^ (key >> 30)) * 0xBF58476D1CE4E5B9ull;
+    key = (key ^ (key >> 27)) * 0x94D049BB133111EBull;
+    key = key ^ (key >> 31);
+    long long n = (long long)ctx->g.env_x * ctx->g.env_y * ctx->g.env_z;
+    gen_kernel<0><<<launch_blocks(n), 256, 0, ctx->stream>>>(d_field, ctx->k, key, 0);
+    B2R_LAUNCH_CHECK();
+    return B2R_OK;
+}
+
+extern "C" int b2r_field_init_bernoulli(b2r_ctx* ctx, void* d_field, uint64_t seed, uint32_t p_q24)
+{
+    B2R_CHECK(ctx, B2R_E_ARG, "null context");
+    if (int rc = b2r_check_field(ctx, d_field)) return rc;
+    B2R_CHECK(ctx->g.elem_size == 1, B2R_E_UNSUPPORTED, "bernoulli init needs u8 cells");
+    uint64_t key = seed;
+    key += 0x9E3779B97F4A7C15ull;
+    key = (key ^ (key >> 30)) * 0xBF58476D1CE4E5B9ull;
+    key = (key ^ (key >> 27)) * 0x94D049BB133111EBull;
+    key = key ^ (key >> 31);
+    long long n = (long long)ctx->g.env_x * ctx->g.env_y * ctx->g.env_z;
+    gen_kernel<1><<<launch_blocks(n), 256, 0, ctx->stream>>>(d_field, ctx->k, key, p_q24);
+    B2R_LAUNCH_CHECK();
+    return B2R_OK;
+}
+
+__global__ void blobs_kernel(float* __restrict__ f, KGeo g, int nb, const double* __restrict__ amp,
+                             const double* __restrict__ ex, const double* __restrict__ ey,
+                             const double* __restrict__ ez)
+{
+    long long gx = (long long)g.B[0] * g.nb[0], gy = (long long)g.B[1] * g.nb[1];
+    long long gz = (long long)g.B[2] * g.nb[2];
+    long long n = gx * gy * gz;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        int x = (int)(i % gx);
+        int y = (int)((i / gx) % gy);
+        int z = (int)(i / (gx * gy));
+        double acc = 0.0;
+        for (int k = 0; k < nb; ++k) {
+            double t = __dmul_rn(amp[k], ex[k * gx + x]);
+            t = __dmul_rn(t, ey[k * gy + y]);
+            t = __dmul_rn(t, ez[k * gz + z]);
+            acc = __dadd_rn(acc, t);
+        }
+        int b = ((z / g.B[2]) * g.nb[1] + y / g.B[1]) * g.nb[0] + x / g.B[0];
+        f[cell_index(g, b, g.h + x % g.B[0], g.h + y % g.B[1], g.h + z % g.B[2])] = (float)acc;
+    }
+}
+
+extern "C" int b2r_field_init_blobs(b2r_ctx* ctx, void* d_field, int nb, const double* amp,
+                                    const double* ex, const double* ey, const double* ez)
+{
+    B2R_CHECK(ctx && amp && ex && ey && ez && nb >= 1, B2R_E_ARG, "bad argument");
+    if (int rc = b2r_check_field(ctx, d_field)) return rc;
+    B2R_CHECK(ctx->g.elem_size == 4, B2R_E_UNSUPPORTED, "blob init needs f32 cells");
+    if (int rc = b2r_activate(ctx)) return rc;
+    size_t nx = (size_t)nb * ctx->g.env_x, ny = (size_t)nb * ctx->g.env_y,
+           nz = (size_t)nb * ctx->g.env_z;
+    double* d = nullptr;
+    B2R_HIP(hipMalloc(&d, sizeof(double) * (nb + nx + ny + nz)));
+    (void)hipMemcpyAsync(d, amp, sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream);
+    (void)hipMemcpyAsync(d + nb, ex, sizeof(double) * nx, hipMemcpyHostToDevice, ctx->stream);
+    (void)hipMemcpyAsync(d + nb + nx, ey, sizeof(double) * ny, hipMemcpyHostToDevice, ctx->stream);
+    (void)hipMemcpyAsync(d + nb + nx + ny, ez, sizeof(double) * nz, hipMemcpyHostToDevice, ctx->stream);
+    long long n = (long long)ctx->g.env_x * ctx->g.env_y * ctx->g.env_z;
+    blobs_kernel<<<launch_blocks(n), 256, 0, ctx->stream>>>((float*)d_field, ctx->k, nb, d, d + nb,
+                                                           d + nb + nx, d + nb + nx + ny);
+    hipError_t e = hipGetLastError();
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    B2R_CHECK(e == hipSuccess, B2R_E_HIP, "blobs kernel: %s", hipGetErrorString(e));
+    return B2R_OK;
+}
+
+// ---------------------------------------------------------------------------
+// halo exchange
+// ---------------------------------------------------------------------------
+// Face slab of dim: n0 x n1 in-face cells times h slabs.
+//   X: in-face (y,z) over B_Y x B_Z    Y: (x,z) over S_X x B_Z    Z: (x,y) over S_X x S_Y
+__host__ __device__ inline void face_dims(const KGeo& g, int dim, int& n0, int& n1)
+{
+    if (dim == 0) { n0 = g.B[1]; n1 = g.B[2]; }
+    else if (dim == 1) { n0 = g.S[0]; n1 = g.B[2]; }
+    else { n0 = g.S[0]; n1 = g.S[1]; }
+}
+
+// Local coordinates of face cell (j, q0, q1) at slab coordinate a along dim.
+__device__ __forceinline__ void face_cell(const KGeo& g, int dim, int a, int q0, int q1,
+                                          int& x, int& y, int& z)
+{
+    if (dim == 0) { x = a; y = g.h + q0; z = g.h + q1; }
+    else if (dim == 1) { x = q0; y = a; z = g.h + q1; }
+    else { x = q0; y = q1; z = a; }
+}
+
+// Message element of (j, q0, q1) in the layouts of b2r_halo_sync.cu:
+// X [j][y][z] (z fastest, :210), Y [j][z][x] (:243), Z [j][y][x] (:275).
+__device__ __forceinline__ long long msg_elem(int dim, int j, int q0, int q1, int n0, int n1)
+{
+    if (dim == 0) return ((long long)j * n0 + q0) * n1 + q1;
+    return ((long long)j * n1 + q1) * n0 + q0;
+}
+
+struct FieldList {
+    void* p[16];
+};
+
+// Direct exchange along dim: halo cell <- owned cell of the neighbour block,
+// threads run x-fastest over (q0 or x) so Y and Z faces are coalesced.
+template <typename T>
+__global__ void direct_exchange_kernel(FieldList fl, KGeo g, int dim)
+{
+    T* f = (T*)fl.p[blockIdx.y];
+    int n0, n1;
+    face_dims(g, dim, n0, n1);
+    const int h = g.h;
+    // enumerate (b, side, j, q0, q1) with the in-row coordinate fastest
+    long long per_side = (long long)h * n0 * n1;
+    long long total = per_side * 2 * g.nblk;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        long long r = t / per_side;
+        long long q = t - r * per_side;
+        int side = (int)(r & 1);
+        int b = (int)(r >> 1);
+        int src = neighbour_block(g, b, dim, side);
+        if (src < 0) continue;
+        int j, q0, q1;
+        if (dim == 0) {  // x is the slab axis: run j fastest (contiguous in memory)
+            j = (int)(q % h);
+            long long u = q / h;
+            q0 = (int)(u % n0);
+            q1 = (int)(u / n0);
+        } else {
+            q0 = (int)(q % n0);
+            long long u = q / n0;
+            j = (int)(u % h);
+            q1 = (int)(u / h);
+        }
+        int Bd = g.B[dim];
+        int a_dst = side ? Bd + h + j : j;
+        int a_src = side ? h + j : Bd + j;
+        int x, y, z, xs, ys, zs;
+        face_cell(g, dim, a_dst, q0, q1, x, y, z);
+        face_cell(g, dim, a_src, q0, q1, xs, ys, zs);
+        f[cell_index(g, b, x, y, z)] = f[cell_index(g, src, xs, ys, zs)];
+    }
+}
+
+// pack: message (b, side) <- owned slab of block b
+template <typename T>
+__global__ void pack_kernel(const T* __restrict__ f, T* __restrict__ msgs, KGeo g, int dim,
+                            long long me)
+{
+    int n0, n1;
+    face_dims(g, dim, n0, n1);
+    const int h = g.h;
+    long long total = me * 2 * g.nblk;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        long long r = t / me;
+        long long e = t - r * me;
+        int side = (int)(r & 1);
+        int b = (int)(r >> 1);
+        int j, q0, q1;
+        if (dim == 0) {
+            q1 = (int)(e % n1);
+            long long u = e / n1;
+            q0 = (int)(u % n0);
+            j = (int)(u / n0);
+        } else {
+            q0 = (int)(e % n0);
+            long long u = e / n0;
+            q1 = (int)(u % n1);
+            j = (int)(u / n1);
+        }
+        int a = side ? g.B[dim] + j : h + j;
+        int x, y, z;
+        face_cell(g, dim, a, q0, q1, x, y, z);
+        msgs[t] = f[cell_index(g, b, x, y, z)];
+    }
+}
+
+// unpack: halo slab of block b (side) <- message (neighbour, other side)
+template <typename T>
+__global__ void unpack_kernel(T* __restrict__ f, const T* __restrict__ msgs, KGeo g, int dim,
+                              long long me)
+{
+    int n0, n1;
+    face_dims(g, dim, n0, n1);
+    const int h = g.h;
+    long long total = me * 2 * g.nblk;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        long long r = t / me;
+        long long e = t - r * me;
+        int side = (int)(r & 1);
+        int b = (int)(r >> 1);
+        int src = neighbour_block(g, b, dim, side);
+        if (src < 0) continue;
+        int j, q0, q1;
+        if (dim == 0) {
+            q1 = (int)(e % n1);
+            long long u = e / n1;
+            q0 = (int)(u % n0);
+            j = (int)(u / n0);
+        } else {
+            q0 = (int)(e % n0);
+            long long u = e / n0;
+            q1 = (int)(u % n1);
+            j = (int)(u / n1);
+        }
+        int a = side ? g.B[dim] + h + j : j;
+        int x, y, z;
+        face_cell(g, dim, a, q0, q1, x, y, z);
+        f[cell_index(g, b, x, y, z)] = msgs[((long long)src * 2 + (1 - side)) * me + e];
+    }
+}
+
+static long long msg_elems_k(const KGeo& g, int dim)
+{
+    int n0, n1;
+    face_dims(g, dim, n0, n1);
+    return (long long)g.h * n0 * n1;
+}
+
+extern "C" long long b2r_msg_elems(const b2r_ctx* ctx, int dim)
+{
+    if (!ctx || dim < 0 || dim > 2) {
+        b2r_set_error("bad argument");
+        return B2R_E_ARG;
+    }
+    return msg_elems_k(ctx->k, dim);
+}
+
+extern "C" int b2r_pack(b2r_ctx* ctx, int dim, const void* d_field, void* d_msgs)
+{
+    B2R_CHECK(ctx && d_msgs && dim >= 0 && dim <= 2, B2R_E_ARG, "bad argument");
+    if (int rc = b2r_check_field(ctx, d_field)) return rc;
+    long long me = msg_elems_k(ctx->k, dim);
+    int nb = launch_blocks(me * 2 * ctx->k.nblk);
+    switch (ctx->g.elem_size) {
+    case 1: pack_kernel<uint8_t><<<nb, 256, 0, ctx->stream>>>((const uint8_t*)d_field, (uint8_t*)d_msgs, ctx->k, dim, me); break;
+    case 4: pack_kernel<uint32_t><<<nb, 256, 0, ctx->stream>>>((const uint32_t*)d_field, (uint32_t*)d_msgs, ctx->k, dim, me); break;
+    default: pack_kernel<uint64_t><<<nb, 256, 0, ctx->stream>>>((const uint64_t*)d_field, (uint64_t*)d_msgs, ctx->k, dim, me); break;
+    }
+    B2R_LAUNCH_CHECK();
+    return B2R_OK;
+}
+
+extern "C" int b2r_unpack(b2r_ctx* ctx, int dim, void* d_field, const void* d_msgs)
+{
+    B2R_CHECK(ctx && d_msgs && dim >= 0 && dim <= 2, B2R_E_ARG, "bad argument");
+    if (int rc = b2r_check_field(ctx, d_field)) return rc;
+    long long me = msg_elems_k(ctx->k, dim);
+    int nb = launch_blocks(me * 2 * ctx->k.nblk);
+    switch (ctx->g.elem_size) {
+    case 1: unpack_kernel<uint8_t><<<nb, 256, 0, ctx->stream>>>((uint8_t*)d_field, (const uint8_t*)d_msgs, ctx->k, dim, me); break;
+    case 4: unpack_kernel<uint32_t><<<nb, 256, 0, ctx->stream>>>((uint32_t*)d_field, (const uint32_t*)d_msgs, ctx->k, dim, me); break;
+    default: unpack_kernel<uint64_t><<<nb, 256, 0, ctx->stream>>>((uint64_t*)d_field, (const uint64_t*)d_msgs, ctx->k, dim, me); break;
+    }
+    B2R_LAUNCH_CHECK();
+    return B2R_OK;
+}
+
+extern "C" int b2r_boundary_updating(b2r_ctx* ctx, int time_step, void* d_field)
+{
+    (void)time_step;
+    B2R_CHECK(ctx, B2R_E_ARG, "null context");
+    if (int rc = b2r_check_field(ctx, d_field)) return rc;
+    if (int rc = b2r_activate(ctx)) return rc;
+    // one message buffer sized for the largest direction (b2r_halo_sync.cu:82-84)
+    size_t need = 0;
+    for (int d = 0; d < 3; ++d) {
+        size_t s = (size_t)msg_elems_k(ctx->k, d) * 2 * ctx->k.nblk * ctx->g.elem_size;
+        need = s > need ? s : need;
+    }
+    if (ctx->msgbuf_bytes < need) {
+        if (ctx->msgbuf) {
+            B2R_HIP(hipStreamSynchronize(ctx->stream));
+            B2R_HIP(hipFree(ctx->msgbuf));
+            ctx->msgbuf = nullptr;
+        }
+        B2R_HIP(hipMalloc(&ctx->msgbuf, need));
+        ctx->msgbuf_bytes = need;
+    }
+    for (int d = 0; d < 3; ++d) {
+        if (int rc = b2r_pack(ctx, d, d_field, ctx->msgbuf)) return rc;
+        if (int rc = b2r_unpack(ctx, d, d_field, ctx->msgbuf)) return rc;
+    }
+    return B2R_OK;
+}
+
+int b2r_direct_exchange(b2r_ctx* ctx, void* const* fields, int nfields)
+{
+    for (int base = 0; base < nfields; base += 16) {
+        int n = nfields - base < 16 ? nfields - base : 16;
+        FieldList fl;
+        for (int i = 0; i < n; ++i) fl.p[i] = fields[base + i];
+        for (int d = 0; d < 3; ++d) {
+            long long total = msg_elems_k(ctx->k, d) * 2 * ctx->k.nblk;
+            dim3 grid(launch_blocks(total, 256, 16384), n);
+            switch (ctx->g.elem_size) {
+            case 1: direct_exchange_kernel<uint8_t><<<grid, 256, 0, ctx->stream>>>(fl, ctx->k, d); break;
+            case 4: direct_exchange_kernel<uint32_t><<<grid, 256, 0, ctx->stream>>>(fl, ctx->k, d); break;
+            default: direct_exchange_kernel<uint64_t><<<grid, 256, 0, ctx->stream>>>(fl, ctx->k, d); break;
+            }
+            B2R_LAUNCH_CHECK();
+        }
+    }
+    return B2R_OK;
+}
+
+extern "C" int b2r_boundary_updating_direct(b2r_ctx* ctx, int time_step, void* const* d_fields,
+                                            int nfields)
+{
+    (void)time_step;
+    B2R_CHECK(ctx && d_fields && nfields >= 1, B2R_E_ARG, "bad argument");
+    for (int i = 0; i < nfields; ++i)
+        if (int rc = b2r_check_field(ctx, d_fields[i])) return rc;
+    if (int rc = b2r_activate(ctx)) return rc;
+    return b2r_direct_exchange(ctx, d_fields, nfields);
+}
+
+// ---------------------------------------------------------------------------
+// measurement helpers
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restrict__ src,
+                                                          uint4* __restrict__ dst, size_t n)
+{
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+extern "C" int b2r_stream_copy(b2r_ctx* ctx, void* d_dst, const void* d_src, size_t bytes,
+                               int iters, float* ms)
+{
+    B2R_CHECK(ctx && d_dst && d_src && iters >= 1, B2R_E_ARG, "bad argument");
+    B2R_CHECK(bytes % 16 == 0, B2R_E_ARG, "bytes must be a multiple of 16");
+    if (int rc = b2r_activate(ctx)) return rc;
+    size_t n = bytes / 16;
+    int nb = launch_blocks((long long)((n + 3) / 4), 256, 256 * 64);
+    if (int rc = b2r_time_begin(ctx, ms)) return rc;
+    for (int it = 0; it < iters; ++it)
+        stream_copy_kernel<<<nb, 256, 0, ctx->stream>>>((const uint4*)d_src, (uint4*)d_dst, n);
+    B2R_LAUNCH_CHECK();
+    return b2r_time_end(ctx, ms);
+}
+
+extern "C" int b2r_dev_alloc(b2r_ctx* ctx, size_t bytes, void** d_ptr)
+{
+    B2R_CHECK(ctx && d_ptr, B2R_E_ARG, "null argument");
+    if (int rc = b2r_activate(ctx)) return rc;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        b2r_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        return B2R_E_NOMEM;
+    }
+    ctx->scratch.insert(p);
+    *d_ptr = p;
+    return B2R_OK;
+}
+
+extern "C" int b2r_dev_free(b2r_ctx* ctx, void* d_ptr)
+{
+    B2R_CHECK(ctx && ctx->scratch.count(d_ptr), B2R_E_ARG, "not a scratch allocation");
+    B2R_HIP(hipStreamSynchronize(ctx->stream));
+    B2R_HIP(hipFree(d_ptr));
+    ctx->scratch.erase(d_ptr);
+    return B2R_OK;
+}
+
+extern "C" int b2r_memcpy_h2d(b2r_ctx* ctx, void* d_dst, const void* h_src, size_t bytes)
+{
+    B2R_CHECK(ctx && d_dst && h_src, B2R_E_ARG, "null argument");
+    B2R_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    B2R_HIP(hipStreamSynchronize(ctx->stream));
+    return B2R_OK;
+}
+
+extern "C" int b2r_memcpy_d2h(b2r_ctx* ctx, void* h_dst, const void* d_src, size_t bytes)
+{
+    B2R_CHECK(ctx && h_dst && d_src, B2R_E_ARG, "null argument");
+    B2R_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    B2R_HIP(hipStreamSynchronize(ctx->stream));
+    return B2R_OK;
+}
+
+// Back-to-back launches of the inner-step kernel (b2r_i cycling 0..R-1, so the
+// average covers a whole B2R period), alternating the two buffers, between two
+// events on the context stream.
+extern "C" int b2r_time_steps(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
+                              const b2r_heat_params* p, int iters, float* ms)
+{
+    B2R_CHECK(ctx && ms && iters >= 1 && (kind == 0 || kind == 1), B2R_E_ARG, "bad argument");
+    if (int rc = b2r_check_field(ctx, d_a)) return rc;
+    if (int rc = b2r_check_field(ctx, d_b)) return rc;
+    B2R_CHECK(kind == 1 || p, B2R_E_ARG, "heat timing needs parameters");
+    if (int rc = b2r_activate(ctx)) return rc;
+    if (int rc = b2r_time_begin(ctx, ms)) return rc;
+    for (int it = 0; it < iters; ++it) {
+        const void* in = (it & 1) ? d_b : d_a;
+        void* out = (it & 1) ? d_a : d_b;
+        const int i = it % ctx->g.R;
+        int rc = kind == 0 ? b2r_heat_launch(ctx, in, out, p, i) : b2r_gol_launch(ctx, in, out, i);
+        if (rc) return rc;
+    }
+    return b2r_time_end(ctx, ms);
+}

@@ -1,0 +1,238 @@
+"""Benchmark of the B2R stencil hot path on one MI355X.
+
+Headline (value): 3D Game of Life, 512^3 u8 cells Bernoulli(0.3) seed 1,
+periodic, 2x2x2 subdomain blocks with R-deep halos exchanged on the device every
+R generations (R in {1,2,4}, best reported), --steps generations timed.  A step
+is one generation of the whole 512^3 grid; GLUP/s = cells * steps / time.
+
+Also reported (same JSON line, "also"): measured STREAM copy bandwidth; heat
+delta1 at 768^3 (FP32, % of measured STREAM); heat delta4 768^3 with 8 Gaussian
+blobs on 2x2x2 blocks; heat delta1 128^3 (the reference CPU case) next to the
+reference's own CPU code; CPU baselines on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--quick]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GLUP/s & % HBM roofline, 1 MI355X; batched+adjoint vs plain at stated FP64 err"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def read_traffic():
+    """Per-launch HBM bytes of the dominant kernel from the committed rocprofv3
+    --pmc summary (profiles/traffic.txt: 'key: value' lines), or None."""
+    p = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(p):
+        return None
+    d = {}
+    for line in open(p):
+        if ":" in line and not line.lstrip().startswith("#"):
+            k, v = line.split(":", 1)
+            d[k.strip()] = v.strip()
+    return d
+
+
+def gol_bench(A, R, steps, warmup):
+    env = (512, 512, 512)
+    with A.Context(env, (2, 2, 2), R, 1, A.BOUNDARY_PERIODIC, 1) as c:
+        a, b = c.field(), c.field()
+        c.init_bernoulli(a, 1, int(0.3 * 16777216))
+        if warmup:
+            c.gol_run(a, b, warmup)
+        c.init_bernoulli(a, 1, int(0.3 * 16777216))
+        c.sync()
+        t0 = time.perf_counter()
+        _, ms = c.gol_run(a, b, steps, timed=True)
+        c.sync()
+        wall = time.perf_counter() - t0
+        # the step kernel alone, HIP events on the context stream
+        kiters = 200
+        kms = c.time_steps(1, a, b, kiters) / kiters
+    cells = 512 ** 3
+    return dict(R=R, ms=ms, wall_s=wall, glups=cells * steps / (ms * 1e-3) / 1e9,
+                kernel_ms=kms)
+
+
+def stream_bw(A):
+    nbytes = 4 << 30
+    with A.Context((16, 16, 16), (1, 1, 1), 1, 1, 0, 4) as c:
+        src = c.dev_alloc(nbytes)
+        dst = c.dev_alloc(nbytes)
+        c.stream_copy(dst, src, nbytes, 2)
+        iters = 10
+        ms = c.stream_copy(dst, src, nbytes, iters)
+    return 2.0 * nbytes * iters / (ms * 1e-3) / 1e9
+
+
+def heat_bench(A, env, blocks, R, D, k, steps, init, warmup=3):
+    p = A.heat_params(D, A.HEAT_F32, k)
+    with A.Context(env, blocks, R, D, A.BOUNDARY_FROZEN, 4) as c:
+        a, b = c.field(), c.field()
+        init(c, a)
+        c.heat_run(a, b, p, warmup)
+        init(c, a)
+        c.sync()
+        _, ms = c.heat_run(a, b, p, steps, timed=True)
+        kiters = 20
+        kms = c.time_steps(0, a, b, kiters, p) / kiters
+    cells = env[0] * env[1] * env[2]
+    return dict(ms=ms, glups=cells * steps / (ms * 1e-3) / 1e9, kernel_ms=kms,
+                kernel_gbs=8.0 * cells / (kms * 1e-3) / 1e9)
+
+
+def blob_tables(env, nb=8, sigma=20.0, seed=2):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0, [env[0], env[1], env[2]], size=(nb, 3))
+    amp = rng.uniform(0.5, 2.0, nb)
+    tabs = []
+    for a in range(3):
+        x = np.arange(env[a], dtype=np.float64)
+        tabs.append(np.exp(-((x[None, :] - centres[:, a:a + 1]) ** 2) / (2 * sigma * sigma)).ravel())
+    return amp, tabs
+
+
+def cpu_gol_baseline(budget_s=10.0):
+    """The oracle's Game of Life (C, OpenMP over z) on the host cores: 512^3
+    periodic, as many generations as fit the time budget."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    u = O.bernoulli((512, 512, 512), 1, 0.3)
+    gens = 0
+    t0 = time.perf_counter()
+    while True:
+        u = O.gol(u, 1, True)
+        gens += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or gens >= 1000:
+            break
+    return dict(value=512 ** 3 * gens / el / 1e9, unit="GLUP/s", cores=O.lib.orc_num_threads(),
+                kind="port", sample=f"512^3 periodic Game of Life, {gens} generations "
+                                    f"({el:.1f} s), oracle C code, OpenMP")
+
+
+def cpu_heat_reference(steps=100):
+    """The reference's own cpu_fdm_heat_diffuse1 (heat-3D.cpp:48-62, built from
+    its sources into oracle/_ref) on 128^3 + zero frame, 100 steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    import numpy as np
+    if not O.ref_available():
+        return None
+    u = O.frame(O.uniform((128, 128, 128), 0), 1)
+    v = u.copy()
+    t0 = time.perf_counter()
+    r = O.ref_lib()
+    pu, pv = u.ctypes.data, v.ctypes.data
+    for _ in range(steps):
+        r.ref_cpu_fdm_heat_diffuse1(pu, pv, 130, 130, 130, 0)
+        pu, pv = pv, pu
+    el = time.perf_counter() - t0
+    return dict(value=128 ** 3 * steps / el / 1e9, unit="GLUP/s", cores=O.lib.orc_num_threads(),
+                kind="reference", sample=f"128^3 heat delta1, {steps} steps ({el:.2f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--quick", action="store_true", help="headline only, no extras")
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the halo "
+              "exchange between GPUs is designed but not built", file=sys.stderr)
+        sys.exit(2)
+
+    from b2r_amd import _abi as A  # raises if the HIP library is missing
+
+    runs = []
+    for R in (1, 2, 4):
+        log(f"gol 512^3 R={R}")
+        runs.append(gol_bench(A, R, args.steps, args.warmup))
+        log(json.dumps(runs[-1]))
+    best = max(runs, key=lambda r: r["glups"])
+    cells = 512 ** 3
+    achieved = 2.0 * cells / (best["kernel_ms"] * 1e-3) / 1e9
+    tr = read_traffic()
+    traffic = None
+    if tr and "gol_step_bytes_per_launch" in tr:
+        traffic = float(tr["gol_step_bytes_per_launch"])
+
+    also = {}
+    if not args.quick:
+        log("stream")
+        bw = stream_bw(A)
+        also["stream_copy_GBs"] = round(bw, 1)
+        log("heat delta1 768^3")
+        h1 = heat_bench(A, (768, 768, 768), (1, 1, 1), 1, 1, (0.1, 0.1, 0.1), 100,
+                        lambda c, f: c.init_uniform(f, 0))
+        h1["pct_stream_kernel"] = round(100 * h1["kernel_gbs"] / bw, 1)
+        h1["pct_stream_run"] = round(100 * h1["glups"] * 8 / bw, 1)
+        also["heat_delta1_768"] = h1
+        log(json.dumps(h1))
+        log("heat delta4 768^3 blobs 2x2x2")
+        amp, (ex, ey, ez) = blob_tables((768, 768, 768))
+        h4 = heat_bench(A, (768, 768, 768), (2, 2, 2), 1, 4, (0.05, 0.05, 0.05), 100,
+                        lambda c, f: c.init_blobs(f, amp, ex, ey, ez))
+        h4["pct_stream_run"] = round(100 * h4["glups"] * 8 / bw, 1)
+        also["heat_delta4_768_2x2x2"] = h4
+        log(json.dumps(h4))
+        h0 = heat_bench(A, (128, 128, 128), (1, 1, 1), 1, 1, (0.1, 0.1, 0.1), 100,
+                        lambda c, f: c.init_uniform(f, 0))
+        also["heat_delta1_128"] = h0
+        ref = cpu_heat_reference()
+        if ref:
+            also["heat_delta1_128_cpu_reference"] = ref
+
+    log("cpu baseline")
+    cpu = cpu_gol_baseline(args.cpu_budget)
+    out = {
+        "metric": METRIC,
+        "value": round(best["glups"], 3),
+        "unit": "GLUP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(best["ms"] / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (SplitMix64 Bernoulli(0.3), seed 1)",
+        "config": {"workload": "gol3d_512^3_u8_periodic_2x2x2", "R": best["R"],
+                   "generations": args.steps,
+                   "glups_by_R": {str(r["R"]): round(r["glups"], 3) for r in runs}},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": "gol_step_kernel", "kernel_ms": round(best["kernel_ms"], 5),
+                     "bytes_per_launch_algorithmic": 2 * cells},
+        "cpu_baseline": cpu,
+        "also": also,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,210 @@
+/*
+ * b2r_hip.h -- C ABI of the MI355X (gfx950) B2R stencil library.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns 0 on
+ * success or a negative B2R_E* code (never exits), and records a message that
+ * b2r_last_error() returns.  A context owns its device memory and one HIP stream;
+ * the caller owns host memory.  A context is not thread-safe.
+ *
+ * Geometry (the B2R scheme of the reference, b2r.cpp:47-67, b2r.h:72-87):
+ *   the global owned grid env_x*env_y*env_z is split into
+ *   block_dim_x*block_dim_y*block_dim_z subdomain blocks of B = env/block_dim
+ *   owned cells, each stored with a halo of h = radius*R cells per side, so a
+ *   block is S = B + 2h cells per axis.  All blocks of a field live on one GPU in
+ *   one allocation, block b = bz*BDx*BDy + by*BDx + bx (b2r_comm.hpp:25-28).
+ *   Within a block, cell (x,y,z) with 0 <= x < S_x ... sits at element
+ *       b*block_stride + x_offset + z*slice + y*pitch + x
+ *   (x fastest, as gg_id in heat-3D.cpp:42-45).  For 4- and 8-byte cells
+ *   x_offset = 0 and pitch = S_x, so one block is exactly the reference's dense
+ *   S_x*S_y*S_z grid; 1-byte cells get a row offset and pitch that put the first
+ *   owned cell on a 16-byte boundary.
+ *
+ * Boundary kinds at the global faces:
+ *   B2R_BOUNDARY_FROZEN   halo cells outside the global grid are never written
+ *                         (Dirichlet frame; the edge-aware semantics of
+ *                         fdm_heat_diffuse_delta1_verify, heat-3D.cu:57-158);
+ *   B2R_BOUNDARY_PERIODIC global faces wrap (the _PERIODIC_ switch of
+ *                         b2r_config.h:12, unimplemented in the reference).
+ */
+#ifndef B2R_HIP_H
+#define B2R_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ------------------------------------------------------- */
+#define B2R_OK            0
+#define B2R_E_ARG        -1   /* bad argument (null pointer, bad value)       */
+#define B2R_E_GEOM       -2   /* geometry check failed (divisibility, R*D)   */
+#define B2R_E_HIP        -3   /* a HIP runtime call failed                   */
+#define B2R_E_NOMEM      -4   /* device allocation failed                    */
+#define B2R_E_UNSUPPORTED -5  /* radius / mode / dtype not supported          */
+
+#define B2R_BOUNDARY_FROZEN   0
+#define B2R_BOUNDARY_PERIODIC 1
+
+/* Heat arithmetic modes.
+ * B2R_HEAT_F32: u' = u + ((kx*Lx + ky*Ly) + kz*Lz) in FP32, L_a the radius-D
+ *   second difference evaluated outermost tap first (heat-3D.cu:314-318 order).
+ * B2R_HEAT_REF: the reference expression with its FP64 promotion (D10):
+ *   u' = (float)((double)u + adt*((Lx/hx2 + Ly/hy2) + Lz/hz2)), L_a in FP32
+ *   exactly as heat-3D.cpp:57-60 (D=1) and heat-3D.cu:314-318, 409-413, 513-517. */
+#define B2R_HEAT_F32 0
+#define B2R_HEAT_REF 1
+
+typedef struct b2r_ctx b2r_ctx;
+
+typedef struct b2r_geom {
+    int env_x, env_y, env_z;                     /* ENV_DIM_X/Y/Z (b2r_config.h:6-8)  */
+    int block_dim_x, block_dim_y, block_dim_z;   /* BLOCK_DIM_X/Y/Z (b2r.cpp:140-143) */
+    int R;                                       /* B2R_R (b2r.cpp:139)               */
+    int radius;                                  /* per-step radius: B2R_D for heat,
+                                                    1 for GoL, 4 for elastic FD4      */
+    int boundary;                                /* B2R_BOUNDARY_*                    */
+    int elem_size;                               /* bytes per cell: 1, 4 or 8         */
+} b2r_geom;
+
+typedef struct b2r_layout {
+    int B[3];              /* owned cells per block and axis (B2R_B_X/Y/Z)            */
+    int S[3];              /* padded block size (B2R_BLOCK_SIZE_X/Y/Z)               */
+    int halo;              /* h = radius*R                                            */
+    int nblocks;           /* block_dim_x*block_dim_y*block_dim_z                     */
+    int x_offset;          /* elements before x = 0 in each row                       */
+    int pitch;             /* elements per row                                        */
+    long long slice;       /* elements per z plane (pitch*S_y)                        */
+    long long block_stride;/* elements per block                                      */
+    long long field_elems; /* elements per field (nblocks*block_stride)               */
+    int elem_size;
+} b2r_layout;
+
+typedef struct b2r_heat_params {
+    int   D;           /* stencil radius 1..4: fdm_heat_diffuse_delta1..4            */
+    int   mode;        /* B2R_HEAT_F32 or B2R_HEAT_REF                               */
+    float k[3];        /* F32 mode: alpha*dt/h_a^2 per axis                          */
+    double adt;        /* REF mode: Alpha*Dt (heat-3D.cu:9-10)                      */
+    double h2[3];      /* REF mode: Dx*Dx, Dy*Dy, Dz*Dz (heat-3D.cu:11-13)          */
+} b2r_heat_params;
+
+/* ---- library / context ---------------------------------------------------- */
+const char* b2r_last_error(void);
+int b2r_version(void);
+
+/* Validates and creates: the checks of b2r.cpp:154-167 and simu_env_init
+ * (b2r.cpp:54-59: radius*R <= min(B)/2), then the per-field layout.  Replaces the
+ * geometry globals of b2r.h:72-87 and b2r_sync_init (b2r_halo_sync.cu:74-111). */
+int b2r_create(const b2r_geom* geom, int device, b2r_ctx** out);
+int b2r_destroy(b2r_ctx* ctx);  /* b2r_sync_finalize (b2r_halo_sync.cu:121-128) */
+int b2r_get_layout(const b2r_ctx* ctx, b2r_layout* out);
+int b2r_synchronize(b2r_ctx* ctx);
+
+/* Which global faces of a FROZEN grid are frozen: bit 2*axis+side (axis 0..2 =
+ * x,y,z; side 0 = minus, 1 = plus).  Default 0x3F (all).  A cleared bit makes
+ * that face open: the blocks on it update their halo there like an interior
+ * one, as the reference's general kernels do (heat-3D.cu:173-235, gol-3D.cu:50),
+ * and nothing refreshes it; the edge flags ef of heat_3D_gpu_main /
+ * fdm_heat_diffuse_delta1_verify (heat-3D.cu:57-114, 583) map to it (ef=-1:
+ * minus face frozen, ef=1: plus face frozen, ef=0: both open). */
+int b2r_set_frozen_faces(b2r_ctx* ctx, int mask);
+
+/* Neighbour-exists flags of one block, order left,right,up,down,infront,behind:
+ * 0 = none (global face), 1 = neighbour, -1 = periodic wrap (b2r.h:37).
+ * Replaces b2r_sync_flag(int rank) (b2r_halo_sync.cu:140-183). */
+int b2r_sync_flag(const b2r_ctx* ctx, int rank, signed char flags[6]);
+
+/* ---- fields (device memory owned by the context) -------------------------- */
+int b2r_field_alloc(b2r_ctx* ctx, void** d_field);
+int b2r_field_free(b2r_ctx* ctx, void* d_field);
+/* Sets every cell of every block (halos included) to *value (elem_size bytes). */
+int b2r_field_fill(b2r_ctx* ctx, void* d_field, const void* value);
+/* Copies the global owned grid env_x*env_y*env_z (x fastest) in or out of the
+ * owned regions of all blocks; halos are untouched. */
+int b2r_field_upload(b2r_ctx* ctx, void* d_field, const void* h_global);
+int b2r_field_download(b2r_ctx* ctx, const void* d_field, void* h_global);
+/* Raw copy of one padded block as a dense S_x*S_y*S_z host grid, the h_in of
+ * heat_3D_gpu_main / gol_3D_gpu_main (heat-3D.cu:583, gol-3D.cu:119). */
+int b2r_block_upload(b2r_ctx* ctx, void* d_field, int block, const void* h_block);
+int b2r_block_download(b2r_ctx* ctx, const void* d_field, int block, void* h_block);
+int b2r_field_copy(b2r_ctx* ctx, void* d_dst, const void* d_src);
+
+/* Seeded synthetic inputs, keyed by the global cell index gid = x+ENVx*(y+ENVy*z)
+ * through SplitMix64 (identical on CPU and GPU):
+ *   uniform:   f32 cells u = (splitmix64(splitmix64(seed)+gid) >> 40) * 2^-24
+ *   bernoulli: u8 cells  (splitmix64(splitmix64(seed)+gid) >> 40) < p_q24
+ * Owned cells only; halos untouched. */
+int b2r_field_init_uniform(b2r_ctx* ctx, void* d_field, uint64_t seed);
+int b2r_field_init_bernoulli(b2r_ctx* ctx, void* d_field, uint64_t seed, uint32_t p_q24);
+/* Separable sum of nb Gaussian blobs, evaluated in FP64 and rounded to f32:
+ * u(x,y,z) = sum_k amp[k]*ex[k*env_x+x]*ey[k*env_y+y]*ez[k*env_z+z], the sum taken
+ * k = 0..nb-1 left to right.  ex/ey/ez are host tables of nb*env_* doubles. */
+int b2r_field_init_blobs(b2r_ctx* ctx, void* d_field, int nb, const double* amp,
+                         const double* ex, const double* ey, const double* ez);
+
+/* ---- halo exchange (boundary_updating) ------------------------------------ */
+/* Ordered X -> Y -> Z exchange of the h-deep halos of every block of one field,
+ * through device pack/unpack kernels and device message buffers in the message
+ * layout of b2r_comm.hpp (X: [x][y][z] over B_Y*h*B_Z, Y: [y][z][x] over
+ * S_X*h*B_Z, Z: [z][y][x] over S_X*S_Y*h).  Replaces boundary_updating
+ * (b2r_halo_sync.cu:442-508); time_step only labels the exchange (MPI tag). */
+int b2r_boundary_updating(b2r_ctx* ctx, int time_step, void* d_field);
+/* Same result with each halo cell copied straight from its source block (no
+ * message buffers): the device-direct path of boundary_updating_direct
+ * (b2r_halo_sync.cu:389-431).  Several fields in one launch per direction. */
+int b2r_boundary_updating_direct(b2r_ctx* ctx, int time_step, void* const* d_fields, int nfields);
+/* Message-level building blocks: pack (sub_matrix_pack_{x,y,z},
+ * b2r_halo_sync.cu:195-282) every block's two faces along dim (0,1,2) into
+ * d_msgs, and unpack (sub_matrix_unpack_{x,y,z}, :292-379) the messages
+ * addressed to each block.  d_msgs holds nblocks*2 messages of
+ * b2r_msg_elems(dim) elements: message (b, side) is what block b sends to its
+ * side-0 (minus) or side-1 (plus) neighbour. */
+long long b2r_msg_elems(const b2r_ctx* ctx, int dim);
+int b2r_pack(b2r_ctx* ctx, int dim, const void* d_field, void* d_msgs);
+int b2r_unpack(b2r_ctx* ctx, int dim, void* d_field, const void* d_msgs);
+
+/* ---- heat diffusion (heat-3D.cu) ------------------------------------------ */
+/* One inner B2R step b2r_i of fdm_heat_diffuse_delta{D} over every block:
+ * out = step(in) on the region [D*(b2r_i+1), S-1-D*(b2r_i+1)] per axis, clipped
+ * at frozen global faces to the owned range (heat-3D.cu:173-235, 250-521,
+ * 57-158).  Cells outside the region are not written.  geom.radius must be D. */
+int b2r_heat_step(b2r_ctx* ctx, const void* d_in, void* d_out,
+                  const b2r_heat_params* p, int b2r_i);
+/* nsteps steps of the B2R loop of b2r.cpp:179-221 with heat_3D_gpu_main's R-step
+ * ping-pong (heat-3D.cu:602-628): every R steps boundary_updating_direct, then
+ * R inner steps.  The result is in d_a if *result_in_b == 0, else in d_b.
+ * d_a and d_b must hold identical halos (the frozen frame) on entry.
+ * fused != 0 runs the R inner steps of each exchange period in one launch
+ * (temporal blocking in LDS) when supported.  *ms receives the device time. */
+int b2r_heat_run(b2r_ctx* ctx, void* d_a, void* d_b, const b2r_heat_params* p,
+                 int nsteps, int fused, int* result_in_b, float* ms);
+
+/* ---- 3D Game of Life (gol-3D.cu) ------------------------------------------ */
+/* One generation (26-neighbour count n; alive & n<=1 -> 0; dead & n==5 -> 1;
+ * alive & n>=8 -> 0; otherwise unchanged; gol-3D.cu:83-99) over the inner-step
+ * region of b2r_i (gol-3D.cu:50, 58).  u8 cells in {0,1}; geom.radius must be 1. */
+int b2r_gol_step(b2r_ctx* ctx, const void* d_in, void* d_out, int b2r_i);
+int b2r_gol_run(b2r_ctx* ctx, void* d_a, void* d_b, int nsteps, int fused,
+                int* result_in_b, float* ms);
+
+/* ---- measurement helpers ---------------------------------------------------- */
+/* iters device-to-device copies of bytes (a STREAM-copy kernel): *ms total. */
+int b2r_stream_copy(b2r_ctx* ctx, void* d_dst, const void* d_src, size_t bytes,
+                    int iters, float* ms);
+/* iters back-to-back launches of the inner-step kernel (b2r_i = it % R) alternating
+ * d_a -> d_b -> d_a ..., bracketed by HIP events on the context stream; *ms is the
+ * total device time (kind 0: heat with *p, kind 1: Game of Life).  No counterpart
+ * in the reference: it times the kernel alone for the roofline figure. */
+int b2r_time_steps(b2r_ctx* ctx, int kind, void* d_a, void* d_b, const b2r_heat_params* p,
+                   int iters, float* ms);
+/* Scratch device buffer owned by the context (freed by b2r_destroy). */
+int b2r_dev_alloc(b2r_ctx* ctx, size_t bytes, void** d_ptr);
+int b2r_dev_free(b2r_ctx* ctx, void* d_ptr);
+int b2r_memcpy_h2d(b2r_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+int b2r_memcpy_d2h(b2r_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* B2R_HIP_H */

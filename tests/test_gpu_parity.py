@@ -1,0 +1,289 @@
+"""GPU parity through the C ABI: layout, generators, halo exchange, heat delta1-4
+and Game of Life against the CPU oracle (bit-exact) and the reference's golden
+fixtures; R- and decomposition-invariance."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from _util import Lay, expected_layout, id_field, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+A = None
+
+
+def setup_module(module):
+    global A
+    from b2r_amd import _abi
+    A = _abi
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    if a.dtype == np.float32:
+        return np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    return np.array_equal(a, b)
+
+
+# ------------------------------------------------------------------ layout
+@pytest.mark.parametrize("env,blocks,R,radius,esize", [
+    ((16, 8, 12), (2, 1, 2), 2, 1, 1), ((20, 20, 20), (2, 2, 2), 2, 2, 4),
+    ((24, 16, 8), (1, 2, 1), 1, 4, 8), ((64, 32, 32), (2, 2, 2), 4, 1, 1)])
+def test_layout_matches_rule(env, blocks, R, radius, esize):
+    say("layout", env, blocks, R, radius, esize)
+    with A.Context(env, blocks, R, radius, 0, esize) as c:
+        lay = c.layout
+        want = expected_layout(env, blocks, R, radius, esize)
+        assert list(lay.B) == want["B"] and list(lay.S) == want["S"]
+        for k in ("halo", "nblocks", "x_offset", "pitch", "slice", "block_stride", "field_elems"):
+            assert getattr(lay, k) == want[k], k
+
+
+# ------------------------------------------------------------------ generators
+def test_generators_bitwise():
+    env = (24, 10, 18)
+    say("generators", env)
+    with A.Context(env, (2, 1, 3), 1, 1, 0, 4) as c:
+        f = c.field()
+        c.init_uniform(f, 0)
+        assert bits_equal(c.download(f), O.uniform((18, 10, 24), 0))
+        rng = np.random.default_rng(0)
+        nb = 3
+        amp = rng.uniform(0.5, 2, nb)
+        ex, ey, ez = (rng.uniform(0, 1, nb * n) for n in env)
+        c.init_blobs(f, amp, ex, ey, ez)
+        assert bits_equal(c.download(f), O.blobs(env, amp, ex, ey, ez))
+    with A.Context(env, (1, 2, 3), 1, 1, 1, 1) as c:
+        f = c.field()
+        c.init_bernoulli(f, 1, int(0.3 * 16777216))
+        assert np.array_equal(c.download(f), O.bernoulli((18, 10, 24), 1, 0.3))
+
+
+def test_upload_download_roundtrip_and_errors():
+    env = (12, 8, 10)
+    with A.Context(env, (2, 2, 1), 1, 1, 0, 4) as c:
+        f = c.field()
+        u = O.uniform((10, 8, 12), 5)
+        c.upload(f, u)
+        assert bits_equal(c.download(f), u)
+        with pytest.raises(A.B2RError):
+            c.heat_step(f, f, A.heat_params(1), 0)          # in == out
+        with pytest.raises(A.B2RError):
+            c.heat_step(f, c.field(), A.heat_params(2), 0)  # D != radius
+        with pytest.raises(A.B2RError):
+            c.gol_step(f, c.field(), 0)                     # wrong cell type
+
+
+# ------------------------------------------------------------------ exchange
+EXCH = [
+    ((8, 6, 10), (2, 1, 2), 1, 1, 4, False),
+    ((8, 6, 10), (2, 1, 2), 2, 1, 4, True),
+    ((12, 12, 12), (3, 2, 1), 1, 2, 8, False),
+    ((12, 16, 16), (1, 2, 2), 2, 2, 8, True),
+    ((16, 8, 12), (2, 2, 3), 1, 1, 1, True),
+    ((16, 8, 12), (2, 1, 1), 3, 1, 1, False),
+    ((8, 8, 8), (1, 1, 1), 2, 2, 4, True),
+    ((40, 24, 32), (2, 2, 2), 2, 2, 4, True),
+]
+
+
+def _id_raw(lay, blocks, esize):
+    dt = {1: np.uint8, 4: np.float32, 8: np.float64}[esize]
+    raw = id_field(lay, blocks, np.float64, -1)
+    if esize == 1:
+        return np.where(raw == -1, 255, raw % 251).astype(np.uint8)
+    return raw.astype(dt)
+
+
+@pytest.mark.parametrize("env,blocks,R,radius,esize,periodic", EXCH)
+def test_exchange_matches_oracle(env, blocks, R, radius, esize, periodic):
+    say("exchange", env, blocks, R, radius, esize, periodic)
+    with A.Context(env, blocks, R, radius, int(periodic), esize) as c:
+        lay = c.layout
+        raw = _id_raw(lay, blocks, esize)
+        want = O.exchange(raw, lay, blocks, esize, periodic)
+        f = c.field()
+        c.raw_upload(f, raw)
+        c.boundary_updating(f, 0)
+        assert np.array_equal(c.raw(f), want), "pack/unpack exchange"
+        g = c.field()
+        c.raw_upload(g, raw)
+        c.boundary_updating_direct([g], 0)
+        assert np.array_equal(c.raw(g), want), "direct exchange"
+        # message contents in the reference's message layout
+        for dim in range(3):
+            me = c.msg_elems(dim)
+            msgs = c.dev_alloc(me * 2 * lay.nblocks * esize)
+            c.raw_upload(f, raw)
+            c.pack(dim, f, msgs)
+            got = c.d2h(msgs, me * 2 * lay.nblocks * esize, raw.dtype)
+            assert np.array_equal(got, O.pack(raw, lay, blocks, esize, periodic, dim)), dim
+            c.dev_free(msgs)
+
+
+# ------------------------------------------------------------------ heat
+HEAT = [
+    # env, blocks, R, D, periodic, nsteps
+    ((20, 18, 16), (1, 1, 1), 1, 1, False, 9),
+    ((20, 18, 16), (2, 1, 2), 2, 1, False, 9),
+    ((24, 16, 20), (2, 2, 1), 3, 1, True, 10),
+    ((32, 24, 32), (2, 1, 2), 2, 2, False, 8),
+    ((24, 24, 24), (2, 2, 2), 1, 2, True, 7),
+    ((36, 24, 30), (2, 1, 1), 2, 3, False, 6),
+    ((30, 24, 36), (1, 2, 2), 1, 3, True, 5),
+    ((32, 24, 32), (2, 1, 2), 2, 4, False, 6),
+    ((40, 32, 48), (1, 1, 2), 1, 4, True, 5),
+    ((130, 66, 70), (2, 1, 1), 2, 1, False, 6),
+    ((96, 80, 72), (2, 2, 2), 2, 4, False, 4),
+]
+
+
+def _heat_case(env, blocks, R, D, periodic, nsteps, mode, k=(0.021, 0.033, 0.017),
+               adt=0.1 * 0.1, h2=(1.0, 1.0, 0.25)):
+    p = A.heat_params(D, mode, k, adt, h2)
+    with A.Context(env, blocks, R, D, int(periodic), 4) as c:
+        a, b = c.field(), c.field()
+        c.init_uniform(a, 17)
+        res, _ = c.heat_run(a, b, p, nsteps)
+        return c.download(res)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("env,blocks,R,D,periodic,nsteps", HEAT)
+def test_heat_bitwise_vs_oracle(env, blocks, R, D, periodic, nsteps, mode):
+    say("heat", env, blocks, "R", R, "D", D, "periodic", periodic, "mode", mode)
+    k = (0.021, 0.033, 0.017)
+    got = _heat_case(env, blocks, R, D, periodic, nsteps, mode, k)
+    owned = O.uniform((env[2], env[1], env[0]), 17)
+    g = owned if periodic else O.frame(owned, D)
+    want = O.heat_f32(g, D, mode, k, 0.1 * 0.1, (1.0, 1.0, 0.25), nsteps, periodic)
+    if not periodic:
+        want = O.unframe(want, D)
+    assert bits_equal(got, want)
+    kk = k if mode == 0 else (0.01, 0.01, 0.04)
+    f64 = O.heat_f64(g.astype(np.float64), D, kk, nsteps, periodic)
+    if not periodic:
+        f64 = O.unframe(f64, D)
+    assert rel_l2(got, f64) < 1e-6
+
+
+@pytest.mark.parametrize("fname", sorted(f for f in os.listdir(os.path.join(os.path.dirname(__file__), "golden"))
+                                         if f.startswith("heat_delta1_ref")))
+def test_heat_ref_mode_matches_reference_golden(fname):
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", fname))
+    inp, out, steps = z["input"], z["output"], int(z["steps"])
+    owned = O.unframe(inp, 1)
+    env = (owned.shape[2], owned.shape[1], owned.shape[0])
+    say("golden", fname, env)
+    with A.Context(env, (1, 1, 1), 1, 1, 0, 4) as c:
+        a, b = c.field(), c.field()
+        c.upload(a, owned)
+        res, _ = c.heat_run(a, b, A.heat_params(1, A.HEAT_REF), steps)
+        assert bits_equal(c.download(res), O.unframe(out, 1))
+
+
+def test_heat_R_and_decomposition_invariance():
+    env = (96, 64, 80)
+    outs = []
+    for blocks, R in [((1, 1, 1), 1), ((2, 2, 2), 1), ((2, 1, 2), 3), ((1, 2, 1), 4)]:
+        say("heat invariance", blocks, R)
+        outs.append(_heat_case(env, blocks, R, 2, False, 12, 0))
+    for o in outs[1:]:
+        assert bits_equal(o, outs[0])
+
+
+def test_heat_step_single_inner_region():
+    # one inner step b2r_i=1 with R=2 updates [2D, S-1-2D] only, the rest of
+    # the output keeps its previous contents
+    env, D, R = (16, 16, 16), 1, 2
+    with A.Context(env, (1, 1, 1), R, D, 1, 4) as c:
+        a, b = c.field(), c.field()
+        c.init_uniform(a, 3)
+        c.boundary_updating(a, 0)
+        c.fill(b, -7.0)
+        c.heat_step(a, b, A.heat_params(D), 1)
+        lay = c.layout
+        from _util import block_view
+        blk_in = block_view(c.raw(a), lay, 0)
+        blk = block_view(c.raw(b), lay, 0)
+        S = lay.S[0]
+        lo, hi = D * 2, S - 1 - D * 2
+        inner = blk[lo:hi + 1, lo:hi + 1, lo:hi + 1]
+        assert not np.any(inner == -7.0)
+        mask = np.ones_like(blk, bool)
+        mask[lo:hi + 1, lo:hi + 1, lo:hi + 1] = False
+        assert np.all(blk[mask] == -7.0)
+        want = O.heat_f32(blk_in, D, 0, (0.1, 0.1, 0.1), 0, (1, 1, 1), 1, periodic=False)
+        # the frozen-frame oracle on the padded block equals the interior update
+        w = want[lo:hi + 1, lo:hi + 1, lo:hi + 1]
+        assert bits_equal(inner, w)
+
+
+# ------------------------------------------------------------------ Game of Life
+GOL = [
+    ((64, 40, 48), (2, 2, 2), 1, True, 12),
+    ((64, 40, 48), (2, 2, 2), 2, True, 12),
+    ((64, 40, 48), (2, 2, 2), 4, True, 12),
+    ((64, 40, 48), (2, 2, 2), 4, False, 13),
+    ((50, 30, 20), (1, 3, 2), 3, True, 7),
+    ((36, 22, 26), (1, 1, 1), 2, False, 9),
+    ((100, 36, 36), (4, 1, 2), 1, True, 5),
+]
+
+
+@pytest.mark.parametrize("env,blocks,R,periodic,nsteps", GOL)
+def test_gol_bitwise_vs_oracle(env, blocks, R, periodic, nsteps):
+    say("gol", env, blocks, "R", R, "periodic", periodic, "steps", nsteps)
+    with A.Context(env, blocks, R, 1, int(periodic), 1) as c:
+        a, b = c.field(), c.field()
+        c.init_bernoulli(a, 1, int(0.3 * 16777216))
+        res, _ = c.gol_run(a, b, nsteps)
+        got = c.download(res)
+    u = O.bernoulli((env[2], env[1], env[0]), 1, 0.3)
+    g = u if periodic else O.frame(u, 1)
+    want = O.gol(g, nsteps, periodic)
+    if not periodic:
+        want = O.unframe(want, 1)
+    assert np.array_equal(got, want)
+
+
+def test_gol_known_patterns():
+    env = (32, 32, 32)
+    u = np.zeros((32, 32, 32), np.uint8)
+    u[10:12, 10:12, 10:12] = 1   # still life
+    u[20, 5, 7] = 1              # dies
+    u[2, 2, 2] = u[2, 2, 3] = u[2, 3, 2] = u[3, 2, 2] = u[4, 4, 4] = 1
+    with A.Context(env, (2, 2, 2), 1, 1, 1, 1) as c:
+        a, b = c.field(), c.field()
+        c.upload(a, u)
+        res, _ = c.gol_run(a, b, 1)
+        got = c.download(res)
+    assert np.array_equal(got, O.gol(u, 1, periodic=True))
+    assert got[10:12, 10:12, 10:12].all() and got[20, 5, 7] == 0 and got[3, 3, 3] == 1
+
+
+def test_gol_full_size_R_invariance_and_population():
+    # 512^3, 2x2x2 periodic: R = 1, 2, 4 agree bit for bit; population checked
+    # against the oracle for the first generations
+    env = (512, 512, 512)
+    outs = {}
+    for R in (1, 2, 4):
+        say("gol 512^3 R", R)
+        with A.Context(env, (2, 2, 2), R, 1, 1, 1) as c:
+            a, b = c.field(), c.field()
+            c.init_bernoulli(a, 1, int(0.3 * 16777216))
+            res, _ = c.gol_run(a, b, 8)
+            outs[R] = c.download(res)
+    assert np.array_equal(outs[1], outs[2]) and np.array_equal(outs[1], outs[4])
+    u = O.bernoulli((512, 512, 512), 1, 0.3)
+    want = O.gol(u, 8, periodic=True)
+    assert np.array_equal(outs[1], want)
