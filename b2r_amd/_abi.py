@@ -55,6 +55,11 @@ class Layout(C.Structure):
                 ("field_elems", C.c_longlong), ("elem_size", C.c_int)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("heat_ry", C.c_int), ("heat_zc", C.c_int), ("gol_zc", C.c_int),
+                ("gol_step_zc", C.c_int)]
+
+
 class HeatParams(C.Structure):
     _fields_ = [("D", C.c_int), ("mode", C.c_int), ("k", C.c_float * 3),
                 ("adt", C.c_double), ("h2", C.c_double * 3)]
@@ -74,6 +79,8 @@ _SIGS = {
     "b2r_synchronize": (_i, [_vp]),
     "b2r_sync_flag": (_i, [_vp, _i, C.POINTER(C.c_byte)]),
     "b2r_set_frozen_faces": (_i, [_vp, _i]),
+    "b2r_set_tuning": (_i, [_vp, C.POINTER(Tuning)]),
+    "b2r_get_tuning": (_i, [_vp, C.POINTER(Tuning)]),
     "b2r_field_alloc": (_i, [_vp, C.POINTER(_vp)]),
     "b2r_field_free": (_i, [_vp, _vp]),
     "b2r_field_fill": (_i, [_vp, _vp, _vp]),
@@ -96,6 +103,25 @@ _SIGS = {
     "b2r_gol_run": (_i, [_vp, _vp, _vp, _i, _i, _pi, _pf]),
     "b2r_stream_copy": (_i, [_vp, _vp, _vp, C.c_size_t, _i, _pf]),
     "b2r_time_steps": (_i, [_vp, _i, _vp, _vp, C.POINTER(HeatParams), _i, _pf]),
+    "b2r_el_create": (_i, [_vp, C.c_float, _pf, C.POINTER(_vp)]),
+    "b2r_el_destroy": (_i, [_vp]),
+    "b2r_el_field": (_i, [_vp, _i, C.POINTER(_vp)]),
+    "b2r_el_set_medium": (_i, [_vp, _vp, _vp, _vp]),
+    "b2r_el_reset": (_i, [_vp]),
+    "b2r_el_enable_displacement": (_i, [_vp]),
+    "b2r_el_set_sources": (_i, [_vp, _i, _vp, _vp, _i, _vp]),
+    "b2r_el_set_receivers": (_i, [_vp, _i, _vp, _i, _i]),
+    "b2r_el_get_receivers": (_i, [_vp, _vp]),
+    "b2r_el_get_state": (_i, [_vp, _i, _vp]),
+    "b2r_el_set_state": (_i, [_vp, _i, _vp]),
+    "b2r_el_forward": (_i, [_vp, _i, _i, _pf]),
+    "b2r_el_forward_batch": (_i, [C.POINTER(_vp), _i, _i, _i, _pf]),
+    "b2r_el_misfit": (_i, [_vp, _vp, C.POINTER(C.c_double)]),
+    "b2r_el_gradient": (_i, [_vp, _vp, _i, _i, C.POINTER(C.c_double), _vp, _vp, _vp]),
+    "b2r_el_gradient_ex": (_i, [_vp, _vp, _i, _i, _i, C.POINTER(C.c_double), _vp, _vp, _vp,
+                                _vp]),
+    "b2r_autotune": (_i, [_vp, _i, _vp, _vp, C.POINTER(HeatParams), C.c_char_p,
+                          C.POINTER(Tuning)]),
     "b2r_dev_alloc": (_i, [_vp, C.c_size_t, C.POINTER(_vp)]),
     "b2r_dev_free": (_i, [_vp, _vp]),
     "b2r_memcpy_h2d": (_i, [_vp, _vp, _vp, C.c_size_t]),
@@ -165,6 +191,24 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        check(_lib.b2r_get_tuning(self.h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        check(_lib.b2r_set_tuning(self.h, C.byref(t)))
+
+    def get_tuning(self):
+        t = Tuning()
+        check(_lib.b2r_get_tuning(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+    def autotune(self, kind, fa, fb, params=None, record=None):
+        t = Tuning()
+        check(_lib.b2r_autotune(self.h, kind, fa, fb, C.byref(params) if params else None,
+                                record.encode() if record else None, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
     def set_frozen_faces(self, mask):
         check(_lib.b2r_set_frozen_faces(self.h, mask))
@@ -302,3 +346,100 @@ def heat_params(D, mode=HEAT_F32, k=(0.1, 0.1, 0.1), adt=0.1 * 0.1, h2=(1.0, 1.0
         p.h2[a] = h2[a]
     p.adt = adt
     return p
+
+
+# ------------------------------------------------------------------ elastic
+EL_FIELDS = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz", "lam", "mu", "b")
+
+
+class ElasticShot:
+    """One elastic shot on a Context of radius 4 (see b2r_el_* in b2r_hip.h)."""
+
+    def __init__(self, ctx: Context, dt, inv_h):
+        self.ctx = ctx
+        ih = (C.c_float * 3)(*inv_h)
+        h = _vp()
+        check(_lib.b2r_el_create(ctx.h, dt, ih, C.byref(h)))
+        self.h = h
+        self.nrec = 0
+        self.rec_nt = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.b2r_el_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_medium(self, lam, mu, b):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (lam, mu, b)]
+        check(_lib.b2r_el_set_medium(self.h, *[_ptr(a) for a in arrs]))
+
+    def reset(self):
+        check(_lib.b2r_el_reset(self.h))
+
+    def enable_displacement(self):
+        check(_lib.b2r_el_enable_displacement(self.h))
+
+    def set_sources(self, xyz, comp, wav):
+        xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        comp = np.ascontiguousarray(comp, np.int32).ravel()
+        wav = np.ascontiguousarray(wav, np.float32)
+        nt = wav.shape[0] if wav.ndim == 2 else (wav.size // max(len(comp), 1))
+        check(_lib.b2r_el_set_sources(self.h, len(comp), _ptr(xyz), _ptr(comp), nt, _ptr(wav)))
+
+    def set_receivers(self, xyz, comp, nt):
+        xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        check(_lib.b2r_el_set_receivers(self.h, len(xyz), _ptr(xyz), comp, nt))
+        self.nrec, self.rec_nt = len(xyz), nt
+
+    def receivers(self):
+        out = np.empty((self.rec_nt, self.nrec), np.float32)
+        check(_lib.b2r_el_get_receivers(self.h, _ptr(out)))
+        return out
+
+    def state(self, which):
+        gx, gy, gz = self.ctx.env
+        a = np.empty((gz, gy, gx), np.float32)
+        idx = which if isinstance(which, int) else (-1 if which == "disp" else EL_FIELDS.index(which))
+        check(_lib.b2r_el_get_state(self.h, idx, _ptr(a)))
+        return a
+
+    def set_state(self, which, a):
+        idx = which if isinstance(which, int) else EL_FIELDS.index(which)
+        a = np.ascontiguousarray(a, np.float32)
+        check(_lib.b2r_el_set_state(self.h, idx, _ptr(a)))
+
+    def forward(self, nsteps, mode=1, timed=False):
+        ms = C.c_float(0.0)
+        check(_lib.b2r_el_forward(self.h, nsteps, mode, C.byref(ms) if timed else None))
+        return ms.value if timed else None
+
+    def misfit(self, obs):
+        obs = np.ascontiguousarray(obs, np.float32)
+        m = C.c_double(0.0)
+        check(_lib.b2r_el_misfit(self.h, _ptr(obs), C.byref(m)))
+        return m.value
+
+    def gradient(self, obs, nsteps, ckpt, from_current=False, adjoint_state=False):
+        """misfit and [g_lam, g_mu, g_b] (plus the 9 adjoint fields at t=0)."""
+        gx, gy, gz = self.ctx.env
+        obs = np.ascontiguousarray(obs, np.float32)
+        g = [np.empty((gz, gy, gx), np.float32) for _ in range(3)]
+        adj = np.empty((9, gz, gy, gx), np.float32) if adjoint_state else None
+        m = C.c_double(0.0)
+        check(_lib.b2r_el_gradient_ex(self.h, _ptr(obs), nsteps, ckpt, int(from_current),
+                                      C.byref(m), *[_ptr(a) for a in g],
+                                      _ptr(adj) if adjoint_state else None))
+        return (m.value, g, adj) if adjoint_state else (m.value, g)
+
+
+def el_forward_batch(shots, nsteps, mode=1, timed=False):
+    arr = (_vp * len(shots))(*[s.h for s in shots])
+    ms = C.c_float(0.0)
+    check(_lib.b2r_el_forward_batch(arr, len(shots), nsteps, mode, C.byref(ms) if timed else None))
+    return ms.value if timed else None

@@ -71,6 +71,7 @@ struct b2r_ctx {
     std::set<void*> scratch;  // allocations made by b2r_dev_alloc
     void* msgbuf;             // boundary_updating message buffers
     size_t msgbuf_bytes;
+    b2r_tuning tune;          // launch shapes (0 = automatic)
 };
 
 int b2r_activate(b2r_ctx* ctx);  // hipSetDevice(ctx->device)
@@ -80,6 +81,8 @@ int b2r_time_begin(b2r_ctx* ctx, float* ms);
 int b2r_time_end(b2r_ctx* ctx, float* ms);
 int b2r_heat_launch(b2r_ctx* ctx, const void* in, void* out, const b2r_heat_params* p, int i);
 int b2r_gol_launch(b2r_ctx* ctx, const void* in, void* out, int i);
+int b2r_gol_fused_launch(b2r_ctx* ctx, const void* in, void* out);
+int b2r_heat_fused_launch(b2r_ctx* ctx, const void* in, void* out, const b2r_heat_params* p);
 
 // ---------------------------------------------------------------------------
 // device helpers

@@ -100,6 +100,7 @@ extern "C" int b2r_create(const b2r_geom* geom, int device, b2r_ctx** out)
     c->device = device;
     c->msgbuf = nullptr;
     c->msgbuf_bytes = 0;
+    memset(&c->tune, 0, sizeof(c->tune));
     b2r_layout& L = c->L;
     memset(&L, 0, sizeof(L));
     for (int a = 0; a < 3; ++a) {
@@ -175,6 +176,24 @@ extern "C" int b2r_synchronize(b2r_ctx* ctx)
 {
     B2R_CHECK(ctx, B2R_E_ARG, "null context");
     B2R_HIP(hipStreamSynchronize(ctx->stream));
+    return B2R_OK;
+}
+
+extern "C" int b2r_set_tuning(b2r_ctx* ctx, const b2r_tuning* t)
+{
+    B2R_CHECK(ctx && t, B2R_E_ARG, "null argument");
+    B2R_CHECK(t->heat_ry == 0 || t->heat_ry == 1 || t->heat_ry == 2 || t->heat_ry == 4 ||
+              t->heat_ry == 8, B2R_E_ARG, "heat_ry must be 0, 1, 2, 4 or 8");
+    B2R_CHECK(t->heat_zc >= 0 && t->gol_zc >= 0 && t->gol_step_zc >= 0, B2R_E_ARG,
+              "negative z chunk");
+    ctx->tune = *t;
+    return B2R_OK;
+}
+
+extern "C" int b2r_get_tuning(const b2r_ctx* ctx, b2r_tuning* t)
+{
+    B2R_CHECK(ctx && t, B2R_E_ARG, "null argument");
+    *t = ctx->tune;
     return B2R_OK;
 }
 
@@ -515,46 +534,66 @@ struct FieldList {
     void* p[16];
 };
 
-// Direct exchange along dim: halo cell <- owned cell of the neighbour block,
-// threads run x-fastest over (q0 or x) so Y and Z faces are coalesced.
-template <typename T>
-__global__ void direct_exchange_kernel(FieldList fl, KGeo g, int dim)
+// Direct exchange along dim: every halo run is copied from the neighbour
+// block with V-byte vectors.  X faces: one run of h cells per (y,z) row of the
+// owned y,z range.  Y and Z faces: whole rows (the full pitch, row padding
+// included, whose contents are unspecified) of the halo rows (y halo x owned z)
+// and halo planes, so they move with 16-byte vectors when the pitch allows.
+template <int V> struct VecT;
+template <> struct VecT<1> { typedef uint8_t T; };
+template <> struct VecT<2> { typedef uint16_t T; };
+template <> struct VecT<4> { typedef uint32_t T; };
+template <> struct VecT<8> { typedef uint2 T; };
+template <> struct VecT<16> { typedef uint4 T; };
+
+template <int V>
+__global__ void direct_exchange_kernel(FieldList fl, KGeo g, int dim, int esize, int nvec,
+                                       long long rows)
 {
-    T* f = (T*)fl.p[blockIdx.y];
-    int n0, n1;
-    face_dims(g, dim, n0, n1);
+    typedef typename VecT<V>::T T;
+    char* f = (char*)fl.p[blockIdx.y];
     const int h = g.h;
-    // enumerate (b, side, j, q0, q1) with the in-row coordinate fastest
-    long long per_side = (long long)h * n0 * n1;
-    long long total = per_side * 2 * g.nblk;
+    const long long total = rows * nvec * 2 * g.nblk;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
-        long long r = t / per_side;
-        long long q = t - r * per_side;
-        int side = (int)(r & 1);
-        int b = (int)(r >> 1);
-        int src = neighbour_block(g, b, dim, side);
+        const int v = (int)(t % nvec);
+        const long long r = t / nvec;
+        const long long row = r % rows;
+        const int bs = (int)(r / rows);
+        const int side = bs & 1, b = bs >> 1;
+        const int src = neighbour_block(g, b, dim, side);
         if (src < 0) continue;
-        int j, q0, q1;
-        if (dim == 0) {  // x is the slab axis: run j fastest (contiguous in memory)
-            j = (int)(q % h);
-            long long u = q / h;
-            q0 = (int)(u % n0);
-            q1 = (int)(u / n0);
+        int xd, xs, yd, ys, zd, zs;
+        if (dim == 0) {
+            yd = ys = h + (int)(row % g.B[1]);
+            zd = zs = h + (int)(row / g.B[1]);
+            xd = side ? g.B[0] + h : 0;
+            xs = side ? h : g.B[0];
+        } else if (dim == 1) {
+            const int j = (int)(row % h);
+            zd = zs = h + (int)(row / h);
+            yd = side ? g.B[1] + h + j : j;
+            ys = side ? h + j : g.B[1] + j;
+            xd = xs = -g.xoff;
         } else {
-            q0 = (int)(q % n0);
-            long long u = q / n0;
-            j = (int)(u % h);
-            q1 = (int)(u / h);
+            yd = ys = (int)(row % g.S[1]);
+            const int j = (int)(row / g.S[1]);
+            zd = side ? g.B[2] + h + j : j;
+            zs = side ? h + j : g.B[2] + j;
+            xd = xs = -g.xoff;
         }
-        int Bd = g.B[dim];
-        int a_dst = side ? Bd + h + j : j;
-        int a_src = side ? h + j : Bd + j;
-        int x, y, z, xs, ys, zs;
-        face_cell(g, dim, a_dst, q0, q1, x, y, z);
-        face_cell(g, dim, a_src, q0, q1, xs, ys, zs);
-        f[cell_index(g, b, x, y, z)] = f[cell_index(g, src, xs, ys, zs)];
+        const long long od = cell_index(g, b, xd, yd, zd) * esize + (long long)v * V;
+        const long long os = cell_index(g, src, xs, ys, zs) * esize + (long long)v * V;
+        *reinterpret_cast<T*>(f + od) = *reinterpret_cast<const T*>(f + os);
     }
+}
+
+// widest power-of-two vector (<= 16 bytes) dividing every byte offset in use
+static int vec_bytes(long long a, long long b, long long c, long long d)
+{
+    int v = 16;
+    while (v > 1 && ((a % v) || (b % v) || (c % v) || (d % v))) v >>= 1;
+    return v;
 }
 
 // pack: message (b, side) <- owned slab of block b
@@ -703,17 +742,34 @@ extern "C" int b2r_boundary_updating(b2r_ctx* ctx, int time_step, void* d_field)
 
 int b2r_direct_exchange(b2r_ctx* ctx, void* const* fields, int nfields)
 {
+    const KGeo& g = ctx->k;
+    const long long es = ctx->g.elem_size;
+    const long long pb = g.pitch * es, sb = g.slice * es, bb = g.bstride * es;
     for (int base = 0; base < nfields; base += 16) {
         int n = nfields - base < 16 ? nfields - base : 16;
         FieldList fl;
         for (int i = 0; i < n; ++i) fl.p[i] = fields[base + i];
         for (int d = 0; d < 3; ++d) {
-            long long total = msg_elems_k(ctx->k, d) * 2 * ctx->k.nblk;
+            int V;
+            long long rows;
+            if (d == 0) {
+                V = vec_bytes(g.h * es, g.B[0] * es, g.xoff * es, pb);
+                V = vec_bytes(V, sb, bb, V);
+                rows = (long long)g.B[1] * g.B[2];
+            } else {
+                V = vec_bytes(pb, sb, bb, 16);
+                rows = d == 1 ? (long long)g.h * g.B[2] : (long long)g.h * g.S[1];
+            }
+            const int nvec = (int)((d == 0 ? g.h * es : pb) / V);
+            const long long total = rows * nvec * 2 * g.nblk;
             dim3 grid(launch_blocks(total, 256, 16384), n);
-            switch (ctx->g.elem_size) {
-            case 1: direct_exchange_kernel<uint8_t><<<grid, 256, 0, ctx->stream>>>(fl, ctx->k, d); break;
-            case 4: direct_exchange_kernel<uint32_t><<<grid, 256, 0, ctx->stream>>>(fl, ctx->k, d); break;
-            default: direct_exchange_kernel<uint64_t><<<grid, 256, 0, ctx->stream>>>(fl, ctx->k, d); break;
+            const int e = (int)es;
+            switch (V) {
+            case 16: direct_exchange_kernel<16><<<grid, 256, 0, ctx->stream>>>(fl, g, d, e, nvec, rows); break;
+            case 8: direct_exchange_kernel<8><<<grid, 256, 0, ctx->stream>>>(fl, g, d, e, nvec, rows); break;
+            case 4: direct_exchange_kernel<4><<<grid, 256, 0, ctx->stream>>>(fl, g, d, e, nvec, rows); break;
+            case 2: direct_exchange_kernel<2><<<grid, 256, 0, ctx->stream>>>(fl, g, d, e, nvec, rows); break;
+            default: direct_exchange_kernel<1><<<grid, 256, 0, ctx->stream>>>(fl, g, d, e, nvec, rows); break;
             }
             B2R_LAUNCH_CHECK();
         }
@@ -805,23 +861,30 @@ extern "C" int b2r_memcpy_d2h(b2r_ctx* ctx, void* h_dst, const void* d_src, size
     return B2R_OK;
 }
 
-// Back-to-back launches of the inner-step kernel (b2r_i cycling 0..R-1, so the
-// average covers a whole B2R period), alternating the two buffers, between two
-// events on the context stream.
+// Back-to-back launches of one kernel between two events on the context
+// stream, alternating the two buffers.  Inner-step kernels cycle b2r_i over
+// 0..R-1 so the average covers a whole B2R period; fused kernels run R steps
+// per launch.
 extern "C" int b2r_time_steps(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
                               const b2r_heat_params* p, int iters, float* ms)
 {
-    B2R_CHECK(ctx && ms && iters >= 1 && (kind == 0 || kind == 1), B2R_E_ARG, "bad argument");
+    B2R_CHECK(ctx && ms && iters >= 1 && kind >= 0 && kind <= 3, B2R_E_ARG, "bad argument");
     if (int rc = b2r_check_field(ctx, d_a)) return rc;
     if (int rc = b2r_check_field(ctx, d_b)) return rc;
-    B2R_CHECK(kind == 1 || p, B2R_E_ARG, "heat timing needs parameters");
+    B2R_CHECK((kind != 0 && kind != 3) || p, B2R_E_ARG, "heat timing needs parameters");
     if (int rc = b2r_activate(ctx)) return rc;
     if (int rc = b2r_time_begin(ctx, ms)) return rc;
     for (int it = 0; it < iters; ++it) {
         const void* in = (it & 1) ? d_b : d_a;
         void* out = (it & 1) ? d_a : d_b;
         const int i = it % ctx->g.R;
-        int rc = kind == 0 ? b2r_heat_launch(ctx, in, out, p, i) : b2r_gol_launch(ctx, in, out, i);
+        int rc;
+        switch (kind) {
+        case 0: rc = b2r_heat_launch(ctx, in, out, p, i); break;
+        case 1: rc = b2r_gol_launch(ctx, in, out, i); break;
+        case 2: rc = b2r_gol_fused_launch(ctx, in, out); break;
+        default: rc = b2r_heat_fused_launch(ctx, in, out, p); break;
+        }
         if (rc) return rc;
     }
     return b2r_time_end(ctx, ms);

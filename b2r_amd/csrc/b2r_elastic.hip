@@ -1,0 +1,974 @@
+// b2r_elastic.hip -- collocated 4th-order elastic velocity-stress propagation
+// (the reference's eqwp model) for gfx950: forward, sources, receivers, FP64
+// misfit, hand-written discrete adjoint with checkpointing.
+//
+// Reference: eqwp_fd4_stress eqwp.cu:341-477, eqwp_fd4_vx/vy/vz :60-331,
+// eqwp_gpu_main :661-714 (stress, then vx, vy, vz per step).  First
+// derivative (-1/12 (f[+2]-f[-2]) + 2/3 (f[+1]-f[-1]))/h (eqwp.cu:22-23),
+// stress law of eqwp.cu:460-466 with per-cell lambda, mu and b = 1/rho.
+//
+// B2R geometry: one step is two dependent radius-2 stages, so a block carries
+// a halo of 4R (radius 4 in the context geometry) and stage s of an R-step
+// window updates [2(s+1), S-1-2(s+1)] (the reference's halo of R*D with D=2 is
+// too shallow, SURVEY defect D6).  Updates are in place: the stress stage reads
+// v and writes sigma, the velocity stage reads sigma and writes v.
+//
+// Two forward paths run the same kernels (identical results):
+//   plain: per shot and step, stress + vx + vy + vz launches and a device
+//          synchronisation (the reference's structure);
+//   batched: one stress and one three-component velocity launch per step for
+//          all shots of a batch (a device table maps workgroups to shots of
+//          different sizes), the time loop captured in HIP graphs.
+// Arithmetic is FP32 without contraction and matches the CPU oracle bit for
+// bit; the misfit is reduced in FP64.
+#include "b2r_common.h"
+
+#include <cstring>
+#include <vector>
+
+enum { F_VX, F_VY, F_VZ, F_SXX, F_SYY, F_SZZ, F_SXY, F_SXZ, F_SYZ, F_LAM, F_MU, F_B, F_NUM };
+
+constexpr int EL_TX = 64, EL_TY = 4;
+constexpr int EL_MAX_BATCH = 64;
+
+struct ElDev {
+    KGeo g;
+    float* f[F_NUM];
+    float* disp;      // optional displacement accumulator (eqwp.cu:231, 234)
+    float* adj[9];    // adjoint state
+    float* fw0[9];    // forward state before the step being reversed
+    float* fw1[9];    // forward state after it
+    float* grad[3];   // d misfit / d lambda, mu, b
+    float dt;
+    float ih[3];
+    int ntx, nty, nzc, zc;
+    int tile0;        // first workgroup of this shot in a batched launch
+    int ntiles;
+};
+
+struct b2r_el_shot {
+    b2r_ctx* ctx;
+    ElDev d;
+    int t;                      // steps taken since the last reset
+    int nsrc = 0, src_nt = 0;
+    int* d_src = nullptr;       // x,y,z,comp per source (global coordinates)
+    float* d_wav = nullptr;     // src_nt x nsrc
+    int nrec = 0, rec_comp = 1, rec_nt = 0;
+    int* d_rec = nullptr;       // x,y,z per receiver
+    float* d_recv = nullptr;    // rec_nt x nrec samples
+    void* fields[F_NUM];
+};
+
+constexpr float EL_C2 = -1.f / 12.f;
+constexpr float EL_C1 = 2.f / 3.f;
+
+__device__ __forceinline__ float fd4(const float* __restrict__ p, long long c, long long s, float ih)
+{
+    return (EL_C2 * (p[c + 2 * s] - p[c - 2 * s]) + EL_C1 * (p[c + s] - p[c - s])) * ih;
+}
+
+// workgroup -> (shot, block, tile) for batched launches
+struct Tile {
+    int shot, b, x, y, zs, ze;
+};
+
+__device__ __forceinline__ Tile decode_tile(const ElDev* __restrict__ tab, int nshot)
+{
+    const int gt = blockIdx.x;
+    int s = 0;
+    while (s + 1 < nshot && tab[s + 1].tile0 <= gt) ++s;
+    const ElDev& d = tab[s];
+    int lt = gt - d.tile0;
+    Tile T;
+    T.shot = s;
+    const int txi = lt % d.ntx;
+    lt /= d.ntx;
+    const int tyi = lt % d.nty;
+    lt /= d.nty;
+    const int zci = lt % d.nzc;
+    T.b = lt / d.nzc;
+    T.x = 2 + txi * EL_TX + threadIdx.x;
+    T.y = 2 + tyi * EL_TY + threadIdx.y;
+    T.zs = 2 + zci * d.zc;
+    T.ze = min(T.zs + d.zc, d.g.S[2] - 2);  // exclusive
+    return T;
+}
+
+// ---------------------------------------------------------------- forward
+// stress stage s = 2i: sigma += dt*C:grad(v) over update_box(radius 2, s)
+__global__ __launch_bounds__(256) void el_stress_kernel(const ElDev* __restrict__ tab, int nshot,
+                                                        int stage)
+{
+    const Tile T = decode_tile(tab, nshot);
+    const ElDev& d = tab[T.shot];
+    if (T.b >= d.g.nblk) return;
+    const KGeo& g = d.g;
+    const Box box = update_box(g, T.b, 2, stage);
+    const int x = T.x, y = T.y;
+    if (x < box.lo[0] || x > box.hi[0] || y < box.lo[1] || y > box.hi[1]) return;
+    const long long boff = (long long)T.b * g.bstride + g.xoff;
+    const float* __restrict__ vx = d.f[F_VX] + boff;
+    const float* __restrict__ vy = d.f[F_VY] + boff;
+    const float* __restrict__ vz = d.f[F_VZ] + boff;
+    const float* __restrict__ lam = d.f[F_LAM] + boff;
+    const float* __restrict__ mu = d.f[F_MU] + boff;
+    float* __restrict__ sxx = d.f[F_SXX] + boff;
+    float* __restrict__ syy = d.f[F_SYY] + boff;
+    float* __restrict__ szz = d.f[F_SZZ] + boff;
+    float* __restrict__ sxy = d.f[F_SXY] + boff;
+    float* __restrict__ sxz = d.f[F_SXZ] + boff;
+    float* __restrict__ syz = d.f[F_SYZ] + boff;
+    const long long sy = g.pitch, sz = g.slice;
+    const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
+    const int z0 = max(T.zs, box.lo[2]), z1 = min(T.ze - 1, box.hi[2]);
+    for (int z = z0; z <= z1; ++z) {
+        const long long c = z * sz + y * sy + x;
+        const float exx = fd4(vx, c, 1, i0);
+        const float eyy = fd4(vy, c, sy, i1);
+        const float ezz = fd4(vz, c, sz, i2);
+        const float dxy = fd4(vx, c, sy, i1) + fd4(vy, c, 1, i0);
+        const float dxz = fd4(vx, c, sz, i2) + fd4(vz, c, 1, i0);
+        const float dyz = fd4(vy, c, sz, i2) + fd4(vz, c, sy, i1);
+        const float l = lam[c], m = mu[c];
+        const float l2m = l + 2.0f * m;
+        sxx[c] = sxx[c] + dt * (l2m * exx + l * (eyy + ezz));
+        syy[c] = syy[c] + dt * (l2m * eyy + l * (exx + ezz));
+        szz[c] = szz[c] + dt * (l2m * ezz + l * (exx + eyy));
+        sxy[c] = sxy[c] + dt * (m * dxy);
+        sxz[c] = sxz[c] + dt * (m * dxz);
+        syz[c] = syz[c] + dt * (m * dyz);
+    }
+}
+
+// velocity stage s = 2i+1 for the components in MASK (1 x, 2 y, 4 z)
+template <int MASK>
+__global__ __launch_bounds__(256) void el_velocity_kernel(const ElDev* __restrict__ tab, int nshot,
+                                                          int stage)
+{
+    const Tile T = decode_tile(tab, nshot);
+    const ElDev& d = tab[T.shot];
+    if (T.b >= d.g.nblk) return;
+    const KGeo& g = d.g;
+    const Box box = update_box(g, T.b, 2, stage);
+    const int x = T.x, y = T.y;
+    if (x < box.lo[0] || x > box.hi[0] || y < box.lo[1] || y > box.hi[1]) return;
+    const long long boff = (long long)T.b * g.bstride + g.xoff;
+    const float* __restrict__ sxx = d.f[F_SXX] + boff;
+    const float* __restrict__ syy = d.f[F_SYY] + boff;
+    const float* __restrict__ szz = d.f[F_SZZ] + boff;
+    const float* __restrict__ sxy = d.f[F_SXY] + boff;
+    const float* __restrict__ sxz = d.f[F_SXZ] + boff;
+    const float* __restrict__ syz = d.f[F_SYZ] + boff;
+    const float* __restrict__ bb = d.f[F_B] + boff;
+    float* __restrict__ vx = d.f[F_VX] + boff;
+    float* __restrict__ vy = d.f[F_VY] + boff;
+    float* __restrict__ vz = d.f[F_VZ] + boff;
+    float* __restrict__ disp = d.disp ? d.disp + boff : nullptr;
+    const long long sy = g.pitch, sz = g.slice;
+    const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
+    const int z0 = max(T.zs, box.lo[2]), z1 = min(T.ze - 1, box.hi[2]);
+    for (int z = z0; z <= z1; ++z) {
+        const long long c = z * sz + y * sy + x;
+        const float tb = dt * bb[c];
+        if (MASK & 1) {
+            const float ax = (fd4(sxx, c, 1, i0) + fd4(sxy, c, sy, i1)) + fd4(sxz, c, sz, i2);
+            vx[c] = vx[c] + tb * ax;
+        }
+        if (MASK & 2) {
+            const float ay = (fd4(sxy, c, 1, i0) + fd4(syy, c, sy, i1)) + fd4(syz, c, sz, i2);
+            vy[c] = vy[c] + tb * ay;
+            if (disp) disp[c] = disp[c] + (dt * tb) * ay;
+        }
+        if (MASK & 4) {
+            const float az = (fd4(sxz, c, 1, i0) + fd4(syz, c, sy, i1)) + fd4(szz, c, sz, i2);
+            vz[c] = vz[c] + tb * az;
+        }
+    }
+}
+
+struct SrcRec {
+    const int* src;     // x,y,z,comp
+    const float* wav;   // nt x nsrc
+    int nsrc, src_nt;
+    const int* rec;     // x,y,z
+    float* recv;        // nt x nrec
+    int nrec, rec_comp, rec_nt;
+};
+
+// Point forces after the velocity stage of inner step i: every block whose
+// stage box holds the source cell adds dt*b*f there, so redundant halo
+// computation between exchanges stays consistent with the owner.
+__global__ void el_inject_kernel(ElDev d, SrcRec sr, int stage, const int* __restrict__ tcount)
+{
+    const int t = *tcount;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= sr.nsrc * d.g.nblk || t >= sr.src_nt) return;
+    const int s = idx / d.g.nblk, b = idx % d.g.nblk;
+    int bc[3];
+    block_coords(d.g, b, bc);
+    const int* p = sr.src + 4 * s;
+    int lc[3];
+    for (int a = 0; a < 3; ++a) lc[a] = p[a] - bc[a] * d.g.B[a] + d.g.h;
+    const Box box = update_box(d.g, b, 2, stage);
+    for (int a = 0; a < 3; ++a)
+        if (lc[a] < box.lo[a] || lc[a] > box.hi[a]) return;
+    const long long c = cell_index(d.g, b, lc[0], lc[1], lc[2]);
+    const float tb = d.dt * d.f[F_B][c];
+    float* v = d.f[p[3]];
+    v[c] = v[c] + tb * sr.wav[(long long)t * sr.nsrc + s];
+}
+
+// Receivers sample their component from the owning block after the step.
+__global__ void el_record_kernel(ElDev d, SrcRec sr, const int* __restrict__ tcount)
+{
+    const int t = *tcount;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= sr.nrec || t >= sr.rec_nt) return;
+    const int* p = sr.rec + 3 * r;
+    int bc[3], lc[3];
+    for (int a = 0; a < 3; ++a) {
+        bc[a] = p[a] / d.g.B[a];
+        lc[a] = p[a] - bc[a] * d.g.B[a] + d.g.h;
+    }
+    const int b = (bc[2] * d.g.nb[1] + bc[1]) * d.g.nb[0] + bc[0];
+    sr.recv[(long long)t * sr.nrec + r] = d.f[sr.rec_comp][cell_index(d.g, b, lc[0], lc[1], lc[2])];
+}
+
+__global__ void el_tick_kernel(int* tcount) { *tcount += 1; }
+
+// ---------------------------------------------------------------- adjoint
+// Reverse of one forward step t (single block, all global faces frozen, owned
+// cells = update region).  Exact transposes on the zero-framed grid: D^T = -D.
+// adjoint of the velocity stage: sh -= dt * D-form of w = b*vh, and
+// g_b += dt * vh . (D sigma'), sigma' = stresses after the stress stage.
+__global__ __launch_bounds__(256) void el_adj_velocity_kernel(const ElDev* __restrict__ tab, int nshot)
+{
+    const Tile T = decode_tile(tab, nshot);
+    const ElDev& d = tab[T.shot];
+    const KGeo& g = d.g;
+    const int h = g.h;
+    const int x = T.x, y = T.y;
+    if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
+    const long long sy = g.pitch, sz = g.slice;
+    const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
+    const long long boff = g.xoff;
+    const float* __restrict__ b = d.f[F_B] + boff;
+    const float* __restrict__ hx = d.adj[0] + boff;
+    const float* __restrict__ hy = d.adj[1] + boff;
+    const float* __restrict__ hz = d.adj[2] + boff;
+    float* sh[6];
+    const float* s1[6];
+    for (int k = 0; k < 6; ++k) {
+        sh[k] = d.adj[3 + k] + boff;
+        s1[k] = d.fw1[3 + k] + boff;
+    }
+    auto dw = [&](const float* hv, long long c, long long s, float ih) {
+        return (EL_C2 * (b[c + 2 * s] * hv[c + 2 * s] - b[c - 2 * s] * hv[c - 2 * s]) +
+                EL_C1 * (b[c + s] * hv[c + s] - b[c - s] * hv[c - s])) * ih;
+    };
+    const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
+    for (int z = z0; z <= z1; ++z) {
+        const long long c = z * sz + y * sy + x;
+        // g_b with the stresses after the stress stage (sxx syy szz sxy sxz syz)
+        const float ax = fd4(s1[0], c, 1, i0) + fd4(s1[3], c, sy, i1) + fd4(s1[4], c, sz, i2);
+        const float ay = fd4(s1[3], c, 1, i0) + fd4(s1[1], c, sy, i1) + fd4(s1[5], c, sz, i2);
+        const float az = fd4(s1[4], c, 1, i0) + fd4(s1[5], c, sy, i1) + fd4(s1[2], c, sz, i2);
+        d.grad[2][boff + c] += dt * (hx[c] * ax + hy[c] * ay + hz[c] * az);
+        sh[0][c] -= dt * dw(hx, c, 1, i0);
+        sh[1][c] -= dt * dw(hy, c, sy, i1);
+        sh[2][c] -= dt * dw(hz, c, sz, i2);
+        sh[3][c] -= dt * (dw(hx, c, sy, i1) + dw(hy, c, 1, i0));
+        sh[4][c] -= dt * (dw(hx, c, sz, i2) + dw(hz, c, 1, i0));
+        sh[5][c] -= dt * (dw(hy, c, sz, i2) + dw(hz, c, sy, i1));
+    }
+}
+
+// adjoint of the stress stage: vh -= dt * D-form of p = C sh, and the
+// lambda / mu gradients with the strain rates of the pre-step velocity.
+__global__ __launch_bounds__(256) void el_adj_stress_kernel(const ElDev* __restrict__ tab, int nshot)
+{
+    const Tile T = decode_tile(tab, nshot);
+    const ElDev& d = tab[T.shot];
+    const KGeo& g = d.g;
+    const int h = g.h;
+    const int x = T.x, y = T.y;
+    if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
+    const long long sy = g.pitch, sz = g.slice;
+    const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
+    const long long boff = g.xoff;
+    const float* __restrict__ lam = d.f[F_LAM] + boff;
+    const float* __restrict__ mu = d.f[F_MU] + boff;
+    const float* sh[6];
+    for (int k = 0; k < 6; ++k) sh[k] = d.adj[3 + k] + boff;
+    float* hx = d.adj[0] + boff;
+    float* hy = d.adj[1] + boff;
+    float* hz = d.adj[2] + boff;
+    const float* v0x = d.fw0[0] + boff;
+    const float* v0y = d.fw0[1] + boff;
+    const float* v0z = d.fw0[2] + boff;
+    // p components at cell q: 0 xx, 1 yy, 2 zz, 3 xy, 4 xz, 5 yz
+    auto pc = [&](int k, long long q) -> float {
+        const float l = lam[q], m = mu[q];
+        if (k < 3) {
+            const float s0 = sh[0][q], s1_ = sh[1][q], s2 = sh[2][q];
+            const float own = k == 0 ? s0 : (k == 1 ? s1_ : s2);
+            return (l + 2.0f * m) * own + l * (s0 + s1_ + s2 - own);
+        }
+        return m * sh[k][q];
+    };
+    auto dp = [&](int k, long long c, long long s, float ih) {
+        return (EL_C2 * (pc(k, c + 2 * s) - pc(k, c - 2 * s)) +
+                EL_C1 * (pc(k, c + s) - pc(k, c - s))) * ih;
+    };
+    const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
+    for (int z = z0; z <= z1; ++z) {
+        const long long c = z * sz + y * sy + x;
+        const float exx = fd4(v0x, c, 1, i0), eyy = fd4(v0y, c, sy, i1), ezz = fd4(v0z, c, sz, i2);
+        const float dxy = fd4(v0x, c, sy, i1) + fd4(v0y, c, 1, i0);
+        const float dxz = fd4(v0x, c, sz, i2) + fd4(v0z, c, 1, i0);
+        const float dyz = fd4(v0y, c, sz, i2) + fd4(v0z, c, sy, i1);
+        const float tr = exx + eyy + ezz;
+        const float a0 = sh[0][c], a1 = sh[1][c], a2 = sh[2][c];
+        d.grad[0][boff + c] += dt * ((a0 + a1 + a2) * tr);
+        d.grad[1][boff + c] += dt * (2.0f * (a0 * exx + a1 * eyy + a2 * ezz) + sh[3][c] * dxy +
+                                     sh[4][c] * dxz + sh[5][c] * dyz);
+        const float ux = dp(0, c, 1, i0) + dp(3, c, sy, i1) + dp(4, c, sz, i2);
+        const float uy = dp(3, c, 1, i0) + dp(1, c, sy, i1) + dp(5, c, sz, i2);
+        const float uz = dp(4, c, 1, i0) + dp(5, c, sy, i1) + dp(2, c, sz, i2);
+        hx[c] -= dt * ux;
+        hy[c] -= dt * uy;
+        hz[c] -= dt * uz;
+    }
+}
+
+// receivers of step t feed the adjoint velocity; sources feed g_b
+__global__ void el_adj_point_kernel(ElDev d, SrcRec sr, const float* __restrict__ res, int t)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const KGeo& g = d.g;
+    if (idx < sr.nrec) {
+        const int* p = sr.rec + 3 * idx;
+        const long long c = cell_index(g, 0, p[0] + g.h, p[1] + g.h, p[2] + g.h);
+        d.adj[sr.rec_comp][c] += res[(long long)t * sr.nrec + idx];
+    }
+}
+
+__global__ void el_adj_source_kernel(ElDev d, SrcRec sr, int t)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= sr.nsrc || t >= sr.src_nt) return;
+    const KGeo& g = d.g;
+    const int* p = sr.src + 4 * i;
+    const long long c = cell_index(g, 0, p[0] + g.h, p[1] + g.h, p[2] + g.h);
+    d.grad[2][c] += d.dt * d.adj[p[3]][c] * sr.wav[(long long)t * sr.nsrc + i];
+}
+
+// res = rec - obs, and per-workgroup FP64 partial sums of res^2 (wave64 shuffles)
+__global__ __launch_bounds__(256) void el_residual_kernel(const float* __restrict__ rec,
+                                                          const float* __restrict__ obs,
+                                                          float* __restrict__ res, long long n,
+                                                          double* __restrict__ part)
+{
+    double acc = 0.0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const float r = rec[i] - obs[i];
+        res[i] = r;
+        acc += (double)r * (double)r;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    __shared__ double ws[4];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+}
+
+__global__ void el_sum_kernel(const double* __restrict__ part, int n, double* __restrict__ out)
+{
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 64) acc += part[i];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (threadIdx.x == 0) *out = acc;
+}
+
+// ---------------------------------------------------------------- host side
+static void el_tiles(ElDev& d)
+{
+    const KGeo& g = d.g;
+    d.ntx = ceil_div(g.S[0] - 4, EL_TX);
+    d.nty = ceil_div(g.S[1] - 4, EL_TY);
+    const int nz = g.S[2] - 4;
+    long long tiles = (long long)d.ntx * d.nty * g.nblk;
+    int nzc = (int)((8192 + tiles - 1) / tiles);
+    int zc = ceil_div(nz, nzc < 1 ? 1 : nzc);
+    if (zc < 8) zc = 8;
+    if (zc > nz) zc = nz;
+    d.zc = zc;
+    d.nzc = ceil_div(nz, zc);
+    d.ntiles = d.ntx * d.nty * d.nzc * g.nblk;
+}
+
+static SrcRec el_srcrec(const b2r_el_shot* s)
+{
+    SrcRec r;
+    r.src = s->d_src;
+    r.wav = s->d_wav;
+    r.nsrc = s->nsrc;
+    r.src_nt = s->src_nt;
+    r.rec = s->d_rec;
+    r.recv = s->d_recv;
+    r.nrec = s->nrec;
+    r.rec_comp = s->rec_comp;
+    r.rec_nt = s->rec_nt;
+    return r;
+}
+
+extern "C" int b2r_el_create(b2r_ctx* ctx, float dt, const float inv_h[3], b2r_el_shot** out)
+{
+    B2R_CHECK(ctx && inv_h && out, B2R_E_ARG, "null argument");
+    B2R_CHECK(ctx->g.elem_size == 4, B2R_E_UNSUPPORTED, "elastic fields are f32");
+    B2R_CHECK(ctx->g.radius == 4, B2R_E_ARG,
+              "elastic needs radius 4 (two radius-2 stages per step), got %d", ctx->g.radius);
+    if (int rc = b2r_activate(ctx)) return rc;
+    b2r_el_shot* s = new b2r_el_shot();
+    s->ctx = ctx;
+    s->t = 0;
+    memset(&s->d, 0, sizeof(s->d));
+    s->d.g = ctx->k;
+    s->d.dt = dt;
+    for (int a = 0; a < 3; ++a) s->d.ih[a] = inv_h[a];
+    for (int k = 0; k < F_NUM; ++k) {
+        if (int rc = b2r_field_alloc(ctx, &s->fields[k])) {
+            delete s;
+            return rc;
+        }
+        s->d.f[k] = (float*)s->fields[k];
+    }
+    el_tiles(s->d);
+    *out = s;
+    return B2R_OK;
+}
+
+extern "C" int b2r_el_destroy(b2r_el_shot* s)
+{
+    if (!s) return B2R_OK;
+    b2r_ctx* c = s->ctx;
+    (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < F_NUM; ++k) b2r_field_free(c, s->fields[k]);
+    if (s->d.disp) b2r_field_free(c, s->d.disp);
+    void* ptrs[] = {s->d_src, s->d_wav, s->d_rec, s->d_recv};
+    for (void* p : ptrs)
+        if (p) b2r_dev_free(c, p);
+    delete s;
+    return B2R_OK;
+}
+
+extern "C" int b2r_el_field(const b2r_el_shot* s, int which, void** d_field)
+{
+    B2R_CHECK(s && d_field && which >= 0 && which < F_NUM, B2R_E_ARG, "bad argument");
+    *d_field = s->fields[which];
+    return B2R_OK;
+}
+
+extern "C" int b2r_el_set_medium(b2r_el_shot* s, const float* lam, const float* mu, const float* b)
+{
+    B2R_CHECK(s && lam && mu && b, B2R_E_ARG, "null argument");
+    b2r_ctx* c = s->ctx;
+    if (int rc = b2r_field_upload(c, s->fields[F_LAM], lam)) return rc;
+    if (int rc = b2r_field_upload(c, s->fields[F_MU], mu)) return rc;
+    if (int rc = b2r_field_upload(c, s->fields[F_B], b)) return rc;
+    // halos of the medium once: it never changes
+    void* m[3] = {s->fields[F_LAM], s->fields[F_MU], s->fields[F_B]};
+    if (int rc = b2r_direct_exchange(c, m, 3)) return rc;
+    B2R_HIP(hipStreamSynchronize(c->stream));
+    return B2R_OK;
+}
+
+extern "C" int b2r_el_reset(b2r_el_shot* s)
+{
+    B2R_CHECK(s, B2R_E_ARG, "null shot");
+    b2r_ctx* c = s->ctx;
+    for (int k = 0; k < 9; ++k)
+        B2R_HIP(hipMemsetAsync(s->fields[k], 0, (size_t)c->L.field_elems * 4, c->stream));
+    if (s->d.disp) B2R_HIP(hipMemsetAsync(s->d.disp, 0, (size_t)c->L.field_elems * 4, c->stream));
+    s->t = 0;
+    return B2R_OK;
+}
+
+extern "C" int b2r_el_enable_displacement(b2r_el_shot* s)
+{
+    B2R_CHECK(s, B2R_E_ARG, "null shot");
+    if (!s->d.disp) {
+        void* p = nullptr;
+        if (int rc = b2r_field_alloc(s->ctx, &p)) return rc;
+        s->d.disp = (float*)p;
+    }
+    return B2R_OK;
+}
+
+static int upload_buf(b2r_ctx* c, void** dst, const void* src, size_t bytes)
+{
+    if (*dst) {
+        if (int rc = b2r_dev_free(c, *dst)) return rc;
+        *dst = nullptr;
+    }
+    if (bytes == 0) return B2R_OK;
+    if (int rc = b2r_dev_alloc(c, bytes, dst)) return rc;
+    B2R_HIP(hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    B2R_HIP(hipStreamSynchronize(c->stream));
+    return B2R_OK;
+}
+
+extern "C" int b2r_el_set_sources(b2r_el_shot* s, int nsrc, const int* xyz, const int* comp, int nt,
+                                  const float* wav)
+{
+    B2R_CHECK(s && nsrc >= 0 && nt >= 0 && (nsrc == 0 || (xyz && comp && wav)), B2R_E_ARG,
+              "bad argument");
+    const b2r_geom& g = s->ctx->g;
+    std::vector<int> t(4 * (size_t)nsrc);
+    for (int i = 0; i < nsrc; ++i) {
+        B2R_CHECK(xyz[3 * i] >= 0 && xyz[3 * i] < g.env_x && xyz[3 * i + 1] >= 0 &&
+                  xyz[3 * i + 1] < g.env_y && xyz[3 * i + 2] >= 0 && xyz[3 * i + 2] < g.env_z,
+                  B2R_E_ARG, "source %d outside the grid", i);
+        B2R_CHECK(comp[i] >= 0 && comp[i] <= 2, B2R_E_ARG, "source component must be 0..2");
+        for (int a = 0; a < 3; ++a) t[4 * i + a] = xyz[3 * i + a];
+        t[4 * i + 3] = comp[i];
+    }
+    if (int rc = upload_buf(s->ctx, (void**)&s->d_src, t.data(), t.size() * sizeof(int))) return rc;
+    if (int rc = upload_buf(s->ctx, (void**)&s->d_wav, wav, (size_t)nt * nsrc * sizeof(float)))
+        return rc;
+    s->nsrc = nsrc;
+    s->src_nt = nt;
+    return B2R_OK;
+}
+
+extern "C" int b2r_el_set_receivers(b2r_el_shot* s, int nrec, const int* xyz, int comp, int nt)
+{
+    B2R_CHECK(s && nrec >= 0 && nt >= 0 && comp >= 0 && comp < 9 && (nrec == 0 || xyz),
+              B2R_E_ARG, "bad argument");
+    const b2r_geom& g = s->ctx->g;
+    for (int i = 0; i < nrec; ++i)
+        B2R_CHECK(xyz[3 * i] >= 0 && xyz[3 * i] < g.env_x && xyz[3 * i + 1] >= 0 &&
+                  xyz[3 * i + 1] < g.env_y && xyz[3 * i + 2] >= 0 && xyz[3 * i + 2] < g.env_z,
+                  B2R_E_ARG, "receiver %d outside the grid", i);
+    if (int rc = upload_buf(s->ctx, (void**)&s->d_rec, xyz, (size_t)nrec * 3 * sizeof(int)))
+        return rc;
+    if (s->d_recv) {
+        if (int rc = b2r_dev_free(s->ctx, s->d_recv)) return rc;
+        s->d_recv = nullptr;
+    }
+    if ((size_t)nrec * nt) {
+        if (int rc = b2r_dev_alloc(s->ctx, (size_t)nrec * nt * sizeof(float), (void**)&s->d_recv))
+            return rc;
+        B2R_HIP(hipMemsetAsync(s->d_recv, 0, (size_t)nrec * nt * sizeof(float), s->ctx->stream));
+    }
+    s->nrec = nrec;
+    s->rec_comp = comp;
+    s->rec_nt = nt;
+    return B2R_OK;
+}
+
+extern "C" int b2r_el_get_receivers(b2r_el_shot* s, float* h_rec)
+{
+    B2R_CHECK(s && h_rec, B2R_E_ARG, "null argument");
+    if ((size_t)s->nrec * s->rec_nt == 0) return B2R_OK;
+    B2R_HIP(hipMemcpyAsync(h_rec, s->d_recv, (size_t)s->nrec * s->rec_nt * sizeof(float),
+                           hipMemcpyDeviceToHost, s->ctx->stream));
+    B2R_HIP(hipStreamSynchronize(s->ctx->stream));
+    return B2R_OK;
+}
+
+extern "C" int b2r_el_get_state(b2r_el_shot* s, int which, float* h_global)
+{
+    B2R_CHECK(s && h_global && which >= -1 && which < F_NUM, B2R_E_ARG, "bad argument");
+    const void* f = which < 0 ? (const void*)s->d.disp : s->fields[which];
+    B2R_CHECK(f, B2R_E_ARG, "displacement not enabled");
+    return b2r_field_download(s->ctx, f, h_global);
+}
+
+extern "C" int b2r_el_set_state(b2r_el_shot* s, int which, const float* h_global)
+{
+    B2R_CHECK(s && h_global && which >= 0 && which < 9, B2R_E_ARG, "bad argument");
+    return b2r_field_upload(s->ctx, s->fields[which], h_global);
+}
+
+// ---------------------------------------------------------------- stepping
+struct ElBatch {
+    std::vector<b2r_el_shot*> shots;
+    ElDev* d_tab = nullptr;   // device copy of the shot table
+    int* d_t = nullptr;       // per shot step counters (device)
+    int total_tiles = 0;
+    hipStream_t stream;
+};
+
+static int el_table(std::vector<b2r_el_shot*>& shots, ElDev** d_tab, int* total)
+{
+    std::vector<ElDev> tab(shots.size());
+    int t0 = 0;
+    for (size_t i = 0; i < shots.size(); ++i) {
+        tab[i] = shots[i]->d;
+        tab[i].tile0 = t0;
+        t0 += tab[i].ntiles;
+    }
+    *total = t0;
+    b2r_ctx* c = shots[0]->ctx;
+    B2R_HIP(hipMalloc(d_tab, sizeof(ElDev) * tab.size()));
+    B2R_HIP(hipMemcpyAsync(*d_tab, tab.data(), sizeof(ElDev) * tab.size(), hipMemcpyHostToDevice,
+                           c->stream));
+    return B2R_OK;
+}
+
+// One step (inner step i of the B2R window) of every shot in the batch.
+// plain: separate vx, vy, vz launches, per shot.
+static int el_step_launch(ElBatch& bt, int i, bool plain, const int* d_t_base)
+{
+    const int nshot = (int)bt.shots.size();
+    dim3 blk(EL_TX, EL_TY);
+    if (plain) {
+        // one shot per launch: the table entry of shot k alone
+        for (int k = 0; k < nshot; ++k) {
+            const ElDev* tk = bt.d_tab + k;
+            const int nt = bt.shots[k]->d.ntiles;
+            // tile0 of entry k is relative to the batch; shift blockIdx by a
+            // one-entry table whose tile0 is subtracted inside decode_tile
+            el_stress_kernel<<<nt, blk, 0, bt.stream>>>(tk, 1, 2 * i);
+            el_velocity_kernel<1><<<nt, blk, 0, bt.stream>>>(tk, 1, 2 * i + 1);
+            el_velocity_kernel<2><<<nt, blk, 0, bt.stream>>>(tk, 1, 2 * i + 1);
+            el_velocity_kernel<4><<<nt, blk, 0, bt.stream>>>(tk, 1, 2 * i + 1);
+        }
+    } else {
+        el_stress_kernel<<<bt.total_tiles, blk, 0, bt.stream>>>(bt.d_tab, nshot, 2 * i);
+        el_velocity_kernel<7><<<bt.total_tiles, blk, 0, bt.stream>>>(bt.d_tab, nshot, 2 * i + 1);
+    }
+    for (int k = 0; k < nshot; ++k) {
+        b2r_el_shot* s = bt.shots[k];
+        SrcRec sr = el_srcrec(s);
+        if (s->nsrc) {
+            const int n = s->nsrc * s->d.g.nblk;
+            el_inject_kernel<<<ceil_div(n, 64), 64, 0, bt.stream>>>(s->d, sr, 2 * i + 1, d_t_base + k);
+        }
+        if (s->nrec)
+            el_record_kernel<<<ceil_div(s->nrec, 64), 64, 0, bt.stream>>>(s->d, sr, d_t_base + k);
+        el_tick_kernel<<<1, 1, 0, bt.stream>>>(bt.d_t + k);
+    }
+    B2R_LAUNCH_CHECK();
+    return B2R_OK;
+}
+
+static int el_exchange(ElBatch& bt)
+{
+    for (b2r_el_shot* s : bt.shots) {
+        const KGeo& g = s->ctx->k;
+        if (g.nblk == 1 && !g.periodic) continue;  // nothing to exchange
+        void* f[9];
+        for (int k = 0; k < 9; ++k) f[k] = s->fields[k];
+        // the exchange runs on the batch stream
+        hipStream_t keep = s->ctx->stream;
+        s->ctx->stream = bt.stream;
+        int rc = b2r_direct_exchange(s->ctx, f, 9);
+        s->ctx->stream = keep;
+        if (rc) return rc;
+    }
+    return B2R_OK;
+}
+
+// the tile0 of a single-shot table entry must be zero for plain launches
+static int el_batch_init(ElBatch& bt, b2r_el_shot* const* shots, int nshots, bool plain)
+{
+    B2R_CHECK(shots && nshots >= 1 && nshots <= EL_MAX_BATCH, B2R_E_ARG, "1..%d shots",
+              EL_MAX_BATCH);
+    bt.shots.assign(shots, shots + nshots);
+    const int R = shots[0]->ctx->g.R;
+    for (int k = 0; k < nshots; ++k) {
+        B2R_CHECK(shots[k], B2R_E_ARG, "null shot %d", k);
+        B2R_CHECK(shots[k]->ctx->device == shots[0]->ctx->device, B2R_E_ARG,
+                  "all shots of a batch must be on one device");
+        B2R_CHECK(shots[k]->ctx->g.R == R, B2R_E_ARG, "all shots of a batch must share R");
+        shots[k]->d.g = shots[k]->ctx->k;
+    }
+    bt.stream = shots[0]->ctx->stream;
+    if (plain) {
+        std::vector<ElDev> tab(nshots);
+        for (int k = 0; k < nshots; ++k) {
+            tab[k] = shots[k]->d;
+            tab[k].tile0 = 0;
+        }
+        B2R_HIP(hipMalloc(&bt.d_tab, sizeof(ElDev) * nshots));
+        B2R_HIP(hipMemcpyAsync(bt.d_tab, tab.data(), sizeof(ElDev) * nshots,
+                               hipMemcpyHostToDevice, bt.stream));
+        bt.total_tiles = 0;
+    } else if (int rc = el_table(bt.shots, &bt.d_tab, &bt.total_tiles)) {
+        return rc;
+    }
+    std::vector<int> t0(nshots);
+    for (int k = 0; k < nshots; ++k) t0[k] = shots[k]->t;
+    B2R_HIP(hipMalloc(&bt.d_t, sizeof(int) * nshots));
+    B2R_HIP(hipMemcpyAsync(bt.d_t, t0.data(), sizeof(int) * nshots, hipMemcpyHostToDevice,
+                           bt.stream));
+    return B2R_OK;
+}
+
+static void el_batch_free(ElBatch& bt)
+{
+    (void)hipStreamSynchronize(bt.stream);
+    if (bt.d_tab) (void)hipFree(bt.d_tab);
+    if (bt.d_t) (void)hipFree(bt.d_t);
+    bt.d_tab = nullptr;
+    bt.d_t = nullptr;
+}
+
+// steps of one R-window group: exchange, then n inner steps
+static int el_group(ElBatch& bt, int n, bool plain)
+{
+    if (int rc = el_exchange(bt)) return rc;
+    for (int i = 0; i < n; ++i) {
+        if (int rc = el_step_launch(bt, i, plain, bt.d_t)) return rc;
+        if (plain) B2R_HIP(hipStreamSynchronize(bt.stream));  // eqwp.cu:685
+    }
+    return B2R_OK;
+}
+
+extern "C" int b2r_el_forward_batch(b2r_el_shot* const* shots, int nshots, int nsteps, int mode,
+                                    float* ms)
+{
+    B2R_CHECK(nsteps >= 0 && mode >= 0 && mode <= 2, B2R_E_ARG, "bad nsteps or mode");
+    const bool plain = mode == 0, graph = mode == 2;
+    ElBatch bt;
+    if (int rc = el_batch_init(bt, shots, nshots, plain)) {
+        el_batch_free(bt);
+        return rc;
+    }
+    b2r_ctx* c0 = shots[0]->ctx;
+    if (int rc = b2r_activate(c0)) return rc;
+    const int R = c0->g.R;
+    int rc = B2R_OK;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph_ = nullptr;
+    // graphs replay one chunk of whole R-windows; the step counters live on
+    // the device so one graph serves every chunk
+    int chunk = R;
+    while (chunk < 32 && chunk * 2 <= nsteps) chunk *= 2;
+    chunk -= chunk % R;
+    if (chunk < R) chunk = R;
+    if (graph && nsteps >= chunk) {
+        if (hipStreamBeginCapture(bt.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            b2r_set_error("hipStreamBeginCapture failed");
+            rc = B2R_E_HIP;
+        }
+        for (int s = 0; s < chunk && !rc; s += R) rc = el_group(bt, R, false);
+        hipError_t e = hipStreamEndCapture(bt.stream, &graph_);
+        if (!rc && e != hipSuccess) {
+            b2r_set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+            rc = B2R_E_HIP;
+        }
+        if (!rc && hipGraphInstantiate(&exec, graph_, nullptr, nullptr, 0) != hipSuccess) {
+            b2r_set_error("hipGraphInstantiate failed");
+            rc = B2R_E_HIP;
+        }
+    }
+    if (!rc) rc = b2r_time_begin(c0, ms);
+    int done = 0;
+    if (!rc && exec) {
+        for (; done + chunk <= nsteps && !rc; done += chunk)
+            if (hipGraphLaunch(exec, bt.stream) != hipSuccess) {
+                b2r_set_error("hipGraphLaunch failed");
+                rc = B2R_E_HIP;
+            }
+    }
+    while (!rc && done < nsteps) {
+        const int n = nsteps - done < R ? nsteps - done : R;
+        rc = el_group(bt, n, plain);
+        done += n;
+    }
+    if (!rc) rc = b2r_time_end(c0, ms);
+    if (!rc) B2R_HIP(hipStreamSynchronize(bt.stream));
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (!rc)
+        for (int k = 0; k < nshots; ++k) shots[k]->t += nsteps;
+    el_batch_free(bt);
+    return rc;
+}
+
+extern "C" int b2r_el_forward(b2r_el_shot* s, int nsteps, int mode, float* ms)
+{
+    return b2r_el_forward_batch(&s, 1, nsteps, mode, ms);
+}
+
+// ---------------------------------------------------------------- gradient
+extern "C" int b2r_el_misfit(b2r_el_shot* s, const float* h_obs, double* misfit)
+{
+    B2R_CHECK(s && h_obs && misfit, B2R_E_ARG, "null argument");
+    b2r_ctx* c = s->ctx;
+    const long long n = (long long)s->nrec * s->rec_nt;
+    B2R_CHECK(n > 0, B2R_E_ARG, "no receivers");
+    float* d_obs = nullptr;
+    float* d_res = nullptr;
+    double* d_part = nullptr;
+    const int nb = 256;
+    B2R_HIP(hipMalloc(&d_obs, n * 4));
+    B2R_HIP(hipMalloc(&d_res, n * 4));
+    B2R_HIP(hipMalloc(&d_part, (nb + 1) * sizeof(double)));
+    (void)hipMemcpyAsync(d_obs, h_obs, n * 4, hipMemcpyHostToDevice, c->stream);
+    el_residual_kernel<<<nb, 256, 0, c->stream>>>(s->d_recv, d_obs, d_res, n, d_part);
+    el_sum_kernel<<<1, 64, 0, c->stream>>>(d_part, nb, d_part + nb);
+    double ss = 0;
+    (void)hipMemcpyAsync(&ss, d_part + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_obs);
+    (void)hipFree(d_res);
+    (void)hipFree(d_part);
+    B2R_CHECK(e == hipSuccess, B2R_E_HIP, "misfit: %s", hipGetErrorString(e));
+    *misfit = 0.5 * ss;
+    return B2R_OK;
+}
+
+static int copy_state(b2r_el_shot* s, float* const* dst, const float* const* src)
+{
+    const size_t bytes = (size_t)s->ctx->L.field_elems * 4;
+    for (int k = 0; k < 9; ++k)
+        B2R_HIP(hipMemcpyAsync(dst[k], src[k], bytes, hipMemcpyDeviceToDevice, s->ctx->stream));
+    return B2R_OK;
+}
+
+// Gradient of J = 1/2 sum (rec - obs)^2 over nsteps steps from a zero state,
+// w.r.t. lambda, mu and b per owned cell (h_g: 3 global grids).  Forward
+// states are checkpointed every ckpt steps and each segment is recomputed
+// (states of the segment kept on the device) before its reverse sweep.
+extern "C" int b2r_el_gradient_ex(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt,
+                                  int flags, double* misfit, float* h_glam, float* h_gmu,
+                                  float* h_gb, float* h_adj0);
+
+extern "C" int b2r_el_gradient(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt,
+                               double* misfit, float* h_glam, float* h_gmu, float* h_gb)
+{
+    return b2r_el_gradient_ex(s, h_obs, nsteps, ckpt, 0, misfit, h_glam, h_gmu, h_gb, nullptr);
+}
+
+// flags bit 0: start from the current state instead of zero.  h_adj0 (optional,
+// 9 global grids): the adjoint state at the start, i.e. d misfit / d state(0).
+extern "C" int b2r_el_gradient_ex(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt,
+                                  int flags, double* misfit, float* h_glam, float* h_gmu,
+                                  float* h_gb, float* h_adj0)
+{
+    B2R_CHECK(s && h_obs && misfit && h_glam && h_gmu && h_gb && nsteps >= 1 && ckpt >= 1,
+              B2R_E_ARG, "bad argument");
+    b2r_ctx* c = s->ctx;
+    B2R_CHECK(c->k.nblk == 1 && c->g.R == 1 && !c->k.periodic && c->k.fmask == 0x3F,
+              B2R_E_UNSUPPORTED, "the adjoint runs on one block, R=1, frozen faces");
+    B2R_CHECK(s->rec_nt >= nsteps && s->nrec > 0, B2R_E_ARG, "receivers must cover nsteps");
+    B2R_CHECK(s->d.disp == nullptr, B2R_E_UNSUPPORTED, "disable the displacement output");
+    if (int rc = b2r_activate(c)) return rc;
+    const size_t fb = (size_t)c->L.field_elems * 4;
+    const int ncp = (nsteps + ckpt - 1) / ckpt;
+    std::vector<void*> mem;
+    auto alloc = [&](size_t bytes) -> float* {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        mem.push_back(p);
+        return (float*)p;
+    };
+    auto release = [&]() {
+        (void)hipStreamSynchronize(c->stream);
+        for (void* p : mem) (void)hipFree(p);
+    };
+    float* cp = alloc(fb * 9 * ncp);
+    float* seg = alloc(fb * 9 * (ckpt + 1));
+    float* adj = alloc(fb * 9);
+    float* grd = alloc(fb * 3);
+    float* res = alloc((size_t)s->nrec * s->rec_nt * 4);
+    if (!cp || !seg || !adj || !grd || !res) {
+        release();
+        b2r_set_error("gradient: device allocation of %zu MB failed",
+                      fb * (9 * ncp + 9 * (ckpt + 1) + 12) >> 20);
+        return B2R_E_NOMEM;
+    }
+    auto slot = [&](float* base, int j, float** out) {
+        for (int k = 0; k < 9; ++k) out[k] = base + ((size_t)j * 9 + k) * (fb / 4);
+    };
+    int rc = B2R_OK;
+    if (flags & 1) s->t = 0;
+    else rc = b2r_el_reset(s);
+    // forward with checkpoints
+    for (int j = 0; j < ncp && !rc; ++j) {
+        float* dst[9];
+        slot(cp, j, dst);
+        rc = copy_state(s, dst, s->d.f);
+        const int n = std::min(ckpt, nsteps - j * ckpt);
+        if (!rc) rc = b2r_el_forward(s, n, 1, nullptr);
+    }
+    if (!rc) rc = b2r_el_misfit(s, h_obs, misfit);
+    // residuals on the device (recomputed: b2r_el_misfit freed its copy)
+    if (!rc) {
+        const long long n = (long long)s->nrec * s->rec_nt;
+        std::vector<float> rec(n);
+        rc = b2r_el_get_receivers(s, rec.data());
+        for (long long i = 0; i < n && !rc; ++i) rec[i] -= h_obs[i];
+        if (!rc) B2R_HIP(hipMemcpyAsync(res, rec.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+        if (!rc) B2R_HIP(hipStreamSynchronize(c->stream));
+    }
+    if (!rc) {
+        B2R_HIP(hipMemsetAsync(adj, 0, fb * 9, c->stream));
+        B2R_HIP(hipMemsetAsync(grd, 0, fb * 3, c->stream));
+    }
+    ElDev d = s->d;
+    d.g = c->k;
+    for (int k = 0; k < 9; ++k) d.adj[k] = adj + (size_t)k * (fb / 4);
+    for (int k = 0; k < 3; ++k) d.grad[k] = grd + (size_t)k * (fb / 4);
+    ElDev* d_tab = alloc(sizeof(ElDev)) ? (ElDev*)mem.back() : nullptr;
+    if (!d_tab && !rc) {
+        b2r_set_error("gradient: table allocation failed");
+        rc = B2R_E_NOMEM;
+    }
+    SrcRec sr = el_srcrec(s);
+    dim3 blk(EL_TX, EL_TY);
+    for (int j = ncp - 1; j >= 0 && !rc; --j) {
+        const int t0 = j * ckpt, t1 = std::min(nsteps, t0 + ckpt);
+        // recompute the segment, keeping every state
+        float* src[9];
+        slot(cp, j, src);
+        float* st[9];
+        slot(seg, 0, st);
+        rc = copy_state(s, st, src);
+        if (!rc) rc = copy_state(s, s->d.f, st);
+        s->t = t0;
+        for (int t = t0; t < t1 && !rc; ++t) {
+            rc = b2r_el_forward(s, 1, 1, nullptr);
+            slot(seg, t - t0 + 1, st);
+            if (!rc) rc = copy_state(s, st, s->d.f);
+        }
+        for (int t = t1 - 1; t >= t0 && !rc; --t) {
+            slot(seg, t - t0, d.fw0);
+            slot(seg, t - t0 + 1, d.fw1);
+            B2R_HIP(hipMemcpyAsync(d_tab, &d, sizeof(ElDev), hipMemcpyHostToDevice, c->stream));
+            el_adj_point_kernel<<<ceil_div(s->nrec, 64), 64, 0, c->stream>>>(d, sr, res, t);
+            el_adj_velocity_kernel<<<d.ntiles, blk, 0, c->stream>>>(d_tab, 1);
+            if (s->nsrc) el_adj_source_kernel<<<ceil_div(s->nsrc, 64), 64, 0, c->stream>>>(d, sr, t);
+            el_adj_stress_kernel<<<d.ntiles, blk, 0, c->stream>>>(d_tab, 1);
+            // the table is rewritten next step: wait for these kernels first
+            B2R_HIP(hipStreamSynchronize(c->stream));
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) {
+                b2r_set_error("adjoint kernels: %s", hipGetErrorString(e));
+                rc = B2R_E_HIP;
+            }
+        }
+    }
+    s->t = nsteps;
+    // gradients (and the adjoint state) out through the owned-cell copy
+    const long long gcells = (long long)c->g.env_x * c->g.env_y * c->g.env_z;
+    for (int k = 0; k < 3 + (h_adj0 ? 9 : 0) && !rc; ++k) {
+        float* out = k == 0 ? h_glam : (k == 1 ? h_gmu : (k == 2 ? h_gb : h_adj0 + (k - 3) * gcells));
+        const float* srcp = k < 3 ? grd + (size_t)k * (fb / 4) : adj + (size_t)(k - 3) * (fb / 4);
+        void* f = nullptr;
+        rc = b2r_field_alloc(c, &f);
+        if (!rc && hipMemcpyAsync(f, srcp, fb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+            b2r_set_error("gradient copy failed");
+            rc = B2R_E_HIP;
+        }
+        if (!rc) rc = b2r_field_download(c, f, out);
+        if (f) b2r_field_free(c, f);
+    }
+    release();
+    return rc;
+}

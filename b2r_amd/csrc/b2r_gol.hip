@@ -136,6 +136,235 @@ __global__ __launch_bounds__(256) void gol_step_kernel(const uint8_t* __restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// R generations in one launch: temporal blocking in LDS (B2R's deep halo on
+// chip).  A workgroup owns an output tile of 16*GF_TXQ cells x TY = GF_NR-2R
+// rows of one block and marches z through a chunk of planes.  For every input
+// plane z_in it computes generation t (t = 1..R) of plane z_in-t from
+// generation t-1, whose last three planes sit in an LDS ring over the tile
+// grown by R-t cells per side (here: one 16-cell quad per side in x, R-t rows
+// in y).  Each generation is two passes over 16-cell quads:
+//   A: P = sum over the 3 planes x 3 rows of the previous generation (SWAR)
+//   B: box = P + P(x-1) + P(x+1) via byte alignment with the neighbour
+//      quads' edge dwords, then the rule; written to the next ring or, for
+//      generation R, to global memory.
+// Cells outside a generation's update box (frozen global faces) keep their
+// previous value, as in the step-by-step path.  Halo cells of the output are
+// never written.
+// ---------------------------------------------------------------------------
+constexpr int GF_TXQ = 8;            // output quads (16 cells) per tile row
+constexpr int GF_NQ = GF_TXQ + 2;    // plus one quad per side
+constexpr int GF_NR = 32;            // rows incl. R halo rows per side
+constexpr int GF_PLANE = GF_NQ * GF_NR;
+
+__device__ __forceinline__ uint4 q_add(uint4 a, uint4 b)
+{
+    return make_uint4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+
+// byte mask of the four cells x .. x+3 inside [lo, hi]
+__device__ __forceinline__ uint32_t dword_mask(int x, int lo, int hi)
+{
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (x + k >= lo && x + k <= hi) m |= 0xFFu << (8 * k);
+    return m;
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void gol_fused_kernel(const uint8_t* __restrict__ in,
+                                                        uint8_t* __restrict__ out, KGeo g,
+                                                        int ntx, int nty, int zc)
+{
+    constexpr int TY = GF_NR - 2 * R;
+    extern __shared__ uint4 smem[];
+    uint4* ring = smem;                   // [R][3][GF_PLANE], level t-1 at ring[t-1]
+    uint4* pbuf = smem + R * 3 * GF_PLANE;
+
+    const int tid = threadIdx.x;
+    int tt = blockIdx.x;
+    const int txi = tt % ntx;
+    tt /= ntx;
+    const int tyi = tt % nty;
+    const int zci = tt / nty;
+    const int b = blockIdx.y;
+    const int h = g.h;
+    const int x0 = h + txi * 16 * GF_TXQ;  // first output cell
+    const int y0 = h + tyi * TY;
+    const int zc0 = h + zci * zc;
+    const int zc1 = min(zc0 + zc, h + g.B[2]);
+    const long long boff = (long long)b * g.bstride + g.xoff;
+    const uint8_t* __restrict__ src = in + boff;
+    uint8_t* __restrict__ dst = out + boff;
+    const long long slice = g.slice;
+    const int pitch = g.pitch;
+
+    // local quad q covers x = x0 + 16*(q-1) .. +15; local row r is y = y0 - R + r
+    auto quad_ok = [&](int q) {
+        const int x = x0 + 16 * (q - 1);
+        return x + 15 >= 0 && x < g.S[0];
+    };
+
+    // level-0 plane loads: quads pos = tid, tid + 256 of the GF_PLANE grid
+    constexpr int NPF = (GF_PLANE + 255) / 256;
+    uint4 pf[NPF];
+    auto load_plane = [&](int z) {
+        const long long pz = (long long)z * slice;
+#pragma unroll
+        for (int s = 0; s < NPF; ++s) {
+            const int pos = tid + 256 * s;
+            pf[s] = make_uint4(0, 0, 0, 0);
+            if (pos < GF_PLANE) {
+                const int r = pos / GF_NQ, q = pos % GF_NQ;
+                const int y = y0 - R + r;
+                if (y < g.S[1] && quad_ok(q))
+                    pf[s] = *reinterpret_cast<const uint4*>(src + pz + (long long)y * pitch +
+                                                            x0 + 16 * (q - 1));
+            }
+        }
+    };
+
+    // per-generation boxes (frozen faces) and whether the tile needs masking
+    Box bx[R];
+    bool clip[R];
+#pragma unroll
+    for (int t = 1; t <= R; ++t) {
+        bx[t - 1] = update_box(g, b, 1, t - 1);
+        const int gr = R - t;  // growth of generation t's region
+        const Box& B_ = bx[t - 1];
+        clip[t - 1] = x0 - 16 < B_.lo[0] || x0 + 16 * (GF_TXQ + 1) - 1 > B_.hi[0] ||
+                      y0 - gr < B_.lo[1] || y0 + TY + gr - 1 > B_.hi[1] ||
+                      zc0 - gr < B_.lo[2] || zc1 + gr - 1 > B_.hi[2];
+    }
+
+    load_plane(zc0 - R);
+    for (int zin = zc0 - R; zin < zc1 + R; ++zin) {
+        uint4* r0 = ring + (zin % 3) * GF_PLANE;
+#pragma unroll
+        for (int s = 0; s < NPF; ++s) {
+            const int pos = tid + 256 * s;
+            if (pos < GF_PLANE) r0[pos] = pf[s];
+        }
+        if (zin + 1 < zc1 + R) load_plane(zin + 1);
+        __syncthreads();
+#pragma unroll
+        for (int t = 1; t <= R; ++t) {
+            if (zin < zc0 - R + 2 * t) continue;  // uniform
+            const int p = zin - t;
+            const uint4* prev = ring + (t - 1) * 3 * GF_PLANE;
+            const uint4* pm = prev + ((p + 2) % 3) * GF_PLANE;  // plane p-1
+            const uint4* pc = prev + (p % 3) * GF_PLANE;        // plane p
+            const uint4* pp = prev + ((p + 1) % 3) * GF_PLANE;  // plane p+1
+            const int nrow = GF_NR - 2 * t;
+            // pass A: 3x3 (z,y) sums of the previous generation
+            for (int pos = tid; pos < nrow * GF_NQ; pos += 256) {
+                const int r = t + pos / GF_NQ, q = pos % GF_NQ;
+                const int i = r * GF_NQ + q;
+                uint4 s = q_add(q_add(pm[i - GF_NQ], pm[i]), pm[i + GF_NQ]);
+                s = q_add(s, q_add(q_add(pc[i - GF_NQ], pc[i]), pc[i + GF_NQ]));
+                s = q_add(s, q_add(q_add(pp[i - GF_NQ], pp[i]), pp[i + GF_NQ]));
+                pbuf[i] = s;
+            }
+            __syncthreads();
+            // pass B: x neighbours and the rule
+            const int qa = t == R ? 1 : 0, qb = t == R ? GF_NQ - 1 : GF_NQ;
+            const int nq = qb - qa;
+            uint4* nxt = t < R ? ring + t * 3 * GF_PLANE + (p % 3) * GF_PLANE : nullptr;
+            for (int pos = tid; pos < nrow * nq; pos += 256) {
+                const int r = t + pos / nq, q = qa + pos % nq;
+                const int i = r * GF_NQ + q;
+                const uint4 P = pbuf[i];
+                const uint32_t el = q > 0 ? pbuf[i - 1].w : 0u;
+                const uint32_t er = q < GF_NQ - 1 ? pbuf[i + 1].x : 0u;
+                const uint4 self = pc[i];
+                uint4 nw;
+                nw.x = gol_rule4(__builtin_amdgcn_alignbyte(P.x, el, 3) + P.x +
+                                 __builtin_amdgcn_alignbyte(P.y, P.x, 1), self.x);
+                nw.y = gol_rule4(__builtin_amdgcn_alignbyte(P.y, P.x, 3) + P.y +
+                                 __builtin_amdgcn_alignbyte(P.z, P.y, 1), self.y);
+                nw.z = gol_rule4(__builtin_amdgcn_alignbyte(P.z, P.y, 3) + P.z +
+                                 __builtin_amdgcn_alignbyte(P.w, P.z, 1), self.z);
+                nw.w = gol_rule4(__builtin_amdgcn_alignbyte(P.w, P.z, 3) + P.w +
+                                 __builtin_amdgcn_alignbyte(er, P.w, 1), self.w);
+                const int y = y0 - R + r;
+                const int x = x0 + 16 * (q - 1);
+                if (clip[t - 1]) {
+                    const Box& B_ = bx[t - 1];
+                    const bool in_yz = y >= B_.lo[1] && y <= B_.hi[1] && p >= B_.lo[2] &&
+                                       p <= B_.hi[2];
+                    uint32_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+                    if (in_yz) {
+                        m0 = dword_mask(x, B_.lo[0], B_.hi[0]);
+                        m1 = dword_mask(x + 4, B_.lo[0], B_.hi[0]);
+                        m2 = dword_mask(x + 8, B_.lo[0], B_.hi[0]);
+                        m3 = dword_mask(x + 12, B_.lo[0], B_.hi[0]);
+                    }
+                    nw.x = (nw.x & m0) | (self.x & ~m0);
+                    nw.y = (nw.y & m1) | (self.y & ~m1);
+                    nw.z = (nw.z & m2) | (self.z & ~m2);
+                    nw.w = (nw.w & m3) | (self.w & ~m3);
+                }
+                if (t < R) {
+                    nxt[i] = nw;
+                } else if (y < y0 + TY && y < h + g.B[1] && x < h + g.B[0]) {
+                    *reinterpret_cast<uint4*>(dst + (long long)p * slice + (long long)y * pitch +
+                                              x) = nw;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+static bool gol_fused_ok(const b2r_ctx* ctx, int R)
+{
+    return R >= 1 && R <= 4 && (ctx->k.periodic || ctx->k.fmask == 0x3F);
+}
+
+template <int R>
+static int gol_fused_launch_t(b2r_ctx* ctx, const void* in, void* out)
+{
+    constexpr int TY = GF_NR - 2 * R;
+    const KGeo& g = ctx->k;
+    const size_t lds = (size_t)(R * 3 + 1) * GF_PLANE * sizeof(uint4);
+    static bool attr_set = false;
+    if (!attr_set) {
+        B2R_HIP(hipFuncSetAttribute((const void*)gol_fused_kernel<R>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr_set = true;
+    }
+    const int ntx = ceil_div(g.B[0], 16 * GF_TXQ);
+    const int nty = ceil_div(g.B[1], TY);
+    long long tiles = (long long)ntx * nty * g.nblk;
+    int zc = ctx->tune.gol_zc;
+    if (zc <= 0) {
+        // about four workgroup waves over 256 CUs (two resident per CU)
+        int nzc = (int)((2048 + tiles - 1) / tiles);
+        zc = ceil_div(g.B[2], nzc < 1 ? 1 : nzc);
+        if (zc < 4 * R) zc = 4 * R;
+    }
+    if (zc > g.B[2]) zc = g.B[2];
+    const int nzc = ceil_div(g.B[2], zc);
+    dim3 grid(ntx * nty * nzc, g.nblk);
+    gol_fused_kernel<R><<<grid, 256, lds, ctx->stream>>>((const uint8_t*)in, (uint8_t*)out, g,
+                                                        ntx, nty, zc);
+    B2R_LAUNCH_CHECK();
+    return B2R_OK;
+}
+
+int b2r_gol_fused_launch(b2r_ctx* ctx, const void* in, void* out)
+{
+    switch (ctx->g.R) {
+    case 1: return gol_fused_launch_t<1>(ctx, in, out);
+    case 2: return gol_fused_launch_t<2>(ctx, in, out);
+    case 3: return gol_fused_launch_t<3>(ctx, in, out);
+    case 4: return gol_fused_launch_t<4>(ctx, in, out);
+    }
+    b2r_set_error("fused Game of Life supports R <= 4 (got %d)", ctx->g.R);
+    return B2R_E_UNSUPPORTED;
+}
+
 static int gol_check(b2r_ctx* ctx)
 {
     B2R_CHECK(ctx, B2R_E_ARG, "null context");
@@ -156,12 +385,15 @@ int b2r_gol_launch(b2r_ctx* ctx, const void* in, void* out, int i)
     const int ntx = ceil_div(jd1 - jd0 + 1, GOL_TXD);
     const int nty = ceil_div(g.S[1] - 2, TY);
     const int nz = g.S[2] - 2;
-    long long tiles = (long long)ntx * nty * g.nblk;
-    int nzc = (int)((4096 + tiles - 1) / tiles);
-    int zc = ceil_div(nz, nzc < 1 ? 1 : nzc);
-    if (zc < 8) zc = 8;
+    int zc = ctx->tune.gol_step_zc;
+    if (zc <= 0) {
+        long long tiles = (long long)ntx * nty * g.nblk;
+        int nzc = (int)((4096 + tiles - 1) / tiles);
+        zc = ceil_div(nz, nzc < 1 ? 1 : nzc);
+        if (zc < 8) zc = 8;
+    }
     if (zc > nz) zc = nz;
-    nzc = ceil_div(nz, zc);
+    const int nzc = ceil_div(nz, zc);
     dim3 grid(ntx * nty * nzc, g.nblk);
     dim3 block(GOL_TXD, GOL_WY);
     gol_step_kernel<RY><<<grid, block, 0, ctx->stream>>>((const uint8_t*)in, (uint8_t*)out, g, i,
@@ -184,7 +416,6 @@ extern "C" int b2r_gol_step(b2r_ctx* ctx, const void* d_in, void* d_out, int b2r
 extern "C" int b2r_gol_run(b2r_ctx* ctx, void* d_a, void* d_b, int nsteps, int fused,
                            int* result_in_b, float* ms)
 {
-    (void)fused;
     if (int rc = gol_check(ctx)) return rc;
     if (int rc = b2r_check_field(ctx, d_a)) return rc;
     if (int rc = b2r_check_field(ctx, d_b)) return rc;
@@ -195,9 +426,17 @@ extern "C" int b2r_gol_run(b2r_ctx* ctx, void* d_a, void* d_b, int nsteps, int f
     void* nxt = d_b;
     if (int rc = b2r_time_begin(ctx, ms)) return rc;
     const int R = ctx->g.R;
+    const bool use_fused = fused && gol_fused_ok(ctx, R);
     for (int s = 0; s < nsteps; s += R) {
         if (int rc = b2r_direct_exchange(ctx, &cur, 1)) return rc;
         const int n = nsteps - s < R ? nsteps - s : R;
+        if (use_fused && n == R) {
+            if (int rc = b2r_gol_fused_launch(ctx, cur, nxt)) return rc;
+            void* t = cur;
+            cur = nxt;
+            nxt = t;
+            continue;
+        }
         for (int i = 0; i < n; ++i) {
             if (int rc = b2r_gol_launch(ctx, cur, nxt, i)) return rc;
             void* t = cur;

@@ -205,25 +205,38 @@ static int heat_check(b2r_ctx* ctx, const b2r_heat_params* p)
     return B2R_OK;
 }
 
-template <int D, int MODE>
-static void launch_heat(b2r_ctx* ctx, const float* in, float* out, const HeatK& hk, int i)
+template <int D, int MODE, int RY>
+static void launch_heat_ry(b2r_ctx* ctx, const float* in, float* out, const HeatK& hk, int i)
 {
-    constexpr int RY = 4;
     constexpr int TY = HEAT_WY * RY;
     const KGeo& g = ctx->k;
     const int ntx = ceil_div(g.S[0] - 2 * D, HEAT_TX);
     const int nty = ceil_div(g.S[1] - 2 * D, TY);
     const int nz = g.S[2] - 2 * D;
-    // enough workgroups to fill 256 CUs several times, chunks >= 4D planes
-    long long tiles = (long long)ntx * nty * g.nblk;
-    int nzc = (int)((4096 + tiles - 1) / tiles);
-    int zc = ceil_div(nz, nzc < 1 ? 1 : nzc);
-    if (zc < 4 * D) zc = 4 * D;
+    int zc = ctx->tune.heat_zc;
+    if (zc <= 0) {
+        // enough workgroups to fill 256 CUs several times, chunks >= 4D planes
+        long long tiles = (long long)ntx * nty * g.nblk;
+        int nzc = (int)((4096 + tiles - 1) / tiles);
+        zc = ceil_div(nz, nzc < 1 ? 1 : nzc);
+        if (zc < 4 * D) zc = 4 * D;
+    }
     if (zc > nz) zc = nz;
-    nzc = ceil_div(nz, zc);
+    const int nzc = ceil_div(nz, zc);
     dim3 grid(ntx * nty * nzc, g.nblk);
     dim3 block(HEAT_TX, HEAT_WY);
     heat_step_kernel<D, MODE, RY><<<grid, block, 0, ctx->stream>>>(in, out, g, hk, i, ntx, nty, zc);
+}
+
+template <int D, int MODE>
+static void launch_heat(b2r_ctx* ctx, const float* in, float* out, const HeatK& hk, int i)
+{
+    switch (ctx->tune.heat_ry) {
+    case 1: launch_heat_ry<D, MODE, 1>(ctx, in, out, hk, i); break;
+    case 2: launch_heat_ry<D, MODE, 2>(ctx, in, out, hk, i); break;
+    case 8: launch_heat_ry<D, MODE, 8>(ctx, in, out, hk, i); break;
+    default: launch_heat_ry<D, MODE, 4>(ctx, in, out, hk, i); break;
+    }
 }
 
 int b2r_heat_launch(b2r_ctx* ctx, const void* in, void* out, const b2r_heat_params* p, int i)
@@ -264,10 +277,11 @@ extern "C" int b2r_heat_step(b2r_ctx* ctx, const void* d_in, void* d_out,
     return b2r_heat_launch(ctx, d_in, d_out, p, b2r_i);
 }
 
+static bool heat_fused_ok(const b2r_ctx* ctx, int D);
+
 extern "C" int b2r_heat_run(b2r_ctx* ctx, void* d_a, void* d_b, const b2r_heat_params* p,
                             int nsteps, int fused, int* result_in_b, float* ms)
 {
-    (void)fused;
     if (int rc = heat_check(ctx, p)) return rc;
     if (int rc = b2r_check_field(ctx, d_a)) return rc;
     if (int rc = b2r_check_field(ctx, d_b)) return rc;
@@ -279,9 +293,17 @@ extern "C" int b2r_heat_run(b2r_ctx* ctx, void* d_a, void* d_b, const b2r_heat_p
     void* nxt = d_b;
     if (int rc = b2r_time_begin(ctx, ms)) return rc;
     const int R = ctx->g.R;
+    const bool use_fused = fused && heat_fused_ok(ctx, p->D);
     for (int s = 0; s < nsteps; s += R) {
         if (int rc = b2r_direct_exchange(ctx, &cur, 1)) return rc;
         const int n = nsteps - s < R ? nsteps - s : R;
+        if (use_fused && n == R) {
+            if (int rc = b2r_heat_fused_launch(ctx, cur, nxt, p)) return rc;
+            void* t = cur;
+            cur = nxt;
+            nxt = t;
+            continue;
+        }
         for (int i = 0; i < n; ++i) {
             if (int rc = b2r_heat_launch(ctx, cur, nxt, p, i)) return rc;
             void* t = cur;
@@ -292,4 +314,189 @@ extern "C" int b2r_heat_run(b2r_ctx* ctx, void* d_a, void* d_b, const b2r_heat_p
     if (int rc = b2r_time_end(ctx, ms)) return rc;
     if (result_in_b) *result_in_b = cur == d_b;
     return B2R_OK;
+}
+
+// ---------------------------------------------------------------------------
+// R steps in one launch: temporal blocking (B2R's deep halo kept on chip).
+// A workgroup of 64 x 4 threads, RY = 8 rows each, covers a 64 x 32 column
+// tile and marches z.  Level t (step t of the window, t = 1..R) of plane z_in-tD
+// is computed from level t-1, whose last 2D+1 planes of the thread's own
+// columns sit in a register queue; the x/y neighbours of the plane being
+// computed come from one shared LDS plane (with a D-wide unused border),
+// reused level after level.  The valid region shrinks by D per level, so the
+// inner (64-2RD) x (32-2RD) columns of level R are written.  Cells outside a
+// level's update box (frozen global faces) keep their previous value; halo
+// cells of the output are never written.  Same arithmetic as the step kernel,
+// so the results are bit-identical.
+// ---------------------------------------------------------------------------
+constexpr int HF_RY = 8;
+constexpr int HF_TYT = HEAT_WY * HF_RY;  // thread-tile rows
+
+template <int D, int MODE, int R>
+__global__ __launch_bounds__(256) void heat_fused_kernel(const float* __restrict__ in,
+                                                         float* __restrict__ out, KGeo g,
+                                                         HeatK hk, int ntx, int nty, int zc)
+{
+    constexpr int RD = R * D;
+    constexpr int TXO = HEAT_TX - 2 * RD, TYO = HF_TYT - 2 * RD;
+    constexpr int LX = HEAT_TX + 2 * D, LY = HF_TYT + 2 * D;
+    constexpr int Q = 2 * D + 1;
+    __shared__ float lds[LY * LX];
+
+    const int lane = threadIdx.x, wy = threadIdx.y;
+    int tt = blockIdx.x;
+    const int txi = tt % ntx;
+    tt /= ntx;
+    const int tyi = tt % nty;
+    const int zci = tt / nty;
+    const int b = blockIdx.y;
+    const int h = g.h;
+    const int xo = h + txi * TXO;  // first output column
+    const int yo = h + tyi * TYO;
+    const int zc0 = h + zci * zc;
+    const int zc1 = min(zc0 + zc, h + g.B[2]);
+
+    const long long boff = (long long)b * g.bstride + g.xoff;
+    const float* __restrict__ src = in + boff;
+    float* __restrict__ dst = out + boff;
+    const long long slice = g.slice;
+    const int pitch = g.pitch;
+
+    const int x = xo - RD + lane;
+    const bool xin = x < g.S[0];
+    int y[HF_RY];
+    bool yin[HF_RY];
+#pragma unroll
+    for (int j = 0; j < HF_RY; ++j) {
+        y[j] = yo - RD + wy * HF_RY + j;
+        yin[j] = y[j] < g.S[1];
+    }
+    Box bx[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) bx[t] = update_box(g, b, D, t);
+
+    // q[t][j][k]: level t at plane (newest - 2D + k) for row j
+    float q[R][HF_RY][Q];
+#pragma unroll
+    for (int t = 0; t < R; ++t)
+#pragma unroll
+        for (int j = 0; j < HF_RY; ++j)
+#pragma unroll
+            for (int k = 0; k < Q; ++k) q[t][j][k] = 0.0f;
+
+    const int lrow0 = wy * HF_RY + D;  // LDS row of this thread's first row
+    for (int zin = zc0 - RD; zin < zc1 + RD; ++zin) {
+        // level 0: push plane zin
+        const long long pz = (long long)zin * slice;
+#pragma unroll
+        for (int j = 0; j < HF_RY; ++j) {
+#pragma unroll
+            for (int k = 0; k < Q - 1; ++k) q[0][j][k] = q[0][j][k + 1];
+            q[0][j][Q - 1] = (xin && yin[j]) ? src[pz + (long long)y[j] * pitch + x] : 0.0f;
+        }
+#pragma unroll
+        for (int t = 1; t <= R; ++t) {
+            const int p = zin - t * D;
+            const bool active = zin >= zc0 - RD + 2 * t * D;  // uniform
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < HF_RY; ++j) lds[(lrow0 + j) * LX + lane + D] = q[t - 1][j][D];
+            __syncthreads();
+            const Box& B_ = bx[t - 1];
+            const bool zin_box = p >= B_.lo[2] && p <= B_.hi[2];
+            const bool xin_box = x >= B_.lo[0] && x <= B_.hi[0];
+#pragma unroll
+            for (int j = 0; j < HF_RY; ++j) {
+                const float uc = q[t - 1][j][D];
+                float r = uc;
+                if (active && zin_box && xin_box && y[j] >= B_.lo[1] && y[j] <= B_.hi[1]) {
+                    const float* row = &lds[(lrow0 + j) * LX + lane + D];
+                    float mx[D + 1], px[D + 1], my[D + 1], py[D + 1], mz[D + 1], pz_[D + 1];
+#pragma unroll
+                    for (int k = 1; k <= D; ++k) {
+                        mx[k] = row[-k];
+                        px[k] = row[k];
+                        my[k] = row[-k * LX];
+                        py[k] = row[k * LX];
+                        mz[k] = q[t - 1][j][D - k];
+                        pz_[k] = q[t - 1][j][D + k];
+                    }
+                    const float lx = lap<D, MODE>(mx, px, uc);
+                    const float ly = lap<D, MODE>(my, py, uc);
+                    const float lz = lap<D, MODE>(mz, pz_, uc);
+                    r = heat_combine<MODE>(uc, lx, ly, lz, hk);
+                }
+                if (t < R) {
+#pragma unroll
+                    for (int k = 0; k < Q - 1; ++k) q[t][j][k] = q[t][j][k + 1];
+                    q[t][j][Q - 1] = r;
+                } else if (active && lane >= RD && lane < HEAT_TX - RD && x < h + g.B[0]) {
+                    const int lr = wy * HF_RY + j;
+                    if (lr >= RD && lr < HF_TYT - RD && y[j] < h + g.B[1] && p < zc1)
+                        dst[(long long)p * slice + (long long)y[j] * pitch + x] = r;
+                }
+            }
+        }
+    }
+}
+
+static bool heat_fused_ok(const b2r_ctx* ctx, int D)
+{
+    const int R = ctx->g.R;
+    return R * D <= 4 && (ctx->k.periodic || ctx->k.fmask == 0x3F);
+}
+
+template <int D, int MODE, int R>
+static int heat_fused_t(b2r_ctx* ctx, const float* in, float* out, const HeatK& hk)
+{
+    constexpr int RD = R * D;
+    constexpr int TXO = HEAT_TX - 2 * RD, TYO = HF_TYT - 2 * RD;
+    const KGeo& g = ctx->k;
+    const int ntx = ceil_div(g.B[0], TXO);
+    const int nty = ceil_div(g.B[1], TYO);
+    int zc = ctx->tune.heat_zc;
+    if (zc <= 0) {
+        long long tiles = (long long)ntx * nty * g.nblk;
+        int nzc = (int)((2048 + tiles - 1) / tiles);
+        zc = ceil_div(g.B[2], nzc < 1 ? 1 : nzc);
+        if (zc < 8 * RD) zc = 8 * RD;
+    }
+    if (zc > g.B[2]) zc = g.B[2];
+    const int nzc = ceil_div(g.B[2], zc);
+    dim3 grid(ntx * nty * nzc, g.nblk);
+    dim3 block(HEAT_TX, HEAT_WY);
+    heat_fused_kernel<D, MODE, R><<<grid, block, 0, ctx->stream>>>(in, out, g, hk, ntx, nty, zc);
+    B2R_LAUNCH_CHECK();
+    return B2R_OK;
+}
+
+template <int MODE>
+static int heat_fused_mode(b2r_ctx* ctx, const float* in, float* out, const HeatK& hk, int D)
+{
+    const int R = ctx->g.R;
+    switch (D * 8 + R) {
+    case 1 * 8 + 1: return heat_fused_t<1, MODE, 1>(ctx, in, out, hk);
+    case 1 * 8 + 2: return heat_fused_t<1, MODE, 2>(ctx, in, out, hk);
+    case 1 * 8 + 3: return heat_fused_t<1, MODE, 3>(ctx, in, out, hk);
+    case 1 * 8 + 4: return heat_fused_t<1, MODE, 4>(ctx, in, out, hk);
+    case 2 * 8 + 1: return heat_fused_t<2, MODE, 1>(ctx, in, out, hk);
+    case 2 * 8 + 2: return heat_fused_t<2, MODE, 2>(ctx, in, out, hk);
+    case 3 * 8 + 1: return heat_fused_t<3, MODE, 1>(ctx, in, out, hk);
+    case 4 * 8 + 1: return heat_fused_t<4, MODE, 1>(ctx, in, out, hk);
+    }
+    b2r_set_error("fused heat needs R*D <= 4 (R=%d, D=%d)", R, D);
+    return B2R_E_UNSUPPORTED;
+}
+
+int b2r_heat_fused_launch(b2r_ctx* ctx, const void* in, void* out, const b2r_heat_params* p)
+{
+    HeatK hk;
+    for (int a = 0; a < 3; ++a) {
+        hk.k[a] = p->k[a];
+        hk.h2[a] = p->h2[a];
+    }
+    hk.adt = p->adt;
+    if (p->mode == B2R_HEAT_F32)
+        return heat_fused_mode<B2R_HEAT_F32>(ctx, (const float*)in, (float*)out, hk, p->D);
+    return heat_fused_mode<B2R_HEAT_REF>(ctx, (const float*)in, (float*)out, hk, p->D);
 }

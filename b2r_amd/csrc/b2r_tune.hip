@@ -1,0 +1,80 @@
+// b2r_tune.hip -- autotuner for launch shapes.
+//
+// No counterpart in the reference (its CUDA_BLOCK_SIZE is a compile-time 32 or
+// 16, heat-3D.cu:8, eqwp.cu:25).  For one kernel kind on the context's actual
+// geometry it times every candidate shape with back-to-back launches between
+// HIP events (b2r_time_steps), keeps the fastest in the context and appends one
+// line per search to a plain-text record:
+//   kind=<k> S=<Sx>x<Sy>x<Sz> blocks=<n> R=<R> radius=<r> best=<field>:<v>,... ms=<t> tried=<n>
+#include "b2r_common.h"
+
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+extern "C" int b2r_time_steps(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
+                              const b2r_heat_params* p, int iters, float* ms);
+
+extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
+                            const b2r_heat_params* p, const char* record_path, b2r_tuning* out)
+{
+    B2R_CHECK(ctx && d_a && d_b && kind >= 0 && kind <= 3, B2R_E_ARG, "bad argument");
+    const b2r_tuning base = ctx->tune;
+    std::vector<b2r_tuning> cand;
+    const int zcs[] = {0, 8, 16, 32, 64, 128, 256};
+    if (kind == 0 || kind == 3) {
+        const int rys[] = {1, 2, 4, 8};
+        for (int ry : rys)
+            for (int zc : zcs) {
+                b2r_tuning t = base;
+                t.heat_ry = ry;
+                t.heat_zc = zc;
+                cand.push_back(t);
+            }
+    } else {
+        for (int zc : zcs) {
+            b2r_tuning t = base;
+            if (kind == 1) t.gol_step_zc = zc;
+            else t.gol_zc = zc;
+            cand.push_back(t);
+        }
+    }
+    const int warm = 2, iters = 6;
+    float best_ms = 1e30f;
+    b2r_tuning best = base;
+    int tried = 0;
+    for (const b2r_tuning& t : cand) {
+        ctx->tune = t;
+        float ms = 0;
+        int rc = b2r_time_steps(ctx, kind, d_a, d_b, p, warm, &ms);
+        if (!rc) rc = b2r_time_steps(ctx, kind, d_a, d_b, p, iters, &ms);
+        if (rc) {
+            ctx->tune = base;
+            return rc;
+        }
+        ++tried;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    }
+    ctx->tune = best;
+    if (out) *out = best;
+    if (record_path && *record_path) {
+        FILE* f = fopen(record_path, "a");
+        B2R_CHECK(f, B2R_E_ARG, "cannot open %s", record_path);
+        static const char* names[] = {"heat_step", "gol_step", "gol_fused", "heat_fused"};
+        fprintf(f, "kind=%s S=%dx%dx%d blocks=%d R=%d radius=%d", names[kind], ctx->L.S[0],
+                ctx->L.S[1], ctx->L.S[2], ctx->L.nblocks, ctx->g.R, ctx->g.radius);
+        if (kind == 0 || kind == 3) {
+            fprintf(f, " heat_D=%d best=heat_ry:%d,heat_zc:%d", p ? p->D : 0, best.heat_ry,
+                    best.heat_zc);
+        } else {
+            fprintf(f, " best=%s:%d", kind == 1 ? "gol_step_zc" : "gol_zc",
+                    kind == 1 ? best.gol_step_zc : best.gol_zc);
+        }
+        fprintf(f, " ms_per_launch=%.5f tried=%d\n", best_ms / iters, tried);
+        fclose(f);
+    }
+    return B2R_OK;
+}

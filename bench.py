@@ -51,25 +51,32 @@ def read_traffic():
     return d
 
 
-def gol_bench(A, R, steps, warmup):
+def gol_bench(A, R, steps, warmup, fused=1, zc=0):
+    """steps generations of 512^3; kernel_ms = one launch of the dominant kernel
+    (the fused R-generation kernel, or the single-generation kernel)."""
     env = (512, 512, 512)
     with A.Context(env, (2, 2, 2), R, 1, A.BOUNDARY_PERIODIC, 1) as c:
+        if zc:
+            c.set_tuning(gol_zc=zc)
         a, b = c.field(), c.field()
         c.init_bernoulli(a, 1, int(0.3 * 16777216))
         if warmup:
-            c.gol_run(a, b, warmup)
+            c.gol_run(a, b, warmup, fused=fused)
         c.init_bernoulli(a, 1, int(0.3 * 16777216))
         c.sync()
         t0 = time.perf_counter()
-        _, ms = c.gol_run(a, b, steps, timed=True)
+        _, ms = c.gol_run(a, b, steps, fused=fused, timed=True)
         c.sync()
         wall = time.perf_counter() - t0
-        # the step kernel alone, HIP events on the context stream
-        kiters = 200
-        kms = c.time_steps(1, a, b, kiters) / kiters
+        # the kernel alone, HIP events on the context stream
+        kiters = 50 if fused else 200
+        kms = c.time_steps(2 if fused else 1, a, b, kiters) / kiters
     cells = 512 ** 3
-    return dict(R=R, ms=ms, wall_s=wall, glups=cells * steps / (ms * 1e-3) / 1e9,
-                kernel_ms=kms)
+    gens_per_launch = R if fused else 1
+    return dict(R=R, fused=fused, ms=ms, wall_s=wall,
+                glups=cells * steps / (ms * 1e-3) / 1e9, kernel_ms=kms,
+                gens_per_launch=gens_per_launch,
+                kernel_gbs_alg=2.0 * cells * gens_per_launch / (kms * 1e-3) / 1e9)
 
 
 def stream_bw(A):
@@ -158,6 +165,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="headline only, no extras")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--gol-zc", type=int, default=0, help="z chunk of the fused kernel (0: auto)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the halo "
@@ -168,12 +176,15 @@ def main():
 
     runs = []
     for R in (1, 2, 4):
-        log(f"gol 512^3 R={R}")
-        runs.append(gol_bench(A, R, args.steps, args.warmup))
+        log(f"gol 512^3 R={R} fused")
+        runs.append(gol_bench(A, R, args.steps, args.warmup, fused=1, zc=args.gol_zc))
         log(json.dumps(runs[-1]))
     best = max(runs, key=lambda r: r["glups"])
     cells = 512 ** 3
-    achieved = 2.0 * cells / (best["kernel_ms"] * 1e-3) / 1e9
+    achieved = best["kernel_gbs_alg"]
+    log("gol 512^3 R=1 step-by-step (plain)")
+    plain = gol_bench(A, 1, args.steps, args.warmup, fused=0)
+    log(json.dumps(plain))
     tr = read_traffic()
     traffic = None
     if tr and "gol_step_bytes_per_launch" in tr:
@@ -221,13 +232,21 @@ def main():
         "dtype": "u8",
         "data": "synthetic (SplitMix64 Bernoulli(0.3), seed 1)",
         "config": {"workload": "gol3d_512^3_u8_periodic_2x2x2", "R": best["R"],
-                   "generations": args.steps,
-                   "glups_by_R": {str(r["R"]): round(r["glups"], 3) for r in runs}},
+                   "generations": args.steps, "fused_R_generations_per_launch": True,
+                   "glups_by_R": {str(r["R"]): round(r["glups"], 3) for r in runs},
+                   "plain_step_by_step_R1_glups": round(plain["glups"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "gol_step_kernel", "kernel_ms": round(best["kernel_ms"], 5),
-                     "bytes_per_launch_algorithmic": 2 * cells},
+                     "kernel": f"gol_fused_kernel<{best['R']}>",
+                     "kernel_ms": round(best["kernel_ms"], 5),
+                     "bytes_per_launch_algorithmic": 2 * cells * best["gens_per_launch"],
+                     "note": "2 B/LUP x 512^3 cells x R generations per launch; a fused "
+                             "launch reads and writes each cell once per R generations, "
+                             "so frac can exceed 1",
+                     "single_pass_kernel": "gol_step_kernel",
+                     "single_pass_achieved": round(plain["kernel_gbs_alg"], 1),
+                     "single_pass_frac": round(plain["kernel_gbs_alg"] / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
         "also": also,
     }

@@ -89,6 +89,16 @@ typedef struct b2r_heat_params {
     double h2[3];      /* REF mode: Dx*Dx, Dy*Dy, Dz*Dz (heat-3D.cu:11-13)          */
 } b2r_heat_params;
 
+/* Launch shapes of a context; 0 selects the built-in automatic choice.  No
+ * counterpart in the reference (CUDA_BLOCK_SIZE is a compile-time 32 or 16,
+ * heat-3D.cu:8, eqwp.cu:25); set by hand or by b2r_autotune. */
+typedef struct b2r_tuning {
+    int heat_ry;      /* rows per thread of the heat z-march: 1, 2, 4 or 8         */
+    int heat_zc;      /* z planes marched per heat workgroup                        */
+    int gol_zc;       /* z planes marched per fused Game of Life workgroup          */
+    int gol_step_zc;  /* z planes marched per single-step Game of Life workgroup    */
+} b2r_tuning;
+
 /* ---- library / context ---------------------------------------------------- */
 const char* b2r_last_error(void);
 int b2r_version(void);
@@ -100,6 +110,8 @@ int b2r_create(const b2r_geom* geom, int device, b2r_ctx** out);
 int b2r_destroy(b2r_ctx* ctx);  /* b2r_sync_finalize (b2r_halo_sync.cu:121-128) */
 int b2r_get_layout(const b2r_ctx* ctx, b2r_layout* out);
 int b2r_synchronize(b2r_ctx* ctx);
+int b2r_set_tuning(b2r_ctx* ctx, const b2r_tuning* t);
+int b2r_get_tuning(const b2r_ctx* ctx, b2r_tuning* t);
 
 /* Which global faces of a FROZEN grid are frozen: bit 2*axis+side (axis 0..2 =
  * x,y,z; side 0 = minus, 1 = plus).  Default 0x3F (all).  A cleared bit makes
@@ -188,6 +200,56 @@ int b2r_gol_step(b2r_ctx* ctx, const void* d_in, void* d_out, int b2r_i);
 int b2r_gol_run(b2r_ctx* ctx, void* d_a, void* d_b, int nsteps, int fused,
                 int* result_in_b, float* ms);
 
+/* ---- elastic velocity-stress FD4 (eqwp.cu) ------------------------------------ */
+/* One shot: 9 state fields (vx vy vz sxx syy szz sxy sxz syz, index 0..8) and
+ * the medium lambda, mu, b = 1/rho (9, 10, 11) on a context of radius 4 (one
+ * step = stress stage + velocity stage, radius 2 each, so an R-step window
+ * needs a 4R halo).  Replaces eqwp_cuda_init (eqwp.cu:515-582); per step:
+ *   sigma += dt*(lambda tr(E) I + 2 mu E),  E = sym grad v  (eqwp_fd4_stress,
+ *            eqwp.cu:341-477, lambda+2mu / lambda / mu of :460-466)
+ *   v     += dt*b*div(sigma)                (eqwp_fd4_vx/vy/vz, :60-331)
+ * first derivatives (-1/12 (f[+2]-f[-2]) + 2/3 (f[+1]-f[-1])) * inv_h (:22-23),
+ * then point forces v_c += dt*b*f(t) and receiver samples after the step. */
+typedef struct b2r_el_shot b2r_el_shot;
+int b2r_el_create(b2r_ctx* ctx, float dt, const float inv_h[3], b2r_el_shot** out);
+int b2r_el_destroy(b2r_el_shot* s);
+int b2r_el_field(const b2r_el_shot* s, int which, void** d_field);
+/* medium on the global owned grid (x fastest); halos filled by exchange */
+int b2r_el_set_medium(b2r_el_shot* s, const float* lam, const float* mu, const float* b);
+int b2r_el_reset(b2r_el_shot* s);  /* zero state, t = 0 */
+/* the displacement accumulator of eqwp_fd4_vy (eqwp.cu:231, 234): d += dt*dt*b*(div sigma)_y */
+int b2r_el_enable_displacement(b2r_el_shot* s);
+/* nsrc point forces at global cells xyz[3i..], component comp[i] (0..2), values
+ * wav[t*nsrc + i] for steps t < nt (the force patch of eqwp.cu:561-566 is one
+ * source per cell with a constant value for t < FRC_TIME). */
+int b2r_el_set_sources(b2r_el_shot* s, int nsrc, const int* xyz, const int* comp, int nt,
+                       const float* wav);
+/* nrec receivers sampling field comp (0..8) after each step t < nt */
+int b2r_el_set_receivers(b2r_el_shot* s, int nrec, const int* xyz, int comp, int nt);
+int b2r_el_get_receivers(b2r_el_shot* s, float* h_rec);           /* nt x nrec */
+int b2r_el_get_state(b2r_el_shot* s, int which, float* h_global); /* -1: displacement */
+int b2r_el_set_state(b2r_el_shot* s, int which, const float* h_global);
+/* nsteps steps of b2r.cpp:179-221 / eqwp_gpu_main (eqwp.cu:661-714): every R
+ * steps the 9 fields are exchanged, then R steps.  mode 0 (plain): per shot,
+ * stress, vx, vy, vz launches and a device synchronisation each step;
+ * mode 1 (batched): one stress and one velocity launch per step for all shots;
+ * mode 2: mode 1 with the time loop replayed from HIP graphs.  The shots of a
+ * batch may differ in size; they must share the device and R. */
+int b2r_el_forward(b2r_el_shot* s, int nsteps, int mode, float* ms);
+int b2r_el_forward_batch(b2r_el_shot* const* shots, int nshots, int nsteps, int mode, float* ms);
+/* misfit 1/2 sum (rec - obs)^2 over the recorded samples, reduced in FP64 */
+int b2r_el_misfit(b2r_el_shot* s, const float* h_obs, double* misfit);
+/* Discrete adjoint (no counterpart in the reference): misfit of nsteps steps
+ * from a zero state and its gradient w.r.t. lambda, mu and b per owned cell
+ * (global grids), forward states checkpointed every ckpt steps and recomputed
+ * per segment.  One block, R = 1, frozen faces. */
+int b2r_el_gradient(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt, double* misfit,
+                    float* h_glam, float* h_gmu, float* h_gb);
+/* flags bit 0: start from the current state (not zero); h_adj0 (optional, 9
+ * global grids): the adjoint state at the start = d misfit / d initial state */
+int b2r_el_gradient_ex(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt, int flags,
+                       double* misfit, float* h_glam, float* h_gmu, float* h_gb, float* h_adj0);
+
 /* ---- measurement helpers ---------------------------------------------------- */
 /* iters device-to-device copies of bytes (a STREAM-copy kernel): *ms total. */
 int b2r_stream_copy(b2r_ctx* ctx, void* d_dst, const void* d_src, size_t bytes,
@@ -198,6 +260,13 @@ int b2r_stream_copy(b2r_ctx* ctx, void* d_dst, const void* d_src, size_t bytes,
  * in the reference: it times the kernel alone for the roofline figure. */
 int b2r_time_steps(b2r_ctx* ctx, int kind, void* d_a, void* d_b, const b2r_heat_params* p,
                    int iters, float* ms);
+/* Autotuner (no counterpart in the reference): times every candidate launch
+ * shape of one kernel kind (as b2r_time_steps: 0 heat step, 1 GoL step, 2 fused
+ * GoL, 3 fused heat) on this context's geometry, keeps the fastest in the
+ * context (b2r_get_tuning) and, if record_path is given, appends one text line
+ * with the geometry, the choice and its time per launch. */
+int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b, const b2r_heat_params* p,
+                 const char* record_path, b2r_tuning* out);
 /* Scratch device buffer owned by the context (freed by b2r_destroy). */
 int b2r_dev_alloc(b2r_ctx* ctx, size_t bytes, void** d_ptr);
 int b2r_dev_free(b2r_ctx* ctx, void* d_ptr);

@@ -42,6 +42,15 @@ def _load():
         "orc_unpack": (None, [_vp, _vp, _i, _vp, _vp, _vp, _i, _i, _i, _ll, _ll, _i, _i]),
         "orc_exchange": (None, [_vp, _vp, _vp, _vp, _i, _i, _i, _ll, _ll, _i, _i]),
         "orc_num_threads": (_i, []),
+        "orc_el_stress_f32": (None, [_vp, _vp, _vp, _i, _i, _i, _vp, C.c_float]),
+        "orc_el_velocity_f32": (None, [_vp, _vp, _vp, _i, _i, _i, _vp, C.c_float]),
+        "orc_el_forward_f32": (None, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, C.c_float, _i,
+                                      _i, _vp, _vp, _vp, _i, _vp, _i, _vp]),
+        "orc_el_forward_f64": (None, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _d, _i, _i, _vp,
+                                      _vp, _vp, _i, _vp, _i, _vp]),
+        "orc_el_sumsq_f64": (_d, [_vp, _i, _i, _i]),
+        "orc_el_adjoint_f64": (None, [_vp, _vp, _vp, _i, _i, _i, _vp, _d, _i, _i, _vp, _vp, _vp,
+                                      _i, _vp, _i, _vp, _vp, _vp, _vp]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(lib, n)

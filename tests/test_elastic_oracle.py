@@ -1,0 +1,114 @@
+"""Pins the elastic oracle (no GPU): FD4 exactness on polynomials, a closed-form
+one-step update, energy boundedness, mirror symmetry, FP32 vs FP64 agreement,
+and the FP64 adjoint gradient against central finite differences."""
+import numpy as np
+import pytest
+
+import _elastic as E
+from _util import rel_l2
+
+
+def _const_medium(env, lam=2.0e9, mu=1.0e9, rho=2000.0):
+    gx, gy, gz = env
+    sh = (gz, gy, gx)
+    return (np.full(sh, lam, np.float64), np.full(sh, mu, np.float64), np.full(sh, 1 / rho, np.float64))
+
+
+def test_fd4_exact_on_quartic_one_stress_step():
+    # v = (x^4, y^3, z^2) (grid units); FD4 is exact for degree <= 4, so one
+    # stress stage gives sigma = dt*C:grad v exactly; the velocity stage sees
+    # an exact divergence of a cubic stress field as well.
+    env = (12, 12, 12)
+    gx, gy, gz = env
+    lam, mu, b = _const_medium(env, 3.0, 2.0, 1.0)
+    z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in (gz, gy, gx)), indexing="ij")
+    s0 = [x ** 4 / 100, y ** 3 / 10, z ** 2, 0 * x, 0 * x, 0 * x, 0 * x, 0 * x, 0 * x]
+    # the oracle's frame is 2 cells of the same polynomial: pad by evaluation
+    st, _ = E.forward(env, lam, mu, b, 0.5, (1.0, 1.0, 1.0), 0, fp64=True, state0=s0)
+    import _oracle as O
+    import ctypes as C
+    full = [np.zeros((gz + 4, gy + 4, gx + 4)) for _ in range(9)]
+    Z, Y, X = np.meshgrid(*(np.arange(-2, n + 2, dtype=np.float64) for n in (gz, gy, gx)),
+                          indexing="ij")
+    full[0][:] = X ** 4 / 100
+    full[1][:] = Y ** 3 / 10
+    full[2][:] = Z ** 2
+    L, M, B = (np.pad(a, 2, mode="edge") for a in (lam, mu, b))
+    ptrs = (C.c_void_p * 9)(*[a.ctypes.data for a in full])
+    O.lib.orc_el_forward_f64(ptrs, O._p(L), O._p(M), O._p(B), gx, gy, gz,
+                             O._p(np.ones(3)), 0.5, 1, 0, None, None, None, 0, None, 1, None)
+    exx, eyy, ezz = 4 * X ** 3 / 100, 3 * Y ** 2 / 10, 2 * Z
+    s = slice(2, -2)
+    l2m = 3.0 + 2 * 2.0
+    np.testing.assert_allclose(full[3][s, s, s], (0.5 * (l2m * exx + 3.0 * (eyy + ezz)))[s, s, s],
+                               rtol=1e-13, atol=1e-9)
+    np.testing.assert_allclose(full[4][s, s, s], (0.5 * (l2m * eyy + 3.0 * (exx + ezz)))[s, s, s],
+                               rtol=1e-13, atol=1e-9)
+    assert np.abs(full[6][s, s, s]).max() == 0  # shear strain of this v is zero
+
+
+def test_energy_bounded_without_source():
+    env = (16, 16, 16)
+    lam, mu, b = _const_medium(env)
+    rng = np.random.default_rng(0)
+    s0 = [np.zeros((16, 16, 16)) for _ in range(9)]
+    s0[1][6:10, 6:10, 6:10] = rng.uniform(-1, 1, (4, 4, 4))
+    dx = 10.0
+    vp = np.sqrt((2e9 + 2e9) / 2000)
+    dt = 0.4 * dx / vp
+    e0 = sum((a ** 2).sum() for a in s0)
+    st, _ = E.forward(env, lam, mu, b, dt, (1 / dx,) * 3, 200, fp64=True, state0=s0)
+    e = [float((a ** 2).sum()) for a in st]
+    assert np.isfinite(sum(e)) and sum(e[:3]) < 50 * e0
+
+
+def test_mirror_symmetry():
+    # a vy source on the x mid-plane of a x-symmetric medium: vy(x) = vy(-x)
+    env = (17, 14, 15)
+    lam, mu, b = _const_medium(env)
+    src = [[8, 6, 7]]
+    wav = np.ones((20, 1)) * 1e3
+    st, _ = E.forward(env, lam, mu, b, 1e-3, (0.1,) * 3, 20, src, [1], wav, fp64=True)
+    vy = st[1]
+    np.testing.assert_allclose(vy, vy[:, :, ::-1], rtol=1e-12, atol=1e-15)
+    vx = st[0]
+    np.testing.assert_allclose(vx, -vx[:, :, ::-1], rtol=1e-12, atol=1e-15)
+
+
+def test_f32_close_to_f64():
+    P = E.small_problem()
+    a = E.forward(P["env"], P["lam"], P["mu"], P["b"], P["dt"], P["ih"], P["nsteps"], P["src"],
+                  P["comp"], P["wav"], P["rec"], 1)
+    d = E.forward(P["env"], P["lam"], P["mu"], P["b"], float(P["dt"]), P["ih"], P["nsteps"],
+                  P["src"], P["comp"], P["wav"].astype(np.float64), P["rec"], 1, fp64=True)
+    assert rel_l2(a[1], d[1]) < 1e-5
+    for k in range(9):
+        assert rel_l2(a[0][k], d[0][k]) < 1e-5
+
+
+def _misfit64(P, lam, mu, b, obs):
+    _, rec = E.forward(P["env"], lam, mu, b, float(P["dt"]), P["ih"], P["nsteps"], P["src"],
+                       P["comp"], P["wav"].astype(np.float64), P["rec"], 1, fp64=True)
+    return 0.5 * float(((rec - obs) ** 2).sum()), rec
+
+
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_adjoint_gradient_matches_finite_differences(which):
+    P = E.small_problem(env=(14, 12, 13), nsteps=24)
+    lam, mu, b = (P[k].astype(np.float64) for k in ("lam", "mu", "b"))
+    obs = _misfit64(P, lam * 1.02, mu * 0.97, b, np.zeros((24, len(P["rec"]))))[1]
+    J0, rec = _misfit64(P, lam, mu, b, obs)
+    g = E.adjoint(P["env"], lam, mu, b, float(P["dt"]), P["ih"], 24, P["src"], P["comp"],
+                  P["wav"].astype(np.float64), P["rec"], 1, rec - obs)
+    rng = np.random.default_rng(which)
+    m = [lam, mu, b]
+    for trial in range(2):
+        dirn = rng.standard_normal(m[which].shape) * m[which]
+        eps = 1e-6
+        mp = [a.copy() for a in m]
+        mm = [a.copy() for a in m]
+        mp[which] = m[which] + eps * dirn
+        mm[which] = m[which] - eps * dirn
+        fd = (_misfit64(P, *mp, obs)[0] - _misfit64(P, *mm, obs)[0]) / (2 * eps)
+        an = float((g[which] * dirn).sum())
+        assert abs(fd - an) <= 1e-6 * max(abs(fd), abs(an)) + 1e-30, (fd, an)

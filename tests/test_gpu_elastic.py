@@ -1,0 +1,198 @@
+"""GPU parity of the elastic path through the C ABI: forward bit-exact against
+the FP32 oracle (plain, batched and graph modes, multi-block B2R windows,
+uneven batches), FP64 error report, and the hand-written adjoint against the
+FP64 CPU adjoint, finite differences and a dot-product test."""
+import sys
+
+import numpy as np
+import pytest
+
+import _elastic as E
+from _util import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+A = None
+
+
+def setup_module(module):
+    global A
+    from b2r_amd import _abi
+    A = _abi
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.ascontiguousarray(a, np.float32).view(np.uint32),
+                          np.ascontiguousarray(b, np.float32).view(np.uint32))
+
+
+def _shot(ctx, P, rec_comp=1, nt=None):
+    s = A.ElasticShot(ctx, float(P["dt"]), P["ih"])
+    s.set_medium(P["lam"], P["mu"], P["b"])
+    s.set_sources(P["src"], P["comp"], P["wav"])
+    s.set_receivers(P["rec"], rec_comp, nt or P["nsteps"])
+    return s
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("env,blocks,R", [((48, 40, 48), (1, 1, 1), 1),
+                                          ((48, 40, 48), (2, 1, 2), 1),
+                                          ((48, 40, 48), (2, 2, 1), 2),
+                                          ((48, 56, 56), (1, 2, 2), 3)])
+def test_forward_bitwise_vs_oracle(env, blocks, R, mode):
+    P = E.small_problem(env=env, nsteps=13)
+    say("elastic forward", blocks, "R", R, "mode", mode)
+    with A.Context(P["env"], blocks, R, 4, A.BOUNDARY_FROZEN, 4) as c:
+        with _shot(c, P) as s:
+            s.forward(P["nsteps"], mode)
+            got = [s.state(k) for k in range(9)]
+            rec = s.receivers()
+    st, want_rec = E.forward(P["env"], P["lam"], P["mu"], P["b"], P["dt"], P["ih"], P["nsteps"],
+                             P["src"], P["comp"], P["wav"], P["rec"], 1)
+    for k in range(9):
+        assert bits_equal(got[k], st[k]), A.EL_FIELDS[k]
+    assert bits_equal(rec, want_rec)
+
+
+def test_forward_displacement_and_split_runs():
+    P = E.small_problem(env=(24, 20, 22), nsteps=10)
+    with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
+        with _shot(c, P) as s:
+            s.enable_displacement()
+            s.forward(4, 0)
+            s.forward(6, 1)   # continues the clock (sources, receivers)
+            disp = s.state("disp")
+            vy = s.state("vy")
+            rec = s.receivers()
+    st, want_rec, d = E.forward(P["env"], P["lam"], P["mu"], P["b"], P["dt"], P["ih"], 10,
+                                P["src"], P["comp"], P["wav"], P["rec"], 1, disp=True)
+    assert bits_equal(vy, st[1]) and bits_equal(disp, d) and bits_equal(rec, want_rec)
+
+
+def test_uneven_batch_matches_single_shots():
+    envs = [(32, 24, 40), (48, 32, 24), (24, 40, 32)]
+    Ps = [E.small_problem(env=e, seed=i, nsteps=9) for i, e in enumerate(envs)]
+    ctxs = [A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) for P in Ps]
+    shots = [_shot(c, P) for c, P in zip(ctxs, Ps)]
+    try:
+        A.el_forward_batch(shots, 9, mode=2)
+        for P, s in zip(Ps, shots):
+            _, want = E.forward(P["env"], P["lam"], P["mu"], P["b"], P["dt"], P["ih"], 9,
+                                P["src"], P["comp"], P["wav"], P["rec"], 1)
+            assert bits_equal(s.receivers(), want)
+    finally:
+        for s in shots:
+            s.close()
+        for c in ctxs:
+            c.close()
+
+
+def test_forward_error_vs_fp64():
+    P = E.small_problem(env=(40, 36, 44), nsteps=60)
+    with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
+        with _shot(c, P) as s:
+            s.forward(P["nsteps"], 2)
+            rec = s.receivers()
+    _, r64 = E.forward(P["env"], P["lam"], P["mu"], P["b"], float(P["dt"]), P["ih"], P["nsteps"],
+                       P["src"], P["comp"], P["wav"].astype(np.float64), P["rec"], 1, fp64=True)
+    err = rel_l2(rec, r64)
+    say("elastic receivers rel-L2 vs FP64:", err, "max abs:", float(np.abs(rec - r64).max()))
+    assert err < 1e-5
+
+
+# ---------------------------------------------------------------- adjoint
+def _obs(P):
+    _, obs = E.forward(P["env"], P["lam"] * np.float32(1.03), P["mu"] * np.float32(0.98), P["b"],
+                       P["dt"], P["ih"], P["nsteps"], P["src"], P["comp"], P["wav"], P["rec"], 1)
+    return obs
+
+
+@pytest.mark.parametrize("ckpt", [1, 7, 100])
+def test_gradient_vs_fp64_cpu_adjoint(ckpt):
+    P = E.small_problem(env=(20, 18, 16), nsteps=30)
+    obs = _obs(P)
+    with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
+        with _shot(c, P) as s:
+            J, g = s.gradient(obs, P["nsteps"], ckpt)
+    lam, mu, b = (P[k].astype(np.float64) for k in ("lam", "mu", "b"))
+    _, rec = E.forward(P["env"], lam, mu, b, float(P["dt"]), P["ih"], P["nsteps"], P["src"],
+                       P["comp"], P["wav"].astype(np.float64), P["rec"], 1, fp64=True)
+    J64 = 0.5 * float(((rec - obs) ** 2).sum())
+    g64 = E.adjoint(P["env"], lam, mu, b, float(P["dt"]), P["ih"], P["nsteps"], P["src"],
+                    P["comp"], P["wav"].astype(np.float64), P["rec"], 1, rec - obs)
+    say("ckpt", ckpt, "misfit", J, J64, "grad rel:", [rel_l2(g[k], g64[k]) for k in range(3)])
+    assert abs(J - J64) <= 1e-4 * J64
+    for k in range(3):
+        assert rel_l2(g[k], g64[k]) < 1e-3
+
+
+def test_gradient_checkpoint_invariance():
+    # recomputation replays the forward bit for bit, so the checkpoint
+    # interval does not change the gradient at all
+    P = E.small_problem(env=(20, 18, 16), nsteps=20)
+    obs = _obs(P)
+    out = []
+    for ckpt in (1, 6, 20):
+        with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
+            with _shot(c, P) as s:
+                out.append(s.gradient(obs, 20, ckpt)[1])
+    for g in out[1:]:
+        for k in range(3):
+            assert bits_equal(g[k], out[0][k])
+
+
+def test_gradient_finite_differences():
+    P = E.small_problem(env=(18, 16, 14), nsteps=24)
+    obs = _obs(P)
+    with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
+        with _shot(c, P) as s:
+            _, g = s.gradient(obs, P["nsteps"], 5)
+    m = [P[k].astype(np.float64) for k in ("lam", "mu", "b")]
+
+    def J(mm):
+        _, rec = E.forward(P["env"], *mm, float(P["dt"]), P["ih"], P["nsteps"], P["src"],
+                           P["comp"], P["wav"].astype(np.float64), P["rec"], 1, fp64=True)
+        return 0.5 * float(((rec - obs) ** 2).sum())
+
+    rng = np.random.default_rng(7)
+    for trial in range(6):
+        which = trial % 3
+        dirn = rng.standard_normal(m[which].shape) * m[which]
+        eps = 1e-5
+        mp = [a.copy() for a in m]
+        mn = [a.copy() for a in m]
+        mp[which] += eps * dirn
+        mn[which] -= eps * dirn
+        fd = (J(mp) - J(mn)) / (2 * eps)
+        an = float((g[which].astype(np.float64) * dirn).sum())
+        say("FD", which, fd, an)
+        assert abs(fd - an) <= 1e-3 * max(abs(fd), abs(an))
+
+
+def test_adjoint_dot_product():
+    # P: initial state -> receiver data (linear).  With obs = 0 the adjoint
+    # state at t = 0 is P^T P x0, so <x0, P^T P x0> = |P x0|^2 = 2 J.
+    P = E.small_problem(env=(18, 16, 20), nsteps=25)
+    gx, gy, gz = P["env"]
+    rng = np.random.default_rng(11)
+    x0 = [rng.standard_normal((gz, gy, gx)).astype(np.float32) * np.float32(1e-3)
+          for _ in range(9)]
+    with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
+        s = A.ElasticShot(c, float(P["dt"]), P["ih"])
+        s.set_medium(P["lam"], P["mu"], P["b"])
+        s.set_receivers(P["rec"], 1, P["nsteps"])
+        s.reset()
+        for k in range(9):
+            s.set_state(k, x0[k])
+        J, _, adj0 = s.gradient(np.zeros((P["nsteps"], len(P["rec"])), np.float32),
+                                P["nsteps"], 6, from_current=True, adjoint_state=True)
+        s.close()
+    lhs = sum(float((x0[k].astype(np.float64) * adj0[k].astype(np.float64)).sum())
+              for k in range(9))
+    say("dot product", lhs, 2 * J)
+    assert abs(lhs - 2 * J) <= 1e-4 * abs(2 * J)

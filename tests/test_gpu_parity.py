@@ -240,13 +240,14 @@ GOL = [
 ]
 
 
+@pytest.mark.parametrize("fused", [0, 1])
 @pytest.mark.parametrize("env,blocks,R,periodic,nsteps", GOL)
-def test_gol_bitwise_vs_oracle(env, blocks, R, periodic, nsteps):
-    say("gol", env, blocks, "R", R, "periodic", periodic, "steps", nsteps)
+def test_gol_bitwise_vs_oracle(env, blocks, R, periodic, nsteps, fused):
+    say("gol", env, blocks, "R", R, "periodic", periodic, "steps", nsteps, "fused", fused)
     with A.Context(env, blocks, R, 1, int(periodic), 1) as c:
         a, b = c.field(), c.field()
         c.init_bernoulli(a, 1, int(0.3 * 16777216))
-        res, _ = c.gol_run(a, b, nsteps)
+        res, _ = c.gol_run(a, b, nsteps, fused=fused)
         got = c.download(res)
     u = O.bernoulli((env[2], env[1], env[0]), 1, 0.3)
     g = u if periodic else O.frame(u, 1)
@@ -277,13 +278,45 @@ def test_gol_full_size_R_invariance_and_population():
     env = (512, 512, 512)
     outs = {}
     for R in (1, 2, 4):
-        say("gol 512^3 R", R)
-        with A.Context(env, (2, 2, 2), R, 1, 1, 1) as c:
-            a, b = c.field(), c.field()
-            c.init_bernoulli(a, 1, int(0.3 * 16777216))
-            res, _ = c.gol_run(a, b, 8)
-            outs[R] = c.download(res)
-    assert np.array_equal(outs[1], outs[2]) and np.array_equal(outs[1], outs[4])
+        for fused in (0, 1):
+            say("gol 512^3 R", R, "fused", fused)
+            with A.Context(env, (2, 2, 2), R, 1, 1, 1) as c:
+                a, b = c.field(), c.field()
+                c.init_bernoulli(a, 1, int(0.3 * 16777216))
+                res, _ = c.gol_run(a, b, 8, fused=fused)
+                outs[(R, fused)] = c.download(res)
     u = O.bernoulli((512, 512, 512), 1, 0.3)
     want = O.gol(u, 8, periodic=True)
-    assert np.array_equal(outs[1], want)
+    for key, o in outs.items():
+        assert np.array_equal(o, want), key
+
+
+@pytest.mark.parametrize("zc", [4, 9, 64])
+def test_gol_fused_chunking(zc):
+    # z-chunk lengths that do not divide B, and a single chunk
+    env = (96, 72, 64)
+    with A.Context(env, (1, 2, 1), 3, 1, 0, 1) as c:
+        c.set_tuning(gol_zc=zc, gol_step_zc=zc)
+        a, b = c.field(), c.field()
+        c.init_bernoulli(a, 5, int(0.3 * 16777216))
+        res, _ = c.gol_run(a, b, 9, fused=1)
+        got = c.download(res)
+    want = O.unframe(O.gol(O.frame(O.bernoulli((64, 72, 96), 5, 0.3), 1), 9, False), 1)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("ry", [1, 2, 4, 8])
+@pytest.mark.parametrize("zc", [0, 5])
+def test_heat_tuning_variants(ry, zc):
+    env, D = (70, 40, 36), 3
+    k = (0.021, 0.033, 0.017)
+    p = A.heat_params(D, 0, k)
+    with A.Context(env, (1, 2, 1), 2, D, 0, 4) as c:
+        c.set_tuning(heat_ry=ry, heat_zc=zc)
+        a, b = c.field(), c.field()
+        c.init_uniform(a, 9)
+        res, _ = c.heat_run(a, b, p, 6)
+        got = c.download(res)
+    g = O.frame(O.uniform((36, 40, 70), 9), D)
+    want = O.unframe(O.heat_f32(g, D, 0, k, 0, (1, 1, 1), 6, False), D)
+    assert bits_equal(got, want)
