@@ -156,6 +156,7 @@ constexpr int GF_TXQ = 8;            // output quads (16 cells) per tile row
 constexpr int GF_NQ = GF_TXQ + 2;    // plus one quad per side
 constexpr int GF_NR = 32;            // rows incl. R halo rows per side
 constexpr int GF_PLANE = GF_NQ * GF_NR;
+constexpr int GF_THREADS = GF_PLANE;  // one quad position per thread and pass
 
 __device__ __forceinline__ uint4 q_add(uint4 a, uint4 b)
 {
@@ -173,7 +174,7 @@ __device__ __forceinline__ uint32_t dword_mask(int x, int lo, int hi)
 }
 
 template <int R>
-__global__ __launch_bounds__(256) void gol_fused_kernel(const uint8_t* __restrict__ in,
+__global__ __launch_bounds__(GF_THREADS) void gol_fused_kernel(const uint8_t* __restrict__ in,
                                                         uint8_t* __restrict__ out, KGeo g,
                                                         int ntx, int nty, int zc)
 {
@@ -206,14 +207,14 @@ __global__ __launch_bounds__(256) void gol_fused_kernel(const uint8_t* __restric
         return x + 15 >= 0 && x < g.S[0];
     };
 
-    // level-0 plane loads: quads pos = tid, tid + 256 of the GF_PLANE grid
-    constexpr int NPF = (GF_PLANE + 255) / 256;
+    // level-0 plane loads: quads pos = tid, tid + GF_THREADS, .. of the plane
+    constexpr int NPF = (GF_PLANE + GF_THREADS - 1) / GF_THREADS;
     uint4 pf[NPF];
     auto load_plane = [&](int z) {
         const long long pz = (long long)z * slice;
 #pragma unroll
         for (int s = 0; s < NPF; ++s) {
-            const int pos = tid + 256 * s;
+            const int pos = tid + GF_THREADS * s;
             pf[s] = make_uint4(0, 0, 0, 0);
             if (pos < GF_PLANE) {
                 const int r = pos / GF_NQ, q = pos % GF_NQ;
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(256) void gol_fused_kernel(const uint8_t* __restric
         uint4* r0 = ring + (zin % 3) * GF_PLANE;
 #pragma unroll
         for (int s = 0; s < NPF; ++s) {
-            const int pos = tid + 256 * s;
+            const int pos = tid + GF_THREADS * s;
             if (pos < GF_PLANE) r0[pos] = pf[s];
         }
         if (zin + 1 < zc1 + R) load_plane(zin + 1);
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(256) void gol_fused_kernel(const uint8_t* __restric
             const uint4* pp = prev + ((p + 1) % 3) * GF_PLANE;  // plane p+1
             const int nrow = GF_NR - 2 * t;
             // pass A: 3x3 (z,y) sums of the previous generation
-            for (int pos = tid; pos < nrow * GF_NQ; pos += 256) {
+            for (int pos = tid; pos < nrow * GF_NQ; pos += GF_THREADS) {
                 const int r = t + pos / GF_NQ, q = pos % GF_NQ;
                 const int i = r * GF_NQ + q;
                 uint4 s = q_add(q_add(pm[i - GF_NQ], pm[i]), pm[i + GF_NQ]);
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(256) void gol_fused_kernel(const uint8_t* __restric
             const int qa = t == R ? 1 : 0, qb = t == R ? GF_NQ - 1 : GF_NQ;
             const int nq = qb - qa;
             uint4* nxt = t < R ? ring + t * 3 * GF_PLANE + (p % 3) * GF_PLANE : nullptr;
-            for (int pos = tid; pos < nrow * nq; pos += 256) {
+            for (int pos = tid; pos < nrow * nq; pos += GF_THREADS) {
                 const int r = t + pos / nq, q = qa + pos % nq;
                 const int i = r * GF_NQ + q;
                 const uint4 P = pbuf[i];
@@ -347,7 +348,7 @@ static int gol_fused_launch_t(b2r_ctx* ctx, const void* in, void* out)
     if (zc > g.B[2]) zc = g.B[2];
     const int nzc = ceil_div(g.B[2], zc);
     dim3 grid(ntx * nty * nzc, g.nblk);
-    gol_fused_kernel<R><<<grid, 256, lds, ctx->stream>>>((const uint8_t*)in, (uint8_t*)out, g,
+    gol_fused_kernel<R><<<grid, GF_THREADS, lds, ctx->stream>>>((const uint8_t*)in, (uint8_t*)out, g,
                                                         ntx, nty, zc);
     B2R_LAUNCH_CHECK();
     return B2R_OK;

@@ -1,0 +1,75 @@
+// b2r_eqwp_host.cpp -- eqwp_cuda_init / eqwp_gpu_main (eqwp.cu:515-582,
+// 661-714) as shims over the elastic C ABI, with the reference's model: a
+// constant medium lambda = 32.00, mu = 174.87, 1/rho = 1/1800 (the law of
+// eqwp.cu:460-466 with the constants of :15-19), Dt = 0.5, Dx = Dy = Dz = 10
+// (:9-12), and a y-force of -10000 on the patch x in [Nx/2, Nx/2+Nx/10),
+// z in [Nz/2, Nz/2+Nz/10) at depth Ny-7 for the first 50 steps (:13-14,
+// 135-139, 561-566), placed once in the global grid.
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "b2r_host.h"
+
+static b2r_el_shot* g_shot = nullptr;
+
+static void die_on(int rc, const char* what)
+{
+    if (rc < 0) {
+        fprintf(stderr, "B2R Error: %s: %s\n", what, b2r_last_error());
+        exit(-1);
+    }
+}
+
+b2r_el_shot* eqwp_shot() { return g_shot; }
+
+void eqwp_cuda_init(int Ngx, int Ngy, int Ngz)
+{
+    (void)Ngx; (void)Ngy; (void)Ngz;  // the global context carries the geometry
+    b2r_ctx* c = b2r_host_ctx();
+    if (!c) {
+        fprintf(stderr, "B2R Error: eqwp_cuda_init before b2r_sync_init\n");
+        exit(-1);
+    }
+    const float ih[3] = {0.1f, 0.1f, 0.1f};
+    die_on(b2r_el_create(c, 0.5f, ih, &g_shot), "eqwp_cuda_init");
+    const long long n = (long long)ENV_DIM_X * ENV_DIM_Y * ENV_DIM_Z;
+    std::vector<float> lam(n, 32.00f), mu(n, 174.87f), b(n, (float)(1.0 / 1800.0));
+    die_on(b2r_el_set_medium(g_shot, lam.data(), mu.data(), b.data()), "medium");
+    std::vector<int> xyz, comp;
+    const int y = ENV_DIM_Y - 7 >= 0 ? ENV_DIM_Y - 7 : 0;
+    for (int x = ENV_DIM_X / 2; x < ENV_DIM_X / 2 + ENV_DIM_X / 10; ++x)
+        for (int z = ENV_DIM_Z / 2; z < ENV_DIM_Z / 2 + ENV_DIM_Z / 10; ++z) {
+            xyz.push_back(x);
+            xyz.push_back(y);
+            xyz.push_back(z);
+            comp.push_back(1);
+        }
+    const int nsrc = (int)comp.size(), frc_time = 50;
+    std::vector<float> wav((size_t)frc_time * nsrc, -10000.0f);
+    die_on(b2r_el_set_sources(g_shot, nsrc, xyz.data(), comp.data(), frc_time, wav.data()),
+           "sources");
+    die_on(b2r_el_enable_displacement(g_shot), "displacement");
+}
+
+void eqwp_cuda_finalize()
+{
+    if (g_shot) b2r_el_destroy(g_shot);
+    g_shot = nullptr;
+}
+
+double eqwp_gpu_main(int ts, int b2r_R, int Ngx, int Ngy, int Ngz, bool op_flag)
+{
+    (void)ts; (void)Ngx; (void)Ngy; (void)Ngz; (void)op_flag;
+    float ms = 0.0f;
+    // the reference structure: stress, vx, vy, vz per step, synchronised
+    die_on(b2r_el_forward(g_shot, b2r_R, 0, &ms), "eqwp_gpu_main");
+    return ms / 1000.0;
+}
+
+CELL_DT* eqwp_field(int which)
+{
+    void* f = nullptr;
+    die_on(b2r_el_field(g_shot, which, &f), "eqwp field");
+    return (CELL_DT*)f;
+}

@@ -13,6 +13,7 @@ int B2R_R = 1;
 int ENV_DIM_X = 256, ENV_DIM_Y = 256, ENV_DIM_Z = 256;
 int B2R_D = 1;
 int B2R_PERIODIC = 0;
+int B2R_ELEM_SIZE = sizeof(CELL_DT);
 char b2r_halo_update[6];
 
 static b2r_ctx* g_ctx = nullptr;
@@ -61,7 +62,7 @@ void b2r_sync_init()
     g.R = B2R_R;
     g.radius = B2R_D;
     g.boundary = B2R_PERIODIC ? B2R_BOUNDARY_PERIODIC : B2R_BOUNDARY_FROZEN;
-    g.elem_size = sizeof(CELL_DT);
+    g.elem_size = B2R_ELEM_SIZE;
     die_on(b2r_create(&g, 0, &g_ctx), "b2r_sync_init");
 }
 

@@ -19,6 +19,7 @@ extern int B2R_R;
 extern int ENV_DIM_X, ENV_DIM_Y, ENV_DIM_Z;  // compile-time in b2r_config.h:6-8
 extern int B2R_D;                            // compile-time in b2r_config.h:10
 extern int B2R_PERIODIC;                     // _PERIODIC_ (b2r_config.h:12)
+extern int B2R_ELEM_SIZE;                    // sizeof(CELL_DT): 4, or 1 for gol-3D.cu:6
 extern char b2r_halo_update[6];              // b2r.h:37-42
 
 b2r_ctx* b2r_host_ctx();  // the context behind the globals (created by b2r_sync_init)
@@ -40,3 +41,11 @@ double heat_3D_gpu_main(CELL_DT* h_in, int b2r_R, char* ef, int Ngx, int Ngy, in
 double gol_3D_gpu_main(unsigned char* h_in, int b2r_R, char* ef, int Ngx, int Ngy, int Ngz);
 // R inner steps of the resident heat field (the device half of heat_3D_gpu_main)
 double heat_3D_gpu_steps(CELL_DT** d_cur, CELL_DT** d_nxt, int b2r_R);
+
+// eqwp.cu:515-582, 592-605, 661-714 over the elastic ABI (B2R_D must be 4:
+// two radius-2 stages per step); eqwp_field(k) is d_vx .. d_sigma_zz (k 0..8)
+void eqwp_cuda_init(int Ngx, int Ngy, int Ngz);
+void eqwp_cuda_finalize();
+double eqwp_gpu_main(int ts, int b2r_R, int Ngx, int Ngy, int Ngz, bool op_flag);
+CELL_DT* eqwp_field(int which);
+b2r_el_shot* eqwp_shot();

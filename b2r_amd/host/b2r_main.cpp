@@ -1,17 +1,26 @@
 // b2r_main.cpp -- the B2R driver of b2r.cpp:125-250 on one GPU.
 //
-//   b2r_main R Bx By Bz [--env X Y Z] [--D d] [--iters N] [--periodic]
+//   b2r_main R Bx By Bz [--model heat|gol|eqwp] [--env X Y Z] [--D d]
+//            [--iters N] [--periodic]
 //
 // Checks the arguments like b2r.cpp:132-167, then runs the time loop of
-// b2r.cpp:179-221 for the heat model: every R steps boundary_updating of the
-// field, then R inner steps, and reports the timing lines of b2r.cpp:230-237.
+// b2r.cpp:179-221: every R steps boundary_updating of the model's fields
+// (the nine eqwp fields, b2r.cpp:197-206), then R model steps, and reports the
+// timing lines of b2r.cpp:230-237.  All Bx*By*Bz blocks live on the one GPU.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <vector>
+#include <string>
 
 #include "b2r_host.h"
+
+typedef std::chrono::steady_clock clk;
+
+static double since(clk::time_point t0)
+{
+    return std::chrono::duration<double>(clk::now() - t0).count();
+}
 
 int main(int argc, char* argv[])
 {
@@ -25,6 +34,7 @@ int main(int argc, char* argv[])
     BLOCK_DIM_Y = atoi(argv[3]);
     BLOCK_DIM_Z = atoi(argv[4]);
     int n_iter = 600;  // N_ITER (b2r.h:12)
+    std::string model = "heat";
     for (int i = 5; i < argc; ++i) {
         if (!strcmp(argv[i], "--env") && i + 3 < argc) {
             ENV_DIM_X = atoi(argv[++i]);
@@ -34,9 +44,20 @@ int main(int argc, char* argv[])
             B2R_D = atoi(argv[++i]);
         } else if (!strcmp(argv[i], "--iters") && i + 1 < argc) {
             n_iter = atoi(argv[++i]);
+        } else if (!strcmp(argv[i], "--model") && i + 1 < argc) {
+            model = argv[++i];
         } else if (!strcmp(argv[i], "--periodic")) {
             B2R_PERIODIC = 1;
         }
+    }
+    if (model == "gol") {
+        B2R_D = 1;
+        B2R_ELEM_SIZE = 1;
+    } else if (model == "eqwp") {
+        B2R_D = 4;  // stress + velocity stage, radius 2 each
+    } else if (model != "heat") {
+        printf("unknown model %s (heat, gol, eqwp)\n", model.c_str());
+        return 1;
     }
     printf("arguments received, where: B2R_R = %d, BLOCK_DIM_X = %d, BLOCK_DIM_Y = %d, "
            "BLOCK_DIM_Z = %d\n", B2R_R, BLOCK_DIM_X, BLOCK_DIM_Y, BLOCK_DIM_Z);
@@ -47,33 +68,64 @@ int main(int argc, char* argv[])
     }
     b2r_sync_init();  // validates R*D <= min(B)/2 and exits like b2r.cpp:54-59
     for (int r = 0; r < BLOCK_DIM_X * BLOCK_DIM_Y * BLOCK_DIM_Z; ++r) b2r_sync_flag(r);
+    b2r_ctx* ctx = b2r_host_ctx();
 
-    CELL_DT* cur = b2r_field_new();
-    CELL_DT* nxt = b2r_field_new();
-    if (b2r_field_init_uniform(b2r_host_ctx(), cur, 0) < 0 ||
-        b2r_field_copy(b2r_host_ctx(), nxt, cur) < 0) {
-        fprintf(stderr, "B2R Error: init: %s\n", b2r_last_error());
-        return -1;
+    CELL_DT* cur = nullptr;
+    CELL_DT* nxt = nullptr;
+    if (model == "eqwp") {
+        eqwp_cuda_init(B2R_BLOCK_SIZE_X, B2R_BLOCK_SIZE_Y, B2R_BLOCK_SIZE_Z);
+    } else {
+        cur = b2r_field_new();
+        nxt = b2r_field_new();
+        int rc = model == "gol" ? b2r_field_init_bernoulli(ctx, cur, 1, (uint32_t)(0.3 * 16777216))
+                                : b2r_field_init_uniform(ctx, cur, 0);
+        if (rc < 0 || b2r_field_copy(ctx, nxt, cur) < 0) {
+            fprintf(stderr, "B2R Error: init: %s\n", b2r_last_error());
+            return -1;
+        }
     }
-    b2r_synchronize(b2r_host_ctx());
+    b2r_synchronize(ctx);
     double sync_time = 0, compu_time = 0;
-    auto t_all = std::chrono::steady_clock::now();
+    auto t_all = clk::now();
     for (int i = 0; i < n_iter; i += B2R_R) {
-        auto s0 = std::chrono::steady_clock::now();
-        boundary_updating(i, cur);
-        b2r_synchronize(b2r_host_ctx());
-        sync_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - s0).count();
-        int r = n_iter - i < B2R_R ? n_iter - i : B2R_R;
-        compu_time += heat_3D_gpu_steps(&cur, &nxt, r);
+        const int r = n_iter - i < B2R_R ? n_iter - i : B2R_R;
+        auto s0 = clk::now();
+        if (model == "eqwp") {
+            for (int k = 0; k < 9; ++k) boundary_updating(i, eqwp_field(k));
+        } else {
+            boundary_updating(i, cur);
+        }
+        b2r_synchronize(ctx);
+        sync_time += since(s0);
+        if (model == "eqwp") {
+            compu_time += eqwp_gpu_main(i, r, B2R_BLOCK_SIZE_X, B2R_BLOCK_SIZE_Y,
+                                        B2R_BLOCK_SIZE_Z, i + B2R_R >= n_iter);
+        } else if (model == "heat") {
+            compu_time += heat_3D_gpu_steps(&cur, &nxt, r);
+        } else {
+            auto c0 = clk::now();
+            for (int k = 0; k < r; ++k) {
+                if (b2r_gol_step(ctx, cur, nxt, k) < 0) {
+                    fprintf(stderr, "B2R Error: gol step: %s\n", b2r_last_error());
+                    return -1;
+                }
+                CELL_DT* t = cur;
+                cur = nxt;
+                nxt = t;
+            }
+            b2r_synchronize(ctx);
+            compu_time += since(c0);
+        }
     }
-    double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_all).count();
-    printf("Execution/Model Parameters: B2R_R = %d, B2R_D: %d, ENV: %d X %d X %d, BLOCK_DIM_X = "
-           "%d, BLOCK_DIM_Y = %d, BLOCK_DIM_Z = %d\n", B2R_R, B2R_D, ENV_DIM_X, ENV_DIM_Y,
-           ENV_DIM_Z, BLOCK_DIM_X, BLOCK_DIM_Y, BLOCK_DIM_Z);
+    const double total = since(t_all);
+    printf("Execution/Model Parameters: model %s, B2R_R = %d, B2R_D: %d, ENV: %d X %d X %d, "
+           "BLOCK_DIM_X = %d, BLOCK_DIM_Y = %d, BLOCK_DIM_Z = %d\n", model.c_str(), B2R_R, B2R_D,
+           ENV_DIM_X, ENV_DIM_Y, ENV_DIM_Z, BLOCK_DIM_X, BLOCK_DIM_Y, BLOCK_DIM_Z);
     printf("%d Iterations with B2R_R=%d took: %g seconds\n", n_iter, B2R_R, total);
     printf("device computation took: %g seconds\n", compu_time);
     printf("halo synchronization took: %g seconds\n", sync_time);
     printf("GLUP/s: %g\n", (double)ENV_DIM_X * ENV_DIM_Y * ENV_DIM_Z * n_iter / total / 1e9);
+    if (model == "eqwp") eqwp_cuda_finalize();
     b2r_sync_finalize();
     return 0;
 }

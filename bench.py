@@ -158,8 +158,45 @@ def cpu_heat_reference(steps=100):
                 kind="reference", sample=f"128^3 heat delta1, {steps} steps ({el:.2f} s)")
 
 
+def profile_kernels(A, R=2):
+    """A short fixed sequence for rocprofv3 counter runs: a copy kernel of known
+    size (1 GiB read + 1 GiB written per launch, the scale for FETCH_SIZE /
+    WRITE_SIZE), then 5 launches each of the fused Game of Life kernel (512^3,
+    2x2x2, R), the heat delta1 step kernel (768^3) and the elastic stress and
+    velocity kernels (256^3)."""
+    nbytes = 1 << 30
+    with A.Context((16, 16, 16), (1, 1, 1), 1, 1, 0, 4) as c:
+        src, dst = c.dev_alloc(nbytes), c.dev_alloc(nbytes)
+        log("stream_copy 1 GiB x3")
+        c.stream_copy(dst, src, nbytes, 3)
+    with A.Context((512, 512, 512), (2, 2, 2), R, 1, A.BOUNDARY_PERIODIC, 1) as c:
+        a, b = c.field(), c.field()
+        c.init_bernoulli(a, 1, int(0.3 * 16777216))
+        c.boundary_updating_direct([a])
+        log(f"gol_fused_kernel<{R}> x5")
+        c.time_steps(2, a, b, 5)
+    prm = A.heat_params(1, A.HEAT_F32, (0.1, 0.1, 0.1))
+    with A.Context((768, 768, 768), (1, 1, 1), 1, 1, 0, 4) as c:
+        a, b = c.field(), c.field()
+        c.init_uniform(a, 0)
+        log("heat_step_kernel<1> x5")
+        c.time_steps(0, a, b, 5, prm)
+    from b2r_amd import workloads as W
+    env = (256, 256, 256)
+    vp, vs, rho = W.layered_medium(env, 0)
+    with A.Context(env, (1, 1, 1), 1, 4, 0, 4) as c:
+        s = A.ElasticShot(c, W.cfl_dt(float(vp.max()), 10.0), (0.1, 0.1, 0.1))
+        s.set_medium(*W.lame(vp, vs, rho))
+        log("elastic stress + velocity x5")
+        s.forward(5, 1)
+        s.close()
+    print(json.dumps({"profile_kernels": "done"}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="run the fixed kernel sequence used by the rocprofv3 counter runs")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
@@ -173,6 +210,10 @@ def main():
         sys.exit(2)
 
     from b2r_amd import _abi as A  # raises if the HIP library is missing
+
+    if args.profile_kernels:
+        profile_kernels(A)
+        return
 
     runs = []
     for R in (1, 2, 4):

@@ -174,6 +174,44 @@ def test_gradient_finite_differences():
         assert abs(fd - an) <= 1e-3 * max(abs(fd), abs(an))
 
 
+def test_autograd_wrapper_vp_vs_rho():
+    # the torch wrapper's (vp, vs, rho) gradient against FP64 central
+    # differences of the oracle misfit, on random directions
+    import torch
+    from b2r_amd.autograd import elastic_misfit
+    from b2r_amd import workloads as W
+    P = E.small_problem(env=(16, 14, 18), nsteps=20)
+    obs = _obs(P)
+    vp, vs, rho = (torch.tensor(P[k], requires_grad=True) for k in ("vp", "vs", "rho"))
+    with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
+        with _shot(c, P) as s:
+            J = elastic_misfit(vp, vs, rho, s, obs, P["nsteps"], 6)
+            J.backward()
+
+    def J64(v, w, r):
+        # medium in FP64 from FP64 inputs
+        v, w, r = (np.asarray(a, np.float64) for a in (v, w, r))
+        lam, mu, b = r * (v * v - 2 * w * w), r * w * w, 1.0 / r
+        _, rec = E.forward(P["env"], lam, mu, b, float(P["dt"]), P["ih"], P["nsteps"], P["src"],
+                           P["comp"], P["wav"].astype(np.float64), P["rec"], 1, fp64=True)
+        return 0.5 * float(((rec - obs) ** 2).sum())
+
+    m = [P[k].astype(np.float64) for k in ("vp", "vs", "rho")]
+    grads = [vp.grad.numpy(), vs.grad.numpy(), rho.grad.numpy()]
+    rng = np.random.default_rng(3)
+    for which in range(3):
+        d = rng.standard_normal(m[which].shape) * m[which]
+        eps = 1e-6
+        mp = [a.copy() for a in m]
+        mn = [a.copy() for a in m]
+        mp[which] += eps * d
+        mn[which] -= eps * d
+        fd = (J64(*mp) - J64(*mn)) / (2 * eps)
+        an = float((grads[which] * d).sum())
+        say("autograd", which, fd, an)
+        assert abs(fd - an) <= 2e-3 * max(abs(fd), abs(an))
+
+
 def test_adjoint_dot_product():
     # P: initial state -> receiver data (linear).  With obs = 0 the adjoint
     # state at t = 0 is P^T P x0, so <x0, P^T P x0> = |P x0|^2 = 2 J.

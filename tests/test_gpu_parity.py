@@ -148,21 +148,22 @@ HEAT = [
 
 
 def _heat_case(env, blocks, R, D, periodic, nsteps, mode, k=(0.021, 0.033, 0.017),
-               adt=0.1 * 0.1, h2=(1.0, 1.0, 0.25)):
+               adt=0.1 * 0.1, h2=(1.0, 1.0, 0.25), fused=0):
     p = A.heat_params(D, mode, k, adt, h2)
     with A.Context(env, blocks, R, D, int(periodic), 4) as c:
         a, b = c.field(), c.field()
         c.init_uniform(a, 17)
-        res, _ = c.heat_run(a, b, p, nsteps)
+        res, _ = c.heat_run(a, b, p, nsteps, fused=fused)
         return c.download(res)
 
 
+@pytest.mark.parametrize("fused", [0, 1])
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("env,blocks,R,D,periodic,nsteps", HEAT)
-def test_heat_bitwise_vs_oracle(env, blocks, R, D, periodic, nsteps, mode):
-    say("heat", env, blocks, "R", R, "D", D, "periodic", periodic, "mode", mode)
+def test_heat_bitwise_vs_oracle(env, blocks, R, D, periodic, nsteps, mode, fused):
+    say("heat", env, blocks, "R", R, "D", D, "periodic", periodic, "mode", mode, "fused", fused)
     k = (0.021, 0.033, 0.017)
-    got = _heat_case(env, blocks, R, D, periodic, nsteps, mode, k)
+    got = _heat_case(env, blocks, R, D, periodic, nsteps, mode, k, fused=fused)
     owned = O.uniform((env[2], env[1], env[0]), 17)
     g = owned if periodic else O.frame(owned, D)
     want = O.heat_f32(g, D, mode, k, 0.1 * 0.1, (1.0, 1.0, 0.25), nsteps, periodic)
@@ -194,9 +195,10 @@ def test_heat_ref_mode_matches_reference_golden(fname):
 def test_heat_R_and_decomposition_invariance():
     env = (96, 64, 80)
     outs = []
-    for blocks, R in [((1, 1, 1), 1), ((2, 2, 2), 1), ((2, 1, 2), 3), ((1, 2, 1), 4)]:
-        say("heat invariance", blocks, R)
-        outs.append(_heat_case(env, blocks, R, 2, False, 12, 0))
+    for blocks, R, fused in [((1, 1, 1), 1, 0), ((2, 2, 2), 1, 1), ((2, 1, 2), 3, 0),
+                             ((1, 2, 1), 4, 0), ((2, 2, 1), 2, 1)]:
+        say("heat invariance", blocks, R, fused)
+        outs.append(_heat_case(env, blocks, R, 2, False, 12, 0, fused=fused))
     for o in outs[1:]:
         assert bits_equal(o, outs[0])
 
