@@ -13,7 +13,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libb2r_hip.so")
+LIB_PATH = os.environ.get("B2R_HIP_LIB") or os.path.join(_HERE, "libb2r_hip.so")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(

@@ -135,6 +135,18 @@ __host__ __device__ inline int neighbour_block(const KGeo& g, int b, int dim, in
     return (c[2] * g.nb[1] + c[1]) * g.nb[0] + c[0];
 }
 
+// XCD-aware workgroup order.  The dispatcher deals workgroups round-robin over
+// the 8 XCDs of an MI355X (linear id mod 8), each with its own L2.  Mapping the
+// linear id L (of N) to t so that XCD k receives a contiguous range of t keeps
+// neighbouring tiles, which share halo rows and planes, in the same L2.
+// A bijection for any N.
+__device__ __forceinline__ int xcd_tile(int L, int N)
+{
+    const int q = N >> 3, r = N & 7;
+    const int k = L & 7, i = L >> 3;
+    return k * q + (k < r ? k : r) + i;
+}
+
 // Element offset of local cell (x,y,z) of block b.
 __host__ __device__ inline long long cell_index(const KGeo& g, int b, int x, int y, int z)
 {

@@ -45,16 +45,17 @@ template <int RY>
 __global__ __launch_bounds__(256) void gol_step_kernel(const uint8_t* __restrict__ in,
                                                        uint8_t* __restrict__ out, KGeo g,
                                                        int b2r_i, int jd0, int ntx, int nty,
-                                                       int zc)
+                                                       int nzc, int zc)
 {
     constexpr int TY = GOL_WY * RY;
     const int lane = threadIdx.x, wy = threadIdx.y;
-    int t = blockIdx.x;
+    int t = xcd_tile(blockIdx.x, gridDim.x);
     const int txi = t % ntx;
     t /= ntx;
     const int tyi = t % nty;
-    const int zci = t / nty;
-    const int b = blockIdx.y;
+    t /= nty;
+    const int zci = t % nzc;
+    const int b = t / nzc;
 
     const Box box = update_box(g, b, 1, b2r_i);
     const int jd = jd0 + txi * GOL_TXD + lane;
@@ -176,7 +177,7 @@ __device__ __forceinline__ uint32_t dword_mask(int x, int lo, int hi)
 template <int R>
 __global__ __launch_bounds__(GF_THREADS) void gol_fused_kernel(const uint8_t* __restrict__ in,
                                                         uint8_t* __restrict__ out, KGeo g,
-                                                        int ntx, int nty, int zc)
+                                                        int ntx, int nty, int nzc, int zc)
 {
     constexpr int TY = GF_NR - 2 * R;
     extern __shared__ uint4 smem[];
@@ -184,12 +185,13 @@ __global__ __launch_bounds__(GF_THREADS) void gol_fused_kernel(const uint8_t* __
     uint4* pbuf = smem + R * 3 * GF_PLANE;
 
     const int tid = threadIdx.x;
-    int tt = blockIdx.x;
+    int tt = xcd_tile(blockIdx.x, gridDim.x);
     const int txi = tt % ntx;
     tt /= ntx;
     const int tyi = tt % nty;
-    const int zci = tt / nty;
-    const int b = blockIdx.y;
+    tt /= nty;
+    const int zci = tt % nzc;
+    const int b = tt / nzc;
     const int h = g.h;
     const int x0 = h + txi * 16 * GF_TXQ;  // first output cell
     const int y0 = h + tyi * TY;
@@ -347,9 +349,9 @@ static int gol_fused_launch_t(b2r_ctx* ctx, const void* in, void* out)
     }
     if (zc > g.B[2]) zc = g.B[2];
     const int nzc = ceil_div(g.B[2], zc);
-    dim3 grid(ntx * nty * nzc, g.nblk);
-    gol_fused_kernel<R><<<grid, GF_THREADS, lds, ctx->stream>>>((const uint8_t*)in, (uint8_t*)out, g,
-                                                        ntx, nty, zc);
+    dim3 grid(ntx * nty * nzc * g.nblk);
+    gol_fused_kernel<R><<<grid, GF_THREADS, lds, ctx->stream>>>((const uint8_t*)in, (uint8_t*)out,
+                                                               g, ntx, nty, nzc, zc);
     B2R_LAUNCH_CHECK();
     return B2R_OK;
 }
@@ -395,10 +397,10 @@ int b2r_gol_launch(b2r_ctx* ctx, const void* in, void* out, int i)
     }
     if (zc > nz) zc = nz;
     const int nzc = ceil_div(nz, zc);
-    dim3 grid(ntx * nty * nzc, g.nblk);
+    dim3 grid(ntx * nty * nzc * g.nblk);
     dim3 block(GOL_TXD, GOL_WY);
     gol_step_kernel<RY><<<grid, block, 0, ctx->stream>>>((const uint8_t*)in, (uint8_t*)out, g, i,
-                                                         jd0, ntx, nty, zc);
+                                                         jd0, ntx, nty, nzc, zc);
     B2R_LAUNCH_CHECK();
     return B2R_OK;
 }

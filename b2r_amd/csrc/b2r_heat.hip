@@ -69,7 +69,7 @@ template <int D, int MODE, int RY>
 __global__ __launch_bounds__(256) void heat_step_kernel(const float* __restrict__ in,
                                                         float* __restrict__ out, KGeo g,
                                                         HeatK hk, int b2r_i, int ntx, int nty,
-                                                        int zc)
+                                                        int nzc, int zc)
 {
     constexpr int TX = HEAT_TX, TY = HEAT_WY * RY;
     constexpr int LX = TX + 2 * D, LY = TY + 2 * D;
@@ -79,12 +79,13 @@ __global__ __launch_bounds__(256) void heat_step_kernel(const float* __restrict_
     __shared__ float lds[LY * LX];
 
     const int lane = threadIdx.x, wy = threadIdx.y, tid = wy * TX + lane;
-    int t = blockIdx.x;
+    int t = xcd_tile(blockIdx.x, gridDim.x);
     const int txi = t % ntx;
     t /= ntx;
     const int tyi = t % nty;
-    const int zci = t / nty;
-    const int b = blockIdx.y;
+    t /= nty;
+    const int zci = t % nzc;
+    const int b = t / nzc;
 
     const Box box = update_box(g, b, D, b2r_i);
     const int x0 = D + txi * TX, y0 = D + tyi * TY;
@@ -223,9 +224,10 @@ static void launch_heat_ry(b2r_ctx* ctx, const float* in, float* out, const Heat
     }
     if (zc > nz) zc = nz;
     const int nzc = ceil_div(nz, zc);
-    dim3 grid(ntx * nty * nzc, g.nblk);
+    dim3 grid(ntx * nty * nzc * g.nblk);
     dim3 block(HEAT_TX, HEAT_WY);
-    heat_step_kernel<D, MODE, RY><<<grid, block, 0, ctx->stream>>>(in, out, g, hk, i, ntx, nty, zc);
+    heat_step_kernel<D, MODE, RY><<<grid, block, 0, ctx->stream>>>(in, out, g, hk, i, ntx, nty,
+                                                                   nzc, zc);
 }
 
 template <int D, int MODE>
@@ -335,7 +337,8 @@ constexpr int HF_TYT = HEAT_WY * HF_RY;  // thread-tile rows
 template <int D, int MODE, int R>
 __global__ __launch_bounds__(256) void heat_fused_kernel(const float* __restrict__ in,
                                                          float* __restrict__ out, KGeo g,
-                                                         HeatK hk, int ntx, int nty, int zc)
+                                                         HeatK hk, int ntx, int nty, int nzc,
+                                                         int zc)
 {
     constexpr int RD = R * D;
     constexpr int TXO = HEAT_TX - 2 * RD, TYO = HF_TYT - 2 * RD;
@@ -344,12 +347,13 @@ __global__ __launch_bounds__(256) void heat_fused_kernel(const float* __restrict
     __shared__ float lds[LY * LX];
 
     const int lane = threadIdx.x, wy = threadIdx.y;
-    int tt = blockIdx.x;
+    int tt = xcd_tile(blockIdx.x, gridDim.x);
     const int txi = tt % ntx;
     tt /= ntx;
     const int tyi = tt % nty;
-    const int zci = tt / nty;
-    const int b = blockIdx.y;
+    tt /= nty;
+    const int zci = tt % nzc;
+    const int b = tt / nzc;
     const int h = g.h;
     const int xo = h + txi * TXO;  // first output column
     const int yo = h + tyi * TYO;
@@ -463,9 +467,10 @@ static int heat_fused_t(b2r_ctx* ctx, const float* in, float* out, const HeatK& 
     }
     if (zc > g.B[2]) zc = g.B[2];
     const int nzc = ceil_div(g.B[2], zc);
-    dim3 grid(ntx * nty * nzc, g.nblk);
+    dim3 grid(ntx * nty * nzc * g.nblk);
     dim3 block(HEAT_TX, HEAT_WY);
-    heat_fused_kernel<D, MODE, R><<<grid, block, 0, ctx->stream>>>(in, out, g, hk, ntx, nty, zc);
+    heat_fused_kernel<D, MODE, R><<<grid, block, 0, ctx->stream>>>(in, out, g, hk, ntx, nty, nzc,
+                                                                   zc);
     B2R_LAUNCH_CHECK();
     return B2R_OK;
 }

@@ -2,19 +2,22 @@
 
 Headline (value): 3D Game of Life, 512^3 u8 cells Bernoulli(0.3) seed 1,
 periodic, 2x2x2 subdomain blocks with R-deep halos exchanged on the device every
-R generations (R in {1,2,4}, best reported), --steps generations timed.  A step
-is one generation of the whole 512^3 grid; GLUP/s = cells * steps / time.
+R generations, R in {1,2,4} (best reported), the R generations of a window in one
+fused launch; --steps generations timed.  A step is one generation of the whole
+512^3 grid; GLUP/s = cells * steps / time.
 
-Also reported (same JSON line, "also"): measured STREAM copy bandwidth; heat
-delta1 at 768^3 (FP32, % of measured STREAM); heat delta4 768^3 with 8 Gaussian
-blobs on 2x2x2 blocks; heat delta1 128^3 (the reference CPU case) next to the
-reference's own CPU code; CPU baselines on the host cores.
+Also reported in the same JSON line ("also"): measured STREAM copy bandwidth;
+heat delta1 at 768^3 (single pass as % of measured STREAM, and the fused R=2
+window); heat delta4 768^3 with 8 Gaussian blobs on 2x2x2 blocks; heat delta1
+128^3 next to the reference's own CPU code; the 16-shot elastic batch (plain
+per-shot loop vs batched graph); one elastic forward+adjoint gradient; CPU
+baselines on the host cores.  Launch shapes are autotuned per size and the
+choices are appended to --tune-record.
 
 Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--quick]
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -28,13 +31,6 @@ HBM_PEAK_GBS = 8000.0
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def host_cores():
-    try:
-        return len(os.sched_getaffinity(0))
-    except Exception:
-        return os.cpu_count() or 1
 
 
 def read_traffic():
@@ -51,14 +47,20 @@ def read_traffic():
     return d
 
 
-def gol_bench(A, R, steps, warmup, fused=1, zc=0):
+# ------------------------------------------------------------------ Game of Life
+def gol_bench(A, R, steps, warmup, fused=1, zc=0, record=None):
     """steps generations of 512^3; kernel_ms = one launch of the dominant kernel
     (the fused R-generation kernel, or the single-generation kernel)."""
     env = (512, 512, 512)
     with A.Context(env, (2, 2, 2), R, 1, A.BOUNDARY_PERIODIC, 1) as c:
+        a, b = c.field(), c.field()
+        c.init_bernoulli(a, 1, int(0.3 * 16777216))
+        c.boundary_updating_direct([a])
+        kind = 2 if fused else 1
         if zc:
             c.set_tuning(gol_zc=zc)
-        a, b = c.field(), c.field()
+        elif record is not None:
+            c.autotune(kind, a, b, None, record)
         c.init_bernoulli(a, 1, int(0.3 * 16777216))
         if warmup:
             c.gol_run(a, b, warmup, fused=fused)
@@ -68,17 +70,18 @@ def gol_bench(A, R, steps, warmup, fused=1, zc=0):
         _, ms = c.gol_run(a, b, steps, fused=fused, timed=True)
         c.sync()
         wall = time.perf_counter() - t0
-        # the kernel alone, HIP events on the context stream
         kiters = 50 if fused else 200
-        kms = c.time_steps(2 if fused else 1, a, b, kiters) / kiters
+        kms = c.time_steps(kind, a, b, kiters) / kiters
+        tune = c.get_tuning()
     cells = 512 ** 3
-    gens_per_launch = R if fused else 1
-    return dict(R=R, fused=fused, ms=ms, wall_s=wall,
-                glups=cells * steps / (ms * 1e-3) / 1e9, kernel_ms=kms,
-                gens_per_launch=gens_per_launch,
-                kernel_gbs_alg=2.0 * cells * gens_per_launch / (kms * 1e-3) / 1e9)
+    gens = R if fused else 1
+    return dict(R=R, fused=fused, ms=ms, wall_s=wall, glups=cells * steps / (ms * 1e-3) / 1e9,
+                kernel_ms=kms, gens_per_launch=gens,
+                kernel_gbs_alg=2.0 * cells * gens / (kms * 1e-3) / 1e9,
+                zc=tune["gol_zc"] if fused else tune["gol_step_zc"])
 
 
+# ------------------------------------------------------------------ heat
 def stream_bw(A):
     nbytes = 4 << 30
     with A.Context((16, 16, 16), (1, 1, 1), 1, 1, 0, 4) as c:
@@ -90,20 +93,28 @@ def stream_bw(A):
     return 2.0 * nbytes * iters / (ms * 1e-3) / 1e9
 
 
-def heat_bench(A, env, blocks, R, D, k, steps, init, warmup=3):
+def heat_bench(A, env, blocks, R, D, k, steps, init, fused=0, record=None, warmup=2):
     p = A.heat_params(D, A.HEAT_F32, k)
     with A.Context(env, blocks, R, D, A.BOUNDARY_FROZEN, 4) as c:
         a, b = c.field(), c.field()
         init(c, a)
-        c.heat_run(a, b, p, warmup)
+        c.heat_run(a, b, p, warmup * R, fused=fused)
+        kind = 3 if fused else 0
+        if record is not None:
+            c.autotune(kind, a, b, p, record)
         init(c, a)
         c.sync()
-        _, ms = c.heat_run(a, b, p, steps, timed=True)
-        kiters = 20
-        kms = c.time_steps(0, a, b, kiters, p) / kiters
+        _, ms = c.heat_run(a, b, p, steps, fused=fused, timed=True)
+        kiters = 10
+        kms = c.time_steps(kind, a, b, kiters, p) / kiters
+        tune = c.get_tuning()
     cells = env[0] * env[1] * env[2]
-    return dict(ms=ms, glups=cells * steps / (ms * 1e-3) / 1e9, kernel_ms=kms,
-                kernel_gbs=8.0 * cells / (kms * 1e-3) / 1e9)
+    per_launch = R if fused else 1
+    return dict(R=R, fused=fused, steps=steps, ms=round(ms, 3),
+                glups=round(cells * steps / (ms * 1e-3) / 1e9, 2),
+                kernel_ms=round(kms, 4), steps_per_launch=per_launch,
+                kernel_gbs_alg=round(8.0 * cells * per_launch / (kms * 1e-3) / 1e9, 1),
+                tuning={"heat_ry": tune["heat_ry"], "heat_zc": tune["heat_zc"]})
 
 
 def blob_tables(env, nb=8, sigma=20.0, seed=2):
@@ -118,6 +129,83 @@ def blob_tables(env, nb=8, sigma=20.0, seed=2):
     return amp, tabs
 
 
+# ------------------------------------------------------------------ elastic
+def elastic_shots(A, W, sizes, nsteps, seed=4):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ctxs, shots = [], []
+    for i, env in enumerate(sizes):
+        vp, vs, rho = W.layered_medium(env, seed + i)
+        dt = W.cfl_dt(4500.0, 10.0, 0.4)
+        c = A.Context(env, (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4)
+        s = A.ElasticShot(c, dt, (0.1, 0.1, 0.1))
+        s.set_medium(*W.lame(vp, vs, rho))
+        src = [[int(rng.integers(env[0] // 4, 3 * env[0] // 4)), int(rng.integers(8, env[1] // 4)),
+                int(rng.integers(env[2] // 4, 3 * env[2] // 4))]]
+        s.set_sources(src, [1], W.ricker(10.0, dt, nsteps)[:, None] * np.float32(1e6))
+        s.set_receivers(W.surface_receivers(env), 1, nsteps)
+        ctxs.append(c)
+        shots.append(s)
+    return ctxs, shots
+
+
+def elastic_batch_bench(A, W, nshots=16, nsteps=100):
+    sizes = W.shot_sizes(nshots, seed=4)
+    cells = sum(e[0] * e[1] * e[2] for e in sizes)
+    ctxs, shots = elastic_shots(A, W, sizes, nsteps)
+    try:
+        out = {"shots": nshots, "steps": nsteps, "cells_per_step": cells,
+               "sizes": ["%dx%dx%d" % e for e in sizes]}
+        for s in shots:
+            s.reset()
+        A.el_forward_batch(shots, 2, mode=2)
+        for s in shots:
+            s.reset()
+        ms_b = A.el_forward_batch(shots, nsteps, mode=2, timed=True)
+        t0 = time.perf_counter()
+        ms_p = 0.0
+        for s in shots:  # the plain per-shot loop in the reference's structure
+            s.reset()
+            ms_p += s.forward(nsteps, 0, timed=True)
+        wall_p = time.perf_counter() - t0
+        out.update(batched_graph_ms=round(ms_b, 2), plain_loop_ms=round(ms_p, 2),
+                   plain_loop_wall_s=round(wall_p, 2),
+                   batched_glups=round(cells * nsteps / (ms_b * 1e-3) / 1e9, 2),
+                   plain_glups=round(cells * nsteps / (ms_p * 1e-3) / 1e9, 2),
+                   speedup_batched_vs_plain=round(ms_p / ms_b, 3),
+                   batched_gbs_84=round(84.0 * cells * nsteps / (ms_b * 1e-3) / 1e9, 1))
+        return out
+    finally:
+        for s in shots:
+            s.close()
+        for c in ctxs:
+            c.close()
+
+
+def elastic_gradient_bench(A, W, nsteps=200, ckpt=50):
+    import numpy as np
+    env = (256, 256, 256)
+    ctxs, shots = elastic_shots(A, W, [env], nsteps, seed=9)
+    try:
+        s = shots[0]
+        s.forward(nsteps, 2)
+        obs = s.receivers() * np.float32(0.95)
+        t0 = time.perf_counter()
+        J, g = s.gradient(obs, nsteps, ckpt)
+        wall = time.perf_counter() - t0
+        cells = env[0] * env[1] * env[2]
+        return {"grid": "256^3", "steps": nsteps, "ckpt": ckpt, "misfit": J,
+                "wall_s": round(wall, 3),
+                "shot_glups_gradient": round(cells * nsteps / wall / 1e9, 3),
+                "grad_finite": bool(all(np.isfinite(a).all() for a in g))}
+    finally:
+        for s in shots:
+            s.close()
+        for c in ctxs:
+            c.close()
+
+
+# ------------------------------------------------------------------ CPU baselines
 def cpu_gol_baseline(budget_s=10.0):
     """The oracle's Game of Life (C, OpenMP over z) on the host cores: 512^3
     periodic, as many generations as fit the time budget."""
@@ -132,9 +220,10 @@ def cpu_gol_baseline(budget_s=10.0):
         el = time.perf_counter() - t0
         if el >= budget_s or gens >= 1000:
             break
-    return dict(value=512 ** 3 * gens / el / 1e9, unit="GLUP/s", cores=O.lib.orc_num_threads(),
-                kind="port", sample=f"512^3 periodic Game of Life, {gens} generations "
-                                    f"({el:.1f} s), oracle C code, OpenMP")
+    return dict(value=round(512 ** 3 * gens / el / 1e9, 4), unit="GLUP/s",
+                cores=O.lib.orc_num_threads(), kind="port",
+                sample=f"512^3 periodic Game of Life, {gens} generations ({el:.1f} s), "
+                       "oracle C code, OpenMP over z")
 
 
 def cpu_heat_reference(steps=100):
@@ -142,7 +231,6 @@ def cpu_heat_reference(steps=100):
     its sources into oracle/_ref) on 128^3 + zero frame, 100 steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
-    import numpy as np
     if not O.ref_available():
         return None
     u = O.frame(O.uniform((128, 128, 128), 0), 1)
@@ -154,8 +242,30 @@ def cpu_heat_reference(steps=100):
         r.ref_cpu_fdm_heat_diffuse1(pu, pv, 130, 130, 130, 0)
         pu, pv = pv, pu
     el = time.perf_counter() - t0
-    return dict(value=128 ** 3 * steps / el / 1e9, unit="GLUP/s", cores=O.lib.orc_num_threads(),
-                kind="reference", sample=f"128^3 heat delta1, {steps} steps ({el:.2f} s)")
+    return dict(value=round(128 ** 3 * steps / el / 1e9, 4), unit="GLUP/s",
+                cores=O.lib.orc_num_threads(), kind="reference",
+                sample=f"128^3 heat delta1, {steps} steps ({el:.2f} s)")
+
+
+def cpu_elastic_baseline(steps=10):
+    """The oracle's FP32 elastic step (C, OpenMP over z) on a 128^3 layered
+    medium, a bounded sample of the per-shot work."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import _elastic as E
+    from b2r_amd import workloads as W
+    env = (128, 128, 128)
+    vp, vs, rho = W.layered_medium(env, 0)
+    lam, mu, b = W.lame(vp, vs, rho)
+    dt = np.float32(W.cfl_dt(4500.0, 10.0))
+    t0 = time.perf_counter()
+    E.forward(env, lam, mu, b, dt, (np.float32(0.1),) * 3, steps, [[64, 20, 64]], [1],
+              np.ones((steps, 1), np.float32))
+    el = time.perf_counter() - t0
+    import _oracle as O
+    return dict(value=round(128 ** 3 * steps / el / 1e9, 5), unit="GLUP/s",
+                cores=O.lib.orc_num_threads(), kind="port",
+                sample=f"128^3 elastic FP32, {steps} steps ({el:.2f} s), oracle C code, OpenMP")
 
 
 def profile_kernels(A, R=2):
@@ -173,12 +283,14 @@ def profile_kernels(A, R=2):
         a, b = c.field(), c.field()
         c.init_bernoulli(a, 1, int(0.3 * 16777216))
         c.boundary_updating_direct([a])
+        c.set_tuning(gol_zc=16)
         log(f"gol_fused_kernel<{R}> x5")
         c.time_steps(2, a, b, 5)
     prm = A.heat_params(1, A.HEAT_F32, (0.1, 0.1, 0.1))
     with A.Context((768, 768, 768), (1, 1, 1), 1, 1, 0, 4) as c:
         a, b = c.field(), c.field()
         c.init_uniform(a, 0)
+        c.set_tuning(heat_ry=4, heat_zc=8)
         log("heat_step_kernel<1> x5")
         c.time_steps(0, a, b, 5, prm)
     from b2r_amd import workloads as W
@@ -195,14 +307,16 @@ def profile_kernels(A, R=2):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--profile-kernels", action="store_true",
-                    help="run the fixed kernel sequence used by the rocprofv3 counter runs")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="headline only, no extras")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
-    ap.add_argument("--gol-zc", type=int, default=0, help="z chunk of the fused kernel (0: auto)")
+    ap.add_argument("--elastic-steps", type=int, default=100)
+    ap.add_argument("--tune-record", default="",
+                    help="append the autotuner's choices to this file")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="run the fixed kernel sequence used by the rocprofv3 counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the halo "
@@ -210,15 +324,17 @@ def main():
         sys.exit(2)
 
     from b2r_amd import _abi as A  # raises if the HIP library is missing
+    from b2r_amd import workloads as W
 
     if args.profile_kernels:
         profile_kernels(A)
         return
+    rec = args.tune_record or None
 
     runs = []
     for R in (1, 2, 4):
         log(f"gol 512^3 R={R} fused")
-        runs.append(gol_bench(A, R, args.steps, args.warmup, fused=1, zc=args.gol_zc))
+        runs.append(gol_bench(A, R, args.steps, args.warmup, fused=1, record=rec))
         log(json.dumps(runs[-1]))
     best = max(runs, key=lambda r: r["glups"])
     cells = 512 ** 3
@@ -228,27 +344,34 @@ def main():
     log(json.dumps(plain))
     tr = read_traffic()
     traffic = None
-    if tr and "gol_step_bytes_per_launch" in tr:
-        traffic = float(tr["gol_step_bytes_per_launch"])
+    if tr and f"gol_fused_R{best['R']}_bytes_per_launch" in tr:
+        traffic = float(tr[f"gol_fused_R{best['R']}_bytes_per_launch"])
 
     also = {}
     if not args.quick:
         log("stream")
         bw = stream_bw(A)
         also["stream_copy_GBs"] = round(bw, 1)
-        log("heat delta1 768^3")
-        h1 = heat_bench(A, (768, 768, 768), (1, 1, 1), 1, 1, (0.1, 0.1, 0.1), 100,
-                        lambda c, f: c.init_uniform(f, 0))
-        h1["pct_stream_kernel"] = round(100 * h1["kernel_gbs"] / bw, 1)
-        h1["pct_stream_run"] = round(100 * h1["glups"] * 8 / bw, 1)
-        also["heat_delta1_768"] = h1
+        env = (768, 768, 768)
+        log("heat delta1 768^3 single pass")
+        h1 = heat_bench(A, env, (1, 1, 1), 1, 1, (0.1, 0.1, 0.1), 100,
+                        lambda c, f: c.init_uniform(f, 0), fused=0, record=rec)
+        h1["kernel_pct_of_stream"] = round(100 * h1["kernel_gbs_alg"] / bw, 1)
+        h1["run_pct_of_stream"] = round(100 * h1["glups"] * 8 / bw, 1)
+        also["heat_delta1_768_single_pass"] = h1
         log(json.dumps(h1))
-        log("heat delta4 768^3 blobs 2x2x2")
-        amp, (ex, ey, ez) = blob_tables((768, 768, 768))
-        h4 = heat_bench(A, (768, 768, 768), (2, 2, 2), 1, 4, (0.05, 0.05, 0.05), 100,
-                        lambda c, f: c.init_blobs(f, amp, ex, ey, ez))
-        h4["pct_stream_run"] = round(100 * h4["glups"] * 8 / bw, 1)
-        also["heat_delta4_768_2x2x2"] = h4
+        log("heat delta1 768^3 fused R=2")
+        h1f = heat_bench(A, env, (1, 1, 1), 2, 1, (0.1, 0.1, 0.1), 100,
+                         lambda c, f: c.init_uniform(f, 0), fused=1, record=rec)
+        h1f["effective_pct_of_stream"] = round(100 * h1f["glups"] * 8 / bw, 1)
+        also["heat_delta1_768_fused_R2"] = h1f
+        log(json.dumps(h1f))
+        log("heat delta4 768^3 blobs 2x2x2, 500 steps")
+        amp, (ex, ey, ez) = blob_tables(env)
+        h4 = heat_bench(A, env, (2, 2, 2), 1, 4, (0.05, 0.05, 0.05), 500,
+                        lambda c, f: c.init_blobs(f, amp, ex, ey, ez), fused=0, record=rec)
+        h4["run_pct_of_stream"] = round(100 * h4["glups"] * 8 / bw, 1)
+        also["heat_delta4_768_2x2x2_R1"] = h4
         log(json.dumps(h4))
         h0 = heat_bench(A, (128, 128, 128), (1, 1, 1), 1, 1, (0.1, 0.1, 0.1), 100,
                         lambda c, f: c.init_uniform(f, 0))
@@ -256,6 +379,13 @@ def main():
         ref = cpu_heat_reference()
         if ref:
             also["heat_delta1_128_cpu_reference"] = ref
+        log("elastic 16-shot batch")
+        also["elastic_16_shots"] = elastic_batch_bench(A, W, 16, args.elastic_steps)
+        log(json.dumps(also["elastic_16_shots"]))
+        log("elastic gradient")
+        also["elastic_gradient_256"] = elastic_gradient_bench(A, W)
+        log(json.dumps(also["elastic_gradient_256"]))
+        also["elastic_cpu_port"] = cpu_elastic_baseline()
 
     log("cpu baseline")
     cpu = cpu_gol_baseline(args.cpu_budget)
@@ -274,6 +404,7 @@ def main():
         "data": "synthetic (SplitMix64 Bernoulli(0.3), seed 1)",
         "config": {"workload": "gol3d_512^3_u8_periodic_2x2x2", "R": best["R"],
                    "generations": args.steps, "fused_R_generations_per_launch": True,
+                   "gol_zc": best["zc"],
                    "glups_by_R": {str(r["R"]): round(r["glups"], 3) for r in runs},
                    "plain_step_by_step_R1_glups": round(plain["glups"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
