@@ -9,8 +9,8 @@
 //
 // B2R geometry: one step is two dependent radius-2 stages, so a block carries
 // a halo of 4R (radius 4 in the context geometry) and stage s of an R-step
-// window updates [2(s+1), S-1-2(s+1)] (the reference's halo of R*D with D=2 is
-// too shallow, SURVEY defect D6).  Updates are in place: the stress stage reads
+// window updates [2(s+1), S-1-2(s+1)] (the reference sizes the halo R*D with
+// D=2, b2r.cpp:61-64, which is too shallow for two radius-2 stages per step).  Updates are in place: the stress stage reads
 // v and writes sigma, the velocity stage reads sigma and writes v.
 //
 // Two forward paths run the same kernels (identical results):
@@ -44,7 +44,6 @@ struct ElDev {
     int ntx, nty, nzc, zc;
     int tile0;        // first workgroup of this shot in a batched launch
     int ntiles;
-    int nty16, ntiles16, tile016;  // the same for 64 x 16 tiles (2.5D kernels)
 };
 
 struct b2r_el_shot {
@@ -75,7 +74,7 @@ struct Tile {
 
 __device__ __forceinline__ Tile decode_tile(const ElDev* __restrict__ tab, int nshot)
 {
-    const int gt = blockIdx.x;
+    const int gt = xcd_tile(blockIdx.x, gridDim.x);
     int s = 0;
     while (s + 1 < nshot && tab[s + 1].tile0 <= gt) ++s;
     const ElDev& d = tab[s];
@@ -184,289 +183,6 @@ __global__ __launch_bounds__(256) void el_velocity_kernel(const ElDev* __restric
             const float az = (fd4(sxz, c, 1, i0) + fd4(syz, c, sy, i1)) + fd4(szz, c, sz, i2);
             vz[c] = vz[c] + tb * az;
         }
-    }
-}
-
-// ---------------------------------------------------------------- 2.5D kernels
-// 64 x 16 tiles: 64 x-lanes x 4 waves, 4 rows per thread.  Fields whose z
-// derivative is needed sit in 5-plane register queues; fields whose x or y
-// derivative is needed are staged per plane in LDS with a 2-cell halo.  The
-// expressions are those of fd4 above, so results are bit-identical.
-constexpr int EL_WY = 4, EL_RY = 4, EL_TY16 = EL_WY * EL_RY;
-constexpr int EL_LX = EL_TX + 4, EL_LY = EL_TY16 + 4, EL_PL = EL_LX * EL_LY;
-constexpr int EL_NH = 4 * EL_TX + 4 * EL_TY16;       // halo cells per field plane
-constexpr int EL_NHS = (EL_NH + 255) / 256;            // halo slots per thread
-
-struct Tile16 {
-    int shot, b, xt, yt, zs, ze;
-};
-
-__device__ __forceinline__ Tile16 decode_tile16(const ElDev* __restrict__ tab, int nshot)
-{
-    const int gt = xcd_tile(blockIdx.x, gridDim.x);
-    int s = 0;
-    while (s + 1 < nshot && tab[s + 1].tile016 <= gt) ++s;
-    const ElDev& d = tab[s];
-    int lt = gt - d.tile016;
-    Tile16 T;
-    T.shot = s;
-    const int txi = lt % d.ntx;
-    lt /= d.ntx;
-    const int tyi = lt % d.nty16;
-    lt /= d.nty16;
-    const int zci = lt % d.nzc;
-    T.b = lt / d.nzc;
-    T.xt = 2 + txi * EL_TX;
-    T.yt = 2 + tyi * EL_TY16;
-    T.zs = 2 + zci * d.zc;
-    T.ze = min(T.zs + d.zc, d.g.S[2] - 2);
-    return T;
-}
-
-__device__ __forceinline__ float dq(const float (&q)[5], float ih)
-{
-    return (EL_C2 * (q[4] - q[0]) + EL_C1 * (q[3] - q[1])) * ih;
-}
-
-__device__ __forceinline__ float dl(const float* r, int s, float ih)
-{
-    return (EL_C2 * (r[2 * s] - r[-2 * s]) + EL_C1 * (r[s] - r[-s])) * ih;
-}
-
-// halo slot e -> (LDS index, row offset in the plane); ok = inside the block
-__device__ __forceinline__ void el_halo_slot(int e, int xt, int yt, const KGeo& g, int& li,
-                                             long long& go, bool& ok)
-{
-    ok = false;
-    li = 0;
-    go = 0;
-    if (e >= EL_NH) return;
-    int lx, ly;
-    if (e < 4 * EL_TX) {
-        const int r = e / EL_TX, c = e % EL_TX;
-        ly = r < 2 ? r : EL_TY16 + r;
-        lx = c + 2;
-    } else {
-        const int e2 = e - 4 * EL_TX;
-        const int r = e2 % EL_TY16, c = e2 / EL_TY16;
-        ly = r + 2;
-        lx = c < 2 ? c : EL_TX + c;
-    }
-    const int gx = xt - 2 + lx, gy = yt - 2 + ly;
-    ok = gx < g.S[0] && gy < g.S[1];
-    li = ly * EL_LX + lx;
-    go = (long long)gy * g.pitch + gx;
-}
-
-__global__ __launch_bounds__(256) void el_stress2_kernel(const ElDev* __restrict__ tab, int nshot,
-                                                         int stage)
-{
-    __shared__ float sv[3 * EL_PL];
-    const Tile16 T = decode_tile16(tab, nshot);
-    const ElDev& d = tab[T.shot];
-    const KGeo& g = d.g;
-    const Box box = update_box(g, T.b, 2, stage);
-    if (T.xt > box.hi[0] || T.xt + EL_TX - 1 < box.lo[0] || T.yt > box.hi[1] ||
-        T.yt + EL_TY16 - 1 < box.lo[1])
-        return;  // uniform
-    const int zlo = max(T.zs, box.lo[2]), zhi = min(T.ze - 1, box.hi[2]);
-    if (zlo > zhi) return;
-    const int lane = threadIdx.x, wy = threadIdx.y, tid = wy * EL_TX + lane;
-    const long long boff = (long long)T.b * g.bstride + g.xoff;
-    const float* __restrict__ v[3] = {d.f[F_VX] + boff, d.f[F_VY] + boff, d.f[F_VZ] + boff};
-    const float* __restrict__ lam = d.f[F_LAM] + boff;
-    const float* __restrict__ mu = d.f[F_MU] + boff;
-    float* __restrict__ sxx = d.f[F_SXX] + boff;
-    float* __restrict__ syy = d.f[F_SYY] + boff;
-    float* __restrict__ szz = d.f[F_SZZ] + boff;
-    float* __restrict__ sxy = d.f[F_SXY] + boff;
-    float* __restrict__ sxz = d.f[F_SXZ] + boff;
-    float* __restrict__ syz = d.f[F_SYZ] + boff;
-    const long long sy = g.pitch, sz = g.slice;
-    const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
-    const int x = T.xt + lane;
-    const bool xin = x < g.S[0];
-    const bool xupd = x >= box.lo[0] && x <= box.hi[0];
-    int y[EL_RY];
-    bool yin[EL_RY], yupd[EL_RY];
-#pragma unroll
-    for (int j = 0; j < EL_RY; ++j) {
-        y[j] = T.yt + wy * EL_RY + j;
-        yin[j] = y[j] < g.S[1];
-        yupd[j] = y[j] >= box.lo[1] && y[j] <= box.hi[1];
-    }
-    int hl[EL_NHS];
-    long long ho[EL_NHS];
-    bool hok[EL_NHS];
-#pragma unroll
-    for (int s = 0; s < EL_NHS; ++s) el_halo_slot(tid + 256 * s, T.xt, T.yt, g, hl[s], ho[s], hok[s]);
-    float q[3][EL_RY][5];
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        const long long pz = (long long)(zlo - 2 + p) * sz;
-#pragma unroll
-        for (int f = 0; f < 3; ++f)
-#pragma unroll
-            for (int j = 0; j < EL_RY; ++j)
-                q[f][j][p] = (xin && yin[j]) ? v[f][pz + y[j] * sy + x] : 0.0f;
-    }
-    for (int z = zlo; z <= zhi; ++z) {
-        const long long pz = (long long)z * sz, pf = pz + 2 * sz;
-#pragma unroll
-        for (int f = 0; f < 3; ++f)
-#pragma unroll
-            for (int j = 0; j < EL_RY; ++j)
-                q[f][j][4] = (xin && yin[j]) ? v[f][pf + y[j] * sy + x] : 0.0f;
-        __syncthreads();
-#pragma unroll
-        for (int f = 0; f < 3; ++f) {
-#pragma unroll
-            for (int j = 0; j < EL_RY; ++j)
-                sv[f * EL_PL + (wy * EL_RY + j + 2) * EL_LX + lane + 2] = q[f][j][2];
-#pragma unroll
-            for (int s = 0; s < EL_NHS; ++s)
-                if (hok[s]) sv[f * EL_PL + hl[s]] = v[f][pz + ho[s]];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < EL_RY; ++j) {
-            if (!(xupd && yupd[j])) continue;
-            const int li = (wy * EL_RY + j + 2) * EL_LX + lane + 2;
-            const float* r0 = sv + li;
-            const float* r1 = sv + EL_PL + li;
-            const float* r2 = sv + 2 * EL_PL + li;
-            const long long c = pz + y[j] * sy + x;
-            const float exx = dl(r0, 1, i0);
-            const float eyy = dl(r1, EL_LX, i1);
-            const float ezz = dq(q[2][j], i2);
-            const float dxy = dl(r0, EL_LX, i1) + dl(r1, 1, i0);
-            const float dxz = dq(q[0][j], i2) + dl(r2, 1, i0);
-            const float dyz = dq(q[1][j], i2) + dl(r2, EL_LX, i1);
-            const float l = lam[c], m = mu[c];
-            const float l2m = l + 2.0f * m;
-            sxx[c] = sxx[c] + dt * (l2m * exx + l * (eyy + ezz));
-            syy[c] = syy[c] + dt * (l2m * eyy + l * (exx + ezz));
-            szz[c] = szz[c] + dt * (l2m * ezz + l * (exx + eyy));
-            sxy[c] = sxy[c] + dt * (m * dxy);
-            sxz[c] = sxz[c] + dt * (m * dxz);
-            syz[c] = syz[c] + dt * (m * dyz);
-        }
-#pragma unroll
-        for (int f = 0; f < 3; ++f)
-#pragma unroll
-            for (int j = 0; j < EL_RY; ++j)
-#pragma unroll
-                for (int p = 0; p < 4; ++p) q[f][j][p] = q[f][j][p + 1];
-    }
-}
-
-// all three velocity components; LDS planes: sxx sxy syy sxz syz, queues: sxz syz szz
-__global__ __launch_bounds__(256) void el_velocity2_kernel(const ElDev* __restrict__ tab,
-                                                           int nshot, int stage)
-{
-    __shared__ float sv[5 * EL_PL];
-    const Tile16 T = decode_tile16(tab, nshot);
-    const ElDev& d = tab[T.shot];
-    const KGeo& g = d.g;
-    const Box box = update_box(g, T.b, 2, stage);
-    if (T.xt > box.hi[0] || T.xt + EL_TX - 1 < box.lo[0] || T.yt > box.hi[1] ||
-        T.yt + EL_TY16 - 1 < box.lo[1])
-        return;  // uniform
-    const int zlo = max(T.zs, box.lo[2]), zhi = min(T.ze - 1, box.hi[2]);
-    if (zlo > zhi) return;
-    const int lane = threadIdx.x, wy = threadIdx.y, tid = wy * EL_TX + lane;
-    const long long boff = (long long)T.b * g.bstride + g.xoff;
-    // LDS fields 0..4: sxx sxy syy sxz syz; queue fields 0..2: sxz syz szz
-    const float* __restrict__ lf[5] = {d.f[F_SXX] + boff, d.f[F_SXY] + boff, d.f[F_SYY] + boff,
-                                       d.f[F_SXZ] + boff, d.f[F_SYZ] + boff};
-    const float* __restrict__ qf[3] = {d.f[F_SXZ] + boff, d.f[F_SYZ] + boff, d.f[F_SZZ] + boff};
-    const float* __restrict__ bb = d.f[F_B] + boff;
-    float* __restrict__ vx = d.f[F_VX] + boff;
-    float* __restrict__ vy = d.f[F_VY] + boff;
-    float* __restrict__ vz = d.f[F_VZ] + boff;
-    float* __restrict__ disp = d.disp ? d.disp + boff : nullptr;
-    const long long sy = g.pitch, sz = g.slice;
-    const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
-    const int x = T.xt + lane;
-    const bool xin = x < g.S[0];
-    const bool xupd = x >= box.lo[0] && x <= box.hi[0];
-    int y[EL_RY];
-    bool yin[EL_RY], yupd[EL_RY];
-#pragma unroll
-    for (int j = 0; j < EL_RY; ++j) {
-        y[j] = T.yt + wy * EL_RY + j;
-        yin[j] = y[j] < g.S[1];
-        yupd[j] = y[j] >= box.lo[1] && y[j] <= box.hi[1];
-    }
-    int hl[EL_NHS];
-    long long ho[EL_NHS];
-    bool hok[EL_NHS];
-#pragma unroll
-    for (int s = 0; s < EL_NHS; ++s) el_halo_slot(tid + 256 * s, T.xt, T.yt, g, hl[s], ho[s], hok[s]);
-    float q[3][EL_RY][5];
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        const long long pz = (long long)(zlo - 2 + p) * sz;
-#pragma unroll
-        for (int f = 0; f < 3; ++f)
-#pragma unroll
-            for (int j = 0; j < EL_RY; ++j)
-                q[f][j][p] = (xin && yin[j]) ? qf[f][pz + y[j] * sy + x] : 0.0f;
-    }
-    for (int z = zlo; z <= zhi; ++z) {
-        const long long pz = (long long)z * sz, pf = pz + 2 * sz;
-#pragma unroll
-        for (int f = 0; f < 3; ++f)
-#pragma unroll
-            for (int j = 0; j < EL_RY; ++j)
-                q[f][j][4] = (xin && yin[j]) ? qf[f][pf + y[j] * sy + x] : 0.0f;
-        float cen[3][EL_RY];
-#pragma unroll
-        for (int f = 0; f < 3; ++f)
-#pragma unroll
-            for (int j = 0; j < EL_RY; ++j)
-                cen[f][j] = (xin && yin[j]) ? lf[f][pz + y[j] * sy + x] : 0.0f;
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < EL_RY; ++j) {
-            const int li = (wy * EL_RY + j + 2) * EL_LX + lane + 2;
-            sv[0 * EL_PL + li] = cen[0][j];
-            sv[1 * EL_PL + li] = cen[1][j];
-            sv[2 * EL_PL + li] = cen[2][j];
-            sv[3 * EL_PL + li] = q[0][j][2];
-            sv[4 * EL_PL + li] = q[1][j][2];
-        }
-#pragma unroll
-        for (int f = 0; f < 5; ++f)
-#pragma unroll
-            for (int s = 0; s < EL_NHS; ++s)
-                if (hok[s]) sv[f * EL_PL + hl[s]] = lf[f][pz + ho[s]];
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < EL_RY; ++j) {
-            if (!(xupd && yupd[j])) continue;
-            const int li = (wy * EL_RY + j + 2) * EL_LX + lane + 2;
-            const long long c = pz + y[j] * sy + x;
-            const float* rxx = sv + li;
-            const float* rxy = sv + EL_PL + li;
-            const float* ryy = sv + 2 * EL_PL + li;
-            const float* rxz = sv + 3 * EL_PL + li;
-            const float* ryz = sv + 4 * EL_PL + li;
-            const float tb = dt * bb[c];
-            const float ax = (dl(rxx, 1, i0) + dl(rxy, EL_LX, i1)) + dq(q[0][j], i2);
-            const float ay = (dl(rxy, 1, i0) + dl(ryy, EL_LX, i1)) + dq(q[1][j], i2);
-            const float az = (dl(rxz, 1, i0) + dl(ryz, EL_LX, i1)) + dq(q[2][j], i2);
-            vx[c] = vx[c] + tb * ax;
-            vy[c] = vy[c] + tb * ay;
-            if (disp) disp[c] = disp[c] + (dt * tb) * ay;
-            vz[c] = vz[c] + tb * az;
-        }
-#pragma unroll
-        for (int f = 0; f < 3; ++f)
-#pragma unroll
-            for (int j = 0; j < EL_RY; ++j)
-#pragma unroll
-                for (int p = 0; p < 4; ++p) q[f][j][p] = q[f][j][p + 1];
     }
 }
 
@@ -690,8 +406,6 @@ static void el_tiles(ElDev& d)
     d.zc = zc;
     d.nzc = ceil_div(nz, zc);
     d.ntiles = d.ntx * d.nty * d.nzc * g.nblk;
-    d.nty16 = ceil_div(g.S[1] - 4, EL_TY16);
-    d.ntiles16 = d.ntx * d.nty16 * d.nzc * g.nblk;
 }
 
 static SrcRec el_srcrec(const b2r_el_shot* s)
@@ -890,15 +604,13 @@ struct ElBatch {
 static int el_table(std::vector<b2r_el_shot*>& shots, ElDev** d_tab, int* total)
 {
     std::vector<ElDev> tab(shots.size());
-    int t0 = 0, t16 = 0;
+    int t0 = 0;
     for (size_t i = 0; i < shots.size(); ++i) {
         tab[i] = shots[i]->d;
         tab[i].tile0 = t0;
-        tab[i].tile016 = t16;
         t0 += tab[i].ntiles;
-        t16 += tab[i].ntiles16;
     }
-    *total = t16;
+    *total = t0;
     b2r_ctx* c = shots[0]->ctx;
     B2R_HIP(hipMalloc(d_tab, sizeof(ElDev) * tab.size()));
     B2R_HIP(hipMemcpyAsync(*d_tab, tab.data(), sizeof(ElDev) * tab.size(), hipMemcpyHostToDevice,
@@ -919,14 +631,14 @@ static int el_step_launch(ElBatch& bt, int i, bool plain, const int* d_t_base)
             const int nt = bt.shots[k]->d.ntiles;
             // tile0 of entry k is relative to the batch; shift blockIdx by a
             // one-entry table whose tile0 is subtracted inside decode_tile
-            el_stress2_kernel<<<bt.shots[k]->d.ntiles16, blk, 0, bt.stream>>>(tk, 1, 2 * i);
+            el_stress_kernel<<<nt, blk, 0, bt.stream>>>(tk, 1, 2 * i);
             el_velocity_kernel<1><<<nt, blk, 0, bt.stream>>>(tk, 1, 2 * i + 1);
             el_velocity_kernel<2><<<nt, blk, 0, bt.stream>>>(tk, 1, 2 * i + 1);
             el_velocity_kernel<4><<<nt, blk, 0, bt.stream>>>(tk, 1, 2 * i + 1);
         }
     } else {
-        el_stress2_kernel<<<bt.total_tiles, blk, 0, bt.stream>>>(bt.d_tab, nshot, 2 * i);
-        el_velocity2_kernel<<<bt.total_tiles, blk, 0, bt.stream>>>(bt.d_tab, nshot, 2 * i + 1);
+        el_stress_kernel<<<bt.total_tiles, blk, 0, bt.stream>>>(bt.d_tab, nshot, 2 * i);
+        el_velocity_kernel<7><<<bt.total_tiles, blk, 0, bt.stream>>>(bt.d_tab, nshot, 2 * i + 1);
     }
     for (int k = 0; k < nshot; ++k) {
         b2r_el_shot* s = bt.shots[k];
@@ -980,7 +692,6 @@ static int el_batch_init(ElBatch& bt, b2r_el_shot* const* shots, int nshots, boo
         for (int k = 0; k < nshots; ++k) {
             tab[k] = shots[k]->d;
             tab[k].tile0 = 0;
-            tab[k].tile016 = 0;
         }
         B2R_HIP(hipMalloc(&bt.d_tab, sizeof(ElDev) * nshots));
         B2R_HIP(hipMemcpyAsync(bt.d_tab, tab.data(), sizeof(ElDev) * nshots,

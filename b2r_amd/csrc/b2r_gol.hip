@@ -4,7 +4,7 @@
 // rule (gol-3D.cu:93-99): alive & n<=1 -> 0, dead & n==5 -> 1, alive & n>=8 -> 0,
 // otherwise unchanged.  The reference leaves tile-corner halos unloaded,
 // applies the rule with an uninitialised count and writes only changed cells
-// into a ping-pong buffer (SURVEY defect D5); none of that is reproduced: every
+// into a ping-pong buffer; none of that is reproduced: every
 // cell of the update region is written with the total rule.
 //
 // SWAR formulation: a thread owns one aligned dword = 4 cells along x and RY

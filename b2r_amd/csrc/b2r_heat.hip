@@ -4,7 +4,7 @@
 // heat-3D.cpp:48-62 (CPU step).  The reference kernels use 32x32 thread blocks,
 // load tile-edge halos with edge threads and march z with a register queue; they
 // read the y-halo from the wrong row for D>1 and start the z march at the wrong
-// plane for R>1 (SURVEY defects D1/D2), which is not reproduced.
+// plane for R>1, which is not reproduced.
 //
 // This kernel: one workgroup = 64 x-lanes (one wave per row group) x 4 waves,
 // each thread owns RY consecutive rows of one x column and marches z through a
