@@ -99,6 +99,8 @@ _SIGS = {
     "b2r_unpack": (_i, [_vp, _i, _vp, _vp]),
     "b2r_heat_step": (_i, [_vp, _vp, _vp, C.POINTER(HeatParams), _i]),
     "b2r_heat_run": (_i, [_vp, _vp, _vp, C.POINTER(HeatParams), _i, _i, _pi, _pf]),
+    "b2r_heat_gradient": (_i, [_vp, _vp, _vp, C.POINTER(HeatParams), _i, _i,
+                               C.POINTER(C.c_double), _vp, C.POINTER(C.c_double)]),
     "b2r_gol_step": (_i, [_vp, _vp, _vp, _i]),
     "b2r_gol_run": (_i, [_vp, _vp, _vp, _i, _i, _pi, _pf]),
     "b2r_stream_copy": (_i, [_vp, _vp, _vp, C.c_size_t, _i, _pf]),
@@ -314,6 +316,15 @@ class Context:
         check(_lib.b2r_heat_run(self.h, fa, fb, C.byref(params), nsteps, fused, C.byref(rib),
                                 C.byref(ms) if timed else None))
         return (fb if rib.value else fa), (ms.value if timed else None)
+
+    def heat_gradient(self, u0, target, params: HeatParams, nsteps, ckpt, grad_u0):
+        """J = 1/2 |u_N - target|^2 over owned cells; writes dJ/du0 into the
+        field grad_u0 and returns (J, dJ/dk as 3 floats)."""
+        J = C.c_double(0.0)
+        gk = (C.c_double * 3)()
+        check(_lib.b2r_heat_gradient(self.h, u0, target, C.byref(params), nsteps, ckpt,
+                                     C.byref(J), grad_u0, gk))
+        return J.value, np.array(list(gk), np.float64)
 
     def gol_step(self, fin, fout, b2r_i):
         check(_lib.b2r_gol_step(self.h, fin, fout, b2r_i))

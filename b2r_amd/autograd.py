@@ -1,8 +1,10 @@
-"""torch.autograd wrapper of the elastic misfit and its hand-written adjoint.
+"""torch.autograd wrappers of the elastic and heat misfits and their
+hand-written adjoints.
 
 Used for checking only (finite differences, autograd plumbing): the forward
-and the gradient both run through the C ABI (b2r_el_gradient); torch only
-carries the arrays.  The medium is parametrised by (vp, vs, rho) with
+and the gradient both run through the C ABI (b2r_el_gradient,
+b2r_heat_gradient); torch only carries the arrays.  The elastic medium is
+parametrised by (vp, vs, rho) with
 lambda = rho (vp^2 - 2 vs^2), mu = rho vs^2, b = 1/rho, so
 
     dJ/dvp  = 2 rho vp g_lambda
@@ -43,3 +45,33 @@ class ElasticMisfit(torch.autograd.Function):
 
 def elastic_misfit(vp, vs, rho, shot, obs, nsteps, ckpt=50):
     return ElasticMisfit.apply(vp, vs, rho, shot, obs, nsteps, ckpt)
+
+
+class HeatMisfit(torch.autograd.Function):
+    """J(u0, k) = 1/2 sum_owned (u_N - target)^2 for nsteps F32 heat steps of
+    delta-D on a Context whose radius is D; target is a field of that context."""
+
+    @staticmethod
+    def forward(ctx, u0, k, context, target, D, nsteps, ckpt):
+        from . import _abi as A
+        kk = [float(v) for v in k.detach().cpu().numpy()]
+        p = A.heat_params(D, A.HEAT_F32, kk)
+        fu, fg = context.field(), context.field()
+        try:
+            context.upload(fu, u0.detach().cpu().numpy().astype(np.float32))
+            J, gk = context.heat_gradient(fu, target, p, nsteps, ckpt, fg)
+            g = context.download(fg)
+        finally:
+            context.free(fu)
+            context.free(fg)
+        ctx.save_for_backward(torch.from_numpy(g.astype(np.float64)), torch.from_numpy(gk))
+        return torch.tensor(J, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, gJ):
+        g, gk = ctx.saved_tensors
+        return gJ * g, gJ * gk, None, None, None, None, None
+
+
+def heat_misfit(u0, k, context, target, D, nsteps, ckpt=50):
+    return HeatMisfit.apply(u0, k, context, target, D, nsteps, ckpt)

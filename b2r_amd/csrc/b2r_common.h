@@ -79,6 +79,7 @@ int b2r_check_field(const b2r_ctx* ctx, const void* f);
 int b2r_direct_exchange(b2r_ctx* ctx, void* const* fields, int nfields);
 int b2r_time_begin(b2r_ctx* ctx, float* ms);
 int b2r_time_end(b2r_ctx* ctx, float* ms);
+int b2r_heat_check(b2r_ctx* ctx, const b2r_heat_params* p);
 int b2r_heat_launch(b2r_ctx* ctx, const void* in, void* out, const b2r_heat_params* p, int i);
 int b2r_gol_launch(b2r_ctx* ctx, const void* in, void* out, int i);
 int b2r_gol_fused_launch(b2r_ctx* ctx, const void* in, void* out);

@@ -18,49 +18,7 @@
 //   L_a = c_D*(m_D+p_D) + ... + c_1*(m_1+p_1) + c_0*u   (heat-3D.cu:314-318 order),
 //   REF and D=1: L_a = (m_1 - 2u) + p_1                  (heat-3D.cpp:58-60).
 #include "b2r_common.h"
-
-struct HeatK {
-    float k[3];
-    double adt;
-    double h2[3];
-};
-
-// second-difference taps (heat-3D.cu:15-28; delta1 weights 1, -2)
-__host__ __device__ constexpr float heat_c(int D, int j)
-{
-    return D == 1 ? (j == 0 ? -2.0f : 1.0f)
-         : D == 2 ? (j == 0 ? -2.5f : j == 1 ? 4.f / 3.f : -1.f / 12.f)
-         : D == 3 ? (j == 0 ? -49.f / 18.f : j == 1 ? 1.5f : j == 2 ? -3.f / 20.f : 1.f / 90.f)
-                  : (j == 0 ? -205.f / 72.f : j == 1 ? 1.6f : j == 2 ? -0.2f
-                     : j == 3 ? 8.f / 315.f : -1.f / 560.f);
-}
-
-// L_a from the two sides m[j] = u[-j], p[j] = u[+j] (j = 1..D) and the centre.
-template <int D, int MODE>
-__device__ __forceinline__ float lap(const float (&m)[D + 1], const float (&p)[D + 1], float uc)
-{
-    if (MODE == B2R_HEAT_REF && D == 1) return (m[1] - 2.0f * uc) + p[1];
-    float acc = heat_c(D, D) * (m[D] + p[D]);
-#pragma unroll
-    for (int j = D - 1; j >= 1; --j) acc = acc + heat_c(D, j) * (m[j] + p[j]);
-    return acc + heat_c(D, 0) * uc;
-}
-
-template <int MODE>
-__device__ __forceinline__ float heat_combine(float uc, float lx, float ly, float lz, const HeatK& hk)
-{
-    if (MODE == B2R_HEAT_F32) {
-        float t = hk.k[0] * lx;
-        t = t + hk.k[1] * ly;
-        t = t + hk.k[2] * lz;
-        return uc + t;
-    } else {
-        double t = (double)lx / hk.h2[0];
-        t = t + (double)ly / hk.h2[1];
-        t = t + (double)lz / hk.h2[2];
-        return (float)((double)uc + hk.adt * t);
-    }
-}
+#include "b2r_heat_ops.h"
 
 constexpr int HEAT_TX = 64;
 constexpr int HEAT_WY = 4;
@@ -193,7 +151,7 @@ __global__ __launch_bounds__(256) void heat_step_kernel(const float* __restrict_
     }
 }
 
-static int heat_check(b2r_ctx* ctx, const b2r_heat_params* p)
+int b2r_heat_check(b2r_ctx* ctx, const b2r_heat_params* p)
 {
     B2R_CHECK(ctx && p, B2R_E_ARG, "null argument");
     B2R_CHECK(ctx->g.elem_size == 4, B2R_E_UNSUPPORTED, "heat needs f32 cells");
@@ -270,7 +228,7 @@ int b2r_heat_launch(b2r_ctx* ctx, const void* in, void* out, const b2r_heat_para
 extern "C" int b2r_heat_step(b2r_ctx* ctx, const void* d_in, void* d_out,
                              const b2r_heat_params* p, int b2r_i)
 {
-    if (int rc = heat_check(ctx, p)) return rc;
+    if (int rc = b2r_heat_check(ctx, p)) return rc;
     if (int rc = b2r_check_field(ctx, d_in)) return rc;
     if (int rc = b2r_check_field(ctx, d_out)) return rc;
     B2R_CHECK(d_in != d_out, B2R_E_ARG, "in and out must differ");
@@ -284,7 +242,7 @@ static bool heat_fused_ok(const b2r_ctx* ctx, int D);
 extern "C" int b2r_heat_run(b2r_ctx* ctx, void* d_a, void* d_b, const b2r_heat_params* p,
                             int nsteps, int fused, int* result_in_b, float* ms)
 {
-    if (int rc = heat_check(ctx, p)) return rc;
+    if (int rc = b2r_heat_check(ctx, p)) return rc;
     if (int rc = b2r_check_field(ctx, d_a)) return rc;
     if (int rc = b2r_check_field(ctx, d_b)) return rc;
     B2R_CHECK(d_a != d_b && nsteps >= 0, B2R_E_ARG, "bad buffers or nsteps");

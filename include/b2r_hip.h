@@ -191,6 +191,18 @@ int b2r_heat_step(b2r_ctx* ctx, const void* d_in, void* d_out,
  * (temporal blocking in LDS) when supported.  *ms receives the device time. */
 int b2r_heat_run(b2r_ctx* ctx, void* d_a, void* d_b, const b2r_heat_params* p,
                  int nsteps, int fused, int* result_in_b, float* ms);
+/* Hand-written discrete adjoint of the heat time loop; no reference
+ * counterpart (the reference runs heat forward only, heat-3D.cu:583-642).
+ * J = 1/2 sum over owned cells of (u_N - target)^2 with u_N = nsteps steps of
+ * delta-D from d_u0.  Writes dJ/du_0 (owned cells) into d_grad_u0 and
+ * dJ/dk_a, a = x, y, z, into grad_k[3] (REF mode: k_a = adt / h2_a).  Needs
+ * periodic or frozen global faces.  Forward states are checkpointed every
+ * ckpt steps and recomputed per segment: ceil(nsteps/ckpt) + max(ckpt, 2) + 2
+ * extra fields of device memory.  FP64 reductions; d_u0 and d_target are
+ * read only (they may alias d_grad_u0). */
+int b2r_heat_gradient(b2r_ctx* ctx, const void* d_u0, const void* d_target,
+                      const b2r_heat_params* p, int nsteps, int ckpt, double* misfit,
+                      void* d_grad_u0, double* grad_k);
 
 /* ---- 3D Game of Life (gol-3D.cu) ------------------------------------------ */
 /* One generation (26-neighbour count n; alive & n<=1 -> 0; dead & n==5 -> 1;

@@ -17,13 +17,16 @@
  *   - the B2R halo pack / unpack and the ordered X->Y->Z exchange
  *     (b2r_halo_sync.cu:195-379, 442-508; neighbours b2r_comm.hpp:25-28,
  *     b2r_halo_sync.cu:140-183);
- *   - SplitMix64 seeded generators shared with the GPU.
+ *   - SplitMix64 seeded generators shared with the GPU;
+ *   - an FP64 adjoint of the heat time loop (no reference counterpart).
  *
  * Pinning (see tests/): delta1 REF mode is compared bit-for-bit with the
  * reference's own heat-3D.cpp compiled under oracle/_ref; FP64 heat against the
  * closed-form Dirichlet sine eigenmode and periodic Fourier modes, polynomial
  * exactness and conservation; GoL against known patterns, a 27-cell brute force
- * and translation equivariance; the exchange against global-index maps.
+ * and translation equivariance; the exchange against global-index maps; the
+ * heat adjoint against central finite differences of the FP64 forward and the
+ * dot-product identity <A^N x, y> = <x, (A^T)^N y>.
  */
 #include <math.h>
 #include <stdint.h>
@@ -284,6 +287,92 @@ void orc_heat_ref1_block_run(float* h_in, float* scratch, int R, int Ngx, int Ng
         float* t = pin; pin = pout; pout = t;
     }
     if (pin != h_in) memcpy(h_in, pin, sizeof(float) * n);
+}
+
+/* ------------------------------------------------------------------------- */
+/* FP64 heat adjoint (no reference counterpart)                              */
+/* ------------------------------------------------------------------------- */
+static inline int owned(const grid_t* g, int x, int y, int z)
+{
+    return x >= 0 && x < g->gx && y >= 0 && y < g->gy && z >= 0 && z < g->gz;
+}
+
+/* J = 1/2 sum_owned (u_N - tgt)^2 with u_N = nsteps FP64 heat steps from u0
+ * (grids as orc_heat_f64).  g_u0 (same shape, frame left zero) = dJ/du0,
+ * g_k[a] = dJ/dk_a.  All states are kept (small grids only).  The adjoint step
+ * is written as the transpose of the forward gather, i.e. a scatter: every
+ * owned cell q sends k_a c_j lam[q] to its owned neighbours q -/+ j e_a and
+ * (1 + c_0 sum_a k_a) lam[q] to itself; frame cells are constants and
+ * receive nothing. */
+void orc_heat_grad_f64(const double* u0, const double* tgt, int gx, int gy, int gz, int D,
+                       const double* k, int nsteps, int periodic, double* misfit,
+                       double* g_u0, double* g_k)
+{
+    grid_t g = mkgrid(gx, gy, gz, D, periodic);
+    long long n = grid_elems(&g);
+    double c[5];
+    coef_f64(D, c);
+    double* st = (double*)malloc(sizeof(double) * n * (nsteps + 1));
+    double* lam = (double*)calloc(n, sizeof(double));
+    double* nl = (double*)calloc(n, sizeof(double));
+    memcpy(st, u0, sizeof(double) * n);
+    for (int s = 0; s < nsteps; ++s) {
+        memcpy(st + (s + 1) * n, st + s * n, sizeof(double) * n);
+        orc_heat_f64(st + (s + 1) * n, gx, gy, gz, D, k, 1, periodic);
+    }
+    double J = 0.0;
+    const double* uN = st + (long long)nsteps * n;
+    for (int z = 0; z < gz; ++z)
+        for (int y = 0; y < gy; ++y)
+            for (int x = 0; x < gx; ++x) {
+                long long i = gidx(&g, x, y, z);
+                double r = uN[i] - tgt[i];
+                lam[i] = r;
+                J += 0.5 * r * r;
+            }
+    double gk[3] = {0.0, 0.0, 0.0};
+    const double self = 1.0 + c[0] * (k[0] + k[1] + k[2]);
+    for (int s = nsteps - 1; s >= 0; --s) {
+        const double* u = st + (long long)s * n;
+        memset(nl, 0, sizeof(double) * n);
+        for (int z = 0; z < gz; ++z)
+            for (int y = 0; y < gy; ++y)
+                for (int x = 0; x < gx; ++x) {
+                    long long q = gidx(&g, x, y, z);
+                    double uc = u[q];
+                    double l[3] = {c[0] * uc, c[0] * uc, c[0] * uc};
+                    for (int j = 1; j <= D; ++j) {
+                        l[0] += c[j] * (u[gidx(&g, x - j, y, z)] + u[gidx(&g, x + j, y, z)]);
+                        l[1] += c[j] * (u[gidx(&g, x, y - j, z)] + u[gidx(&g, x, y + j, z)]);
+                        l[2] += c[j] * (u[gidx(&g, x, y, z - j)] + u[gidx(&g, x, y, z + j)]);
+                    }
+                    for (int a = 0; a < 3; ++a) gk[a] += lam[q] * l[a];
+                    nl[q] += self * lam[q];
+                    for (int j = 1; j <= D; ++j)
+                        for (int sgn = -1; sgn <= 1; sgn += 2) {
+                            int d = sgn * j;
+                            if (periodic || owned(&g, x + d, y, z))
+                                nl[gidx(&g, x + d, y, z)] += k[0] * c[j] * lam[q];
+                            if (periodic || owned(&g, x, y + d, z))
+                                nl[gidx(&g, x, y + d, z)] += k[1] * c[j] * lam[q];
+                            if (periodic || owned(&g, x, y, z + d))
+                                nl[gidx(&g, x, y, z + d)] += k[2] * c[j] * lam[q];
+                        }
+                }
+        double* t = lam; lam = nl; nl = t;
+    }
+    memset(g_u0, 0, sizeof(double) * n);
+    for (int z = 0; z < gz; ++z)
+        for (int y = 0; y < gy; ++y)
+            for (int x = 0; x < gx; ++x) {
+                long long i = gidx(&g, x, y, z);
+                g_u0[i] = lam[i];
+            }
+    for (int a = 0; a < 3; ++a) g_k[a] = gk[a];
+    *misfit = J;
+    free(st);
+    free(lam);
+    free(nl);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -42,6 +42,7 @@ def _load():
         "orc_unpack": (None, [_vp, _vp, _i, _vp, _vp, _vp, _i, _i, _i, _ll, _ll, _i, _i]),
         "orc_exchange": (None, [_vp, _vp, _vp, _vp, _i, _i, _i, _ll, _ll, _i, _i]),
         "orc_num_threads": (_i, []),
+        "orc_heat_grad_f64": (None, [_vp, _vp, _i, _i, _i, _i, _vp, _i, _i, _vp, _vp, _vp]),
         "orc_el_stress_f32": (None, [_vp, _vp, _vp, _i, _i, _i, _vp, C.c_float]),
         "orc_el_velocity_f32": (None, [_vp, _vp, _vp, _i, _i, _i, _vp, C.c_float]),
         "orc_el_forward_f32": (None, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, C.c_float, _i,
@@ -123,6 +124,22 @@ def heat_f64(u, D, k, nsteps, periodic):
     kk = np.asarray(k, np.float64)
     lib.orc_heat_f64(_p(u), gx, gy, gz, D, _p(kk), nsteps, int(periodic))
     return u
+
+
+def heat_grad_f64(u0, tgt, D, k, nsteps, periodic):
+    """FP64 adjoint: (J, dJ/du0 (same shape as u0, frame zero), dJ/dk[3]) for
+    J = 1/2 sum_owned (u_N - tgt)^2; grids as heat_f64."""
+    u0 = np.ascontiguousarray(u0, np.float64)
+    tgt = np.ascontiguousarray(tgt, np.float64)
+    assert u0.shape == tgt.shape
+    gz, gy, gx = u0.shape if periodic else tuple(s - 2 * D for s in u0.shape)
+    kk = np.asarray(k, np.float64)
+    J = np.zeros(1)
+    g = np.zeros_like(u0)
+    gk = np.zeros(3)
+    lib.orc_heat_grad_f64(_p(u0), _p(tgt), gx, gy, gz, D, _p(kk), nsteps, int(periodic),
+                          _p(J), _p(g), _p(gk))
+    return float(J[0]), g, gk
 
 
 def heat_f32(u, D, mode, k, adt, h2, nsteps, periodic):
