@@ -122,6 +122,9 @@ _SIGS = {
     "b2r_el_gradient": (_i, [_vp, _vp, _i, _i, C.POINTER(C.c_double), _vp, _vp, _vp]),
     "b2r_el_gradient_ex": (_i, [_vp, _vp, _i, _i, _i, C.POINTER(C.c_double), _vp, _vp, _vp,
                                 _vp]),
+    "b2r_el_gradient_batch": (_i, [C.POINTER(_vp), _i, C.POINTER(_vp), _i, _i, _i,
+                                   C.POINTER(C.c_double), C.POINTER(_vp), C.POINTER(_vp),
+                                   C.POINTER(_vp), _pf]),
     "b2r_autotune": (_i, [_vp, _i, _vp, _vp, C.POINTER(HeatParams), C.c_char_p,
                           C.POINTER(Tuning)]),
     "b2r_dev_alloc": (_i, [_vp, C.c_size_t, C.POINTER(_vp)]),
@@ -436,14 +439,17 @@ class ElasticShot:
         check(_lib.b2r_el_misfit(self.h, _ptr(obs), C.byref(m)))
         return m.value
 
-    def gradient(self, obs, nsteps, ckpt, from_current=False, adjoint_state=False):
-        """misfit and [g_lam, g_mu, g_b] (plus the 9 adjoint fields at t=0)."""
+    def gradient(self, obs, nsteps, ckpt, from_current=False, adjoint_state=False, plain=False):
+        """misfit and [g_lam, g_mu, g_b] (plus the 9 adjoint fields at t=0).
+        plain: the per-step store-and-recompute algorithm instead of the
+        reverse sweep."""
         gx, gy, gz = self.ctx.env
         obs = np.ascontiguousarray(obs, np.float32)
         g = [np.empty((gz, gy, gx), np.float32) for _ in range(3)]
         adj = np.empty((9, gz, gy, gx), np.float32) if adjoint_state else None
         m = C.c_double(0.0)
-        check(_lib.b2r_el_gradient_ex(self.h, _ptr(obs), nsteps, ckpt, int(from_current),
+        flags = int(bool(from_current)) | (2 if plain else 0)
+        check(_lib.b2r_el_gradient_ex(self.h, _ptr(obs), nsteps, ckpt, flags,
                                       C.byref(m), *[_ptr(a) for a in g],
                                       _ptr(adj) if adjoint_state else None))
         return (m.value, g, adj) if adjoint_state else (m.value, g)
@@ -454,3 +460,23 @@ def el_forward_batch(shots, nsteps, mode=1, timed=False):
     ms = C.c_float(0.0)
     check(_lib.b2r_el_forward_batch(arr, len(shots), nsteps, mode, C.byref(ms) if timed else None))
     return ms.value if timed else None
+
+
+def el_gradient_batch(shots, obs, nsteps, ckpt, mode=1, timed=False):
+    """Gradients of several shots: mode 0 plain per-shot loop, 1 batched
+    reverse sweep, 2 batched with HIP graphs.  Returns (misfits, grads, ms)
+    with grads[k] = [g_lam, g_mu, g_b] of shot k."""
+    n = len(shots)
+    arr = (_vp * n)(*[s.h for s in shots])
+    obs = [np.ascontiguousarray(o, np.float32) for o in obs]
+    grads = []
+    for s in shots:
+        gx, gy, gz = s.ctx.env
+        grads.append([np.empty((gz, gy, gx), np.float32) for _ in range(3)])
+    po = (_vp * n)(*[_ptr(o) for o in obs])
+    pl, pm, pb = ((_vp * n)(*[_ptr(g[i]) for g in grads]) for i in range(3))
+    mis = (C.c_double * n)()
+    ms = C.c_float(0.0)
+    check(_lib.b2r_el_gradient_batch(arr, n, po, nsteps, ckpt, mode, mis, pl, pm, pb,
+                                     C.byref(ms) if timed else None))
+    return list(mis), grads, (ms.value if timed else None)

@@ -31,6 +31,15 @@ enum { F_VX, F_VY, F_VZ, F_SXX, F_SYY, F_SZZ, F_SXY, F_SXZ, F_SYZ, F_LAM, F_MU, 
 constexpr int EL_TX = 64, EL_TY = 4;
 constexpr int EL_MAX_BATCH = 64;
 
+struct SrcRec {
+    const int* src;     // x,y,z,comp
+    const float* wav;   // nt x nsrc
+    int nsrc, src_nt;
+    const int* rec;     // x,y,z
+    float* recv;        // nt x nrec
+    int nrec, rec_comp, rec_nt;
+};
+
 struct ElDev {
     KGeo g;
     float* f[F_NUM];
@@ -44,6 +53,11 @@ struct ElDev {
     int ntx, nty, nzc, zc;
     int tile0;        // first workgroup of this shot in a batched launch
     int ntiles;
+    // reverse sweep of the batched gradient
+    SrcRec sr;        // sources and receivers
+    const float* res; // receiver residuals, rec_nt x nrec
+    float* cpt;       // checkpoint slots: slot j-1 holds the state at step j*ckpt
+    long long fstride;// elements per field allocation
 };
 
 struct b2r_el_shot {
@@ -58,6 +72,17 @@ struct b2r_el_shot {
     float* d_recv = nullptr;    // rec_nt x nrec samples
     void* fields[F_NUM];
 };
+
+// occupancy hint for the stencil kernels (waves per SIMD the register
+// allocation must allow; 0 leaves the choice to the compiler)
+#ifndef EL_WAVES
+#define EL_WAVES 0
+#endif
+#if EL_WAVES
+#define EL_OCC __attribute__((amdgpu_waves_per_eu(EL_WAVES, 8)))
+#else
+#define EL_OCC
+#endif
 
 constexpr float EL_C2 = -1.f / 12.f;
 constexpr float EL_C1 = 2.f / 3.f;
@@ -96,7 +121,7 @@ __device__ __forceinline__ Tile decode_tile(const ElDev* __restrict__ tab, int n
 
 // ---------------------------------------------------------------- forward
 // stress stage s = 2i: sigma += dt*C:grad(v) over update_box(radius 2, s)
-__global__ __launch_bounds__(256) void el_stress_kernel(const ElDev* __restrict__ tab, int nshot,
+__global__ __launch_bounds__(256) EL_OCC void el_stress_kernel(const ElDev* __restrict__ tab, int nshot,
                                                         int stage)
 {
     const Tile T = decode_tile(tab, nshot);
@@ -142,7 +167,7 @@ __global__ __launch_bounds__(256) void el_stress_kernel(const ElDev* __restrict_
 
 // velocity stage s = 2i+1 for the components in MASK (1 x, 2 y, 4 z)
 template <int MASK>
-__global__ __launch_bounds__(256) void el_velocity_kernel(const ElDev* __restrict__ tab, int nshot,
+__global__ __launch_bounds__(256) EL_OCC void el_velocity_kernel(const ElDev* __restrict__ tab, int nshot,
                                                           int stage)
 {
     const Tile T = decode_tile(tab, nshot);
@@ -186,21 +211,17 @@ __global__ __launch_bounds__(256) void el_velocity_kernel(const ElDev* __restric
     }
 }
 
-struct SrcRec {
-    const int* src;     // x,y,z,comp
-    const float* wav;   // nt x nsrc
-    int nsrc, src_nt;
-    const int* rec;     // x,y,z
-    float* recv;        // nt x nrec
-    int nrec, rec_comp, rec_nt;
-};
 
 // Point forces after the velocity stage of inner step i: every block whose
 // stage box holds the source cell adds dt*b*f there, so redundant halo
-// computation between exchanges stays consistent with the owner.
-__global__ void el_inject_kernel(ElDev d, SrcRec sr, int stage, const int* __restrict__ tcount)
+// computation between exchanges stays consistent with the owner.  One launch
+// serves a table of shots: blockIdx.y is the shot, tcount[shot] its step.
+__global__ void el_inject_kernel(const ElDev* __restrict__ tab, int stage,
+                                 const int* __restrict__ tcount)
 {
-    const int t = *tcount;
+    const ElDev& d = tab[blockIdx.y];
+    const SrcRec& sr = d.sr;
+    const int t = tcount[blockIdx.y];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= sr.nsrc * d.g.nblk || t >= sr.src_nt) return;
     const int s = idx / d.g.nblk, b = idx % d.g.nblk;
@@ -219,9 +240,11 @@ __global__ void el_inject_kernel(ElDev d, SrcRec sr, int stage, const int* __res
 }
 
 // Receivers sample their component from the owning block after the step.
-__global__ void el_record_kernel(ElDev d, SrcRec sr, const int* __restrict__ tcount)
+__global__ void el_record_kernel(const ElDev* __restrict__ tab, const int* __restrict__ tcount)
 {
-    const int t = *tcount;
+    const ElDev& d = tab[blockIdx.y];
+    const SrcRec& sr = d.sr;
+    const int t = tcount[blockIdx.y];
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= sr.nrec || t >= sr.rec_nt) return;
     const int* p = sr.rec + 3 * r;
@@ -234,14 +257,19 @@ __global__ void el_record_kernel(ElDev d, SrcRec sr, const int* __restrict__ tco
     sr.recv[(long long)t * sr.nrec + r] = d.f[sr.rec_comp][cell_index(d.g, b, lc[0], lc[1], lc[2])];
 }
 
-__global__ void el_tick_kernel(int* tcount) { *tcount += 1; }
+// step counters of n shots += delta
+__global__ void el_tick_kernel(int* tcount, int n, int delta)
+{
+    const int i = threadIdx.x;
+    if (i < n) tcount[i] += delta;
+}
 
 // ---------------------------------------------------------------- adjoint
 // Reverse of one forward step t (single block, all global faces frozen, owned
 // cells = update region).  Exact transposes on the zero-framed grid: D^T = -D.
 // adjoint of the velocity stage: sh -= dt * D-form of w = b*vh, and
 // g_b += dt * vh . (D sigma'), sigma' = stresses after the stress stage.
-__global__ __launch_bounds__(256) void el_adj_velocity_kernel(const ElDev* __restrict__ tab, int nshot)
+__global__ __launch_bounds__(256) EL_OCC void el_adj_velocity_kernel(const ElDev* __restrict__ tab, int nshot)
 {
     const Tile T = decode_tile(tab, nshot);
     const ElDev& d = tab[T.shot];
@@ -285,7 +313,7 @@ __global__ __launch_bounds__(256) void el_adj_velocity_kernel(const ElDev* __res
 
 // adjoint of the stress stage: vh -= dt * D-form of p = C sh, and the
 // lambda / mu gradients with the strain rates of the pre-step velocity.
-__global__ __launch_bounds__(256) void el_adj_stress_kernel(const ElDev* __restrict__ tab, int nshot)
+__global__ __launch_bounds__(256) EL_OCC void el_adj_stress_kernel(const ElDev* __restrict__ tab, int nshot)
 {
     const Tile T = decode_tile(tab, nshot);
     const ElDev& d = tab[T.shot];
@@ -361,6 +389,199 @@ __global__ void el_adj_source_kernel(ElDev d, SrcRec sr, int t)
     const int* p = sr.src + 4 * i;
     const long long c = cell_index(g, 0, p[0] + g.h, p[1] + g.h, p[2] + g.h);
     d.grad[2][c] += d.dt * d.adj[p[3]][c] * sr.wav[(long long)t * sr.nsrc + i];
+}
+
+// ---------------------------------------------------------------- reverse sweep
+// The velocity-stress scheme is time-reversible.  From the state after step t,
+// (v', sigma'):  v = v' - dt b D.sigma' - dt b f(t) at the sources, then
+// sigma = sigma' - dt C:eps(v).  The batched gradient reverses each step this
+// way together with its adjoint, so nothing is stored between checkpoints;
+// the state is reset from a checkpoint every ckpt steps, which bounds the
+// rounding drift of the reconstruction.  Single block per shot, all global
+// faces frozen (the owned cells are the update region, the halo stays zero).
+// Each kernel writes only cell-local values and reads with a stencil only
+// fields it does not write, so both run in place.
+
+// velocity stage: a = D.sigma' (rounded as the forward velocity stage),
+//   v -= dt b a,  g_b += dt vh.a,  sh -= dt D-form(b vh)
+__global__ __launch_bounds__(256) EL_OCC void el_rev_velocity_kernel(const ElDev* __restrict__ tab,
+                                                                     int nshot)
+{
+    const Tile T = decode_tile(tab, nshot);
+    const ElDev& d = tab[T.shot];
+    const KGeo& g = d.g;
+    const int h = g.h, x = T.x, y = T.y;
+    if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
+    const long long sy = g.pitch, sz = g.slice, boff = g.xoff;
+    const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
+    const float* __restrict__ sxx = d.f[F_SXX] + boff;
+    const float* __restrict__ syy = d.f[F_SYY] + boff;
+    const float* __restrict__ szz = d.f[F_SZZ] + boff;
+    const float* __restrict__ sxy = d.f[F_SXY] + boff;
+    const float* __restrict__ sxz = d.f[F_SXZ] + boff;
+    const float* __restrict__ syz = d.f[F_SYZ] + boff;
+    const float* __restrict__ b = d.f[F_B] + boff;
+    float* __restrict__ vx = d.f[F_VX] + boff;
+    float* __restrict__ vy = d.f[F_VY] + boff;
+    float* __restrict__ vz = d.f[F_VZ] + boff;
+    const float* __restrict__ hx = d.adj[0] + boff;
+    const float* __restrict__ hy = d.adj[1] + boff;
+    const float* __restrict__ hz = d.adj[2] + boff;
+    float* __restrict__ gb = d.grad[2] + boff;
+    // D-form of w = b*vh: the transpose of the forward divergence
+    auto dw = [&](const float* __restrict__ hv, long long c, long long s, float ih) {
+        return (EL_C2 * (b[c + 2 * s] * hv[c + 2 * s] - b[c - 2 * s] * hv[c - 2 * s]) +
+                EL_C1 * (b[c + s] * hv[c + s] - b[c - s] * hv[c - s])) * ih;
+    };
+    const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
+    for (int z = z0; z <= z1; ++z) {
+        const long long c = z * sz + y * sy + x;
+        const float ax = (fd4(sxx, c, 1, i0) + fd4(sxy, c, sy, i1)) + fd4(sxz, c, sz, i2);
+        const float ay = (fd4(sxy, c, 1, i0) + fd4(syy, c, sy, i1)) + fd4(syz, c, sz, i2);
+        const float az = (fd4(sxz, c, 1, i0) + fd4(syz, c, sy, i1)) + fd4(szz, c, sz, i2);
+        const float wxx = dw(hx, c, 1, i0);
+        const float wyy = dw(hy, c, sy, i1);
+        const float wzz = dw(hz, c, sz, i2);
+        const float wxy = dw(hx, c, sy, i1) + dw(hy, c, 1, i0);
+        const float wxz = dw(hx, c, sz, i2) + dw(hz, c, 1, i0);
+        const float wyz = dw(hy, c, sz, i2) + dw(hz, c, sy, i1);
+        const float tb = dt * b[c];
+        gb[c] += dt * (hx[c] * ax + hy[c] * ay + hz[c] * az);
+        vx[c] = vx[c] - tb * ax;
+        vy[c] = vy[c] - tb * ay;
+        vz[c] = vz[c] - tb * az;
+        d.adj[3][boff + c] -= dt * wxx;
+        d.adj[4][boff + c] -= dt * wyy;
+        d.adj[5][boff + c] -= dt * wzz;
+        d.adj[6][boff + c] -= dt * wxy;
+        d.adj[7][boff + c] -= dt * wxz;
+        d.adj[8][boff + c] -= dt * wyz;
+    }
+}
+
+// stress stage: e = eps(v) (rounded as the forward stress stage),
+//   sigma -= dt C:e,  g_lambda, g_mu += dt (sh : dC/d. e),  vh -= dt D-form(C sh)
+__global__ __launch_bounds__(256) EL_OCC void el_rev_stress_kernel(const ElDev* __restrict__ tab,
+                                                                   int nshot)
+{
+    const Tile T = decode_tile(tab, nshot);
+    const ElDev& d = tab[T.shot];
+    const KGeo& g = d.g;
+    const int h = g.h, x = T.x, y = T.y;
+    if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
+    const long long sy = g.pitch, sz = g.slice, boff = g.xoff;
+    const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
+    const float* __restrict__ vx = d.f[F_VX] + boff;
+    const float* __restrict__ vy = d.f[F_VY] + boff;
+    const float* __restrict__ vz = d.f[F_VZ] + boff;
+    const float* __restrict__ lam = d.f[F_LAM] + boff;
+    const float* __restrict__ mu = d.f[F_MU] + boff;
+    const float* __restrict__ hxx = d.adj[3] + boff;
+    const float* __restrict__ hyy = d.adj[4] + boff;
+    const float* __restrict__ hzz = d.adj[5] + boff;
+    const float* __restrict__ hxy = d.adj[6] + boff;
+    const float* __restrict__ hxz = d.adj[7] + boff;
+    const float* __restrict__ hyz = d.adj[8] + boff;
+    // (C sh) components at cell q
+    auto pxx = [&](long long q) { return (lam[q] + 2.0f * mu[q]) * hxx[q] + lam[q] * (hyy[q] + hzz[q]); };
+    auto pyy = [&](long long q) { return (lam[q] + 2.0f * mu[q]) * hyy[q] + lam[q] * (hxx[q] + hzz[q]); };
+    auto pzz = [&](long long q) { return (lam[q] + 2.0f * mu[q]) * hzz[q] + lam[q] * (hxx[q] + hyy[q]); };
+    auto pxy = [&](long long q) { return mu[q] * hxy[q]; };
+    auto pxz = [&](long long q) { return mu[q] * hxz[q]; };
+    auto pyz = [&](long long q) { return mu[q] * hyz[q]; };
+    auto dp = [&](auto pf, long long c, long long s, float ih) {
+        return (EL_C2 * (pf(c + 2 * s) - pf(c - 2 * s)) + EL_C1 * (pf(c + s) - pf(c - s))) * ih;
+    };
+    const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
+    for (int z = z0; z <= z1; ++z) {
+        const long long c = z * sz + y * sy + x;
+        const float exx = fd4(vx, c, 1, i0);
+        const float eyy = fd4(vy, c, sy, i1);
+        const float ezz = fd4(vz, c, sz, i2);
+        const float dxy = fd4(vx, c, sy, i1) + fd4(vy, c, 1, i0);
+        const float dxz = fd4(vx, c, sz, i2) + fd4(vz, c, 1, i0);
+        const float dyz = fd4(vy, c, sz, i2) + fd4(vz, c, sy, i1);
+        const float ux = (dp(pxx, c, 1, i0) + dp(pxy, c, sy, i1)) + dp(pxz, c, sz, i2);
+        const float uy = (dp(pxy, c, 1, i0) + dp(pyy, c, sy, i1)) + dp(pyz, c, sz, i2);
+        const float uz = (dp(pxz, c, 1, i0) + dp(pyz, c, sy, i1)) + dp(pzz, c, sz, i2);
+        const float l = lam[c], m = mu[c];
+        const float l2m = l + 2.0f * m;
+        const float a0 = hxx[c], a1 = hyy[c], a2 = hzz[c];
+        const float tr = exx + eyy + ezz;
+        d.grad[0][boff + c] += dt * ((a0 + a1 + a2) * tr);
+        d.grad[1][boff + c] += dt * (2.0f * (a0 * exx + a1 * eyy + a2 * ezz) + hxy[c] * dxy +
+                                     hxz[c] * dxz + hyz[c] * dyz);
+        float* sg = d.f[F_SXX] + boff + c;
+        sg[0] = sg[0] - dt * (l2m * exx + l * (eyy + ezz));
+        sg = d.f[F_SYY] + boff + c;
+        sg[0] = sg[0] - dt * (l2m * eyy + l * (exx + ezz));
+        sg = d.f[F_SZZ] + boff + c;
+        sg[0] = sg[0] - dt * (l2m * ezz + l * (exx + eyy));
+        sg = d.f[F_SXY] + boff + c;
+        sg[0] = sg[0] - dt * (m * dxy);
+        sg = d.f[F_SXZ] + boff + c;
+        sg[0] = sg[0] - dt * (m * dxz);
+        sg = d.f[F_SYZ] + boff + c;
+        sg[0] = sg[0] - dt * (m * dyz);
+        d.adj[0][boff + c] -= dt * ux;
+        d.adj[1][boff + c] -= dt * uy;
+        d.adj[2][boff + c] -= dt * uz;
+    }
+}
+
+// Reverse of the point terms of step tcount[shot] (blockIdx.y = shot):
+// receivers feed the adjoint velocity (x < nrec), then (a second launch,
+// phase 1) sources are taken out of the state and feed g_b.
+__global__ void el_rev_point_kernel(const ElDev* __restrict__ tab, const int* __restrict__ tcount,
+                                    int phase)
+{
+    const ElDev& d = tab[blockIdx.y];
+    const SrcRec& sr = d.sr;
+    const int t = tcount[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const KGeo& g = d.g;
+    if (phase == 0) {
+        if (i >= sr.nrec || t < 0 || t >= sr.rec_nt) return;
+        const int* p = sr.rec + 3 * i;
+        const long long c = cell_index(g, 0, p[0] + g.h, p[1] + g.h, p[2] + g.h);
+        d.adj[sr.rec_comp][c] += d.res[(long long)t * sr.nrec + i];
+        return;
+    }
+    if (i >= sr.nsrc || t < 0 || t >= sr.src_nt) return;
+    const int* p = sr.src + 4 * i;
+    const long long c = cell_index(g, 0, p[0] + g.h, p[1] + g.h, p[2] + g.h);
+    const float w = sr.wav[(long long)t * sr.nsrc + i];
+    const float tb = d.dt * d.f[F_B][c];
+    float* v = d.f[p[3]];
+    v[c] = v[c] - tb * w;
+    d.grad[2][c] += d.dt * d.adj[p[3]][c] * w;
+}
+
+// Checkpoint slot j-1 <-> state (owned cells of the 9 fields).  save: after
+// the forward reached step t = j*ckpt < nsteps; restore: before reversing
+// step t with t+1 = j*ckpt < nsteps.  Other steps leave everything alone.
+__global__ __launch_bounds__(256) void el_ckpt_kernel(const ElDev* __restrict__ tab, int nshot,
+                                                      const int* __restrict__ tcount, int ckpt,
+                                                      int nsteps, int restore)
+{
+    const Tile T = decode_tile(tab, nshot);
+    const ElDev& d = tab[T.shot];
+    const KGeo& g = d.g;
+    const int ts = tcount[T.shot] + (restore ? 1 : 0);
+    if (ts % ckpt != 0 || ts < ckpt || ts >= nsteps) return;
+    const int h = g.h, x = T.x, y = T.y;
+    if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
+    float* slot = d.cpt + (long long)(ts / ckpt - 1) * 9 * d.fstride;
+    const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
+    for (int k = 0; k < 9; ++k) {
+        float* st = d.f[k];
+        float* cp = slot + (long long)k * d.fstride;
+        for (int z = z0; z <= z1; ++z) {
+            const long long c = g.xoff + z * g.slice + (long long)y * g.pitch + x;
+            if (restore) st[c] = cp[c];
+            else cp[c] = st[c];
+        }
+    }
 }
 
 // res = rec - obs, and per-workgroup FP64 partial sums of res^2 (wave64 shuffles)
@@ -597,60 +818,48 @@ struct ElBatch {
     std::vector<b2r_el_shot*> shots;
     ElDev* d_tab = nullptr;   // device copy of the shot table
     int* d_t = nullptr;       // per shot step counters (device)
-    int total_tiles = 0;
+    int total_tiles = 0;      // workgroups of a batched launch
+    int max_src = 0;          // max over shots of sources x blocks
+    int max_rec = 0;          // max over shots of receivers
     hipStream_t stream;
 };
 
-static int el_table(std::vector<b2r_el_shot*>& shots, ElDev** d_tab, int* total)
-{
-    std::vector<ElDev> tab(shots.size());
-    int t0 = 0;
-    for (size_t i = 0; i < shots.size(); ++i) {
-        tab[i] = shots[i]->d;
-        tab[i].tile0 = t0;
-        t0 += tab[i].ntiles;
-    }
-    *total = t0;
-    b2r_ctx* c = shots[0]->ctx;
-    B2R_HIP(hipMalloc(d_tab, sizeof(ElDev) * tab.size()));
-    B2R_HIP(hipMemcpyAsync(*d_tab, tab.data(), sizeof(ElDev) * tab.size(), hipMemcpyHostToDevice,
-                           c->stream));
-    return B2R_OK;
-}
-
 // One step (inner step i of the B2R window) of every shot in the batch.
-// plain: separate vx, vy, vz launches, per shot.
-static int el_step_launch(ElBatch& bt, int i, bool plain, const int* d_t_base)
+// plain: per shot, a stress launch, separate vx, vy, vz launches and the
+// point terms (eqwp_gpu_main's structure, eqwp.cu:679-684); batched: one
+// stress and one three-component velocity launch over the table of shots,
+// one launch each for the point terms of all shots.
+static int el_step_launch(ElBatch& bt, int i, bool plain)
 {
     const int nshot = (int)bt.shots.size();
     dim3 blk(EL_TX, EL_TY);
     if (plain) {
-        // one shot per launch: the table entry of shot k alone
         for (int k = 0; k < nshot; ++k) {
+            // a one-entry table (tile0 = 0 for plain tables)
             const ElDev* tk = bt.d_tab + k;
-            const int nt = bt.shots[k]->d.ntiles;
-            // tile0 of entry k is relative to the batch; shift blockIdx by a
-            // one-entry table whose tile0 is subtracted inside decode_tile
+            const b2r_el_shot* s = bt.shots[k];
+            const int nt = s->d.ntiles;
             el_stress_kernel<<<nt, blk, 0, bt.stream>>>(tk, 1, 2 * i);
             el_velocity_kernel<1><<<nt, blk, 0, bt.stream>>>(tk, 1, 2 * i + 1);
             el_velocity_kernel<2><<<nt, blk, 0, bt.stream>>>(tk, 1, 2 * i + 1);
             el_velocity_kernel<4><<<nt, blk, 0, bt.stream>>>(tk, 1, 2 * i + 1);
+            if (s->nsrc)
+                el_inject_kernel<<<dim3(ceil_div(s->nsrc * s->d.g.nblk, 64), 1), 64, 0, bt.stream>>>(
+                    tk, 2 * i + 1, bt.d_t + k);
+            if (s->nrec)
+                el_record_kernel<<<dim3(ceil_div(s->nrec, 64), 1), 64, 0, bt.stream>>>(tk, bt.d_t + k);
         }
     } else {
         el_stress_kernel<<<bt.total_tiles, blk, 0, bt.stream>>>(bt.d_tab, nshot, 2 * i);
         el_velocity_kernel<7><<<bt.total_tiles, blk, 0, bt.stream>>>(bt.d_tab, nshot, 2 * i + 1);
+        if (bt.max_src)
+            el_inject_kernel<<<dim3(ceil_div(bt.max_src, 64), nshot), 64, 0, bt.stream>>>(
+                bt.d_tab, 2 * i + 1, bt.d_t);
+        if (bt.max_rec)
+            el_record_kernel<<<dim3(ceil_div(bt.max_rec, 64), nshot), 64, 0, bt.stream>>>(bt.d_tab,
+                                                                                         bt.d_t);
     }
-    for (int k = 0; k < nshot; ++k) {
-        b2r_el_shot* s = bt.shots[k];
-        SrcRec sr = el_srcrec(s);
-        if (s->nsrc) {
-            const int n = s->nsrc * s->d.g.nblk;
-            el_inject_kernel<<<ceil_div(n, 64), 64, 0, bt.stream>>>(s->d, sr, 2 * i + 1, d_t_base + k);
-        }
-        if (s->nrec)
-            el_record_kernel<<<ceil_div(s->nrec, 64), 64, 0, bt.stream>>>(s->d, sr, d_t_base + k);
-        el_tick_kernel<<<1, 1, 0, bt.stream>>>(bt.d_t + k);
-    }
+    el_tick_kernel<<<1, 64, 0, bt.stream>>>(bt.d_t, nshot, 1);
     B2R_LAUNCH_CHECK();
     return B2R_OK;
 }
@@ -672,36 +881,41 @@ static int el_exchange(ElBatch& bt)
     return B2R_OK;
 }
 
-// the tile0 of a single-shot table entry must be zero for plain launches
+// Device table of the shots (their ElDev with sources and receivers) and
+// step counters.  Batched tables number the workgroups of all shots
+// consecutively (tile0); plain tables give every shot tile0 = 0.
 static int el_batch_init(ElBatch& bt, b2r_el_shot* const* shots, int nshots, bool plain)
 {
     B2R_CHECK(shots && nshots >= 1 && nshots <= EL_MAX_BATCH, B2R_E_ARG, "1..%d shots",
               EL_MAX_BATCH);
     bt.shots.assign(shots, shots + nshots);
+    for (int k = 0; k < nshots; ++k) B2R_CHECK(shots[k], B2R_E_ARG, "null shot %d", k);
     const int R = shots[0]->ctx->g.R;
-    for (int k = 0; k < nshots; ++k) {
-        B2R_CHECK(shots[k], B2R_E_ARG, "null shot %d", k);
-        B2R_CHECK(shots[k]->ctx->device == shots[0]->ctx->device, B2R_E_ARG,
-                  "all shots of a batch must be on one device");
-        B2R_CHECK(shots[k]->ctx->g.R == R, B2R_E_ARG, "all shots of a batch must share R");
-        shots[k]->d.g = shots[k]->ctx->k;
-    }
-    bt.stream = shots[0]->ctx->stream;
-    if (plain) {
-        std::vector<ElDev> tab(nshots);
-        for (int k = 0; k < nshots; ++k) {
-            tab[k] = shots[k]->d;
-            tab[k].tile0 = 0;
-        }
-        B2R_HIP(hipMalloc(&bt.d_tab, sizeof(ElDev) * nshots));
-        B2R_HIP(hipMemcpyAsync(bt.d_tab, tab.data(), sizeof(ElDev) * nshots,
-                               hipMemcpyHostToDevice, bt.stream));
-        bt.total_tiles = 0;
-    } else if (int rc = el_table(bt.shots, &bt.d_tab, &bt.total_tiles)) {
-        return rc;
-    }
+    std::vector<ElDev> tab(nshots);
     std::vector<int> t0(nshots);
-    for (int k = 0; k < nshots; ++k) t0[k] = shots[k]->t;
+    int tiles = 0;
+    for (int k = 0; k < nshots; ++k) {
+        b2r_el_shot* s = shots[k];
+        B2R_CHECK(s->ctx->device == shots[0]->ctx->device, B2R_E_ARG,
+                  "all shots of a batch must be on one device");
+        B2R_CHECK(s->ctx->g.R == R, B2R_E_ARG, "all shots of a batch must share R");
+        s->d.g = s->ctx->k;
+        s->d.sr = el_srcrec(s);
+        tab[k] = s->d;
+        tab[k].tile0 = plain ? 0 : tiles;
+        tiles += s->d.ntiles;
+        bt.max_src = std::max(bt.max_src, s->nsrc * s->d.g.nblk);
+        bt.max_rec = std::max(bt.max_rec, s->nrec);
+        t0[k] = s->t;
+    }
+    bt.total_tiles = plain ? 0 : tiles;
+    bt.stream = shots[0]->ctx->stream;
+    // work queued on the shots' own context streams comes first
+    for (int k = 0; k < nshots; ++k)
+        if (shots[k]->ctx->stream != bt.stream) B2R_HIP(hipStreamSynchronize(shots[k]->ctx->stream));
+    B2R_HIP(hipMalloc(&bt.d_tab, sizeof(ElDev) * nshots));
+    B2R_HIP(hipMemcpyAsync(bt.d_tab, tab.data(), sizeof(ElDev) * nshots, hipMemcpyHostToDevice,
+                           bt.stream));
     B2R_HIP(hipMalloc(&bt.d_t, sizeof(int) * nshots));
     B2R_HIP(hipMemcpyAsync(bt.d_t, t0.data(), sizeof(int) * nshots, hipMemcpyHostToDevice,
                            bt.stream));
@@ -722,7 +936,7 @@ static int el_group(ElBatch& bt, int n, bool plain)
 {
     if (int rc = el_exchange(bt)) return rc;
     for (int i = 0; i < n; ++i) {
-        if (int rc = el_step_launch(bt, i, plain, bt.d_t)) return rc;
+        if (int rc = el_step_launch(bt, i, plain)) return rc;
         if (plain) B2R_HIP(hipStreamSynchronize(bt.stream));  // eqwp.cu:685
     }
     return B2R_OK;
@@ -802,124 +1016,149 @@ extern "C" int b2r_el_misfit(b2r_el_shot* s, const float* h_obs, double* misfit)
     b2r_ctx* c = s->ctx;
     const long long n = (long long)s->nrec * s->rec_nt;
     B2R_CHECK(n > 0, B2R_E_ARG, "no receivers");
-    float* d_obs = nullptr;
-    float* d_res = nullptr;
-    double* d_part = nullptr;
+    if (int rc = b2r_activate(c)) return rc;
     const int nb = 256;
-    B2R_HIP(hipMalloc(&d_obs, n * 4));
-    B2R_HIP(hipMalloc(&d_res, n * 4));
-    B2R_HIP(hipMalloc(&d_part, (nb + 1) * sizeof(double)));
-    (void)hipMemcpyAsync(d_obs, h_obs, n * 4, hipMemcpyHostToDevice, c->stream);
-    el_residual_kernel<<<nb, 256, 0, c->stream>>>(s->d_recv, d_obs, d_res, n, d_part);
-    el_sum_kernel<<<1, 64, 0, c->stream>>>(d_part, nb, d_part + nb);
+    void* buf = nullptr;  // [obs n][res n][partials nb + 1]
+    B2R_CHECK(hipMalloc(&buf, n * 8 + (nb + 1) * sizeof(double)) == hipSuccess, B2R_E_NOMEM,
+              "misfit: device allocation failed");
+    float* d_obs = (float*)buf;
+    float* d_res = d_obs + n;
+    double* d_part = (double*)(d_res + n + (n & 1));
     double ss = 0;
-    (void)hipMemcpyAsync(&ss, d_part + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    hipError_t e = hipStreamSynchronize(c->stream);
-    (void)hipFree(d_obs);
-    (void)hipFree(d_res);
-    (void)hipFree(d_part);
+    hipError_t e = hipMemcpyAsync(d_obs, h_obs, n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        el_residual_kernel<<<nb, 256, 0, c->stream>>>(s->d_recv, d_obs, d_res, n, d_part);
+        el_sum_kernel<<<1, 64, 0, c->stream>>>(d_part, nb, d_part + nb);
+        e = hipMemcpyAsync(&ss, d_part + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
     B2R_CHECK(e == hipSuccess, B2R_E_HIP, "misfit: %s", hipGetErrorString(e));
     *misfit = 0.5 * ss;
     return B2R_OK;
 }
 
-static int copy_state(b2r_el_shot* s, float* const* dst, const float* const* src)
+static int copy_state(b2r_el_shot* s, hipStream_t st, float* const* dst, const float* const* src)
 {
     const size_t bytes = (size_t)s->ctx->L.field_elems * 4;
     for (int k = 0; k < 9; ++k)
-        B2R_HIP(hipMemcpyAsync(dst[k], src[k], bytes, hipMemcpyDeviceToDevice, s->ctx->stream));
+        B2R_HIP(hipMemcpyAsync(dst[k], src[k], bytes, hipMemcpyDeviceToDevice, st));
     return B2R_OK;
 }
 
-// Gradient of J = 1/2 sum (rec - obs)^2 over nsteps steps from a zero state,
-// w.r.t. lambda, mu and b per owned cell (h_g: 3 global grids).  Forward
-// states are checkpointed every ckpt steps and each segment is recomputed
-// (states of the segment kept on the device) before its reverse sweep.
-extern "C" int b2r_el_gradient_ex(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt,
-                                  int flags, double* misfit, float* h_glam, float* h_gmu,
-                                  float* h_gb, float* h_adj0);
-
-extern "C" int b2r_el_gradient(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt,
-                               double* misfit, float* h_glam, float* h_gmu, float* h_gb)
+static int el_grad_check(b2r_el_shot* s, int nsteps)
 {
-    return b2r_el_gradient_ex(s, h_obs, nsteps, ckpt, 0, misfit, h_glam, h_gmu, h_gb, nullptr);
-}
-
-// flags bit 0: start from the current state instead of zero.  h_adj0 (optional,
-// 9 global grids): the adjoint state at the start, i.e. d misfit / d state(0).
-extern "C" int b2r_el_gradient_ex(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt,
-                                  int flags, double* misfit, float* h_glam, float* h_gmu,
-                                  float* h_gb, float* h_adj0)
-{
-    B2R_CHECK(s && h_obs && misfit && h_glam && h_gmu && h_gb && nsteps >= 1 && ckpt >= 1,
-              B2R_E_ARG, "bad argument");
+    B2R_CHECK(s, B2R_E_ARG, "null shot");
     b2r_ctx* c = s->ctx;
     B2R_CHECK(c->k.nblk == 1 && c->g.R == 1 && !c->k.periodic && c->k.fmask == 0x3F,
               B2R_E_UNSUPPORTED, "the adjoint runs on one block, R=1, frozen faces");
     B2R_CHECK(s->rec_nt >= nsteps && s->nrec > 0, B2R_E_ARG, "receivers must cover nsteps");
     B2R_CHECK(s->d.disp == nullptr, B2R_E_UNSUPPORTED, "disable the displacement output");
+    return B2R_OK;
+}
+
+// Gradients out through the owned-cell copy: 3 grids, then optionally the 9
+// fields of the adjoint state.
+static int el_grad_out(b2r_el_shot* s, const float* grd, const float* adj, float* h_glam,
+                       float* h_gmu, float* h_gb, float* h_adj0)
+{
+    b2r_ctx* c = s->ctx;
+    const size_t fe = (size_t)c->L.field_elems;
+    const long long gcells = (long long)c->g.env_x * c->g.env_y * c->g.env_z;
+    void* f = nullptr;
+    int rc = b2r_field_alloc(c, &f);
+    for (int k = 0; k < 3 + (h_adj0 ? 9 : 0) && !rc; ++k) {
+        float* out = k == 0 ? h_glam : (k == 1 ? h_gmu : (k == 2 ? h_gb : h_adj0 + (k - 3) * gcells));
+        const float* srcp = k < 3 ? grd + k * fe : adj + (k - 3) * fe;
+        if (hipMemcpyAsync(f, srcp, fe * 4, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+            b2r_set_error("gradient copy failed");
+            rc = B2R_E_HIP;
+        }
+        if (!rc) rc = b2r_field_download(c, f, out);
+    }
+    if (f) b2r_field_free(c, f);
+    return rc;
+}
+
+// residuals rec - obs on the device (res) and the FP64 misfit
+static int el_residuals(b2r_el_shot* s, hipStream_t st, const float* h_obs, float* d_obs,
+                        float* d_res, double* d_part, double* misfit)
+{
+    const long long n = (long long)s->nrec * s->rec_nt;
+    const int nb = 256;
+    B2R_HIP(hipMemcpyAsync(d_obs, h_obs, n * 4, hipMemcpyHostToDevice, st));
+    el_residual_kernel<<<nb, 256, 0, st>>>(s->d_recv, d_obs, d_res, n, d_part);
+    el_sum_kernel<<<1, 64, 0, st>>>(d_part, nb, d_part + nb);
+    B2R_LAUNCH_CHECK();
+    B2R_HIP(hipMemcpyAsync(misfit, d_part + nb, sizeof(double), hipMemcpyDeviceToHost, st));
+    return B2R_OK;
+}
+
+// Plain per-shot gradient: the forward in the reference's structure (stress,
+// vx, vy, vz launches and a synchronisation per step), states checkpointed
+// every ckpt steps; every segment is recomputed with all its states kept on
+// the device, then reversed with the adjoint kernels, one step at a time.
+static int el_gradient_plain(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt, int flags,
+                             double* misfit, float* h_glam, float* h_gmu, float* h_gb,
+                             float* h_adj0)
+{
+    if (int rc = el_grad_check(s, nsteps)) return rc;
+    b2r_ctx* c = s->ctx;
     if (int rc = b2r_activate(c)) return rc;
-    const size_t fb = (size_t)c->L.field_elems * 4;
+    const size_t fe = (size_t)c->L.field_elems, fb = fe * 4;
     const int ncp = (nsteps + ckpt - 1) / ckpt;
+    const long long nobs = (long long)s->nrec * s->rec_nt;
     std::vector<void*> mem;
-    auto alloc = [&](size_t bytes) -> float* {
+    auto alloc = [&](size_t bytes) -> void* {
         void* p = nullptr;
         if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
         mem.push_back(p);
-        return (float*)p;
+        return p;
     };
+    ElBatch bt;
     auto release = [&]() {
+        el_batch_free(bt);
         (void)hipStreamSynchronize(c->stream);
         for (void* p : mem) (void)hipFree(p);
     };
-    float* cp = alloc(fb * 9 * ncp);
-    float* seg = alloc(fb * 9 * (ckpt + 1));
-    float* adj = alloc(fb * 9);
-    float* grd = alloc(fb * 3);
-    float* res = alloc((size_t)s->nrec * s->rec_nt * 4);
-    if (!cp || !seg || !adj || !grd || !res) {
+    float* cp = (float*)alloc(fb * 9 * ncp);
+    float* seg = (float*)alloc(fb * 9 * (ckpt + 1));
+    float* adj = (float*)alloc(fb * 9);
+    float* grd = (float*)alloc(fb * 3);
+    float* obs = (float*)alloc(nobs * 8 + 8 + 257 * sizeof(double));
+    ElDev* d_tab = (ElDev*)alloc(sizeof(ElDev));
+    if (!cp || !seg || !adj || !grd || !obs || !d_tab) {
         release();
         b2r_set_error("gradient: device allocation of %zu MB failed",
                       fb * (9 * ncp + 9 * (ckpt + 1) + 12) >> 20);
         return B2R_E_NOMEM;
     }
+    float* res = obs + nobs;
+    double* part = (double*)(res + nobs + (nobs & 1));
     auto slot = [&](float* base, int j, float** out) {
-        for (int k = 0; k < 9; ++k) out[k] = base + ((size_t)j * 9 + k) * (fb / 4);
+        for (int k = 0; k < 9; ++k) out[k] = base + ((size_t)j * 9 + k) * fe;
     };
     int rc = B2R_OK;
     if (flags & 1) s->t = 0;
     else rc = b2r_el_reset(s);
+    if (!rc) rc = el_batch_init(bt, &s, 1, true);
     // forward with checkpoints
     for (int j = 0; j < ncp && !rc; ++j) {
         float* dst[9];
         slot(cp, j, dst);
-        rc = copy_state(s, dst, s->d.f);
+        rc = copy_state(s, c->stream, dst, s->d.f);
         const int n = std::min(ckpt, nsteps - j * ckpt);
-        if (!rc) rc = b2r_el_forward(s, n, 1, nullptr);
+        for (int i = 0; i < n && !rc; ++i) rc = el_group(bt, 1, true);
     }
-    if (!rc) rc = b2r_el_misfit(s, h_obs, misfit);
-    // residuals on the device (recomputed: b2r_el_misfit freed its copy)
-    if (!rc) {
-        const long long n = (long long)s->nrec * s->rec_nt;
-        std::vector<float> rec(n);
-        rc = b2r_el_get_receivers(s, rec.data());
-        for (long long i = 0; i < n && !rc; ++i) rec[i] -= h_obs[i];
-        if (!rc) B2R_HIP(hipMemcpyAsync(res, rec.data(), n * 4, hipMemcpyHostToDevice, c->stream));
-        if (!rc) B2R_HIP(hipStreamSynchronize(c->stream));
-    }
-    if (!rc) {
-        B2R_HIP(hipMemsetAsync(adj, 0, fb * 9, c->stream));
-        B2R_HIP(hipMemsetAsync(grd, 0, fb * 3, c->stream));
-    }
+    double ms_h = 0;
+    if (!rc) rc = el_residuals(s, c->stream, h_obs, obs, res, part, &ms_h);
+    if (!rc && hipMemsetAsync(adj, 0, fb * 9, c->stream) != hipSuccess) rc = B2R_E_HIP;
+    if (!rc && hipMemsetAsync(grd, 0, fb * 3, c->stream) != hipSuccess) rc = B2R_E_HIP;
+    if (!rc) B2R_HIP(hipStreamSynchronize(c->stream));
     ElDev d = s->d;
     d.g = c->k;
-    for (int k = 0; k < 9; ++k) d.adj[k] = adj + (size_t)k * (fb / 4);
-    for (int k = 0; k < 3; ++k) d.grad[k] = grd + (size_t)k * (fb / 4);
-    ElDev* d_tab = alloc(sizeof(ElDev)) ? (ElDev*)mem.back() : nullptr;
-    if (!d_tab && !rc) {
-        b2r_set_error("gradient: table allocation failed");
-        rc = B2R_E_NOMEM;
-    }
+    for (int k = 0; k < 9; ++k) d.adj[k] = adj + k * fe;
+    for (int k = 0; k < 3; ++k) d.grad[k] = grd + k * fe;
     SrcRec sr = el_srcrec(s);
     dim3 blk(EL_TX, EL_TY);
     for (int j = ncp - 1; j >= 0 && !rc; --j) {
@@ -929,13 +1168,15 @@ extern "C" int b2r_el_gradient_ex(b2r_el_shot* s, const float* h_obs, int nsteps
         slot(cp, j, src);
         float* st[9];
         slot(seg, 0, st);
-        rc = copy_state(s, st, src);
-        if (!rc) rc = copy_state(s, s->d.f, st);
-        s->t = t0;
+        rc = copy_state(s, c->stream, st, src);
+        if (!rc) rc = copy_state(s, c->stream, s->d.f, st);
+        if (!rc && hipMemcpyAsync(bt.d_t, &t0, sizeof(int), hipMemcpyHostToDevice, c->stream) !=
+                       hipSuccess)
+            rc = B2R_E_HIP;
         for (int t = t0; t < t1 && !rc; ++t) {
-            rc = b2r_el_forward(s, 1, 1, nullptr);
+            rc = el_group(bt, 1, true);
             slot(seg, t - t0 + 1, st);
-            if (!rc) rc = copy_state(s, st, s->d.f);
+            if (!rc) rc = copy_state(s, c->stream, st, s->d.f);
         }
         for (int t = t1 - 1; t >= t0 && !rc; --t) {
             slot(seg, t - t0, d.fw0);
@@ -955,20 +1196,204 @@ extern "C" int b2r_el_gradient_ex(b2r_el_shot* s, const float* h_obs, int nsteps
         }
     }
     s->t = nsteps;
-    // gradients (and the adjoint state) out through the owned-cell copy
-    const long long gcells = (long long)c->g.env_x * c->g.env_y * c->g.env_z;
-    for (int k = 0; k < 3 + (h_adj0 ? 9 : 0) && !rc; ++k) {
-        float* out = k == 0 ? h_glam : (k == 1 ? h_gmu : (k == 2 ? h_gb : h_adj0 + (k - 3) * gcells));
-        const float* srcp = k < 3 ? grd + (size_t)k * (fb / 4) : adj + (size_t)(k - 3) * (fb / 4);
-        void* f = nullptr;
-        rc = b2r_field_alloc(c, &f);
-        if (!rc && hipMemcpyAsync(f, srcp, fb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
-            b2r_set_error("gradient copy failed");
-            rc = B2R_E_HIP;
+    if (!rc) rc = el_grad_out(s, grd, adj, h_glam, h_gmu, h_gb, h_adj0);
+    if (!rc) *misfit = 0.5 * ms_h;
+    release();
+    return rc;
+}
+
+// Batched reverse-sweep gradient of nshots shots (one block each, R = 1,
+// frozen faces, sizes may differ).  Forward: batched steps, the state saved
+// every ckpt steps.  Reverse: each step is undone (the scheme is
+// time-reversible) together with its adjoint, the state reset from the
+// checkpoint at every segment boundary; no per-step storage.  graph != 0
+// replays whole segments from HIP graphs (step counters on the device).
+static int el_gradient_rev(b2r_el_shot* const* shots, int nshots, const float* const* h_obs,
+                           int nsteps, int ckpt, bool graph, int flags, double* misfit,
+                           float* const* h_glam, float* const* h_gmu, float* const* h_gb,
+                           float* h_adj0, float* ms)
+{
+    B2R_CHECK(shots && h_obs && misfit && h_glam && h_gmu && h_gb && nshots >= 1 &&
+                  nshots <= EL_MAX_BATCH && nsteps >= 1 && ckpt >= 1,
+              B2R_E_ARG, "bad argument");
+    B2R_CHECK(!h_adj0 || nshots == 1, B2R_E_ARG, "the adjoint state is returned for one shot");
+    for (int k = 0; k < nshots; ++k) {
+        if (int rc = el_grad_check(shots[k], nsteps)) return rc;
+        B2R_CHECK(h_obs[k] && h_glam[k] && h_gmu[k] && h_gb[k], B2R_E_ARG, "null array, shot %d", k);
+    }
+    b2r_ctx* c0 = shots[0]->ctx;
+    if (int rc = b2r_activate(c0)) return rc;
+    hipStream_t st = c0->stream;
+    const int ncp = (nsteps + ckpt - 1) / ckpt;
+    // per shot: [checkpoints (ncp-1) x 9][adjoint 9][gradient 3] fields, then
+    // [obs][res] samples and FP64 partials
+    std::vector<void*> mem(nshots, nullptr);
+    std::vector<double> h_mis(nshots, 0.0);
+    ElBatch bt;
+    auto release = [&]() {
+        el_batch_free(bt);
+        (void)hipStreamSynchronize(st);
+        for (void* p : mem)
+            if (p) (void)hipFree(p);
+        for (int k = 0; k < nshots; ++k) {
+            shots[k]->d.cpt = nullptr;
+            shots[k]->d.res = nullptr;
+            for (int i = 0; i < 9; ++i) shots[k]->d.adj[i] = nullptr;
+            for (int i = 0; i < 3; ++i) shots[k]->d.grad[i] = nullptr;
         }
-        if (!rc) rc = b2r_field_download(c, f, out);
-        if (f) b2r_field_free(c, f);
+    };
+    int rc = B2R_OK;
+    for (int k = 0; k < nshots && !rc; ++k) {
+        b2r_el_shot* s = shots[k];
+        const size_t fe = (size_t)s->ctx->L.field_elems;
+        const long long nobs = (long long)s->nrec * s->rec_nt;
+        const size_t nf = (size_t)9 * (ncp - 1) + 12;
+        const size_t bytes = nf * fe * 4 + (size_t)nobs * 8 + 8 + 257 * sizeof(double);
+        if (hipMalloc(&mem[k], bytes) != hipSuccess) {
+            mem[k] = nullptr;
+            b2r_set_error("gradient: device allocation of %zu MB for shot %d failed", bytes >> 20, k);
+            rc = B2R_E_NOMEM;
+            break;
+        }
+        float* base = (float*)mem[k];
+        s->d.cpt = base;
+        s->d.fstride = (long long)fe;
+        float* adj = base + (size_t)9 * (ncp - 1) * fe;
+        for (int i = 0; i < 9; ++i) s->d.adj[i] = adj + i * fe;
+        for (int i = 0; i < 3; ++i) s->d.grad[i] = adj + (9 + i) * fe;
+        float* obs = adj + 12 * fe;
+        s->d.res = obs + nobs;
+        if (hipMemsetAsync(adj, 0, 12 * fe * 4, st) != hipSuccess) rc = B2R_E_HIP;
+        if (!rc) {
+            if (flags & 1) s->t = 0;
+            else rc = b2r_el_reset(s);
+        }
+    }
+    if (!rc) rc = el_batch_init(bt, shots, nshots, false);
+    if (!rc) rc = b2r_time_begin(c0, ms);
+    const dim3 blk(EL_TX, EL_TY);
+    // ---- forward: segments of ckpt steps, each followed by a checkpoint save
+    auto fwd_segment = [&](int n) {
+        for (int i = 0; i < n; ++i)
+            if (int r = el_group(bt, 1, false)) return r;
+        el_ckpt_kernel<<<bt.total_tiles, blk, 0, st>>>(bt.d_tab, nshots, bt.d_t, ckpt, nsteps, 0);
+        return hipGetLastError() == hipSuccess ? B2R_OK : B2R_E_HIP;
+    };
+    // ---- reverse: n steps, after a checkpoint restore at the segment top
+    auto rev_segment = [&](int n) {
+        el_ckpt_kernel<<<bt.total_tiles, blk, 0, st>>>(bt.d_tab, nshots, bt.d_t, ckpt, nsteps, 1);
+        for (int i = 0; i < n; ++i) {
+            if (bt.max_rec)
+                el_rev_point_kernel<<<dim3(ceil_div(bt.max_rec, 64), nshots), 64, 0, st>>>(
+                    bt.d_tab, bt.d_t, 0);
+            if (bt.max_src)
+                el_rev_point_kernel<<<dim3(ceil_div(bt.max_src, 64), nshots), 64, 0, st>>>(
+                    bt.d_tab, bt.d_t, 1);
+            el_rev_velocity_kernel<<<bt.total_tiles, blk, 0, st>>>(bt.d_tab, nshots);
+            el_rev_stress_kernel<<<bt.total_tiles, blk, 0, st>>>(bt.d_tab, nshots);
+            el_tick_kernel<<<1, 64, 0, st>>>(bt.d_t, nshots, -1);
+        }
+        return hipGetLastError() == hipSuccess ? B2R_OK : B2R_E_HIP;
+    };
+    // a segment graph, captured once and replayed (its counters advance on the device)
+    auto run_graph = [&](bool reverse, int count) {
+        if (count <= 0) return B2R_OK;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ex = nullptr;
+        int r = B2R_OK;
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            b2r_set_error("hipStreamBeginCapture failed");
+            return B2R_E_HIP;
+        }
+        r = reverse ? rev_segment(ckpt) : fwd_segment(ckpt);
+        hipError_t e = hipStreamEndCapture(st, &g);
+        if (!r && e != hipSuccess) r = B2R_E_HIP;
+        if (!r && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) r = B2R_E_HIP;
+        for (int i = 0; i < count && !r; ++i)
+            if (hipGraphLaunch(ex, st) != hipSuccess) r = B2R_E_HIP;
+        if (ex) (void)hipGraphExecDestroy(ex);
+        if (g) (void)hipGraphDestroy(g);
+        if (r) b2r_set_error("gradient: graph capture or launch failed");
+        return r;
+    };
+    const int nfull = nsteps / ckpt, tail = nsteps - nfull * ckpt;
+    if (!rc) {
+        if (graph) rc = run_graph(false, nfull);
+        else
+            for (int j = 0; j < nfull && !rc; ++j) rc = fwd_segment(ckpt);
+    }
+    if (!rc && tail) rc = fwd_segment(tail);
+    for (int k = 0; k < nshots && !rc; ++k) {
+        b2r_el_shot* s = shots[k];
+        const long long nobs = (long long)s->nrec * s->rec_nt;
+        float* obs = (float*)s->d.res - nobs;
+        double* part = (double*)(s->d.res + nobs + (nobs & 1));
+        rc = el_residuals(s, st, h_obs[k], obs, (float*)s->d.res, part, &h_mis[k]);
+    }
+    // ---- reverse sweep: the top segment (the tail, or a full one) starts from
+    // the final state, every other one from its checkpoint
+    if (!rc) {
+        el_tick_kernel<<<1, 64, 0, st>>>(bt.d_t, nshots, -1);
+        rc = rev_segment(tail ? tail : ckpt);
+    }
+    const int nrest = tail ? nfull : nfull - 1;
+    if (!rc) {
+        if (graph) rc = run_graph(true, nrest);
+        else
+            for (int j = 0; j < nrest && !rc; ++j) rc = rev_segment(ckpt);
+    }
+    if (!rc) rc = b2r_time_end(c0, ms);
+    if (!rc) B2R_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < nshots && !rc; ++k) {
+        b2r_el_shot* s = shots[k];
+        rc = el_grad_out(s, s->d.grad[0], s->d.adj[0], h_glam[k], h_gmu[k], h_gb[k],
+                         k == 0 ? h_adj0 : nullptr);
+        misfit[k] = 0.5 * h_mis[k];
+        s->t = nsteps;
     }
     release();
     return rc;
+}
+
+extern "C" int b2r_el_gradient_ex(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt,
+                                  int flags, double* misfit, float* h_glam, float* h_gmu,
+                                  float* h_gb, float* h_adj0)
+{
+    B2R_CHECK(s && h_obs && misfit && h_glam && h_gmu && h_gb && nsteps >= 1 && ckpt >= 1,
+              B2R_E_ARG, "bad argument");
+    if (flags & 2)
+        return el_gradient_plain(s, h_obs, nsteps, ckpt, flags, misfit, h_glam, h_gmu, h_gb,
+                                 h_adj0);
+    return el_gradient_rev(&s, 1, &h_obs, nsteps, ckpt, false, flags, misfit, &h_glam, &h_gmu,
+                           &h_gb, h_adj0, nullptr);
+}
+
+extern "C" int b2r_el_gradient(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt,
+                               double* misfit, float* h_glam, float* h_gmu, float* h_gb)
+{
+    return b2r_el_gradient_ex(s, h_obs, nsteps, ckpt, 0, misfit, h_glam, h_gmu, h_gb, nullptr);
+}
+
+extern "C" int b2r_el_gradient_batch(b2r_el_shot* const* shots, int nshots,
+                                     const float* const* h_obs, int nsteps, int ckpt, int mode,
+                                     double* misfit, float* const* h_glam, float* const* h_gmu,
+                                     float* const* h_gb, float* ms)
+{
+    B2R_CHECK(shots && h_obs && misfit && h_glam && h_gmu && h_gb && nshots >= 1 &&
+                  mode >= 0 && mode <= 2 && nsteps >= 1 && ckpt >= 1,
+              B2R_E_ARG, "bad argument");
+    if (mode > 0)
+        return el_gradient_rev(shots, nshots, h_obs, nsteps, ckpt, mode == 2, 0, misfit, h_glam,
+                               h_gmu, h_gb, nullptr, ms);
+    // plain: one shot after the other, timed as a whole
+    if (int rc = b2r_activate(shots[0]->ctx)) return rc;
+    if (int rc = b2r_time_begin(shots[0]->ctx, ms)) return rc;
+    for (int k = 0; k < nshots; ++k) {
+        B2R_CHECK(shots[k] && shots[k]->ctx->device == shots[0]->ctx->device, B2R_E_ARG,
+                  "all shots of a batch must be on one device");
+        if (int rc = el_gradient_plain(shots[k], h_obs[k], nsteps, ckpt, 0, &misfit[k],
+                                       h_glam[k], h_gmu[k], h_gb[k], nullptr))
+            return rc;
+    }
+    return b2r_time_end(shots[0]->ctx, ms);
 }

@@ -28,6 +28,19 @@ def layered_medium(env, seed, nlayers=10, vp_range=(2000.0, 4500.0), vs_ratio=1.
     return vp, vs, rho
 
 
+def perturbed_layers(arrays, env, seed, nlayers=10, frac=0.05):
+    """The observed-data model: every array scaled per layer along y by
+    (1 + frac*xi), xi ~ U(-1, 1), independently per array and layer."""
+    gx, gy, gz = env
+    rng = np.random.default_rng(seed)
+    layer = np.minimum((np.arange(gy) * nlayers) // gy, nlayers - 1)
+    out = []
+    for a in arrays:
+        f = 1.0 + frac * rng.uniform(-1.0, 1.0, nlayers)
+        out.append((np.asarray(a, np.float64) * f[layer][None, :, None]).astype(np.float32))
+    return out
+
+
 def lame(vp, vs, rho):
     """lambda = rho (vp^2 - 2 vs^2), mu = rho vs^2, b = 1/rho (FP64, rounded to f32)."""
     vp, vs, rho = (np.asarray(a, np.float64) for a in (vp, vs, rho))

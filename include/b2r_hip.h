@@ -253,14 +253,29 @@ int b2r_el_forward_batch(b2r_el_shot* const* shots, int nshots, int nsteps, int 
 int b2r_el_misfit(b2r_el_shot* s, const float* h_obs, double* misfit);
 /* Discrete adjoint (no counterpart in the reference): misfit of nsteps steps
  * from a zero state and its gradient w.r.t. lambda, mu and b per owned cell
- * (global grids), forward states checkpointed every ckpt steps and recomputed
- * per segment.  One block, R = 1, frozen faces. */
+ * (global grids).  The state is checkpointed every ckpt steps; the reverse
+ * sweep undoes each step (the velocity-stress scheme is time-reversible)
+ * together with its adjoint and resets the state from the checkpoint at every
+ * segment boundary, so nothing is stored between checkpoints:
+ * (ceil(nsteps/ckpt) - 1) * 9 + 12 extra fields.  One block, R = 1, frozen
+ * faces; sources at distinct cells. */
 int b2r_el_gradient(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt, double* misfit,
                     float* h_glam, float* h_gmu, float* h_gb);
-/* flags bit 0: start from the current state (not zero); h_adj0 (optional, 9
- * global grids): the adjoint state at the start = d misfit / d initial state */
+/* flags bit 0: start from the current state (not zero); bit 1: the plain
+ * algorithm (forward in the reference's per-step structure, each segment
+ * recomputed with all its states kept, ckpt + 1 + ceil(nsteps/ckpt) state
+ * copies); h_adj0 (optional, 9 global grids): the adjoint state at the start
+ * = d misfit / d initial state */
 int b2r_el_gradient_ex(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt, int flags,
                        double* misfit, float* h_glam, float* h_gmu, float* h_gb, float* h_adj0);
+/* Gradients of nshots shots (sizes may differ), shot k with observations
+ * h_obs[k] and outputs misfit[k], h_glam[k], h_gmu[k], h_gb[k].  mode 0: the
+ * plain per-shot loop (flags bit 1 above, one shot after the other); mode 1:
+ * all shots per launch, reverse sweep as b2r_el_gradient; mode 2: mode 1 with
+ * every checkpoint segment replayed from a HIP graph.  *ms: device time. */
+int b2r_el_gradient_batch(b2r_el_shot* const* shots, int nshots, const float* const* h_obs,
+                          int nsteps, int ckpt, int mode, double* misfit, float* const* h_glam,
+                          float* const* h_gmu, float* const* h_gb, float* ms);
 
 /* ---- measurement helpers ---------------------------------------------------- */
 /* iters device-to-device copies of bytes (a STREAM-copy kernel): *ms total. */

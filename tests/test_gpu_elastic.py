@@ -112,13 +112,16 @@ def _obs(P):
     return obs
 
 
+@pytest.mark.parametrize("plain", [False, True])
 @pytest.mark.parametrize("ckpt", [1, 7, 100])
-def test_gradient_vs_fp64_cpu_adjoint(ckpt):
+def test_gradient_vs_fp64_cpu_adjoint(ckpt, plain):
+    # both algorithms: the reverse sweep (default) and the plain
+    # store-and-recompute loop
     P = E.small_problem(env=(20, 18, 16), nsteps=30)
     obs = _obs(P)
     with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
         with _shot(c, P) as s:
-            J, g = s.gradient(obs, P["nsteps"], ckpt)
+            J, g = s.gradient(obs, P["nsteps"], ckpt, plain=plain)
     lam, mu, b = (P[k].astype(np.float64) for k in ("lam", "mu", "b"))
     _, rec = E.forward(P["env"], lam, mu, b, float(P["dt"]), P["ih"], P["nsteps"], P["src"],
                        P["comp"], P["wav"].astype(np.float64), P["rec"], 1, fp64=True)
@@ -132,18 +135,67 @@ def test_gradient_vs_fp64_cpu_adjoint(ckpt):
 
 
 def test_gradient_checkpoint_invariance():
-    # recomputation replays the forward bit for bit, so the checkpoint
-    # interval does not change the gradient at all
+    # plain: recomputation replays the forward bit for bit, so the checkpoint
+    # interval does not change the gradient at all; reverse sweep: the
+    # reconstructed states differ from the forward ones by rounding only
     P = E.small_problem(env=(20, 18, 16), nsteps=20)
     obs = _obs(P)
-    out = []
+    out, rev = [], []
     for ckpt in (1, 6, 20):
         with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
             with _shot(c, P) as s:
-                out.append(s.gradient(obs, 20, ckpt)[1])
+                out.append(s.gradient(obs, 20, ckpt, plain=True)[1])
+                rev.append(s.gradient(obs, 20, ckpt)[1])
     for g in out[1:]:
         for k in range(3):
             assert bits_equal(g[k], out[0][k])
+    for g in rev:
+        for k in range(3):
+            assert rel_l2(g[k], out[0][k]) < 1e-5, (k, rel_l2(g[k], out[0][k]))
+
+
+def test_gradient_reverse_sweep_long_segment():
+    # 240 steps reconstructed backwards without a checkpoint reset against the
+    # plain algorithm's stored states: the drift of the reconstruction stays
+    # at rounding level
+    P = E.small_problem(env=(22, 20, 18), nsteps=240)
+    obs = _obs(P)
+    with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
+        with _shot(c, P) as s:
+            J0, g0 = s.gradient(obs, P["nsteps"], 48, plain=True)
+            J1, g1 = s.gradient(obs, P["nsteps"], 240)
+    say("long segment", J0, J1, [rel_l2(g1[k], g0[k]) for k in range(3)])
+    assert J1 == J0
+    for k in range(3):
+        assert rel_l2(g1[k], g0[k]) < 1e-4
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_gradient_batch_uneven_shots(mode):
+    # three shots of different sizes in one batched gradient: each equals its
+    # single-shot reverse-sweep gradient bit for bit; the plain per-shot loop
+    # (mode 0) agrees to rounding
+    sizes = [(20, 18, 16), (24, 16, 20), (16, 22, 18)]
+    probs = [E.small_problem(env=e, nsteps=26, seed=3 + i) for i, e in enumerate(sizes)]
+    obs = [_obs(P) for P in probs]
+    ctxs = [A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) for P in probs]
+    shots = [_shot(c, P) for c, P in zip(ctxs, probs)]
+    try:
+        single = [s.gradient(o, 26, 7) for s, o in zip(shots, obs)]
+        mis, grads, _ = A.el_gradient_batch(shots, obs, 26, 7, mode)
+        mis0, grads0, _ = A.el_gradient_batch(shots, obs, 26, 7, 0)
+        for k in range(3):
+            say("batch shot", k, mis[k], single[k][0], mis0[k])
+            assert mis[k] == single[k][0]
+            assert abs(mis0[k] - mis[k]) <= 1e-6 * mis[k]
+            for i in range(3):
+                assert bits_equal(grads[k][i], single[k][1][i])
+                assert rel_l2(grads0[k][i], grads[k][i]) < 1e-5
+    finally:
+        for s in shots:
+            s.close()
+        for c in ctxs:
+            c.close()
 
 
 def test_gradient_finite_differences():
