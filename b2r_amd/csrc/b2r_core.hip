@@ -184,7 +184,7 @@ extern "C" int b2r_set_tuning(b2r_ctx* ctx, const b2r_tuning* t)
     B2R_CHECK(ctx && t, B2R_E_ARG, "null argument");
     B2R_CHECK(t->heat_ry == 0 || t->heat_ry == 1 || t->heat_ry == 2 || t->heat_ry == 4 ||
               t->heat_ry == 8, B2R_E_ARG, "heat_ry must be 0, 1, 2, 4 or 8");
-    B2R_CHECK(t->heat_zc >= 0 && t->gol_zc >= 0 && t->gol_step_zc >= 0, B2R_E_ARG,
+    B2R_CHECK(t->heat_zc >= 0 && t->gol_zc >= 0 && t->gol_step_zc >= 0 && t->el_zc >= 0, B2R_E_ARG,
               "negative z chunk");
     ctx->tune = *t;
     return B2R_OK;

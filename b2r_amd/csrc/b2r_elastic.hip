@@ -29,6 +29,7 @@
 enum { F_VX, F_VY, F_VZ, F_SXX, F_SYY, F_SZZ, F_SXY, F_SXZ, F_SYZ, F_LAM, F_MU, F_B, F_NUM };
 
 constexpr int EL_TX = 64, EL_TY = 4;
+constexpr int EL_WG = EL_TX * EL_TY;  // threads of a tile workgroup
 constexpr int EL_MAX_BATCH = 64;
 
 struct SrcRec {
@@ -51,6 +52,7 @@ struct ElDev {
     float dt;
     float ih[3];
     int ntx, nty, nzc, zc;
+    int o[3], w[3];   // tiled region per axis: [o, o + w)
     int tile0;        // first workgroup of this shot in a batched launch
     int ntiles;
     // reverse sweep of the batched gradient
@@ -72,17 +74,6 @@ struct b2r_el_shot {
     float* d_recv = nullptr;    // rec_nt x nrec samples
     void* fields[F_NUM];
 };
-
-// occupancy hint for the stencil kernels (waves per SIMD the register
-// allocation must allow; 0 leaves the choice to the compiler)
-#ifndef EL_WAVES
-#define EL_WAVES 0
-#endif
-#if EL_WAVES
-#define EL_OCC __attribute__((amdgpu_waves_per_eu(EL_WAVES, 8)))
-#else
-#define EL_OCC
-#endif
 
 constexpr float EL_C2 = -1.f / 12.f;
 constexpr float EL_C1 = 2.f / 3.f;
@@ -112,16 +103,30 @@ __device__ __forceinline__ Tile decode_tile(const ElDev* __restrict__ tab, int n
     lt /= d.nty;
     const int zci = lt % d.nzc;
     T.b = lt / d.nzc;
-    T.x = 2 + txi * EL_TX + threadIdx.x;
-    T.y = 2 + tyi * EL_TY + threadIdx.y;
-    T.zs = 2 + zci * d.zc;
-    T.ze = min(T.zs + d.zc, d.g.S[2] - 2);  // exclusive
+    T.x = d.o[0] + txi * EL_TX + threadIdx.x;
+    T.y = d.o[1] + tyi * EL_TY + threadIdx.y;
+    T.zs = d.o[2] + zci * d.zc;
+    T.ze = min(T.zs + d.zc, d.o[2] + d.w[2]);  // exclusive
     return T;
+}
+
+// z-derivative from a register queue over planes z-2..z+2 (the rounding of fd4)
+__device__ __forceinline__ float fdq(const float (&q)[5], float ih)
+{
+    return (EL_C2 * (q[4] - q[0]) + EL_C1 * (q[3] - q[1])) * ih;
+}
+
+__device__ __forceinline__ void qshift(float (&q)[5])
+{
+    q[0] = q[1];
+    q[1] = q[2];
+    q[2] = q[3];
+    q[3] = q[4];
 }
 
 // ---------------------------------------------------------------- forward
 // stress stage s = 2i: sigma += dt*C:grad(v) over update_box(radius 2, s)
-__global__ __launch_bounds__(256) EL_OCC void el_stress_kernel(const ElDev* __restrict__ tab, int nshot,
+__global__ __launch_bounds__(EL_WG) void el_stress_kernel(const ElDev* __restrict__ tab, int nshot,
                                                         int stage)
 {
     const Tile T = decode_tile(tab, nshot);
@@ -146,14 +151,31 @@ __global__ __launch_bounds__(256) EL_OCC void el_stress_kernel(const ElDev* __re
     const long long sy = g.pitch, sz = g.slice;
     const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
     const int z0 = max(T.zs, box.lo[2]), z1 = min(T.ze - 1, box.hi[2]);
+    if (z0 > z1) return;
+    float qx[5], qy[5], qz[5];  // v along z, planes z-2..z+2
+    {
+        const long long c0 = z0 * sz + y * sy + x;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            qx[k] = vx[c0 + (k - 2) * sz];
+            qy[k] = vy[c0 + (k - 2) * sz];
+            qz[k] = vz[c0 + (k - 2) * sz];
+        }
+    }
     for (int z = z0; z <= z1; ++z) {
         const long long c = z * sz + y * sy + x;
+        qx[4] = vx[c + 2 * sz];
+        qy[4] = vy[c + 2 * sz];
+        qz[4] = vz[c + 2 * sz];
         const float exx = fd4(vx, c, 1, i0);
         const float eyy = fd4(vy, c, sy, i1);
-        const float ezz = fd4(vz, c, sz, i2);
+        const float ezz = fdq(qz, i2);
         const float dxy = fd4(vx, c, sy, i1) + fd4(vy, c, 1, i0);
-        const float dxz = fd4(vx, c, sz, i2) + fd4(vz, c, 1, i0);
-        const float dyz = fd4(vy, c, sz, i2) + fd4(vz, c, sy, i1);
+        const float dxz = fdq(qx, i2) + fd4(vz, c, 1, i0);
+        const float dyz = fdq(qy, i2) + fd4(vz, c, sy, i1);
+        qshift(qx);
+        qshift(qy);
+        qshift(qz);
         const float l = lam[c], m = mu[c];
         const float l2m = l + 2.0f * m;
         sxx[c] = sxx[c] + dt * (l2m * exx + l * (eyy + ezz));
@@ -167,7 +189,7 @@ __global__ __launch_bounds__(256) EL_OCC void el_stress_kernel(const ElDev* __re
 
 // velocity stage s = 2i+1 for the components in MASK (1 x, 2 y, 4 z)
 template <int MASK>
-__global__ __launch_bounds__(256) EL_OCC void el_velocity_kernel(const ElDev* __restrict__ tab, int nshot,
+__global__ __launch_bounds__(EL_WG) void el_velocity_kernel(const ElDev* __restrict__ tab, int nshot,
                                                           int stage)
 {
     const Tile T = decode_tile(tab, nshot);
@@ -192,20 +214,38 @@ __global__ __launch_bounds__(256) EL_OCC void el_velocity_kernel(const ElDev* __
     const long long sy = g.pitch, sz = g.slice;
     const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
     const int z0 = max(T.zs, box.lo[2]), z1 = min(T.ze - 1, box.hi[2]);
+    if (z0 > z1) return;
+    // z-derivative fields (sxz for vx, syz for vy, szz for vz) from register
+    // queues over planes z-2..z+2, rounded exactly as direct loads
+    float qa[5], qb[5], qc[5];
+    const long long cb = (long long)y * sy + x;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long long q = (long long)(z0 - 2 + k) * sz + cb;
+        if (MASK & 1) qa[k] = sxz[q];
+        if (MASK & 2) qb[k] = syz[q];
+        if (MASK & 4) qc[k] = szz[q];
+    }
     for (int z = z0; z <= z1; ++z) {
-        const long long c = z * sz + y * sy + x;
+        const long long c = z * sz + cb, q = c + 2 * sz;
         const float tb = dt * bb[c];
         if (MASK & 1) {
-            const float ax = (fd4(sxx, c, 1, i0) + fd4(sxy, c, sy, i1)) + fd4(sxz, c, sz, i2);
+            qa[4] = sxz[q];
+            const float ax = (fd4(sxx, c, 1, i0) + fd4(sxy, c, sy, i1)) + fdq(qa, i2);
+            qshift(qa);
             vx[c] = vx[c] + tb * ax;
         }
         if (MASK & 2) {
-            const float ay = (fd4(sxy, c, 1, i0) + fd4(syy, c, sy, i1)) + fd4(syz, c, sz, i2);
+            qb[4] = syz[q];
+            const float ay = (fd4(sxy, c, 1, i0) + fd4(syy, c, sy, i1)) + fdq(qb, i2);
+            qshift(qb);
             vy[c] = vy[c] + tb * ay;
             if (disp) disp[c] = disp[c] + (dt * tb) * ay;
         }
         if (MASK & 4) {
-            const float az = (fd4(sxz, c, 1, i0) + fd4(syz, c, sy, i1)) + fd4(szz, c, sz, i2);
+            qc[4] = szz[q];
+            const float az = (fd4(sxz, c, 1, i0) + fd4(syz, c, sy, i1)) + fdq(qc, i2);
+            qshift(qc);
             vz[c] = vz[c] + tb * az;
         }
     }
@@ -265,275 +305,168 @@ __global__ void el_tick_kernel(int* tcount, int n, int delta)
 }
 
 // ---------------------------------------------------------------- adjoint
-// Reverse of one forward step t (single block, all global faces frozen, owned
-// cells = update region).  Exact transposes on the zero-framed grid: D^T = -D.
-// adjoint of the velocity stage: sh -= dt * D-form of w = b*vh, and
-// g_b += dt * vh . (D sigma'), sigma' = stresses after the stress stage.
-__global__ __launch_bounds__(256) EL_OCC void el_adj_velocity_kernel(const ElDev* __restrict__ tab, int nshot)
-{
-    const Tile T = decode_tile(tab, nshot);
-    const ElDev& d = tab[T.shot];
-    const KGeo& g = d.g;
-    const int h = g.h;
-    const int x = T.x, y = T.y;
-    if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
-    const long long sy = g.pitch, sz = g.slice;
-    const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
-    const long long boff = g.xoff;
-    const float* __restrict__ b = d.f[F_B] + boff;
-    const float* __restrict__ hx = d.adj[0] + boff;
-    const float* __restrict__ hy = d.adj[1] + boff;
-    const float* __restrict__ hz = d.adj[2] + boff;
-    float* sh[6];
-    const float* s1[6];
-    for (int k = 0; k < 6; ++k) {
-        sh[k] = d.adj[3 + k] + boff;
-        s1[k] = d.fw1[3 + k] + boff;
-    }
-    auto dw = [&](const float* hv, long long c, long long s, float ih) {
-        return (EL_C2 * (b[c + 2 * s] * hv[c + 2 * s] - b[c - 2 * s] * hv[c - 2 * s]) +
-                EL_C1 * (b[c + s] * hv[c + s] - b[c - s] * hv[c - s])) * ih;
-    };
-    const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
-    for (int z = z0; z <= z1; ++z) {
-        const long long c = z * sz + y * sy + x;
-        // g_b with the stresses after the stress stage (sxx syy szz sxy sxz syz)
-        const float ax = fd4(s1[0], c, 1, i0) + fd4(s1[3], c, sy, i1) + fd4(s1[4], c, sz, i2);
-        const float ay = fd4(s1[3], c, 1, i0) + fd4(s1[1], c, sy, i1) + fd4(s1[5], c, sz, i2);
-        const float az = fd4(s1[4], c, 1, i0) + fd4(s1[5], c, sy, i1) + fd4(s1[2], c, sz, i2);
-        d.grad[2][boff + c] += dt * (hx[c] * ax + hy[c] * ay + hz[c] * az);
-        sh[0][c] -= dt * dw(hx, c, 1, i0);
-        sh[1][c] -= dt * dw(hy, c, sy, i1);
-        sh[2][c] -= dt * dw(hz, c, sz, i2);
-        sh[3][c] -= dt * (dw(hx, c, sy, i1) + dw(hy, c, 1, i0));
-        sh[4][c] -= dt * (dw(hx, c, sz, i2) + dw(hz, c, 1, i0));
-        sh[5][c] -= dt * (dw(hy, c, sz, i2) + dw(hz, c, sy, i1));
-    }
-}
+// Adjoint of one forward step (single block, all global faces frozen, owned
+// cells = update region).  With E the strain-rate operator and M the force
+// divergence (M = -E^T on the zero-framed grid, D^T = -D), a step is
+//   sigma' = sigma + dt C E v,   v' = v + dt B M sigma'      (B = diag(b))
+// and its transpose, in the variables w = B vh and p = C sh (vh, sh the
+// adjoints of v and sigma), is the same scheme run with -dt:
+//   p <- p - dt C E w,           w <- w - dt B M p,
+// so the adjoint propagation reuses el_stress_kernel and el_velocity_kernel
+// on a table whose fields are (w, p) and whose dt is negated.  The kernels
+// below add the gradients,
+//   g_b      += dt (w/b) . M sigma'          (sigma' = stresses after the stress stage)
+//   g_lambda += dt tr(sh) tr(e),  g_mu += dt (2 sh_ii e_ii + sh_ij d_ij),
+//   sh = C^-1 p,  e = E v                     (v = velocity before the step)
+// and serve two algorithms:
+//   REV = false (plain): sigma' and v come from stored copies (fw1, fw0);
+//   REV = true (reverse sweep): d.f holds the state after the step and is
+//     reconstructed backwards in place, the scheme being time-reversible:
+//     v = v' - dt b M sigma' - dt b f(t) at the sources, then
+//     sigma = sigma' - dt C E v.  Nothing is stored between checkpoints; the
+//     checkpoint reset every ckpt steps bounds the rounding drift.
+// Both kernels write cell-local values only and read with a stencil only
+// fields they do not write, so they run in place; z neighbours come from
+// register queues rounded exactly as direct loads.
 
-// adjoint of the stress stage: vh -= dt * D-form of p = C sh, and the
-// lambda / mu gradients with the strain rates of the pre-step velocity.
-__global__ __launch_bounds__(256) EL_OCC void el_adj_stress_kernel(const ElDev* __restrict__ tab, int nshot)
-{
-    const Tile T = decode_tile(tab, nshot);
-    const ElDev& d = tab[T.shot];
-    const KGeo& g = d.g;
-    const int h = g.h;
-    const int x = T.x, y = T.y;
-    if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
-    const long long sy = g.pitch, sz = g.slice;
-    const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
-    const long long boff = g.xoff;
-    const float* __restrict__ lam = d.f[F_LAM] + boff;
-    const float* __restrict__ mu = d.f[F_MU] + boff;
-    const float* sh[6];
-    for (int k = 0; k < 6; ++k) sh[k] = d.adj[3 + k] + boff;
-    float* hx = d.adj[0] + boff;
-    float* hy = d.adj[1] + boff;
-    float* hz = d.adj[2] + boff;
-    const float* v0x = d.fw0[0] + boff;
-    const float* v0y = d.fw0[1] + boff;
-    const float* v0z = d.fw0[2] + boff;
-    // p components at cell q: 0 xx, 1 yy, 2 zz, 3 xy, 4 xz, 5 yz
-    auto pc = [&](int k, long long q) -> float {
-        const float l = lam[q], m = mu[q];
-        if (k < 3) {
-            const float s0 = sh[0][q], s1_ = sh[1][q], s2 = sh[2][q];
-            const float own = k == 0 ? s0 : (k == 1 ? s1_ : s2);
-            return (l + 2.0f * m) * own + l * (s0 + s1_ + s2 - own);
-        }
-        return m * sh[k][q];
-    };
-    auto dp = [&](int k, long long c, long long s, float ih) {
-        return (EL_C2 * (pc(k, c + 2 * s) - pc(k, c - 2 * s)) +
-                EL_C1 * (pc(k, c + s) - pc(k, c - s))) * ih;
-    };
-    const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
-    for (int z = z0; z <= z1; ++z) {
-        const long long c = z * sz + y * sy + x;
-        const float exx = fd4(v0x, c, 1, i0), eyy = fd4(v0y, c, sy, i1), ezz = fd4(v0z, c, sz, i2);
-        const float dxy = fd4(v0x, c, sy, i1) + fd4(v0y, c, 1, i0);
-        const float dxz = fd4(v0x, c, sz, i2) + fd4(v0z, c, 1, i0);
-        const float dyz = fd4(v0y, c, sz, i2) + fd4(v0z, c, sy, i1);
-        const float tr = exx + eyy + ezz;
-        const float a0 = sh[0][c], a1 = sh[1][c], a2 = sh[2][c];
-        d.grad[0][boff + c] += dt * ((a0 + a1 + a2) * tr);
-        d.grad[1][boff + c] += dt * (2.0f * (a0 * exx + a1 * eyy + a2 * ezz) + sh[3][c] * dxy +
-                                     sh[4][c] * dxz + sh[5][c] * dyz);
-        const float ux = dp(0, c, 1, i0) + dp(3, c, sy, i1) + dp(4, c, sz, i2);
-        const float uy = dp(3, c, 1, i0) + dp(1, c, sy, i1) + dp(5, c, sz, i2);
-        const float uz = dp(4, c, 1, i0) + dp(5, c, sy, i1) + dp(2, c, sz, i2);
-        hx[c] -= dt * ux;
-        hy[c] -= dt * uy;
-        hz[c] -= dt * uz;
-    }
-}
-
-// receivers of step t feed the adjoint velocity; sources feed g_b
-__global__ void el_adj_point_kernel(ElDev d, SrcRec sr, const float* __restrict__ res, int t)
-{
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    const KGeo& g = d.g;
-    if (idx < sr.nrec) {
-        const int* p = sr.rec + 3 * idx;
-        const long long c = cell_index(g, 0, p[0] + g.h, p[1] + g.h, p[2] + g.h);
-        d.adj[sr.rec_comp][c] += res[(long long)t * sr.nrec + idx];
-    }
-}
-
-__global__ void el_adj_source_kernel(ElDev d, SrcRec sr, int t)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= sr.nsrc || t >= sr.src_nt) return;
-    const KGeo& g = d.g;
-    const int* p = sr.src + 4 * i;
-    const long long c = cell_index(g, 0, p[0] + g.h, p[1] + g.h, p[2] + g.h);
-    d.grad[2][c] += d.dt * d.adj[p[3]][c] * sr.wav[(long long)t * sr.nsrc + i];
-}
-
-// ---------------------------------------------------------------- reverse sweep
-// The velocity-stress scheme is time-reversible.  From the state after step t,
-// (v', sigma'):  v = v' - dt b D.sigma' - dt b f(t) at the sources, then
-// sigma = sigma' - dt C:eps(v).  The batched gradient reverses each step this
-// way together with its adjoint, so nothing is stored between checkpoints;
-// the state is reset from a checkpoint every ckpt steps, which bounds the
-// rounding drift of the reconstruction.  Single block per shot, all global
-// faces frozen (the owned cells are the update region, the halo stays zero).
-// Each kernel writes only cell-local values and reads with a stencil only
-// fields it does not write, so both run in place.
-
-// velocity stage: a = D.sigma' (rounded as the forward velocity stage),
-//   v -= dt b a,  g_b += dt vh.a,  sh -= dt D-form(b vh)
-__global__ __launch_bounds__(256) EL_OCC void el_rev_velocity_kernel(const ElDev* __restrict__ tab,
-                                                                     int nshot)
+// velocity stage: a = M sigma' (rounded as the forward velocity stage);
+// g_b += dt (w/b).a;  REV: v -= dt b a
+template <bool REV>
+__global__ __launch_bounds__(EL_WG) void el_adj_vel_kernel(const ElDev* __restrict__ tab, int nshot)
 {
     const Tile T = decode_tile(tab, nshot);
     const ElDev& d = tab[T.shot];
     const KGeo& g = d.g;
     const int h = g.h, x = T.x, y = T.y;
     if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
+    const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
+    if (z0 > z1) return;
     const long long sy = g.pitch, sz = g.slice, boff = g.xoff;
     const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
-    const float* __restrict__ sxx = d.f[F_SXX] + boff;
-    const float* __restrict__ syy = d.f[F_SYY] + boff;
-    const float* __restrict__ szz = d.f[F_SZZ] + boff;
-    const float* __restrict__ sxy = d.f[F_SXY] + boff;
-    const float* __restrict__ sxz = d.f[F_SXZ] + boff;
-    const float* __restrict__ syz = d.f[F_SYZ] + boff;
+    float* const* sg = REV ? d.f + F_SXX : d.fw1 + F_SXX;
+    const float* __restrict__ sxx = sg[0] + boff;
+    const float* __restrict__ syy = sg[1] + boff;
+    const float* __restrict__ szz = sg[2] + boff;
+    const float* __restrict__ sxy = sg[3] + boff;
+    const float* __restrict__ sxz = sg[4] + boff;
+    const float* __restrict__ syz = sg[5] + boff;
     const float* __restrict__ b = d.f[F_B] + boff;
-    float* __restrict__ vx = d.f[F_VX] + boff;
-    float* __restrict__ vy = d.f[F_VY] + boff;
-    float* __restrict__ vz = d.f[F_VZ] + boff;
-    const float* __restrict__ hx = d.adj[0] + boff;
-    const float* __restrict__ hy = d.adj[1] + boff;
-    const float* __restrict__ hz = d.adj[2] + boff;
+    const float* __restrict__ wx = d.adj[0] + boff;
+    const float* __restrict__ wy = d.adj[1] + boff;
+    const float* __restrict__ wz = d.adj[2] + boff;
     float* __restrict__ gb = d.grad[2] + boff;
-    // D-form of w = b*vh: the transpose of the forward divergence
-    auto dw = [&](const float* __restrict__ hv, long long c, long long s, float ih) {
-        return (EL_C2 * (b[c + 2 * s] * hv[c + 2 * s] - b[c - 2 * s] * hv[c - 2 * s]) +
-                EL_C1 * (b[c + s] * hv[c + s] - b[c - s] * hv[c - s])) * ih;
-    };
-    const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
+    const long long cb = (long long)y * sy + x;
+    float qa[5], qb[5], qc[5];  // sxz, syz, szz along z
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long long q = (long long)(z0 - 2 + k) * sz + cb;
+        qa[k] = sxz[q];
+        qb[k] = syz[q];
+        qc[k] = szz[q];
+    }
     for (int z = z0; z <= z1; ++z) {
-        const long long c = z * sz + y * sy + x;
-        const float ax = (fd4(sxx, c, 1, i0) + fd4(sxy, c, sy, i1)) + fd4(sxz, c, sz, i2);
-        const float ay = (fd4(sxy, c, 1, i0) + fd4(syy, c, sy, i1)) + fd4(syz, c, sz, i2);
-        const float az = (fd4(sxz, c, 1, i0) + fd4(syz, c, sy, i1)) + fd4(szz, c, sz, i2);
-        const float wxx = dw(hx, c, 1, i0);
-        const float wyy = dw(hy, c, sy, i1);
-        const float wzz = dw(hz, c, sz, i2);
-        const float wxy = dw(hx, c, sy, i1) + dw(hy, c, 1, i0);
-        const float wxz = dw(hx, c, sz, i2) + dw(hz, c, 1, i0);
-        const float wyz = dw(hy, c, sz, i2) + dw(hz, c, sy, i1);
-        const float tb = dt * b[c];
-        gb[c] += dt * (hx[c] * ax + hy[c] * ay + hz[c] * az);
-        vx[c] = vx[c] - tb * ax;
-        vy[c] = vy[c] - tb * ay;
-        vz[c] = vz[c] - tb * az;
-        d.adj[3][boff + c] -= dt * wxx;
-        d.adj[4][boff + c] -= dt * wyy;
-        d.adj[5][boff + c] -= dt * wzz;
-        d.adj[6][boff + c] -= dt * wxy;
-        d.adj[7][boff + c] -= dt * wxz;
-        d.adj[8][boff + c] -= dt * wyz;
+        const long long c = z * sz + cb, q = c + 2 * sz;
+        qa[4] = sxz[q];
+        qb[4] = syz[q];
+        qc[4] = szz[q];
+        const float ax = (fd4(sxx, c, 1, i0) + fd4(sxy, c, sy, i1)) + fdq(qa, i2);
+        const float ay = (fd4(sxy, c, 1, i0) + fd4(syy, c, sy, i1)) + fdq(qb, i2);
+        const float az = (fd4(sxz, c, 1, i0) + fd4(syz, c, sy, i1)) + fdq(qc, i2);
+        qshift(qa);
+        qshift(qb);
+        qshift(qc);
+        const float bc = b[c];
+        gb[c] += dt * ((wx[c] * ax + wy[c] * ay + wz[c] * az) / bc);
+        if (REV) {
+            const float tb = dt * bc;
+            float* v = d.f[F_VX] + boff + c;
+            v[0] = v[0] - tb * ax;
+            v = d.f[F_VY] + boff + c;
+            v[0] = v[0] - tb * ay;
+            v = d.f[F_VZ] + boff + c;
+            v[0] = v[0] - tb * az;
+        }
     }
 }
 
-// stress stage: e = eps(v) (rounded as the forward stress stage),
-//   sigma -= dt C:e,  g_lambda, g_mu += dt (sh : dC/d. e),  vh -= dt D-form(C sh)
-__global__ __launch_bounds__(256) EL_OCC void el_rev_stress_kernel(const ElDev* __restrict__ tab,
-                                                                   int nshot)
+// stress stage: e = E v (rounded as the forward stress stage), sh = C^-1 p;
+// g_lambda, g_mu += ...;  REV: sigma -= dt C e
+template <bool REV>
+__global__ __launch_bounds__(EL_WG) void el_adj_str_kernel(const ElDev* __restrict__ tab, int nshot)
 {
     const Tile T = decode_tile(tab, nshot);
     const ElDev& d = tab[T.shot];
     const KGeo& g = d.g;
     const int h = g.h, x = T.x, y = T.y;
     if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
+    const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
+    if (z0 > z1) return;
     const long long sy = g.pitch, sz = g.slice, boff = g.xoff;
     const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
-    const float* __restrict__ vx = d.f[F_VX] + boff;
-    const float* __restrict__ vy = d.f[F_VY] + boff;
-    const float* __restrict__ vz = d.f[F_VZ] + boff;
+    float* const* vv = REV ? d.f : d.fw0;
+    const float* __restrict__ vx = vv[0] + boff;
+    const float* __restrict__ vy = vv[1] + boff;
+    const float* __restrict__ vz = vv[2] + boff;
     const float* __restrict__ lam = d.f[F_LAM] + boff;
     const float* __restrict__ mu = d.f[F_MU] + boff;
-    const float* __restrict__ hxx = d.adj[3] + boff;
-    const float* __restrict__ hyy = d.adj[4] + boff;
-    const float* __restrict__ hzz = d.adj[5] + boff;
-    const float* __restrict__ hxy = d.adj[6] + boff;
-    const float* __restrict__ hxz = d.adj[7] + boff;
-    const float* __restrict__ hyz = d.adj[8] + boff;
-    // (C sh) components at cell q
-    auto pxx = [&](long long q) { return (lam[q] + 2.0f * mu[q]) * hxx[q] + lam[q] * (hyy[q] + hzz[q]); };
-    auto pyy = [&](long long q) { return (lam[q] + 2.0f * mu[q]) * hyy[q] + lam[q] * (hxx[q] + hzz[q]); };
-    auto pzz = [&](long long q) { return (lam[q] + 2.0f * mu[q]) * hzz[q] + lam[q] * (hxx[q] + hyy[q]); };
-    auto pxy = [&](long long q) { return mu[q] * hxy[q]; };
-    auto pxz = [&](long long q) { return mu[q] * hxz[q]; };
-    auto pyz = [&](long long q) { return mu[q] * hyz[q]; };
-    auto dp = [&](auto pf, long long c, long long s, float ih) {
-        return (EL_C2 * (pf(c + 2 * s) - pf(c - 2 * s)) + EL_C1 * (pf(c + s) - pf(c - s))) * ih;
-    };
-    const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
+    const long long cb = (long long)y * sy + x;
+    float qx[5], qy[5], qz[5];  // v along z
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long long q = (long long)(z0 - 2 + k) * sz + cb;
+        qx[k] = vx[q];
+        qy[k] = vy[q];
+        qz[k] = vz[q];
+    }
     for (int z = z0; z <= z1; ++z) {
-        const long long c = z * sz + y * sy + x;
+        const long long c = z * sz + cb, q = c + 2 * sz;
+        qx[4] = vx[q];
+        qy[4] = vy[q];
+        qz[4] = vz[q];
         const float exx = fd4(vx, c, 1, i0);
         const float eyy = fd4(vy, c, sy, i1);
-        const float ezz = fd4(vz, c, sz, i2);
+        const float ezz = fdq(qz, i2);
         const float dxy = fd4(vx, c, sy, i1) + fd4(vy, c, 1, i0);
-        const float dxz = fd4(vx, c, sz, i2) + fd4(vz, c, 1, i0);
-        const float dyz = fd4(vy, c, sz, i2) + fd4(vz, c, sy, i1);
-        const float ux = (dp(pxx, c, 1, i0) + dp(pxy, c, sy, i1)) + dp(pxz, c, sz, i2);
-        const float uy = (dp(pxy, c, 1, i0) + dp(pyy, c, sy, i1)) + dp(pyz, c, sz, i2);
-        const float uz = (dp(pxz, c, 1, i0) + dp(pyz, c, sy, i1)) + dp(pzz, c, sz, i2);
+        const float dxz = fdq(qx, i2) + fd4(vz, c, 1, i0);
+        const float dyz = fdq(qy, i2) + fd4(vz, c, sy, i1);
+        qshift(qx);
+        qshift(qy);
+        qshift(qz);
         const float l = lam[c], m = mu[c];
-        const float l2m = l + 2.0f * m;
-        const float a0 = hxx[c], a1 = hyy[c], a2 = hzz[c];
-        const float tr = exx + eyy + ezz;
-        d.grad[0][boff + c] += dt * ((a0 + a1 + a2) * tr);
-        d.grad[1][boff + c] += dt * (2.0f * (a0 * exx + a1 * eyy + a2 * ezz) + hxy[c] * dxy +
-                                     hxz[c] * dxz + hyz[c] * dyz);
-        float* sg = d.f[F_SXX] + boff + c;
-        sg[0] = sg[0] - dt * (l2m * exx + l * (eyy + ezz));
-        sg = d.f[F_SYY] + boff + c;
-        sg[0] = sg[0] - dt * (l2m * eyy + l * (exx + ezz));
-        sg = d.f[F_SZZ] + boff + c;
-        sg[0] = sg[0] - dt * (l2m * ezz + l * (exx + eyy));
-        sg = d.f[F_SXY] + boff + c;
-        sg[0] = sg[0] - dt * (m * dxy);
-        sg = d.f[F_SXZ] + boff + c;
-        sg[0] = sg[0] - dt * (m * dxz);
-        sg = d.f[F_SYZ] + boff + c;
-        sg[0] = sg[0] - dt * (m * dyz);
-        d.adj[0][boff + c] -= dt * ux;
-        d.adj[1][boff + c] -= dt * uy;
-        d.adj[2][boff + c] -= dt * uz;
+        const long long ca = boff + c;
+        const float pxx = d.adj[3][ca], pyy = d.adj[4][ca], pzz = d.adj[5][ca];
+        const float trs = ((pxx + pyy) + pzz) / (3.0f * l + 2.0f * m);
+        const float im2 = 0.5f / m;
+        const float hxx = (pxx - l * trs) * im2, hyy = (pyy - l * trs) * im2,
+                    hzz = (pzz - l * trs) * im2;
+        const float hxy = d.adj[6][ca] / m, hxz = d.adj[7][ca] / m, hyz = d.adj[8][ca] / m;
+        d.grad[0][ca] += dt * (trs * ((exx + eyy) + ezz));
+        d.grad[1][ca] += dt * (2.0f * (hxx * exx + hyy * eyy + hzz * ezz) + hxy * dxy +
+                               hxz * dxz + hyz * dyz);
+        if (REV) {
+            const float l2m = l + 2.0f * m;
+            float* sgp = d.f[F_SXX] + ca;
+            sgp[0] = sgp[0] - dt * (l2m * exx + l * (eyy + ezz));
+            sgp = d.f[F_SYY] + ca;
+            sgp[0] = sgp[0] - dt * (l2m * eyy + l * (exx + ezz));
+            sgp = d.f[F_SZZ] + ca;
+            sgp[0] = sgp[0] - dt * (l2m * ezz + l * (exx + eyy));
+            sgp = d.f[F_SXY] + ca;
+            sgp[0] = sgp[0] - dt * (m * dxy);
+            sgp = d.f[F_SXZ] + ca;
+            sgp[0] = sgp[0] - dt * (m * dxz);
+            sgp = d.f[F_SYZ] + ca;
+            sgp[0] = sgp[0] - dt * (m * dyz);
+        }
     }
 }
 
-// Reverse of the point terms of step tcount[shot] (blockIdx.y = shot):
-// receivers feed the adjoint velocity (x < nrec), then (a second launch,
-// phase 1) sources are taken out of the state and feed g_b.
-__global__ void el_rev_point_kernel(const ElDev* __restrict__ tab, const int* __restrict__ tcount,
-                                    int phase)
+// Point terms of the reverse of step tcount[shot] (blockIdx.y = shot).
+// phase 0: receivers feed the adjoint, vh += res, i.e. w += b res;
+// phase 1: sources feed g_b += dt (w/b) f(t) and, REV, leave the state
+// (v -= dt b f(t)).
+__global__ void el_adj_point_kernel(const ElDev* __restrict__ tab, const int* __restrict__ tcount,
+                                    int phase, int rev)
 {
     const ElDev& d = tab[blockIdx.y];
     const SrcRec& sr = d.sr;
@@ -544,23 +477,49 @@ __global__ void el_rev_point_kernel(const ElDev* __restrict__ tab, const int* __
         if (i >= sr.nrec || t < 0 || t >= sr.rec_nt) return;
         const int* p = sr.rec + 3 * i;
         const long long c = cell_index(g, 0, p[0] + g.h, p[1] + g.h, p[2] + g.h);
-        d.adj[sr.rec_comp][c] += d.res[(long long)t * sr.nrec + i];
+        d.adj[sr.rec_comp][c] += d.f[F_B][c] * d.res[(long long)t * sr.nrec + i];
         return;
     }
     if (i >= sr.nsrc || t < 0 || t >= sr.src_nt) return;
     const int* p = sr.src + 4 * i;
     const long long c = cell_index(g, 0, p[0] + g.h, p[1] + g.h, p[2] + g.h);
-    const float w = sr.wav[(long long)t * sr.nsrc + i];
-    const float tb = d.dt * d.f[F_B][c];
-    float* v = d.f[p[3]];
-    v[c] = v[c] - tb * w;
-    d.grad[2][c] += d.dt * d.adj[p[3]][c] * w;
+    const float wv = sr.wav[(long long)t * sr.nsrc + i];
+    const float bc = d.f[F_B][c];
+    d.grad[2][c] += d.dt * ((d.adj[p[3]][c] / bc) * wv);
+    if (rev) {
+        float* v = d.f[p[3]];
+        v[c] = v[c] - (d.dt * bc) * wv;
+    }
+}
+
+// (w, p) -> (vh, sh) = (w / b, C^-1 p) on the owned cells, for returning the
+// adjoint state itself
+__global__ __launch_bounds__(EL_WG) void el_adj_unscale_kernel(const ElDev* __restrict__ tab, int nshot)
+{
+    const Tile T = decode_tile(tab, nshot);
+    const ElDev& d = tab[T.shot];
+    const KGeo& g = d.g;
+    const int h = g.h, x = T.x, y = T.y;
+    if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
+    const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
+    for (int z = z0; z <= z1; ++z) {
+        const long long c = g.xoff + z * g.slice + (long long)y * g.pitch + x;
+        const float l = d.f[F_LAM][c], m = d.f[F_MU][c], bc = d.f[F_B][c];
+        for (int k = 0; k < 3; ++k) d.adj[k][c] = d.adj[k][c] / bc;
+        const float pxx = d.adj[3][c], pyy = d.adj[4][c], pzz = d.adj[5][c];
+        const float trs = ((pxx + pyy) + pzz) / (3.0f * l + 2.0f * m);
+        const float im2 = 0.5f / m;
+        d.adj[3][c] = (pxx - l * trs) * im2;
+        d.adj[4][c] = (pyy - l * trs) * im2;
+        d.adj[5][c] = (pzz - l * trs) * im2;
+        for (int k = 6; k < 9; ++k) d.adj[k][c] = d.adj[k][c] / m;
+    }
 }
 
 // Checkpoint slot j-1 <-> state (owned cells of the 9 fields).  save: after
 // the forward reached step t = j*ckpt < nsteps; restore: before reversing
 // step t with t+1 = j*ckpt < nsteps.  Other steps leave everything alone.
-__global__ __launch_bounds__(256) void el_ckpt_kernel(const ElDev* __restrict__ tab, int nshot,
+__global__ __launch_bounds__(EL_WG) void el_ckpt_kernel(const ElDev* __restrict__ tab, int nshot,
                                                       const int* __restrict__ tcount, int ckpt,
                                                       int nsteps, int restore)
 {
@@ -613,16 +572,26 @@ __global__ void el_sum_kernel(const double* __restrict__ part, int n, double* __
 }
 
 // ---------------------------------------------------------------- host side
-static void el_tiles(ElDev& d)
+static void el_tiles(ElDev& d, int zc_want)
 {
     const KGeo& g = d.g;
-    d.ntx = ceil_div(g.S[0] - 4, EL_TX);
-    d.nty = ceil_div(g.S[1] - 4, EL_TY);
-    const int nz = g.S[2] - 4;
+    // Stage update boxes lie in [2, S-2); on an axis with one block and both
+    // global faces frozen they are the owned cells [h, h+B) at every stage.
+    for (int a = 0; a < 3; ++a) {
+        const bool owned = g.nb[a] == 1 && !g.periodic && ((g.fmask >> (2 * a)) & 3) == 3;
+        d.o[a] = owned ? g.h : 2;
+        d.w[a] = owned ? g.B[a] : g.S[a] - 4;
+    }
+    d.ntx = ceil_div(d.w[0], EL_TX);
+    d.nty = ceil_div(d.w[1], EL_TY);
+    const int nz = d.w[2];
     long long tiles = (long long)d.ntx * d.nty * g.nblk;
-    int nzc = (int)((8192 + tiles - 1) / tiles);
-    int zc = ceil_div(nz, nzc < 1 ? 1 : nzc);
-    if (zc < 8) zc = 8;
+    int zc = zc_want;
+    if (zc <= 0) {  // automatic: about 8192 workgroups, at least 8 planes each
+        const int nzc = (int)((8192 + tiles - 1) / tiles);
+        zc = ceil_div(nz, nzc < 1 ? 1 : nzc);
+        if (zc < 8) zc = 8;
+    }
     if (zc > nz) zc = nz;
     d.zc = zc;
     d.nzc = ceil_div(nz, zc);
@@ -665,7 +634,7 @@ extern "C" int b2r_el_create(b2r_ctx* ctx, float dt, const float inv_h[3], b2r_e
         }
         s->d.f[k] = (float*)s->fields[k];
     }
-    el_tiles(s->d);
+    el_tiles(s->d, ctx->tune.el_zc);
     *out = s;
     return B2R_OK;
 }
@@ -900,6 +869,7 @@ static int el_batch_init(ElBatch& bt, b2r_el_shot* const* shots, int nshots, boo
                   "all shots of a batch must be on one device");
         B2R_CHECK(s->ctx->g.R == R, B2R_E_ARG, "all shots of a batch must share R");
         s->d.g = s->ctx->k;
+        el_tiles(s->d, s->ctx->tune.el_zc);
         s->d.sr = el_srcrec(s);
         tab[k] = s->d;
         tab[k].tile0 = plain ? 0 : tiles;
@@ -1046,6 +1016,17 @@ static int copy_state(b2r_el_shot* s, hipStream_t st, float* const* dst, const f
     return B2R_OK;
 }
 
+// Table entry of the adjoint propagation: the forward kernels on (w, p)
+// with the step negated (see the adjoint kernels above).
+static ElDev el_adj_entry(const ElDev& d)
+{
+    ElDev a = d;
+    for (int k = 0; k < 9; ++k) a.f[k] = d.adj[k];
+    a.dt = -d.dt;
+    a.disp = nullptr;
+    return a;
+}
+
 static int el_grad_check(b2r_el_shot* s, int nsteps)
 {
     B2R_CHECK(s, B2R_E_ARG, "null shot");
@@ -1126,7 +1107,8 @@ static int el_gradient_plain(b2r_el_shot* s, const float* h_obs, int nsteps, int
     float* adj = (float*)alloc(fb * 9);
     float* grd = (float*)alloc(fb * 3);
     float* obs = (float*)alloc(nobs * 8 + 8 + 257 * sizeof(double));
-    ElDev* d_tab = (ElDev*)alloc(sizeof(ElDev));
+    // [state entry][adjoint entry][step counter]
+    ElDev* d_tab = (ElDev*)alloc(2 * sizeof(ElDev) + 64);
     if (!cp || !seg || !adj || !grd || !obs || !d_tab) {
         release();
         b2r_set_error("gradient: device allocation of %zu MB failed",
@@ -1159,7 +1141,14 @@ static int el_gradient_plain(b2r_el_shot* s, const float* h_obs, int nsteps, int
     d.g = c->k;
     for (int k = 0; k < 9; ++k) d.adj[k] = adj + k * fe;
     for (int k = 0; k < 3; ++k) d.grad[k] = grd + k * fe;
-    SrcRec sr = el_srcrec(s);
+    ElDev* d_atab = d_tab + 1;
+    int* d_tt = (int*)(d_tab + 2);
+    d.sr = el_srcrec(s);
+    d.res = res;
+    ElDev ad = el_adj_entry(d);
+    if (!rc && hipMemcpyAsync(d_atab, &ad, sizeof(ElDev), hipMemcpyHostToDevice, c->stream) !=
+                   hipSuccess)
+        rc = B2R_E_HIP;
     dim3 blk(EL_TX, EL_TY);
     for (int j = ncp - 1; j >= 0 && !rc; --j) {
         const int t0 = j * ckpt, t1 = std::min(nsteps, t0 + ckpt);
@@ -1182,10 +1171,16 @@ static int el_gradient_plain(b2r_el_shot* s, const float* h_obs, int nsteps, int
             slot(seg, t - t0, d.fw0);
             slot(seg, t - t0 + 1, d.fw1);
             B2R_HIP(hipMemcpyAsync(d_tab, &d, sizeof(ElDev), hipMemcpyHostToDevice, c->stream));
-            el_adj_point_kernel<<<ceil_div(s->nrec, 64), 64, 0, c->stream>>>(d, sr, res, t);
-            el_adj_velocity_kernel<<<d.ntiles, blk, 0, c->stream>>>(d_tab, 1);
-            if (s->nsrc) el_adj_source_kernel<<<ceil_div(s->nsrc, 64), 64, 0, c->stream>>>(d, sr, t);
-            el_adj_stress_kernel<<<d.ntiles, blk, 0, c->stream>>>(d_tab, 1);
+            B2R_HIP(hipMemcpyAsync(d_tt, &t, sizeof(int), hipMemcpyHostToDevice, c->stream));
+            el_adj_point_kernel<<<dim3(ceil_div(s->nrec, 64), 1), 64, 0, c->stream>>>(d_tab, d_tt, 0, 0);
+            el_adj_vel_kernel<false><<<d.ntiles, blk, 0, c->stream>>>(d_tab, 1);
+            if (s->nsrc)
+                el_adj_point_kernel<<<dim3(ceil_div(s->nsrc, 64), 1), 64, 0, c->stream>>>(d_tab, d_tt, 1, 0);
+            el_stress_kernel<<<d.ntiles, blk, 0, c->stream>>>(d_atab, 1, 0);
+            el_adj_str_kernel<false><<<d.ntiles, blk, 0, c->stream>>>(d_tab, 1);
+            el_velocity_kernel<1><<<d.ntiles, blk, 0, c->stream>>>(d_atab, 1, 1);
+            el_velocity_kernel<2><<<d.ntiles, blk, 0, c->stream>>>(d_atab, 1, 1);
+            el_velocity_kernel<4><<<d.ntiles, blk, 0, c->stream>>>(d_atab, 1, 1);
             // the table is rewritten next step: wait for these kernels first
             B2R_HIP(hipStreamSynchronize(c->stream));
             hipError_t e = hipGetLastError();
@@ -1196,6 +1191,10 @@ static int el_gradient_plain(b2r_el_shot* s, const float* h_obs, int nsteps, int
         }
     }
     s->t = nsteps;
+    if (!rc && h_adj0) {
+        el_adj_unscale_kernel<<<d.ntiles, blk, 0, c->stream>>>(d_tab, 1);
+        if (hipGetLastError() != hipSuccess) rc = B2R_E_HIP;
+    }
     if (!rc) rc = el_grad_out(s, grd, adj, h_glam, h_gmu, h_gb, h_adj0);
     if (!rc) *misfit = 0.5 * ms_h;
     release();
@@ -1270,6 +1269,25 @@ static int el_gradient_rev(b2r_el_shot* const* shots, int nshots, const float* c
         }
     }
     if (!rc) rc = el_batch_init(bt, shots, nshots, false);
+    // the adjoint-propagation table (same workgroup numbering as bt.d_tab)
+    ElDev* d_atab = nullptr;
+    if (!rc) {
+        std::vector<ElDev> at(nshots);
+        int t0 = 0;
+        for (int k = 0; k < nshots; ++k) {
+            at[k] = el_adj_entry(shots[k]->d);
+            at[k].tile0 = t0;
+            t0 += shots[k]->d.ntiles;
+        }
+        if (hipMalloc(&d_atab, sizeof(ElDev) * nshots) != hipSuccess) {
+            d_atab = nullptr;
+            b2r_set_error("gradient: table allocation failed");
+            rc = B2R_E_NOMEM;
+        } else if (hipMemcpyAsync(d_atab, at.data(), sizeof(ElDev) * nshots,
+                                  hipMemcpyHostToDevice, st) != hipSuccess) {
+            rc = B2R_E_HIP;
+        }
+    }
     if (!rc) rc = b2r_time_begin(c0, ms);
     const dim3 blk(EL_TX, EL_TY);
     // ---- forward: segments of ckpt steps, each followed by a checkpoint save
@@ -1284,13 +1302,15 @@ static int el_gradient_rev(b2r_el_shot* const* shots, int nshots, const float* c
         el_ckpt_kernel<<<bt.total_tiles, blk, 0, st>>>(bt.d_tab, nshots, bt.d_t, ckpt, nsteps, 1);
         for (int i = 0; i < n; ++i) {
             if (bt.max_rec)
-                el_rev_point_kernel<<<dim3(ceil_div(bt.max_rec, 64), nshots), 64, 0, st>>>(
-                    bt.d_tab, bt.d_t, 0);
+                el_adj_point_kernel<<<dim3(ceil_div(bt.max_rec, 64), nshots), 64, 0, st>>>(
+                    bt.d_tab, bt.d_t, 0, 1);
             if (bt.max_src)
-                el_rev_point_kernel<<<dim3(ceil_div(bt.max_src, 64), nshots), 64, 0, st>>>(
-                    bt.d_tab, bt.d_t, 1);
-            el_rev_velocity_kernel<<<bt.total_tiles, blk, 0, st>>>(bt.d_tab, nshots);
-            el_rev_stress_kernel<<<bt.total_tiles, blk, 0, st>>>(bt.d_tab, nshots);
+                el_adj_point_kernel<<<dim3(ceil_div(bt.max_src, 64), nshots), 64, 0, st>>>(
+                    bt.d_tab, bt.d_t, 1, 1);
+            el_adj_vel_kernel<true><<<bt.total_tiles, blk, 0, st>>>(bt.d_tab, nshots);
+            el_stress_kernel<<<bt.total_tiles, blk, 0, st>>>(d_atab, nshots, 0);
+            el_adj_str_kernel<true><<<bt.total_tiles, blk, 0, st>>>(bt.d_tab, nshots);
+            el_velocity_kernel<7><<<bt.total_tiles, blk, 0, st>>>(d_atab, nshots, 1);
             el_tick_kernel<<<1, 64, 0, st>>>(bt.d_t, nshots, -1);
         }
         return hipGetLastError() == hipSuccess ? B2R_OK : B2R_E_HIP;
@@ -1343,6 +1363,14 @@ static int el_gradient_rev(b2r_el_shot* const* shots, int nshots, const float* c
             for (int j = 0; j < nrest && !rc; ++j) rc = rev_segment(ckpt);
     }
     if (!rc) rc = b2r_time_end(c0, ms);
+    if (!rc && h_adj0) {
+        el_adj_unscale_kernel<<<bt.total_tiles, blk, 0, st>>>(bt.d_tab, nshots);
+        if (hipGetLastError() != hipSuccess) rc = B2R_E_HIP;
+    }
+    if (d_atab) {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(d_atab);
+    }
     if (!rc) B2R_HIP(hipStreamSynchronize(st));
     for (int k = 0; k < nshots && !rc; ++k) {
         b2r_el_shot* s = shots[k];

@@ -138,26 +138,33 @@ __global__ __launch_bounds__(256) void gol_step_kernel(const uint8_t* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// R generations in one launch: temporal blocking in LDS (B2R's deep halo on
-// chip).  A workgroup owns an output tile of 16*GF_TXQ cells x TY = GF_NR-2R
-// rows of one block and marches z through a chunk of planes.  For every input
-// plane z_in it computes generation t (t = 1..R) of plane z_in-t from
-// generation t-1, whose last three planes sit in an LDS ring over the tile
-// grown by R-t cells per side (here: one 16-cell quad per side in x, R-t rows
-// in y).  Each generation is two passes over 16-cell quads:
-//   A: P = sum over the 3 planes x 3 rows of the previous generation (SWAR)
-//   B: box = P + P(x-1) + P(x+1) via byte alignment with the neighbour
-//      quads' edge dwords, then the rule; written to the next ring or, for
-//      generation R, to global memory.
-// Cells outside a generation's update box (frozen global faces) keep their
-// previous value, as in the step-by-step path.  Halo cells of the output are
-// never written.
+// R generations in one launch: temporal blocking (B2R's deep halo on chip).
+// A workgroup owns an output tile of 16*GF_TXQ cells x TY = GF_NR-2R rows of
+// one block and marches z through a chunk of planes.  Each thread owns one
+// 16-cell quad position of the tile grown by one quad per side in x and R rows
+// per side in y, and keeps, for every generation t = 0..R-1, the last three
+// planes of its own position in registers.  For every input plane z_in it
+// computes generation t (t = 1..R) of plane p = z_in - t from generation t-1:
+//   zs  = sum of its three planes of generation t-1 (registers, SWAR bytes),
+//   one LDS plane of zs (double-buffered, one barrier per generation),
+//   box = zs(y-1) + zs + zs(y+1), plus the neighbour quads' edge dword
+//         columns for x-1 / x+1 via byte alignment, then the rule with the
+//         cell's own previous value; kept in registers for generation t+1 or,
+//         for generation R, written to global memory.
+// The valid region shrinks by one cell per generation, which the 16-cell
+// x halo and the R halo rows cover.  Cells outside a generation's update box
+// (frozen global faces) keep their previous value, as in the step-by-step
+// path.  Output quads start inside the owned block; when B_x is not a
+// multiple of 16 the last quad of a row also writes halo cells beyond B_x:
+// at a frozen face with their unchanged values, elsewhere with values the
+// next exchange overwrites.
 // ---------------------------------------------------------------------------
 constexpr int GF_TXQ = 8;            // output quads (16 cells) per tile row
 constexpr int GF_NQ = GF_TXQ + 2;    // plus one quad per side
 constexpr int GF_NR = 32;            // rows incl. R halo rows per side
 constexpr int GF_PLANE = GF_NQ * GF_NR;
-constexpr int GF_THREADS = GF_PLANE;  // one quad position per thread and pass
+constexpr int GF_THREADS = GF_PLANE;  // one quad position per thread
+constexpr int GF_PAD = GF_NQ + 1;     // LDS rows read at +-(NQ+1) stay in bounds
 
 __device__ __forceinline__ uint4 q_add(uint4 a, uint4 b)
 {
@@ -180,11 +187,11 @@ __global__ __launch_bounds__(GF_THREADS) void gol_fused_kernel(const uint8_t* __
                                                         int ntx, int nty, int nzc, int zc)
 {
     constexpr int TY = GF_NR - 2 * R;
-    extern __shared__ uint4 smem[];
-    uint4* ring = smem;                   // [R][3][GF_PLANE], level t-1 at ring[t-1]
-    uint4* pbuf = smem + R * 3 * GF_PLANE;
+    constexpr int ZSB = GF_PLANE + 2 * GF_PAD;
+    __shared__ uint4 zsbuf[2 * ZSB];
 
     const int tid = threadIdx.x;
+    const int r = tid / GF_NQ, q = tid % GF_NQ;
     int tt = xcd_tile(blockIdx.x, gridDim.x);
     const int txi = tt % ntx;
     tt /= ntx;
@@ -203,32 +210,18 @@ __global__ __launch_bounds__(GF_THREADS) void gol_fused_kernel(const uint8_t* __
     const long long slice = g.slice;
     const int pitch = g.pitch;
 
-    // local quad q covers x = x0 + 16*(q-1) .. +15; local row r is y = y0 - R + r
-    auto quad_ok = [&](int q) {
-        const int x = x0 + 16 * (q - 1);
-        return x + 15 >= 0 && x < g.S[0];
+    // this thread's quad: x = x .. x+15, row y
+    const int x = x0 + 16 * (q - 1);
+    const int y = y0 - R + r;
+    const bool ld_ok = y < g.S[1] && x + 15 >= 0 && x < g.S[0];
+    const uint8_t* __restrict__ col = src + (long long)y * pitch + x;
+    auto load = [&](int z) {
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (ld_ok) v = *reinterpret_cast<const uint4*>(col + (long long)z * slice);
+        return v;
     };
 
-    // level-0 plane loads: quads pos = tid, tid + GF_THREADS, .. of the plane
-    constexpr int NPF = (GF_PLANE + GF_THREADS - 1) / GF_THREADS;
-    uint4 pf[NPF];
-    auto load_plane = [&](int z) {
-        const long long pz = (long long)z * slice;
-#pragma unroll
-        for (int s = 0; s < NPF; ++s) {
-            const int pos = tid + GF_THREADS * s;
-            pf[s] = make_uint4(0, 0, 0, 0);
-            if (pos < GF_PLANE) {
-                const int r = pos / GF_NQ, q = pos % GF_NQ;
-                const int y = y0 - R + r;
-                if (y < g.S[1] && quad_ok(q))
-                    pf[s] = *reinterpret_cast<const uint4*>(src + pz + (long long)y * pitch +
-                                                            x0 + 16 * (q - 1));
-            }
-        }
-    };
-
-    // per-generation boxes (frozen faces) and whether the tile needs masking
+    // per-generation boxes (frozen faces) and this thread's update masks
     Box bx[R];
     bool clip[R];
 #pragma unroll
@@ -240,82 +233,65 @@ __global__ __launch_bounds__(GF_THREADS) void gol_fused_kernel(const uint8_t* __
                       y0 - gr < B_.lo[1] || y0 + TY + gr - 1 > B_.hi[1] ||
                       zc0 - gr < B_.lo[2] || zc1 + gr - 1 > B_.hi[2];
     }
+    const bool out_ok = q >= 1 && q <= GF_NQ - 2 && r >= R && r < R + TY && y < h + g.B[1] &&
+                        x < h + g.B[0];
 
-    load_plane(zc0 - R);
+    // gen[t][0..2]: generation t at planes (newest-2, newest-1, newest)
+    uint4 gen[R][3];
+    uint4 pf = load(zc0 - R);
+    int lc = 0;  // generations computed so far (selects the LDS buffer)
     for (int zin = zc0 - R; zin < zc1 + R; ++zin) {
-        uint4* r0 = ring + (zin % 3) * GF_PLANE;
-#pragma unroll
-        for (int s = 0; s < NPF; ++s) {
-            const int pos = tid + GF_THREADS * s;
-            if (pos < GF_PLANE) r0[pos] = pf[s];
-        }
-        if (zin + 1 < zc1 + R) load_plane(zin + 1);
-        __syncthreads();
+        gen[0][0] = gen[0][1];
+        gen[0][1] = gen[0][2];
+        gen[0][2] = pf;
+        if (zin + 1 < zc1 + R) pf = load(zin + 1);
 #pragma unroll
         for (int t = 1; t <= R; ++t) {
-            if (zin < zc0 - R + 2 * t) continue;  // uniform
+            if (zin < zc0 - R + 2 * t) continue;  // uniform: generation t-1 not yet 3 planes
             const int p = zin - t;
-            const uint4* prev = ring + (t - 1) * 3 * GF_PLANE;
-            const uint4* pm = prev + ((p + 2) % 3) * GF_PLANE;  // plane p-1
-            const uint4* pc = prev + (p % 3) * GF_PLANE;        // plane p
-            const uint4* pp = prev + ((p + 1) % 3) * GF_PLANE;  // plane p+1
-            const int nrow = GF_NR - 2 * t;
-            // pass A: 3x3 (z,y) sums of the previous generation
-            for (int pos = tid; pos < nrow * GF_NQ; pos += GF_THREADS) {
-                const int r = t + pos / GF_NQ, q = pos % GF_NQ;
-                const int i = r * GF_NQ + q;
-                uint4 s = q_add(q_add(pm[i - GF_NQ], pm[i]), pm[i + GF_NQ]);
-                s = q_add(s, q_add(q_add(pc[i - GF_NQ], pc[i]), pc[i + GF_NQ]));
-                s = q_add(s, q_add(q_add(pp[i - GF_NQ], pp[i]), pp[i + GF_NQ]));
-                pbuf[i] = s;
-            }
+            const uint4 zs = q_add(q_add(gen[t - 1][0], gen[t - 1][1]), gen[t - 1][2]);
+            uint4* buf = zsbuf + (lc & 1) * ZSB + GF_PAD;
+            ++lc;
+            buf[tid] = zs;
             __syncthreads();
-            // pass B: x neighbours and the rule
-            const int qa = t == R ? 1 : 0, qb = t == R ? GF_NQ - 1 : GF_NQ;
-            const int nq = qb - qa;
-            uint4* nxt = t < R ? ring + t * 3 * GF_PLANE + (p % 3) * GF_PLANE : nullptr;
-            for (int pos = tid; pos < nrow * nq; pos += GF_THREADS) {
-                const int r = t + pos / nq, q = qa + pos % nq;
-                const int i = r * GF_NQ + q;
-                const uint4 P = pbuf[i];
-                const uint32_t el = q > 0 ? pbuf[i - 1].w : 0u;
-                const uint32_t er = q < GF_NQ - 1 ? pbuf[i + 1].x : 0u;
-                const uint4 self = pc[i];
-                uint4 nw;
-                nw.x = gol_rule4(__builtin_amdgcn_alignbyte(P.x, el, 3) + P.x +
-                                 __builtin_amdgcn_alignbyte(P.y, P.x, 1), self.x);
-                nw.y = gol_rule4(__builtin_amdgcn_alignbyte(P.y, P.x, 3) + P.y +
-                                 __builtin_amdgcn_alignbyte(P.z, P.y, 1), self.y);
-                nw.z = gol_rule4(__builtin_amdgcn_alignbyte(P.z, P.y, 3) + P.z +
-                                 __builtin_amdgcn_alignbyte(P.w, P.z, 1), self.z);
-                nw.w = gol_rule4(__builtin_amdgcn_alignbyte(P.w, P.z, 3) + P.w +
-                                 __builtin_amdgcn_alignbyte(er, P.w, 1), self.w);
-                const int y = y0 - R + r;
-                const int x = x0 + 16 * (q - 1);
-                if (clip[t - 1]) {
-                    const Box& B_ = bx[t - 1];
-                    const bool in_yz = y >= B_.lo[1] && y <= B_.hi[1] && p >= B_.lo[2] &&
-                                       p <= B_.hi[2];
-                    uint32_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-                    if (in_yz) {
-                        m0 = dword_mask(x, B_.lo[0], B_.hi[0]);
-                        m1 = dword_mask(x + 4, B_.lo[0], B_.hi[0]);
-                        m2 = dword_mask(x + 8, B_.lo[0], B_.hi[0]);
-                        m3 = dword_mask(x + 12, B_.lo[0], B_.hi[0]);
-                    }
-                    nw.x = (nw.x & m0) | (self.x & ~m0);
-                    nw.y = (nw.y & m1) | (self.y & ~m1);
-                    nw.z = (nw.z & m2) | (self.z & ~m2);
-                    nw.w = (nw.w & m3) | (self.w & ~m3);
+            const uint4 P = q_add(q_add(buf[tid - GF_NQ], zs), buf[tid + GF_NQ]);
+            const uint32_t* bw = reinterpret_cast<const uint32_t*>(buf);
+            const int il = 4 * (tid - 1) + 3, ir = 4 * (tid + 1);
+            const uint32_t el = bw[il - 4 * GF_NQ] + bw[il] + bw[il + 4 * GF_NQ];
+            const uint32_t er = bw[ir - 4 * GF_NQ] + bw[ir] + bw[ir + 4 * GF_NQ];
+            const uint4 self = gen[t - 1][1];
+            uint4 nw;
+            nw.x = gol_rule4(__builtin_amdgcn_alignbyte(P.x, el, 3) + P.x +
+                             __builtin_amdgcn_alignbyte(P.y, P.x, 1), self.x);
+            nw.y = gol_rule4(__builtin_amdgcn_alignbyte(P.y, P.x, 3) + P.y +
+                             __builtin_amdgcn_alignbyte(P.z, P.y, 1), self.y);
+            nw.z = gol_rule4(__builtin_amdgcn_alignbyte(P.z, P.y, 3) + P.z +
+                             __builtin_amdgcn_alignbyte(P.w, P.z, 1), self.z);
+            nw.w = gol_rule4(__builtin_amdgcn_alignbyte(P.w, P.z, 3) + P.w +
+                             __builtin_amdgcn_alignbyte(er, P.w, 1), self.w);
+            if (clip[t - 1]) {
+                const Box& B_ = bx[t - 1];
+                const bool in_yz =
+                    y >= B_.lo[1] && y <= B_.hi[1] && p >= B_.lo[2] && p <= B_.hi[2];
+                uint32_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+                if (in_yz) {
+                    m0 = dword_mask(x, B_.lo[0], B_.hi[0]);
+                    m1 = dword_mask(x + 4, B_.lo[0], B_.hi[0]);
+                    m2 = dword_mask(x + 8, B_.lo[0], B_.hi[0]);
+                    m3 = dword_mask(x + 12, B_.lo[0], B_.hi[0]);
                 }
-                if (t < R) {
-                    nxt[i] = nw;
-                } else if (y < y0 + TY && y < h + g.B[1] && x < h + g.B[0]) {
-                    *reinterpret_cast<uint4*>(dst + (long long)p * slice + (long long)y * pitch +
-                                              x) = nw;
-                }
+                nw.x = (nw.x & m0) | (self.x & ~m0);
+                nw.y = (nw.y & m1) | (self.y & ~m1);
+                nw.z = (nw.z & m2) | (self.z & ~m2);
+                nw.w = (nw.w & m3) | (self.w & ~m3);
             }
-            __syncthreads();
+            if (t < R) {
+                gen[t][0] = gen[t][1];
+                gen[t][1] = gen[t][2];
+                gen[t][2] = nw;
+            } else if (out_ok) {
+                *reinterpret_cast<uint4*>(dst + (long long)p * slice + (long long)y * pitch + x) = nw;
+            }
         }
     }
 }
@@ -330,13 +306,6 @@ static int gol_fused_launch_t(b2r_ctx* ctx, const void* in, void* out)
 {
     constexpr int TY = GF_NR - 2 * R;
     const KGeo& g = ctx->k;
-    const size_t lds = (size_t)(R * 3 + 1) * GF_PLANE * sizeof(uint4);
-    static bool attr_set = false;
-    if (!attr_set) {
-        B2R_HIP(hipFuncSetAttribute((const void*)gol_fused_kernel<R>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        attr_set = true;
-    }
     const int ntx = ceil_div(g.B[0], 16 * GF_TXQ);
     const int nty = ceil_div(g.B[1], TY);
     long long tiles = (long long)ntx * nty * g.nblk;
@@ -350,8 +319,8 @@ static int gol_fused_launch_t(b2r_ctx* ctx, const void* in, void* out)
     if (zc > g.B[2]) zc = g.B[2];
     const int nzc = ceil_div(g.B[2], zc);
     dim3 grid(ntx * nty * nzc * g.nblk);
-    gol_fused_kernel<R><<<grid, GF_THREADS, lds, ctx->stream>>>((const uint8_t*)in, (uint8_t*)out,
-                                                               g, ntx, nty, nzc, zc);
+    gol_fused_kernel<R><<<grid, GF_THREADS, 0, ctx->stream>>>((const uint8_t*)in, (uint8_t*)out,
+                                                             g, ntx, nty, nzc, zc);
     B2R_LAUNCH_CHECK();
     return B2R_OK;
 }

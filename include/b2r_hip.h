@@ -97,6 +97,7 @@ typedef struct b2r_tuning {
     int heat_zc;      /* z planes marched per heat workgroup                        */
     int gol_zc;       /* z planes marched per fused Game of Life workgroup          */
     int gol_step_zc;  /* z planes marched per single-step Game of Life workgroup    */
+    int el_zc;        /* z planes marched per elastic workgroup                     */
 } b2r_tuning;
 
 /* ---- library / context ---------------------------------------------------- */
