@@ -165,6 +165,7 @@ constexpr int GF_NR = 32;            // rows incl. R halo rows per side
 constexpr int GF_PLANE = GF_NQ * GF_NR;
 constexpr int GF_THREADS = GF_PLANE;  // one quad position per thread
 constexpr int GF_PAD = GF_NQ + 1;     // LDS rows read at +-(NQ+1) stay in bounds
+constexpr int GF_PD = 2;              // input planes in flight per thread
 
 __device__ __forceinline__ uint4 q_add(uint4 a, uint4 b)
 {
@@ -238,13 +239,20 @@ __global__ __launch_bounds__(GF_THREADS) void gol_fused_kernel(const uint8_t* __
 
     // gen[t][0..2]: generation t at planes (newest-2, newest-1, newest)
     uint4 gen[R][3];
-    uint4 pf = load(zc0 - R);
+    // input planes are loaded GF_PD planes ahead of their use
+    const int zend = zc1 + R;
+    uint4 pf[GF_PD];
+#pragma unroll
+    for (int k = 0; k < GF_PD; ++k)
+        pf[k] = zc0 - R + k < zend ? load(zc0 - R + k) : make_uint4(0, 0, 0, 0);
     int lc = 0;  // generations computed so far (selects the LDS buffer)
-    for (int zin = zc0 - R; zin < zc1 + R; ++zin) {
+    for (int zin = zc0 - R; zin < zend; ++zin) {
         gen[0][0] = gen[0][1];
         gen[0][1] = gen[0][2];
-        gen[0][2] = pf;
-        if (zin + 1 < zc1 + R) pf = load(zin + 1);
+        gen[0][2] = pf[0];
+#pragma unroll
+        for (int k = 0; k + 1 < GF_PD; ++k) pf[k] = pf[k + 1];
+        if (zin + GF_PD < zend) pf[GF_PD - 1] = load(zin + GF_PD);
 #pragma unroll
         for (int t = 1; t <= R; ++t) {
             if (zin < zc0 - R + 2 * t) continue;  // uniform: generation t-1 not yet 3 planes

@@ -252,7 +252,7 @@ def cpu_gol_baseline(budget_s=10.0):
     return dict(value=round(512 ** 3 * gens / el / 1e9, 4), unit="GLUP/s",
                 cores=O.lib.orc_num_threads(), kind="port",
                 sample=f"512^3 periodic Game of Life, {gens} generations ({el:.1f} s), "
-                       "oracle C code, OpenMP over z")
+                       "oracle C code, OpenMP over z"), u, gens
 
 
 def cpu_heat_reference(steps=100):
@@ -273,7 +273,32 @@ def cpu_heat_reference(steps=100):
     el = time.perf_counter() - t0
     return dict(value=round(128 ** 3 * steps / el / 1e9, 4), unit="GLUP/s",
                 cores=O.lib.orc_num_threads(), kind="reference",
-                sample=f"128^3 heat delta1, {steps} steps ({el:.2f} s)")
+                sample=f"128^3 heat delta1, {steps} steps ({el:.2f} s)"), u
+
+
+def heat_reference_check(A, u_ref, steps=100):
+    """The GPU REF-mode path on the same 128^3 problem against the reference's
+    own CPU result (bitwise) and against the FP64 oracle."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    p = A.heat_params(1, A.HEAT_REF)
+    with A.Context((128, 128, 128), (1, 1, 1), 1, 1, A.BOUNDARY_FROZEN, 4) as c:
+        a, b = c.field(), c.field()
+        c.init_uniform(a, 0)
+        res, _ = c.heat_run(a, b, p, steps)
+        g = c.download(res)
+    ref = O.unframe(u_ref, 1)
+    u64 = O.unframe(O.heat_f64(O.frame(O.uniform((128, 128, 128), 0).astype(np.float64), 1), 1,
+                               [0.01, 0.01, 0.04], steps, False), 1)
+
+    def rel(x):
+        x = x.astype(np.float64)
+        return float(np.linalg.norm((x - u64).ravel()) / np.linalg.norm(u64.ravel()))
+    return {"bit_exact_vs_reference_cpu": bool(np.array_equal(g.view(np.uint32),
+                                                              ref.view(np.uint32))),
+            "gpu_rel_l2_vs_fp64": rel(g), "gpu_max_abs_vs_fp64": float(np.abs(g - u64).max()),
+            "reference_cpu_rel_l2_vs_fp64": rel(ref)}
 
 
 def cpu_elastic_baseline(steps=10):
@@ -295,6 +320,129 @@ def cpu_elastic_baseline(steps=10):
     return dict(value=round(128 ** 3 * steps / el / 1e9, 5), unit="GLUP/s",
                 cores=O.lib.orc_num_threads(), kind="port",
                 sample=f"128^3 elastic FP32, {steps} steps ({el:.2f} s), oracle C code, OpenMP")
+
+
+# ------------------------------------------------------------------ accuracy
+def heat_eigenmode_error(A, env, D, k, steps, blocks, R, fused, periodic):
+    """FP32 GPU heat against the closed form of an exact eigenmode (FP64):
+    frozen zero frame, delta1: u0 = prod sin(pi m_a i_a / (N_a + 1)); periodic,
+    any D: u0 = prod cos(2 pi m_a i_a / N_a).  Then u_n = lam^n u0 with lam
+    from the FP32 taps and diffusion numbers the kernel uses."""
+    import numpy as np
+    gx, gy, gz = env
+    k32 = [float(np.float32(k))] * 3
+    c32 = [float(np.float32(v)) for v in _heat_taps(D)]
+    if periodic:
+        th = [2.0 * np.pi / n for n in env]
+        modes = [np.cos(t * np.arange(n)) for t, n in zip(th, env)]
+        lam = 1.0 + sum(ka * (c32[0] + 2.0 * sum(c32[j] * np.cos(j * t) for j in range(1, D + 1)))
+                        for ka, t in zip(k32, th))
+    else:
+        assert D == 1
+        modes = [np.sin(np.pi * np.arange(1, n + 1) / (n + 1)) for n in env]
+        lam = 1.0 + sum(ka * (2.0 * np.cos(np.pi / (n + 1)) - 2.0) for ka, n in zip(k32, env))
+    sx, sy, sz = modes
+    plane = np.outer(sy, sx)
+    u0 = np.empty((gz, gy, gx), np.float32)
+    for z in range(gz):
+        u0[z] = (sz[z] * plane).astype(np.float32)
+    p = A.heat_params(D, A.HEAT_F32, k32)
+    bnd = A.BOUNDARY_PERIODIC if periodic else A.BOUNDARY_FROZEN
+    with A.Context(env, blocks, R, D, bnd, 4) as c:
+        a, b = c.field(), c.field()
+        c.upload(a, u0)
+        res, _ = c.heat_run(a, b, p, steps, fused=fused)
+        u = c.download(res)
+    del u0
+    lamn = lam ** steps
+    num = den = mx = 0.0
+    for z in range(gz):
+        ex = (lamn * sz[z]) * plane
+        d = u[z].astype(np.float64) - ex
+        num += float((d * d).sum())
+        den += float((ex * ex).sum())
+        mx = max(mx, float(np.abs(d).max()))
+    return {"rel_l2_vs_fp64": num ** 0.5 / den ** 0.5, "max_abs_vs_fp64": mx,
+            "max_abs_over_peak": mx / abs(lamn)}
+
+
+def _heat_taps(D):
+    return {1: (-2.0, 1.0), 2: (-2.5, 4 / 3, -1 / 12), 3: (-49 / 18, 1.5, -3 / 20, 1 / 90),
+            4: (-205 / 72, 1.6, -0.2, 8 / 315, -1 / 560)}[D]
+
+
+def gol_bitexact_check(A, grid_cpu, gens, R=4):
+    """The GPU path (512^3, 2x2x2, R-generation fused launches) against the
+    oracle's result after the same generations."""
+    import numpy as np
+    env = (512, 512, 512)
+    with A.Context(env, (2, 2, 2), R, 1, A.BOUNDARY_PERIODIC, 1) as c:
+        a, b = c.field(), c.field()
+        c.init_bernoulli(a, 1, int(0.3 * 16777216))
+        res, _ = c.gol_run(a, b, gens, fused=1)
+        g = c.download(res)
+    diff = int(np.count_nonzero(g != grid_cpu))
+    return {"generations": gens, "R": R, "cells_differing": diff, "bit_exact": diff == 0,
+            "population": int(g.sum(dtype=np.int64))}
+
+
+def elastic_checks(A, W):
+    """Elastic forward FP32 (GPU) against the oracle's FP32 (bitwise) and FP64
+    forward, and the batched gradient against the FP64 adjoint, on small grids."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import _elastic as E
+    out = {}
+    P = E.small_problem(env=(64, 56, 48), nsteps=200)
+    with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
+        s = A.ElasticShot(c, float(P["dt"]), P["ih"])
+        s.set_medium(P["lam"], P["mu"], P["b"])
+        s.set_sources(P["src"], P["comp"], P["wav"])
+        s.set_receivers(P["rec"], 1, P["nsteps"])
+        s.forward(P["nsteps"], 2)
+        rec = s.receivers()
+        vy = s.state(1)
+        s.close()
+    st32, rec32 = E.forward(P["env"], P["lam"], P["mu"], P["b"], P["dt"], P["ih"], P["nsteps"],
+                            P["src"], P["comp"], P["wav"], P["rec"], 1)
+    lam, mu, b = (P[k].astype(np.float64) for k in ("lam", "mu", "b"))
+    st64, rec64 = E.forward(P["env"], lam, mu, b, float(P["dt"]), P["ih"], P["nsteps"], P["src"],
+                            P["comp"], P["wav"].astype(np.float64), P["rec"], 1, fp64=True)
+
+    def rel(a, r):
+        a, r = np.asarray(a, np.float64), np.asarray(r, np.float64)
+        return float(np.linalg.norm((a - r).ravel()) / np.linalg.norm(r.ravel()))
+
+    out["forward_64x56x48_200_steps"] = {
+        "bitwise_vs_fp32_oracle": bool(np.array_equal(rec.view(np.uint32), rec32.view(np.uint32))
+                                       and np.array_equal(vy.view(np.uint32),
+                                                          st32[1].view(np.uint32))),
+        "receivers_rel_l2_vs_fp64": rel(rec, rec64),
+        "receivers_max_abs_vs_fp64": float(np.abs(rec.astype(np.float64) - rec64).max()),
+        "vy_rel_l2_vs_fp64": rel(vy, st64[1]),
+        "vy_max_abs_vs_fp64": float(np.abs(vy.astype(np.float64) - st64[1]).max())}
+    P = E.small_problem(env=(24, 22, 20), nsteps=60)
+    _, obs = E.forward(P["env"], P["lam"] * np.float32(1.03), P["mu"] * np.float32(0.98), P["b"],
+                       P["dt"], P["ih"], P["nsteps"], P["src"], P["comp"], P["wav"], P["rec"], 1)
+    with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
+        s = A.ElasticShot(c, float(P["dt"]), P["ih"])
+        s.set_medium(P["lam"], P["mu"], P["b"])
+        s.set_sources(P["src"], P["comp"], P["wav"])
+        s.set_receivers(P["rec"], 1, P["nsteps"])
+        mis, grads, _ = A.el_gradient_batch([s], [obs], P["nsteps"], 10, 2)
+        s.close()
+    lam, mu, b = (P[k].astype(np.float64) for k in ("lam", "mu", "b"))
+    _, r64 = E.forward(P["env"], lam, mu, b, float(P["dt"]), P["ih"], P["nsteps"], P["src"],
+                       P["comp"], P["wav"].astype(np.float64), P["rec"], 1, fp64=True)
+    g64 = E.adjoint(P["env"], lam, mu, b, float(P["dt"]), P["ih"], P["nsteps"], P["src"],
+                    P["comp"], P["wav"].astype(np.float64), P["rec"], 1, r64 - obs)
+    J64 = 0.5 * float(((r64 - obs) ** 2).sum())
+    out["gradient_24x22x20_60_steps_ckpt10"] = {
+        "misfit_rel_vs_fp64": abs(mis[0] - J64) / J64,
+        "g_lambda_rel_l2_vs_fp64_adjoint": rel(grads[0][0], g64[0]),
+        "g_mu_rel_l2_vs_fp64_adjoint": rel(grads[0][1], g64[1]),
+        "g_b_rel_l2_vs_fp64_adjoint": rel(grads[0][2], g64[2])}
+    return out
 
 
 def profile_kernels(A, R=2):
@@ -378,6 +526,7 @@ def main():
         traffic = float(tr[f"gol_fused_R{best['R']}_bytes_per_launch"])
 
     also = {}
+    heat_ref_grid = None
     if not args.quick:
         log("stream")
         bw = stream_bw(A)
@@ -413,7 +562,7 @@ def main():
         also["heat_delta1_128"] = h0
         ref = cpu_heat_reference()
         if ref:
-            also["heat_delta1_128_cpu_reference"] = ref
+            also["heat_delta1_128_cpu_reference"], heat_ref_grid = ref
         log("elastic 16-shot batch")
         also["elastic_16_shots"] = elastic_batch_bench(A, W, 16, args.elastic_steps)
         log(json.dumps(also["elastic_16_shots"]))
@@ -422,8 +571,28 @@ def main():
         log(json.dumps(also["elastic_gradient_8x256"]))
         also["elastic_cpu_port"] = cpu_elastic_baseline()
 
+    # cpu_baseline leg: the oracle timed on the host cores, and used as the
+    # checker of the GPU results
     log("cpu baseline")
-    cpu = cpu_gol_baseline(args.cpu_budget)
+    cpu, grid_cpu, gens = cpu_gol_baseline(args.cpu_budget)
+    acc = {"gol_512_vs_oracle": gol_bitexact_check(A, grid_cpu, gens, best["R"])}
+    del grid_cpu
+    log(json.dumps(acc))
+    if not args.quick:
+        log("accuracy: heat eigenmodes, elastic forward and gradient")
+        e1 = heat_eigenmode_error(A, (768, 768, 768), 1, 0.1, 100, (1, 1, 1), 1, 0, False)
+        e1.update(workload="heat delta1 768^3, 100 steps, zero Dirichlet, sine eigenmode")
+        acc["heat_delta1_768"] = e1
+        e2 = heat_eigenmode_error(A, (768, 768, 768), 1, 0.1, 100, (1, 1, 1), 2, 1, False)
+        e2.update(workload="the same through the fused R=2 kernel")
+        acc["heat_delta1_768_fused_R2"] = e2
+        e4 = heat_eigenmode_error(A, (768, 768, 768), 4, 0.05, 500, (2, 2, 2), 1, 0, True)
+        e4.update(workload="heat delta4 768^3, 2x2x2 blocks, 500 steps, periodic Fourier mode")
+        acc["heat_delta4_768"] = e4
+        if heat_ref_grid is not None:
+            acc["heat_delta1_128_100_steps"] = heat_reference_check(A, heat_ref_grid)
+        acc.update(elastic_checks(A, W))
+        log(json.dumps(acc))
     out = {
         "metric": METRIC,
         "value": round(best["glups"], 3),
@@ -455,6 +624,7 @@ def main():
                      "single_pass_achieved": round(plain["kernel_gbs_alg"], 1),
                      "single_pass_frac": round(plain["kernel_gbs_alg"] / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
+        "accuracy": acc,
         "also": also,
     }
     print(json.dumps(out), flush=True)
