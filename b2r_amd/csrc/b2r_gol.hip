@@ -190,6 +190,9 @@ __global__ __launch_bounds__(GF_THREADS) void gol_fused_kernel(const uint8_t* __
     constexpr int TY = GF_NR - 2 * R;
     constexpr int ZSB = GF_PLANE + 2 * GF_PAD;
     __shared__ uint4 zsbuf[2 * ZSB];
+    // the x and w dwords of zs again, densely, so that the neighbour-column
+    // reads of consecutive lanes hit consecutive banks
+    __shared__ uint32_t zxbuf[2 * ZSB], zwbuf[2 * ZSB];
 
     const int tid = threadIdx.x;
     const int r = tid / GF_NQ, q = tid % GF_NQ;
@@ -258,15 +261,19 @@ __global__ __launch_bounds__(GF_THREADS) void gol_fused_kernel(const uint8_t* __
             if (zin < zc0 - R + 2 * t) continue;  // uniform: generation t-1 not yet 3 planes
             const int p = zin - t;
             const uint4 zs = q_add(q_add(gen[t - 1][0], gen[t - 1][1]), gen[t - 1][2]);
-            uint4* buf = zsbuf + (lc & 1) * ZSB + GF_PAD;
+            const int bo = (lc & 1) * ZSB + GF_PAD;
+            uint4* buf = zsbuf + bo;
+            uint32_t* bx_ = zxbuf + bo;
+            uint32_t* bw_ = zwbuf + bo;
             ++lc;
             buf[tid] = zs;
+            bx_[tid] = zs.x;
+            bw_[tid] = zs.w;
             __syncthreads();
             const uint4 P = q_add(q_add(buf[tid - GF_NQ], zs), buf[tid + GF_NQ]);
-            const uint32_t* bw = reinterpret_cast<const uint32_t*>(buf);
-            const int il = 4 * (tid - 1) + 3, ir = 4 * (tid + 1);
-            const uint32_t el = bw[il - 4 * GF_NQ] + bw[il] + bw[il + 4 * GF_NQ];
-            const uint32_t er = bw[ir - 4 * GF_NQ] + bw[ir] + bw[ir + 4 * GF_NQ];
+            const int il = tid - 1, ir = tid + 1;
+            const uint32_t el = bw_[il - GF_NQ] + bw_[il] + bw_[il + GF_NQ];
+            const uint32_t er = bx_[ir - GF_NQ] + bx_[ir] + bx_[ir + GF_NQ];
             const uint4 self = gen[t - 1][1];
             uint4 nw;
             nw.x = gol_rule4(__builtin_amdgcn_alignbyte(P.x, el, 3) + P.x +

@@ -445,35 +445,38 @@ def elastic_checks(A, W):
     return out
 
 
-def profile_kernels(A, R=2):
+def profile_kernels(A):
     """A short fixed sequence for rocprofv3 counter runs: a copy kernel of known
     size (1 GiB read + 1 GiB written per launch, the scale for FETCH_SIZE /
     WRITE_SIZE), then 5 launches each of the fused Game of Life kernel (512^3,
-    2x2x2, R), the heat delta1 step kernel (768^3) and the elastic stress and
-    velocity kernels (256^3)."""
+    2x2x2, R = 2 and 4, the z chunks the autotuner picks), the heat delta1 step
+    kernel (768^3, RY = 8, 8-plane chunks) and the elastic stress and velocity
+    kernels (256^3, 64-plane chunks)."""
     nbytes = 1 << 30
     with A.Context((16, 16, 16), (1, 1, 1), 1, 1, 0, 4) as c:
         src, dst = c.dev_alloc(nbytes), c.dev_alloc(nbytes)
         log("stream_copy 1 GiB x3")
         c.stream_copy(dst, src, nbytes, 3)
-    with A.Context((512, 512, 512), (2, 2, 2), R, 1, A.BOUNDARY_PERIODIC, 1) as c:
-        a, b = c.field(), c.field()
-        c.init_bernoulli(a, 1, int(0.3 * 16777216))
-        c.boundary_updating_direct([a])
-        c.set_tuning(gol_zc=16)
-        log(f"gol_fused_kernel<{R}> x5")
-        c.time_steps(2, a, b, 5)
+    for R, zc in ((2, 32), (4, 64)):
+        with A.Context((512, 512, 512), (2, 2, 2), R, 1, A.BOUNDARY_PERIODIC, 1) as c:
+            a, b = c.field(), c.field()
+            c.init_bernoulli(a, 1, int(0.3 * 16777216))
+            c.boundary_updating_direct([a])
+            c.set_tuning(gol_zc=zc)
+            log(f"gol_fused_kernel<{R}> x5")
+            c.time_steps(2, a, b, 5)
     prm = A.heat_params(1, A.HEAT_F32, (0.1, 0.1, 0.1))
     with A.Context((768, 768, 768), (1, 1, 1), 1, 1, 0, 4) as c:
         a, b = c.field(), c.field()
         c.init_uniform(a, 0)
-        c.set_tuning(heat_ry=4, heat_zc=8)
+        c.set_tuning(heat_ry=8, heat_zc=8)
         log("heat_step_kernel<1> x5")
         c.time_steps(0, a, b, 5, prm)
     from b2r_amd import workloads as W
     env = (256, 256, 256)
     vp, vs, rho = W.layered_medium(env, 0)
     with A.Context(env, (1, 1, 1), 1, 4, 0, 4) as c:
+        c.set_tuning(el_zc=64)
         s = A.ElasticShot(c, W.cfl_dt(float(vp.max()), 10.0), (0.1, 0.1, 0.1))
         s.set_medium(*W.lame(vp, vs, rho))
         log("elastic stress + velocity x5")
@@ -515,6 +518,9 @@ def main():
         runs.append(gol_bench(A, R, args.steps, args.warmup, fused=1, record=rec))
         log(json.dumps(runs[-1]))
     best = max(runs, key=lambda r: r["glups"])
+    log("gol 512^3 R=3 fused (outside the configured R set, reported only)")
+    r3 = gol_bench(A, 3, args.steps, args.warmup, fused=1, record=rec)
+    log(json.dumps(r3))
     cells = 512 ** 3
     achieved = best["kernel_gbs_alg"]
     log("gol 512^3 R=1 step-by-step (plain)")
@@ -525,7 +531,7 @@ def main():
     if tr and f"gol_fused_R{best['R']}_bytes_per_launch" in tr:
         traffic = float(tr[f"gol_fused_R{best['R']}_bytes_per_launch"])
 
-    also = {}
+    also = {"gol_512_R3_fused": {k: r3[k] for k in ("glups", "kernel_ms", "zc")}}
     heat_ref_grid = None
     if not args.quick:
         log("stream")
