@@ -498,6 +498,8 @@ def main():
                     help="append the autotuner's choices to this file")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="run the fixed kernel sequence used by the rocprofv3 counter runs")
+    ap.add_argument("--only", choices=["elastic-gradient", "elastic-batch"],
+                    help="run one elastic section alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the halo "
@@ -509,6 +511,12 @@ def main():
 
     if args.profile_kernels:
         profile_kernels(A)
+        return
+    if args.only == "elastic-gradient":
+        print(json.dumps(elastic_gradient_bench(A, W, 8, args.gradient_steps)), flush=True)
+        return
+    if args.only == "elastic-batch":
+        print(json.dumps(elastic_batch_bench(A, W, 16, args.elastic_steps)), flush=True)
         return
     rec = args.tune_record or None
 
