@@ -122,6 +122,7 @@ _SIGS = {
     "b2r_el_gradient": (_i, [_vp, _vp, _i, _i, C.POINTER(C.c_double), _vp, _vp, _vp]),
     "b2r_el_gradient_ex": (_i, [_vp, _vp, _i, _i, _i, C.POINTER(C.c_double), _vp, _vp, _vp,
                                 _vp]),
+    "b2r_el_autotune": (_i, [_vp, C.c_char_p, C.POINTER(Tuning)]),
     "b2r_el_gradient_batch": (_i, [C.POINTER(_vp), _i, C.POINTER(_vp), _i, _i, _i,
                                    C.POINTER(C.c_double), C.POINTER(_vp), C.POINTER(_vp),
                                    C.POINTER(_vp), _pf]),
@@ -438,6 +439,12 @@ class ElasticShot:
         m = C.c_double(0.0)
         check(_lib.b2r_el_misfit(self.h, _ptr(obs), C.byref(m)))
         return m.value
+
+    def autotune(self, record=None):
+        """Pick the elastic z chunk for this shot's size (resets the shot)."""
+        t = Tuning()
+        check(_lib.b2r_el_autotune(self.h, record.encode() if record else None, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
     def gradient(self, obs, nsteps, ckpt, from_current=False, adjoint_state=False, plain=False):
         """misfit and [g_lam, g_mu, g_b] (plus the 9 adjoint fields at t=0).

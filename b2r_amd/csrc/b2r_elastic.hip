@@ -23,6 +23,7 @@
 // bit; the misfit is reduced in FP64.
 #include "b2r_common.h"
 
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -977,6 +978,54 @@ extern "C" int b2r_el_forward_batch(b2r_el_shot* const* shots, int nshots, int n
 extern "C" int b2r_el_forward(b2r_el_shot* s, int nsteps, int mode, float* ms)
 {
     return b2r_el_forward_batch(&s, 1, nsteps, mode, ms);
+}
+
+// Autotuner of the elastic stencil kernels (no counterpart in the reference):
+// times the batched step of this shot (stress + three-component velocity and
+// the point terms) for each z-chunk candidate, keeps the fastest in the
+// shot's context, appends one record line as b2r_autotune does, and leaves
+// the shot reset (zero state, t = 0).
+extern "C" int b2r_el_autotune(b2r_el_shot* s, const char* record_path, b2r_tuning* out)
+{
+    B2R_CHECK(s, B2R_E_ARG, "null shot");
+    b2r_ctx* c = s->ctx;
+    const b2r_tuning base = c->tune;
+    const int zcs[] = {0, 16, 32, 64, 128, 256};
+    const int warm = 2, iters = 4;
+    float best_ms = 1e30f;
+    int best = base.el_zc, tried = 0;
+    for (int zc : zcs) {
+        c->tune = base;
+        c->tune.el_zc = zc;
+        float ms = 0;
+        int rc = b2r_el_forward(s, warm, 1, nullptr);
+        if (!rc) rc = b2r_el_forward(s, iters, 1, &ms);
+        if (rc) {
+            c->tune = base;
+            (void)b2r_el_reset(s);
+            return rc;
+        }
+        ++tried;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = zc;
+        }
+    }
+    c->tune = base;
+    c->tune.el_zc = best;
+    el_tiles(s->d, best);
+    if (int rc = b2r_el_reset(s)) return rc;
+    if (out) *out = c->tune;
+    if (record_path && *record_path) {
+        FILE* f = fopen(record_path, "a");
+        B2R_CHECK(f, B2R_E_ARG, "cannot open %s", record_path);
+        fprintf(f, "kind=el_step S=%dx%dx%d blocks=%d R=%d radius=%d best=el_zc:%d "
+                   "ms_per_step=%.5f tried=%d\n",
+                c->L.S[0], c->L.S[1], c->L.S[2], c->L.nblocks, c->g.R, c->g.radius, best,
+                best_ms / iters, tried);
+        fclose(f);
+    }
+    return B2R_OK;
 }
 
 // ---------------------------------------------------------------- gradient

@@ -80,8 +80,8 @@ __global__ __launch_bounds__(256) void gol_step_kernel(const uint8_t* __restrict
     const int pitch = g.pitch;
     const int Sx = g.S[0];
 
-    // rows y0-1 .. y0+RY, planes z-1, z, z+1
-    uint32_t c[RY + 2][3], e[RY + 2][3];
+    // rows y0-1 .. y0+RY, planes z-1, z, z+1, and z+2 loaded a plane ahead
+    uint32_t c[RY + 2][4], e[RY + 2][4];
     bool rok[RY + 2];
 #pragma unroll
     for (int r = 0; r < RY + 2; ++r) {
@@ -102,9 +102,10 @@ __global__ __launch_bounds__(256) void gol_step_kernel(const uint8_t* __restrict
     };
     load_plane(zlo - 1, 0);
     load_plane(zlo, 1);
+    load_plane(zlo + 1, 2);
 
     for (int z = zlo; z <= zhi; ++z) {
-        load_plane(z + 1, 2);
+        if (z + 2 <= zhi + 1) load_plane(z + 2, 3);
         uint32_t zc_[RY + 2], ze_[RY + 2];
 #pragma unroll
         for (int r = 0; r < RY + 2; ++r) {
@@ -131,8 +132,10 @@ __global__ __launch_bounds__(256) void gol_step_kernel(const uint8_t* __restrict
         for (int r = 0; r < RY + 2; ++r) {
             c[r][0] = c[r][1];
             c[r][1] = c[r][2];
+            c[r][2] = c[r][3];
             e[r][0] = e[r][1];
             e[r][1] = e[r][2];
+            e[r][2] = e[r][3];
         }
     }
 }

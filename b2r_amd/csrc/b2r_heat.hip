@@ -110,6 +110,10 @@ __global__ __launch_bounds__(256) void heat_step_kernel(const float* __restrict_
             q[j][p] = (xin && yin[j]) ? src[pz + (long long)y[j] * pitch + x] : 0.0f;
     }
 
+    // halo values of the plane after the current one, loaded a plane ahead
+    float hv[NSLOT];
+#pragma unroll
+    for (int s = 0; s < NSLOT; ++s) hv[s] = hok[s] ? src[(long long)zlo * slice + ho[s]] : 0.0f;
     for (int z = zlo; z <= zhi; ++z) {
         const long long pz = (long long)z * slice;
         const long long pf = pz + (long long)D * slice;
@@ -121,7 +125,12 @@ __global__ __launch_bounds__(256) void heat_step_kernel(const float* __restrict_
         for (int j = 0; j < RY; ++j) lds[(wy * RY + j + D) * LX + lane + D] = q[j][D];
 #pragma unroll
         for (int s = 0; s < NSLOT; ++s)
-            if (hok[s]) lds[hl[s]] = src[pz + ho[s]];
+            if (hok[s]) lds[hl[s]] = hv[s];
+        if (z < zhi) {
+#pragma unroll
+            for (int s = 0; s < NSLOT; ++s)
+                if (hok[s]) hv[s] = src[pz + slice + ho[s]];
+        }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < RY; ++j) {

@@ -149,11 +149,13 @@ def elastic_shots(A, W, sizes, nsteps, seed=4):
     return ctxs, shots
 
 
-def elastic_batch_bench(A, W, nshots=16, nsteps=100):
+def elastic_batch_bench(A, W, nshots=16, nsteps=100, record=None):
     sizes = W.shot_sizes(nshots, seed=4)
     cells = sum(e[0] * e[1] * e[2] for e in sizes)
     ctxs, shots = elastic_shots(A, W, sizes, nsteps)
     try:
+        for s in shots:
+            s.autotune(record)
         out = {"shots": nshots, "steps": nsteps, "cells_per_step": cells,
                "sizes": ["%dx%dx%d" % e for e in sizes]}
         for s in shots:
@@ -182,7 +184,7 @@ def elastic_batch_bench(A, W, nshots=16, nsteps=100):
             c.close()
 
 
-def elastic_gradient_bench(A, W, nshots=8, nsteps=300, ckpt=50):
+def elastic_gradient_bench(A, W, nshots=8, nsteps=300, ckpt=50, record=None):
     """Misfit gradients of nshots 256^3 shots over one layered medium: 64x64
     surface receivers of v_y, observed data from the per-layer 5%-perturbed
     model, checkpoints every ckpt steps.  The batched reverse sweep (all shots
@@ -196,6 +198,7 @@ def elastic_gradient_bench(A, W, nshots=8, nsteps=300, ckpt=50):
     ctxs, shots = elastic_shots(A, W, [env] * nshots, nsteps, seed=9)
     try:
         for s in shots:
+            s.autotune(record)
             s.set_medium(*obs_m)
             s.reset()
         A.el_forward_batch(shots, nsteps, mode=2)
@@ -578,10 +581,11 @@ def main():
         if ref:
             also["heat_delta1_128_cpu_reference"], heat_ref_grid = ref
         log("elastic 16-shot batch")
-        also["elastic_16_shots"] = elastic_batch_bench(A, W, 16, args.elastic_steps)
+        also["elastic_16_shots"] = elastic_batch_bench(A, W, 16, args.elastic_steps, rec)
         log(json.dumps(also["elastic_16_shots"]))
         log("elastic gradient, 8 shots of 256^3")
-        also["elastic_gradient_8x256"] = elastic_gradient_bench(A, W, 8, args.gradient_steps)
+        also["elastic_gradient_8x256"] = elastic_gradient_bench(A, W, 8, args.gradient_steps,
+                                                                record=rec)
         log(json.dumps(also["elastic_gradient_8x256"]))
         also["elastic_cpu_port"] = cpu_elastic_baseline()
 

@@ -295,6 +295,11 @@ int b2r_time_steps(b2r_ctx* ctx, int kind, void* d_a, void* d_b, const b2r_heat_
  * with the geometry, the choice and its time per launch. */
 int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b, const b2r_heat_params* p,
                  const char* record_path, b2r_tuning* out);
+/* The elastic kernels' autotuner (no counterpart in the reference): times the
+ * batched step of one shot for each elastic z-chunk candidate, keeps the
+ * fastest (b2r_tuning.el_zc) in the shot's context, appends a record line as
+ * above, and leaves the shot reset (zero state, t = 0). */
+int b2r_el_autotune(b2r_el_shot* s, const char* record_path, b2r_tuning* out);
 /* Scratch device buffer owned by the context (freed by b2r_destroy). */
 int b2r_dev_alloc(b2r_ctx* ctx, size_t bytes, void** d_ptr);
 int b2r_dev_free(b2r_ctx* ctx, void* d_ptr);

@@ -286,3 +286,28 @@ def test_adjoint_dot_product():
               for k in range(9))
     say("dot product", lhs, 2 * J)
     assert abs(lhs - 2 * J) <= 1e-4 * abs(2 * J)
+
+
+def test_elastic_zchunk_tuning_and_autotune(tmp_path):
+    # the z chunk changes the launch shape only: results are bit-identical;
+    # the autotuner records its choice and leaves the shot reset
+    P = E.small_problem(env=(40, 36, 44), nsteps=12)
+    outs = []
+    with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
+        for zc in (0, 5, 16, 64):
+            c.set_tuning(el_zc=zc)
+            with _shot(c, P) as s:
+                s.forward(P["nsteps"], 1)
+                outs.append((s.receivers(), s.state(1)))
+        for rec, vy in outs[1:]:
+            assert bits_equal(rec, outs[0][0]) and bits_equal(vy, outs[0][1])
+        rec_file = str(tmp_path / "tune.txt")
+        with _shot(c, P) as s:
+            s.forward(3, 1)
+            t = s.autotune(rec_file)
+            assert t["el_zc"] in (0, 16, 32, 64, 128, 256)
+            assert not np.any(s.state(1))
+            s.forward(P["nsteps"], 2)
+            assert bits_equal(s.receivers(), outs[0][0])
+        line = open(rec_file).read()
+        assert "kind=el_step" in line and "best=el_zc:" in line
