@@ -10,8 +10,10 @@
 // B2R geometry: one step is two dependent radius-2 stages, so a block carries
 // a halo of 4R (radius 4 in the context geometry) and stage s of an R-step
 // window updates [2(s+1), S-1-2(s+1)] (the reference sizes the halo R*D with
-// D=2, b2r.cpp:61-64, which is too shallow for two radius-2 stages per step).  Updates are in place: the stress stage reads
-// v and writes sigma, the velocity stage reads sigma and writes v.
+// D=2, b2r.cpp:61-64, which is too shallow for two radius-2 stages per step).
+// Updates are in place: the stress stage reads v and writes sigma, the
+// velocity stage reads sigma and writes v.  A thread owns one (x, y) column of
+// a 64 x 4 tile and marches z; z neighbours come from register queues.
 //
 // Two forward paths run the same kernels (identical results):
 //   plain: per shot and step, stress + vx + vy + vz launches and a device
@@ -19,8 +21,12 @@
 //   batched: one stress and one three-component velocity launch per step for
 //          all shots of a batch (a device table maps workgroups to shots of
 //          different sizes), the time loop captured in HIP graphs.
+// Gradients (no reference counterpart): the adjoint propagates as the forward
+// scheme with -dt in scaled variables, plus kernels for the gradient terms;
+// the forward states come either from per-segment recomputation (plain) or
+// from running each step backwards (batched reverse sweep).
 // Arithmetic is FP32 without contraction and matches the CPU oracle bit for
-// bit; the misfit is reduced in FP64.
+// bit; misfits are reduced in FP64.
 #include "b2r_common.h"
 
 #include <cstdio>
@@ -55,7 +61,8 @@ struct ElDev {
     int ntx, nty, nzc, zc;
     int o[3], w[3];   // tiled region per axis: [o, o + w)
     int tile0;        // first workgroup of this shot in a batched launch
-    int ntiles;
+    int ntiles;       // tiles of this shot
+    int ngrid;        // its workgroups: ntiles rounded up to a multiple of 8
     // reverse sweep of the batched gradient
     SrcRec sr;        // sources and receivers
     const float* res; // receiver residuals, rec_nt x nrec
@@ -84,18 +91,31 @@ __device__ __forceinline__ float fd4(const float* __restrict__ p, long long c, l
     return (EL_C2 * (p[c + 2 * s] - p[c - 2 * s]) + EL_C1 * (p[c + s] - p[c - s])) * ih;
 }
 
-// workgroup -> (shot, block, tile) for batched launches
+// workgroup -> (shot, block, tile) for batched launches.  A device table of
+// n > 1 shots is followed by an int array holding each shot's first tile.
 struct Tile {
     int shot, b, x, y, zs, ze;
 };
 
 __device__ __forceinline__ Tile decode_tile(const ElDev* __restrict__ tab, int nshot)
 {
-    const int gt = xcd_tile(blockIdx.x, gridDim.x);
+    const int gt = blockIdx.x;
     int s = 0;
-    while (s + 1 < nshot && tab[s + 1].tile0 <= gt) ++s;
+    if (nshot > 1) {
+        // binary search of the compact first-tile array that follows the table
+        const int* __restrict__ t0 = reinterpret_cast<const int*>(tab + nshot);
+        int hi = nshot - 1;
+        while (s < hi) {
+            const int mid = (s + hi + 1) >> 1;
+            if (t0[mid] <= gt) s = mid;
+            else hi = mid - 1;
+        }
+    }
     const ElDev& d = tab[s];
-    int lt = gt - d.tile0;
+    // every shot starts on a multiple of 8 workgroups, so dispatch order spreads
+    // it over the 8 XCDs; xcd_tile gives each XCD a contiguous part of it.
+    // Tiles past ntiles decode to a block index >= nblk and are skipped.
+    int lt = xcd_tile(gt - d.tile0, d.ngrid);
     Tile T;
     T.shot = s;
     const int txi = lt % d.ntx;
@@ -337,6 +357,7 @@ __global__ __launch_bounds__(EL_WG) void el_adj_vel_kernel(const ElDev* __restri
 {
     const Tile T = decode_tile(tab, nshot);
     const ElDev& d = tab[T.shot];
+    if (T.b >= d.g.nblk) return;  // padding tile
     const KGeo& g = d.g;
     const int h = g.h, x = T.x, y = T.y;
     if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
@@ -397,6 +418,7 @@ __global__ __launch_bounds__(EL_WG) void el_adj_str_kernel(const ElDev* __restri
 {
     const Tile T = decode_tile(tab, nshot);
     const ElDev& d = tab[T.shot];
+    if (T.b >= d.g.nblk) return;  // padding tile
     const KGeo& g = d.g;
     const int h = g.h, x = T.x, y = T.y;
     if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
@@ -499,6 +521,7 @@ __global__ __launch_bounds__(EL_WG) void el_adj_unscale_kernel(const ElDev* __re
 {
     const Tile T = decode_tile(tab, nshot);
     const ElDev& d = tab[T.shot];
+    if (T.b >= d.g.nblk) return;  // padding tile
     const KGeo& g = d.g;
     const int h = g.h, x = T.x, y = T.y;
     if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
@@ -526,6 +549,7 @@ __global__ __launch_bounds__(EL_WG) void el_ckpt_kernel(const ElDev* __restrict_
 {
     const Tile T = decode_tile(tab, nshot);
     const ElDev& d = tab[T.shot];
+    if (T.b >= d.g.nblk) return;  // padding tile
     const KGeo& g = d.g;
     const int ts = tcount[T.shot] + (restore ? 1 : 0);
     if (ts % ckpt != 0 || ts < ckpt || ts >= nsteps) return;
@@ -597,6 +621,7 @@ static void el_tiles(ElDev& d, int zc_want)
     d.zc = zc;
     d.nzc = ceil_div(nz, zc);
     d.ntiles = d.ntx * d.nty * d.nzc * g.nblk;
+    d.ngrid = (d.ntiles + 7) & ~7;
 }
 
 static SrcRec el_srcrec(const b2r_el_shot* s)
@@ -808,7 +833,7 @@ static int el_step_launch(ElBatch& bt, int i, bool plain)
             // a one-entry table (tile0 = 0 for plain tables)
             const ElDev* tk = bt.d_tab + k;
             const b2r_el_shot* s = bt.shots[k];
-            const int nt = s->d.ntiles;
+            const int nt = s->d.ngrid;
             el_stress_kernel<<<nt, blk, 0, bt.stream>>>(tk, 1, 2 * i);
             el_velocity_kernel<1><<<nt, blk, 0, bt.stream>>>(tk, 1, 2 * i + 1);
             el_velocity_kernel<2><<<nt, blk, 0, bt.stream>>>(tk, 1, 2 * i + 1);
@@ -851,6 +876,19 @@ static int el_exchange(ElBatch& bt)
     return B2R_OK;
 }
 
+// Uploads a shot table followed by its first-tile array (see decode_tile).
+static int el_upload_table(const std::vector<ElDev>& tab, hipStream_t st, ElDev** d_tab)
+{
+    const size_t n = tab.size();
+    std::vector<int> t0(n);
+    for (size_t k = 0; k < n; ++k) t0[k] = tab[k].tile0;
+    const size_t bytes = sizeof(ElDev) * n + sizeof(int) * n;
+    B2R_CHECK(hipMalloc(d_tab, bytes) == hipSuccess, B2R_E_NOMEM, "table allocation failed");
+    B2R_HIP(hipMemcpyAsync(*d_tab, tab.data(), sizeof(ElDev) * n, hipMemcpyHostToDevice, st));
+    B2R_HIP(hipMemcpyAsync(*d_tab + n, t0.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    return B2R_OK;
+}
+
 // Device table of the shots (their ElDev with sources and receivers) and
 // step counters.  Batched tables number the workgroups of all shots
 // consecutively (tile0); plain tables give every shot tile0 = 0.
@@ -874,7 +912,7 @@ static int el_batch_init(ElBatch& bt, b2r_el_shot* const* shots, int nshots, boo
         s->d.sr = el_srcrec(s);
         tab[k] = s->d;
         tab[k].tile0 = plain ? 0 : tiles;
-        tiles += s->d.ntiles;
+        tiles += s->d.ngrid;
         bt.max_src = std::max(bt.max_src, s->nsrc * s->d.g.nblk);
         bt.max_rec = std::max(bt.max_rec, s->nrec);
         t0[k] = s->t;
@@ -884,9 +922,7 @@ static int el_batch_init(ElBatch& bt, b2r_el_shot* const* shots, int nshots, boo
     // work queued on the shots' own context streams comes first
     for (int k = 0; k < nshots; ++k)
         if (shots[k]->ctx->stream != bt.stream) B2R_HIP(hipStreamSynchronize(shots[k]->ctx->stream));
-    B2R_HIP(hipMalloc(&bt.d_tab, sizeof(ElDev) * nshots));
-    B2R_HIP(hipMemcpyAsync(bt.d_tab, tab.data(), sizeof(ElDev) * nshots, hipMemcpyHostToDevice,
-                           bt.stream));
+    if (int rc = el_upload_table(tab, bt.stream, &bt.d_tab)) return rc;
     B2R_HIP(hipMalloc(&bt.d_t, sizeof(int) * nshots));
     B2R_HIP(hipMemcpyAsync(bt.d_t, t0.data(), sizeof(int) * nshots, hipMemcpyHostToDevice,
                            bt.stream));
@@ -1222,14 +1258,14 @@ static int el_gradient_plain(b2r_el_shot* s, const float* h_obs, int nsteps, int
             B2R_HIP(hipMemcpyAsync(d_tab, &d, sizeof(ElDev), hipMemcpyHostToDevice, c->stream));
             B2R_HIP(hipMemcpyAsync(d_tt, &t, sizeof(int), hipMemcpyHostToDevice, c->stream));
             el_adj_point_kernel<<<dim3(ceil_div(s->nrec, 64), 1), 64, 0, c->stream>>>(d_tab, d_tt, 0, 0);
-            el_adj_vel_kernel<false><<<d.ntiles, blk, 0, c->stream>>>(d_tab, 1);
+            el_adj_vel_kernel<false><<<d.ngrid, blk, 0, c->stream>>>(d_tab, 1);
             if (s->nsrc)
                 el_adj_point_kernel<<<dim3(ceil_div(s->nsrc, 64), 1), 64, 0, c->stream>>>(d_tab, d_tt, 1, 0);
-            el_stress_kernel<<<d.ntiles, blk, 0, c->stream>>>(d_atab, 1, 0);
-            el_adj_str_kernel<false><<<d.ntiles, blk, 0, c->stream>>>(d_tab, 1);
-            el_velocity_kernel<1><<<d.ntiles, blk, 0, c->stream>>>(d_atab, 1, 1);
-            el_velocity_kernel<2><<<d.ntiles, blk, 0, c->stream>>>(d_atab, 1, 1);
-            el_velocity_kernel<4><<<d.ntiles, blk, 0, c->stream>>>(d_atab, 1, 1);
+            el_stress_kernel<<<d.ngrid, blk, 0, c->stream>>>(d_atab, 1, 0);
+            el_adj_str_kernel<false><<<d.ngrid, blk, 0, c->stream>>>(d_tab, 1);
+            el_velocity_kernel<1><<<d.ngrid, blk, 0, c->stream>>>(d_atab, 1, 1);
+            el_velocity_kernel<2><<<d.ngrid, blk, 0, c->stream>>>(d_atab, 1, 1);
+            el_velocity_kernel<4><<<d.ngrid, blk, 0, c->stream>>>(d_atab, 1, 1);
             // the table is rewritten next step: wait for these kernels first
             B2R_HIP(hipStreamSynchronize(c->stream));
             hipError_t e = hipGetLastError();
@@ -1241,7 +1277,7 @@ static int el_gradient_plain(b2r_el_shot* s, const float* h_obs, int nsteps, int
     }
     s->t = nsteps;
     if (!rc && h_adj0) {
-        el_adj_unscale_kernel<<<d.ntiles, blk, 0, c->stream>>>(d_tab, 1);
+        el_adj_unscale_kernel<<<d.ngrid, blk, 0, c->stream>>>(d_tab, 1);
         if (hipGetLastError() != hipSuccess) rc = B2R_E_HIP;
     }
     if (!rc) rc = el_grad_out(s, grd, adj, h_glam, h_gmu, h_gb, h_adj0);
@@ -1326,16 +1362,9 @@ static int el_gradient_rev(b2r_el_shot* const* shots, int nshots, const float* c
         for (int k = 0; k < nshots; ++k) {
             at[k] = el_adj_entry(shots[k]->d);
             at[k].tile0 = t0;
-            t0 += shots[k]->d.ntiles;
+            t0 += shots[k]->d.ngrid;
         }
-        if (hipMalloc(&d_atab, sizeof(ElDev) * nshots) != hipSuccess) {
-            d_atab = nullptr;
-            b2r_set_error("gradient: table allocation failed");
-            rc = B2R_E_NOMEM;
-        } else if (hipMemcpyAsync(d_atab, at.data(), sizeof(ElDev) * nshots,
-                                  hipMemcpyHostToDevice, st) != hipSuccess) {
-            rc = B2R_E_HIP;
-        }
+        rc = el_upload_table(at, st, &d_atab);
     }
     if (!rc) rc = b2r_time_begin(c0, ms);
     const dim3 blk(EL_TX, EL_TY);

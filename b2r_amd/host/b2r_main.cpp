@@ -1,17 +1,21 @@
 // b2r_main.cpp -- the B2R driver of b2r.cpp:125-250 on one GPU.
 //
 //   b2r_main R Bx By Bz [--model heat|gol|eqwp] [--env X Y Z] [--D d]
-//            [--iters N] [--periodic]
+//            [--iters N] [--periodic] [--dump FILE]
 //
 // Checks the arguments like b2r.cpp:132-167, then runs the time loop of
 // b2r.cpp:179-221: every R steps boundary_updating of the model's fields
 // (the nine eqwp fields, b2r.cpp:197-206), then R model steps, and reports the
 // timing lines of b2r.cpp:230-237.  All Bx*By*Bz blocks live on the one GPU.
+// --dump writes the final global owned grid (heat/gol: the model field; eqwp:
+// v_y), x fastest, raw cells, for checking the driver end to end; the
+// reference instead redirects its timing report to a file (b2r.cpp:82-97).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "b2r_host.h"
 
@@ -35,6 +39,7 @@ int main(int argc, char* argv[])
     BLOCK_DIM_Z = atoi(argv[4]);
     int n_iter = 600;  // N_ITER (b2r.h:12)
     std::string model = "heat";
+    const char* dump = nullptr;
     for (int i = 5; i < argc; ++i) {
         if (!strcmp(argv[i], "--env") && i + 3 < argc) {
             ENV_DIM_X = atoi(argv[++i]);
@@ -48,6 +53,8 @@ int main(int argc, char* argv[])
             model = argv[++i];
         } else if (!strcmp(argv[i], "--periodic")) {
             B2R_PERIODIC = 1;
+        } else if (!strcmp(argv[i], "--dump") && i + 1 < argc) {
+            dump = argv[++i];
         }
     }
     if (model == "gol") {
@@ -125,6 +132,18 @@ int main(int argc, char* argv[])
     printf("device computation took: %g seconds\n", compu_time);
     printf("halo synchronization took: %g seconds\n", sync_time);
     printf("GLUP/s: %g\n", (double)ENV_DIM_X * ENV_DIM_Y * ENV_DIM_Z * n_iter / total / 1e9);
+    if (dump) {
+        const size_t cells = (size_t)ENV_DIM_X * ENV_DIM_Y * ENV_DIM_Z;
+        std::vector<unsigned char> host(cells * B2R_ELEM_SIZE);
+        const void* f = model == "eqwp" ? (const void*)eqwp_field(1) : (const void*)cur;
+        FILE* fp = fopen(dump, "wb");
+        if (!fp || b2r_field_download(ctx, f, host.data()) < 0 ||
+            fwrite(host.data(), 1, host.size(), fp) != host.size()) {
+            fprintf(stderr, "B2R Error: dump %s: %s\n", dump, fp ? b2r_last_error() : "open");
+            return -1;
+        }
+        fclose(fp);
+    }
     if (model == "eqwp") eqwp_cuda_finalize();
     b2r_sync_finalize();
     return 0;

@@ -48,6 +48,16 @@ def read_traffic():
     return d
 
 
+def record_choice(path, what, best, tried):
+    """Append the measured choice of R (and of fusion) for a size to the
+    autotuner's record file, in its line format."""
+    if not path:
+        return
+    with open(path, "a") as f:
+        f.write("%s best=R:%s glups=%s\n" % (
+            what, best, ",".join("%s:%.1f" % (k, v) for k, v in tried.items())))
+
+
 # ------------------------------------------------------------------ Game of Life
 def gol_bench(A, R, steps, warmup, fused=1, zc=0, record=None):
     """steps generations of 512^3; kernel_ms = one launch of the dominant kernel
@@ -184,7 +194,8 @@ def elastic_batch_bench(A, W, nshots=16, nsteps=100, record=None):
             c.close()
 
 
-def elastic_gradient_bench(A, W, nshots=8, nsteps=300, ckpt=50, record=None):
+def elastic_gradient_bench(A, W, nshots=8, nsteps=300, ckpt=50, record=None,
+                           modes=((2, "batched_graph"), (1, "batched"), (0, "plain_loop"))):
     """Misfit gradients of nshots 256^3 shots over one layered medium: 64x64
     surface receivers of v_y, observed data from the per-layer 5%-perturbed
     model, checkpoints every ckpt steps.  The batched reverse sweep (all shots
@@ -208,7 +219,7 @@ def elastic_gradient_bench(A, W, nshots=8, nsteps=300, ckpt=50, record=None):
         out = {"shots": nshots, "grid": "256^3", "steps": nsteps, "ckpt": ckpt,
                "receivers": "64x64 surface, v_y"}
         res = {}
-        for mode, name in ((2, "batched_graph"), (1, "batched"), (0, "plain_loop")):
+        for mode, name in modes:
             t0 = time.perf_counter()
             mis, grads, ms = A.el_gradient_batch(shots, obs, nsteps, ckpt, mode, timed=True)
             wall = time.perf_counter() - t0
@@ -334,17 +345,7 @@ def heat_eigenmode_error(A, env, D, k, steps, blocks, R, fused, periodic):
     import numpy as np
     gx, gy, gz = env
     k32 = [float(np.float32(k))] * 3
-    c32 = [float(np.float32(v)) for v in _heat_taps(D)]
-    if periodic:
-        th = [2.0 * np.pi / n for n in env]
-        modes = [np.cos(t * np.arange(n)) for t, n in zip(th, env)]
-        lam = 1.0 + sum(ka * (c32[0] + 2.0 * sum(c32[j] * np.cos(j * t) for j in range(1, D + 1)))
-                        for ka, t in zip(k32, th))
-    else:
-        assert D == 1
-        modes = [np.sin(np.pi * np.arange(1, n + 1) / (n + 1)) for n in env]
-        lam = 1.0 + sum(ka * (2.0 * np.cos(np.pi / (n + 1)) - 2.0) for ka, n in zip(k32, env))
-    sx, sy, sz = modes
+    (sx, sy, sz), lam = heat_eigenmode(env, D, k32, periodic)
     plane = np.outer(sy, sx)
     u0 = np.empty((gz, gy, gx), np.float32)
     for z in range(gz):
@@ -367,6 +368,25 @@ def heat_eigenmode_error(A, env, D, k, steps, blocks, R, fused, periodic):
         mx = max(mx, float(np.abs(d).max()))
     return {"rel_l2_vs_fp64": num ** 0.5 / den ** 0.5, "max_abs_vs_fp64": mx,
             "max_abs_over_peak": mx / abs(lamn)}
+
+
+def heat_eigenmode(env, D, k, periodic):
+    """Separable exact eigenmode of the heat step on the owned grid (x, y, z
+    factors) and its eigenvalue, with the FP32 taps and diffusion numbers k
+    the kernel uses: frozen zero frame (delta1 only) sin(pi i/(N+1)),
+    periodic cos(2 pi i/N) for any D."""
+    import numpy as np
+    c32 = [float(np.float32(v)) for v in _heat_taps(D)]
+    if periodic:
+        th = [2.0 * np.pi / n for n in env]
+        modes = [np.cos(t * np.arange(n)) for t, n in zip(th, env)]
+        lam = 1.0 + sum(ka * (c32[0] + 2.0 * sum(c32[j] * np.cos(j * t) for j in range(1, D + 1)))
+                        for ka, t in zip(k, th))
+    else:
+        assert D == 1
+        modes = [np.sin(np.pi * np.arange(1, n + 1) / (n + 1)) for n in env]
+        lam = 1.0 + sum(ka * (2.0 * np.cos(np.pi / (n + 1)) - 2.0) for ka, n in zip(k, env))
+    return modes, lam
 
 
 def _heat_taps(D):
@@ -529,6 +549,8 @@ def main():
         runs.append(gol_bench(A, R, args.steps, args.warmup, fused=1, record=rec))
         log(json.dumps(runs[-1]))
     best = max(runs, key=lambda r: r["glups"])
+    record_choice(rec, "kind=gol_R S=512x512x512 blocks=2x2x2 periodic", best["R"],
+                  {r["R"]: r["glups"] for r in runs})
     log("gol 512^3 R=3 fused (outside the configured R set, reported only)")
     r3 = gol_bench(A, 3, args.steps, args.warmup, fused=1, record=rec)
     log(json.dumps(r3))
@@ -572,6 +594,11 @@ def main():
             h4runs.append(h4)
             log(json.dumps(h4))
         also["heat_delta4_768_2x2x2"] = max(h4runs, key=lambda r: r["glups"])
+        record_choice(rec, "kind=heat_R S=768x768x768 blocks=2x2x2 heat_D=4",
+                      "%d%s" % (also["heat_delta4_768_2x2x2"]["R"],
+                                "_fused" if also["heat_delta4_768_2x2x2"]["fused"] else ""),
+                      {"%d%s" % (r["R"], "_fused" if r["fused"] else ""): r["glups"]
+                       for r in h4runs})
         also["heat_delta4_768_2x2x2_tried"] = [
             {k: r[k] for k in ("R", "fused", "glups", "tuning")} for r in h4runs]
         h0 = heat_bench(A, (128, 128, 128), (1, 1, 1), 1, 1, (0.1, 0.1, 0.1), 100,
@@ -587,6 +614,11 @@ def main():
         also["elastic_gradient_8x256"] = elastic_gradient_bench(A, W, 8, args.gradient_steps,
                                                                 record=rec)
         log(json.dumps(also["elastic_gradient_8x256"]))
+        log("elastic gradient, 16 shots of 256^3")
+        also["elastic_gradient_16x256"] = elastic_gradient_bench(
+            A, W, 16, args.gradient_steps, record=rec,
+            modes=((2, "batched_graph"), (0, "plain_loop")))
+        log(json.dumps(also["elastic_gradient_16x256"]))
         also["elastic_cpu_port"] = cpu_elastic_baseline()
 
     # cpu_baseline leg: the oracle timed on the host cores, and used as the
