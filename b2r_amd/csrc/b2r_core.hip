@@ -186,6 +186,8 @@ extern "C" int b2r_set_tuning(b2r_ctx* ctx, const b2r_tuning* t)
               t->heat_ry == 8, B2R_E_ARG, "heat_ry must be 0, 1, 2, 4 or 8");
     B2R_CHECK(t->heat_zc >= 0 && t->gol_zc >= 0 && t->gol_step_zc >= 0 && t->el_zc >= 0, B2R_E_ARG,
               "negative z chunk");
+    B2R_CHECK(t->heat_wy == 0 || t->heat_wy == 4 || t->heat_wy == 8 || t->heat_wy == 16,
+              B2R_E_ARG, "heat_wy must be 0, 4, 8 or 16");
     ctx->tune = *t;
     return B2R_OK;
 }

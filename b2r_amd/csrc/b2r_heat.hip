@@ -21,18 +21,18 @@
 #include "b2r_heat_ops.h"
 
 constexpr int HEAT_TX = 64;
-constexpr int HEAT_WY = 4;
+constexpr int HEAT_WY = 4;  // thread rows (waves) of the fused kernel
 
-template <int D, int MODE, int RY>
-__global__ __launch_bounds__(256) void heat_step_kernel(const float* __restrict__ in,
+template <int D, int MODE, int RY, int WY>
+__global__ __launch_bounds__(HEAT_TX * WY) void heat_step_kernel(const float* __restrict__ in,
                                                         float* __restrict__ out, KGeo g,
                                                         HeatK hk, int b2r_i, int ntx, int nty,
                                                         int nzc, int zc)
 {
-    constexpr int TX = HEAT_TX, TY = HEAT_WY * RY;
+    constexpr int TX = HEAT_TX, TY = WY * RY, NT = TX * WY;
     constexpr int LX = TX + 2 * D, LY = TY + 2 * D;
     constexpr int NHY = 2 * D * TX, NHX = 2 * D * TY, NH = NHY + NHX;
-    constexpr int NSLOT = (NH + 255) / 256;
+    constexpr int NSLOT = (NH + NT - 1) / NT;
     constexpr int Q = 2 * D + 1;
     __shared__ float lds[LY * LX];
 
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(256) void heat_step_kernel(const float* __restrict_
     bool hok[NSLOT];
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) {
-        const int e = tid + s * 256;
+        const int e = tid + s * NT;
         hok[s] = false;
         hl[s] = 0;
         ho[s] = 0;
@@ -173,10 +173,10 @@ int b2r_heat_check(b2r_ctx* ctx, const b2r_heat_params* p)
     return B2R_OK;
 }
 
-template <int D, int MODE, int RY>
+template <int D, int MODE, int RY, int WY>
 static void launch_heat_ry(b2r_ctx* ctx, const float* in, float* out, const HeatK& hk, int i)
 {
-    constexpr int TY = HEAT_WY * RY;
+    constexpr int TY = WY * RY;
     const KGeo& g = ctx->k;
     const int ntx = ceil_div(g.S[0] - 2 * D, HEAT_TX);
     const int nty = ceil_div(g.S[1] - 2 * D, TY);
@@ -192,19 +192,30 @@ static void launch_heat_ry(b2r_ctx* ctx, const float* in, float* out, const Heat
     if (zc > nz) zc = nz;
     const int nzc = ceil_div(nz, zc);
     dim3 grid(ntx * nty * nzc * g.nblk);
-    dim3 block(HEAT_TX, HEAT_WY);
-    heat_step_kernel<D, MODE, RY><<<grid, block, 0, ctx->stream>>>(in, out, g, hk, i, ntx, nty,
-                                                                   nzc, zc);
+    dim3 block(HEAT_TX, WY);
+    heat_step_kernel<D, MODE, RY, WY><<<grid, block, 0, ctx->stream>>>(in, out, g, hk, i, ntx,
+                                                                       nty, nzc, zc);
 }
 
+template <int D, int MODE, int WY>
+static void launch_heat_wy(b2r_ctx* ctx, const float* in, float* out, const HeatK& hk, int i)
+{
+    switch (ctx->tune.heat_ry) {
+    case 1: launch_heat_ry<D, MODE, 1, WY>(ctx, in, out, hk, i); break;
+    case 2: launch_heat_ry<D, MODE, 2, WY>(ctx, in, out, hk, i); break;
+    case 8: launch_heat_ry<D, MODE, 8, WY>(ctx, in, out, hk, i); break;
+    default: launch_heat_ry<D, MODE, 4, WY>(ctx, in, out, hk, i); break;
+    }
+}
+
+// workgroup = 64 x-lanes x heat_wy waves (4 by default), each thread heat_ry rows
 template <int D, int MODE>
 static void launch_heat(b2r_ctx* ctx, const float* in, float* out, const HeatK& hk, int i)
 {
-    switch (ctx->tune.heat_ry) {
-    case 1: launch_heat_ry<D, MODE, 1>(ctx, in, out, hk, i); break;
-    case 2: launch_heat_ry<D, MODE, 2>(ctx, in, out, hk, i); break;
-    case 8: launch_heat_ry<D, MODE, 8>(ctx, in, out, hk, i); break;
-    default: launch_heat_ry<D, MODE, 4>(ctx, in, out, hk, i); break;
+    switch (ctx->tune.heat_wy) {
+    case 8: launch_heat_wy<D, MODE, 8>(ctx, in, out, hk, i); break;
+    case 16: launch_heat_wy<D, MODE, 16>(ctx, in, out, hk, i); break;
+    default: launch_heat_wy<D, MODE, 4>(ctx, in, out, hk, i); break;
     }
 }
 

@@ -24,13 +24,24 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
     const int zcs[] = {0, 8, 16, 32, 64, 128, 256};
     if (kind == 0 || kind == 3) {
         const int rys[] = {1, 2, 4, 8};
-        for (int ry : rys)
+        const int wys[] = {4, 8, 16};
+        if (kind == 3) {  // the fused kernel's tile is fixed; only its z chunk varies
             for (int zc : zcs) {
                 b2r_tuning t = base;
-                t.heat_ry = ry;
                 t.heat_zc = zc;
                 cand.push_back(t);
             }
+        } else {
+            for (int wy : wys)
+                for (int ry : rys)
+                    for (int zc : zcs) {
+                        b2r_tuning t = base;
+                        t.heat_ry = ry;
+                        t.heat_wy = wy;
+                        t.heat_zc = zc;
+                        cand.push_back(t);
+                    }
+        }
     } else {
         for (int zc : zcs) {
             b2r_tuning t = base;
@@ -66,9 +77,11 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
         static const char* names[] = {"heat_step", "gol_step", "gol_fused", "heat_fused"};
         fprintf(f, "kind=%s S=%dx%dx%d blocks=%d R=%d radius=%d", names[kind], ctx->L.S[0],
                 ctx->L.S[1], ctx->L.S[2], ctx->L.nblocks, ctx->g.R, ctx->g.radius);
-        if (kind == 0 || kind == 3) {
-            fprintf(f, " heat_D=%d best=heat_ry:%d,heat_zc:%d", p ? p->D : 0, best.heat_ry,
-                    best.heat_zc);
+        if (kind == 0) {
+            fprintf(f, " heat_D=%d best=heat_wy:%d,heat_ry:%d,heat_zc:%d", p ? p->D : 0,
+                    best.heat_wy, best.heat_ry, best.heat_zc);
+        } else if (kind == 3) {
+            fprintf(f, " heat_D=%d best=heat_zc:%d", p ? p->D : 0, best.heat_zc);
         } else {
             fprintf(f, " best=%s:%d", kind == 1 ? "gol_step_zc" : "gol_zc",
                     kind == 1 ? best.gol_step_zc : best.gol_zc);

@@ -124,7 +124,8 @@ def heat_bench(A, env, blocks, R, D, k, steps, init, fused=0, record=None, warmu
                 glups=round(cells * steps / (ms * 1e-3) / 1e9, 2),
                 kernel_ms=round(kms, 4), steps_per_launch=per_launch,
                 kernel_gbs_alg=round(8.0 * cells * per_launch / (kms * 1e-3) / 1e9, 1),
-                tuning={"heat_ry": tune["heat_ry"], "heat_zc": tune["heat_zc"]})
+                tuning={"heat_wy": tune["heat_wy"], "heat_ry": tune["heat_ry"],
+                        "heat_zc": tune["heat_zc"]})
 
 
 def blob_tables(env, nb=8, sigma=20.0, seed=2):
@@ -492,7 +493,7 @@ def profile_kernels(A):
     with A.Context((768, 768, 768), (1, 1, 1), 1, 1, 0, 4) as c:
         a, b = c.field(), c.field()
         c.init_uniform(a, 0)
-        c.set_tuning(heat_ry=8, heat_zc=8)
+        c.set_tuning(heat_wy=16, heat_ry=8, heat_zc=32)
         log("heat_step_kernel<1> x5")
         c.time_steps(0, a, b, 5, prm)
     from b2r_amd import workloads as W

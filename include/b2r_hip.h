@@ -98,6 +98,7 @@ typedef struct b2r_tuning {
     int gol_zc;       /* z planes marched per fused Game of Life workgroup          */
     int gol_step_zc;  /* z planes marched per single-step Game of Life workgroup    */
     int el_zc;        /* z planes marched per elastic workgroup                     */
+    int heat_wy;      /* waves (thread rows) per heat step workgroup: 4, 8 or 16   */
 } b2r_tuning;
 
 /* ---- library / context ---------------------------------------------------- */
