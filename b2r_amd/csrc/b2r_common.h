@@ -148,6 +148,26 @@ __device__ __forceinline__ int xcd_tile(int L, int N)
     return k * q + (k < r ? k : r) + i;
 }
 
+// Global-memory access by 32-bit byte offsets.  A stencil kernel takes a base
+// pointer that is uniform over its workgroup (scalar registers) and addresses
+// cells by unsigned byte offsets from it, so loads use the scalar-base form
+// and one offset register serves every field with the same layout.  The
+// caller keeps every offset a kernel forms below 4 GiB.  Casting to the
+// global address space also replaces flat instructions where the compiler
+// cannot tell that a pointer read from memory is device global memory.
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) char gbyte;
+__device__ __forceinline__ gfloat* G(float* p) { return (gfloat*)p; }
+__device__ __forceinline__ const gfloat* G(const float* p) { return (const gfloat*)p; }
+__device__ __forceinline__ float ldo(const gfloat* __restrict__ p, uint32_t o)
+{
+    return *(const gfloat*)((const gbyte*)p + o);
+}
+__device__ __forceinline__ void sto(gfloat* __restrict__ p, uint32_t o, float v)
+{
+    *(gfloat*)((gbyte*)p + o) = v;
+}
+
 // Element offset of local cell (x,y,z) of block b.
 __host__ __device__ inline long long cell_index(const KGeo& g, int b, int x, int y, int z)
 {

@@ -86,9 +86,16 @@ struct b2r_el_shot {
 constexpr float EL_C2 = -1.f / 12.f;
 constexpr float EL_C1 = 2.f / 3.f;
 
-__device__ __forceinline__ float fd4(const float* __restrict__ p, long long c, long long s, float ih)
+// Field pointers come from the shot table in memory; the stencil kernels cast
+// them to global memory (G) and address a block's cells by 32-bit byte offsets
+// from the block's base (ldo, sto).  b2r_el_create keeps a block's field below
+// 4 GiB.
+
+// first derivative at byte offset c along the axis of byte stride s
+__device__ __forceinline__ float fd4(const gfloat* __restrict__ p, uint32_t c, uint32_t s, float ih)
 {
-    return (EL_C2 * (p[c + 2 * s] - p[c - 2 * s]) + EL_C1 * (p[c + s] - p[c - s])) * ih;
+    return (EL_C2 * (ldo(p, c + 2 * s) - ldo(p, c - 2 * s)) + EL_C1 * (ldo(p, c + s) - ldo(p, c - s))) *
+           ih;
 }
 
 // workgroup -> (shot, block, tile) for batched launches.  A device table of
@@ -158,53 +165,51 @@ __global__ __launch_bounds__(EL_WG) void el_stress_kernel(const ElDev* __restric
     const int x = T.x, y = T.y;
     if (x < box.lo[0] || x > box.hi[0] || y < box.lo[1] || y > box.hi[1]) return;
     const long long boff = (long long)T.b * g.bstride + g.xoff;
-    const float* __restrict__ vx = d.f[F_VX] + boff;
-    const float* __restrict__ vy = d.f[F_VY] + boff;
-    const float* __restrict__ vz = d.f[F_VZ] + boff;
-    const float* __restrict__ lam = d.f[F_LAM] + boff;
-    const float* __restrict__ mu = d.f[F_MU] + boff;
-    float* __restrict__ sxx = d.f[F_SXX] + boff;
-    float* __restrict__ syy = d.f[F_SYY] + boff;
-    float* __restrict__ szz = d.f[F_SZZ] + boff;
-    float* __restrict__ sxy = d.f[F_SXY] + boff;
-    float* __restrict__ sxz = d.f[F_SXZ] + boff;
-    float* __restrict__ syz = d.f[F_SYZ] + boff;
-    const long long sy = g.pitch, sz = g.slice;
+    const gfloat* __restrict__ vx = G(d.f[F_VX] + boff);
+    const gfloat* __restrict__ vy = G(d.f[F_VY] + boff);
+    const gfloat* __restrict__ vz = G(d.f[F_VZ] + boff);
+    const gfloat* __restrict__ lam = G(d.f[F_LAM] + boff);
+    const gfloat* __restrict__ mu = G(d.f[F_MU] + boff);
+    gfloat* __restrict__ sxx = G(d.f[F_SXX] + boff);
+    gfloat* __restrict__ syy = G(d.f[F_SYY] + boff);
+    gfloat* __restrict__ szz = G(d.f[F_SZZ] + boff);
+    gfloat* __restrict__ sxy = G(d.f[F_SXY] + boff);
+    gfloat* __restrict__ sxz = G(d.f[F_SXZ] + boff);
+    gfloat* __restrict__ syz = G(d.f[F_SYZ] + boff);
+    const uint32_t sy = 4u * g.pitch, sz = 4u * (uint32_t)g.slice;  // byte strides
     const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
     const int z0 = max(T.zs, box.lo[2]), z1 = min(T.ze - 1, box.hi[2]);
     if (z0 > z1) return;
+    uint32_t c = (uint32_t)z0 * sz + (uint32_t)y * sy + 4u * x;
     float qx[5], qy[5], qz[5];  // v along z, planes z-2..z+2
-    {
-        const long long c0 = z0 * sz + y * sy + x;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            qx[k] = vx[c0 + (k - 2) * sz];
-            qy[k] = vy[c0 + (k - 2) * sz];
-            qz[k] = vz[c0 + (k - 2) * sz];
-        }
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t q = c - 2 * sz + k * sz;
+        qx[k] = ldo(vx, q);
+        qy[k] = ldo(vy, q);
+        qz[k] = ldo(vz, q);
     }
-    for (int z = z0; z <= z1; ++z) {
-        const long long c = z * sz + y * sy + x;
-        qx[4] = vx[c + 2 * sz];
-        qy[4] = vy[c + 2 * sz];
-        qz[4] = vz[c + 2 * sz];
-        const float exx = fd4(vx, c, 1, i0);
+    for (int z = z0; z <= z1; ++z, c += sz) {
+        qx[4] = ldo(vx, c + 2 * sz);
+        qy[4] = ldo(vy, c + 2 * sz);
+        qz[4] = ldo(vz, c + 2 * sz);
+        const float exx = fd4(vx, c, 4, i0);
         const float eyy = fd4(vy, c, sy, i1);
         const float ezz = fdq(qz, i2);
-        const float dxy = fd4(vx, c, sy, i1) + fd4(vy, c, 1, i0);
-        const float dxz = fdq(qx, i2) + fd4(vz, c, 1, i0);
+        const float dxy = fd4(vx, c, sy, i1) + fd4(vy, c, 4, i0);
+        const float dxz = fdq(qx, i2) + fd4(vz, c, 4, i0);
         const float dyz = fdq(qy, i2) + fd4(vz, c, sy, i1);
         qshift(qx);
         qshift(qy);
         qshift(qz);
-        const float l = lam[c], m = mu[c];
+        const float l = ldo(lam, c), m = ldo(mu, c);
         const float l2m = l + 2.0f * m;
-        sxx[c] = sxx[c] + dt * (l2m * exx + l * (eyy + ezz));
-        syy[c] = syy[c] + dt * (l2m * eyy + l * (exx + ezz));
-        szz[c] = szz[c] + dt * (l2m * ezz + l * (exx + eyy));
-        sxy[c] = sxy[c] + dt * (m * dxy);
-        sxz[c] = sxz[c] + dt * (m * dxz);
-        syz[c] = syz[c] + dt * (m * dyz);
+        sto(sxx, c, ldo(sxx, c) + dt * (l2m * exx + l * (eyy + ezz)));
+        sto(syy, c, ldo(syy, c) + dt * (l2m * eyy + l * (exx + ezz)));
+        sto(szz, c, ldo(szz, c) + dt * (l2m * ezz + l * (exx + eyy)));
+        sto(sxy, c, ldo(sxy, c) + dt * (m * dxy));
+        sto(sxz, c, ldo(sxz, c) + dt * (m * dxz));
+        sto(syz, c, ldo(syz, c) + dt * (m * dyz));
     }
 }
 
@@ -221,53 +226,53 @@ __global__ __launch_bounds__(EL_WG) void el_velocity_kernel(const ElDev* __restr
     const int x = T.x, y = T.y;
     if (x < box.lo[0] || x > box.hi[0] || y < box.lo[1] || y > box.hi[1]) return;
     const long long boff = (long long)T.b * g.bstride + g.xoff;
-    const float* __restrict__ sxx = d.f[F_SXX] + boff;
-    const float* __restrict__ syy = d.f[F_SYY] + boff;
-    const float* __restrict__ szz = d.f[F_SZZ] + boff;
-    const float* __restrict__ sxy = d.f[F_SXY] + boff;
-    const float* __restrict__ sxz = d.f[F_SXZ] + boff;
-    const float* __restrict__ syz = d.f[F_SYZ] + boff;
-    const float* __restrict__ bb = d.f[F_B] + boff;
-    float* __restrict__ vx = d.f[F_VX] + boff;
-    float* __restrict__ vy = d.f[F_VY] + boff;
-    float* __restrict__ vz = d.f[F_VZ] + boff;
-    float* __restrict__ disp = d.disp ? d.disp + boff : nullptr;
-    const long long sy = g.pitch, sz = g.slice;
+    const gfloat* __restrict__ sxx = G(d.f[F_SXX] + boff);
+    const gfloat* __restrict__ syy = G(d.f[F_SYY] + boff);
+    const gfloat* __restrict__ szz = G(d.f[F_SZZ] + boff);
+    const gfloat* __restrict__ sxy = G(d.f[F_SXY] + boff);
+    const gfloat* __restrict__ sxz = G(d.f[F_SXZ] + boff);
+    const gfloat* __restrict__ syz = G(d.f[F_SYZ] + boff);
+    const gfloat* __restrict__ bb = G(d.f[F_B] + boff);
+    gfloat* __restrict__ vx = G(d.f[F_VX] + boff);
+    gfloat* __restrict__ vy = G(d.f[F_VY] + boff);
+    gfloat* __restrict__ vz = G(d.f[F_VZ] + boff);
+    gfloat* __restrict__ disp = d.disp ? G(d.disp + boff) : nullptr;
+    const uint32_t sy = 4u * g.pitch, sz = 4u * (uint32_t)g.slice;  // byte strides
     const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
     const int z0 = max(T.zs, box.lo[2]), z1 = min(T.ze - 1, box.hi[2]);
     if (z0 > z1) return;
+    uint32_t c = (uint32_t)z0 * sz + (uint32_t)y * sy + 4u * x;
     // z-derivative fields (sxz for vx, syz for vy, szz for vz) from register
     // queues over planes z-2..z+2, rounded exactly as direct loads
     float qa[5], qb[5], qc[5];
-    const long long cb = (long long)y * sy + x;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const long long q = (long long)(z0 - 2 + k) * sz + cb;
-        if (MASK & 1) qa[k] = sxz[q];
-        if (MASK & 2) qb[k] = syz[q];
-        if (MASK & 4) qc[k] = szz[q];
+        const uint32_t q = c - 2 * sz + k * sz;
+        if (MASK & 1) qa[k] = ldo(sxz, q);
+        if (MASK & 2) qb[k] = ldo(syz, q);
+        if (MASK & 4) qc[k] = ldo(szz, q);
     }
-    for (int z = z0; z <= z1; ++z) {
-        const long long c = z * sz + cb, q = c + 2 * sz;
-        const float tb = dt * bb[c];
+    for (int z = z0; z <= z1; ++z, c += sz) {
+        const uint32_t q = c + 2 * sz;
+        const float tb = dt * ldo(bb, c);
         if (MASK & 1) {
-            qa[4] = sxz[q];
-            const float ax = (fd4(sxx, c, 1, i0) + fd4(sxy, c, sy, i1)) + fdq(qa, i2);
+            qa[4] = ldo(sxz, q);
+            const float ax = (fd4(sxx, c, 4, i0) + fd4(sxy, c, sy, i1)) + fdq(qa, i2);
             qshift(qa);
-            vx[c] = vx[c] + tb * ax;
+            sto(vx, c, ldo(vx, c) + tb * ax);
         }
         if (MASK & 2) {
-            qb[4] = syz[q];
-            const float ay = (fd4(sxy, c, 1, i0) + fd4(syy, c, sy, i1)) + fdq(qb, i2);
+            qb[4] = ldo(syz, q);
+            const float ay = (fd4(sxy, c, 4, i0) + fd4(syy, c, sy, i1)) + fdq(qb, i2);
             qshift(qb);
-            vy[c] = vy[c] + tb * ay;
-            if (disp) disp[c] = disp[c] + (dt * tb) * ay;
+            sto(vy, c, ldo(vy, c) + tb * ay);
+            if (disp) sto(disp, c, ldo(disp, c) + (dt * tb) * ay);
         }
         if (MASK & 4) {
-            qc[4] = szz[q];
-            const float az = (fd4(sxz, c, 1, i0) + fd4(syz, c, sy, i1)) + fdq(qc, i2);
+            qc[4] = ldo(szz, q);
+            const float az = (fd4(sxz, c, 4, i0) + fd4(syz, c, sy, i1)) + fdq(qc, i2);
             qshift(qc);
-            vz[c] = vz[c] + tb * az;
+            sto(vz, c, ldo(vz, c) + tb * az);
         }
     }
 }
@@ -363,50 +368,51 @@ __global__ __launch_bounds__(EL_WG) void el_adj_vel_kernel(const ElDev* __restri
     if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
     const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
     if (z0 > z1) return;
-    const long long sy = g.pitch, sz = g.slice, boff = g.xoff;
+    const long long boff = g.xoff;
+    const uint32_t sy = 4u * g.pitch, sz = 4u * (uint32_t)g.slice;  // byte strides
     const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
     float* const* sg = REV ? d.f + F_SXX : d.fw1 + F_SXX;
-    const float* __restrict__ sxx = sg[0] + boff;
-    const float* __restrict__ syy = sg[1] + boff;
-    const float* __restrict__ szz = sg[2] + boff;
-    const float* __restrict__ sxy = sg[3] + boff;
-    const float* __restrict__ sxz = sg[4] + boff;
-    const float* __restrict__ syz = sg[5] + boff;
-    const float* __restrict__ b = d.f[F_B] + boff;
-    const float* __restrict__ wx = d.adj[0] + boff;
-    const float* __restrict__ wy = d.adj[1] + boff;
-    const float* __restrict__ wz = d.adj[2] + boff;
-    float* __restrict__ gb = d.grad[2] + boff;
-    const long long cb = (long long)y * sy + x;
+    const gfloat* __restrict__ sxx = G(sg[0] + boff);
+    const gfloat* __restrict__ syy = G(sg[1] + boff);
+    const gfloat* __restrict__ szz = G(sg[2] + boff);
+    const gfloat* __restrict__ sxy = G(sg[3] + boff);
+    const gfloat* __restrict__ sxz = G(sg[4] + boff);
+    const gfloat* __restrict__ syz = G(sg[5] + boff);
+    const gfloat* __restrict__ b = G(d.f[F_B] + boff);
+    const gfloat* __restrict__ wx = G(d.adj[0] + boff);
+    const gfloat* __restrict__ wy = G(d.adj[1] + boff);
+    const gfloat* __restrict__ wz = G(d.adj[2] + boff);
+    gfloat* __restrict__ gb = G(d.grad[2] + boff);
+    gfloat* __restrict__ vx = G(d.f[F_VX] + boff);
+    gfloat* __restrict__ vy = G(d.f[F_VY] + boff);
+    gfloat* __restrict__ vz = G(d.f[F_VZ] + boff);
+    uint32_t c = (uint32_t)z0 * sz + (uint32_t)y * sy + 4u * x;
     float qa[5], qb[5], qc[5];  // sxz, syz, szz along z
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const long long q = (long long)(z0 - 2 + k) * sz + cb;
-        qa[k] = sxz[q];
-        qb[k] = syz[q];
-        qc[k] = szz[q];
+        const uint32_t q = c - 2 * sz + k * sz;
+        qa[k] = ldo(sxz, q);
+        qb[k] = ldo(syz, q);
+        qc[k] = ldo(szz, q);
     }
-    for (int z = z0; z <= z1; ++z) {
-        const long long c = z * sz + cb, q = c + 2 * sz;
-        qa[4] = sxz[q];
-        qb[4] = syz[q];
-        qc[4] = szz[q];
-        const float ax = (fd4(sxx, c, 1, i0) + fd4(sxy, c, sy, i1)) + fdq(qa, i2);
-        const float ay = (fd4(sxy, c, 1, i0) + fd4(syy, c, sy, i1)) + fdq(qb, i2);
-        const float az = (fd4(sxz, c, 1, i0) + fd4(syz, c, sy, i1)) + fdq(qc, i2);
+    for (int z = z0; z <= z1; ++z, c += sz) {
+        const uint32_t q = c + 2 * sz;
+        qa[4] = ldo(sxz, q);
+        qb[4] = ldo(syz, q);
+        qc[4] = ldo(szz, q);
+        const float ax = (fd4(sxx, c, 4, i0) + fd4(sxy, c, sy, i1)) + fdq(qa, i2);
+        const float ay = (fd4(sxy, c, 4, i0) + fd4(syy, c, sy, i1)) + fdq(qb, i2);
+        const float az = (fd4(sxz, c, 4, i0) + fd4(syz, c, sy, i1)) + fdq(qc, i2);
         qshift(qa);
         qshift(qb);
         qshift(qc);
-        const float bc = b[c];
-        gb[c] += dt * ((wx[c] * ax + wy[c] * ay + wz[c] * az) / bc);
+        const float bc = ldo(b, c);
+        sto(gb, c, ldo(gb, c) + dt * ((ldo(wx, c) * ax + ldo(wy, c) * ay + ldo(wz, c) * az) / bc));
         if (REV) {
             const float tb = dt * bc;
-            float* v = d.f[F_VX] + boff + c;
-            v[0] = v[0] - tb * ax;
-            v = d.f[F_VY] + boff + c;
-            v[0] = v[0] - tb * ay;
-            v = d.f[F_VZ] + boff + c;
-            v[0] = v[0] - tb * az;
+            sto(vx, c, ldo(vx, c) - tb * ax);
+            sto(vy, c, ldo(vy, c) - tb * ay);
+            sto(vz, c, ldo(vz, c) - tb * az);
         }
     }
 }
@@ -424,62 +430,66 @@ __global__ __launch_bounds__(EL_WG) void el_adj_str_kernel(const ElDev* __restri
     if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
     const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
     if (z0 > z1) return;
-    const long long sy = g.pitch, sz = g.slice, boff = g.xoff;
+    const long long boff = g.xoff;
+    const uint32_t sy = 4u * g.pitch, sz = 4u * (uint32_t)g.slice;  // byte strides
     const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
     float* const* vv = REV ? d.f : d.fw0;
-    const float* __restrict__ vx = vv[0] + boff;
-    const float* __restrict__ vy = vv[1] + boff;
-    const float* __restrict__ vz = vv[2] + boff;
-    const float* __restrict__ lam = d.f[F_LAM] + boff;
-    const float* __restrict__ mu = d.f[F_MU] + boff;
-    const long long cb = (long long)y * sy + x;
+    const gfloat* __restrict__ vx = G(vv[0] + boff);
+    const gfloat* __restrict__ vy = G(vv[1] + boff);
+    const gfloat* __restrict__ vz = G(vv[2] + boff);
+    const gfloat* __restrict__ lam = G(d.f[F_LAM] + boff);
+    const gfloat* __restrict__ mu = G(d.f[F_MU] + boff);
+    uint32_t c = (uint32_t)z0 * sz + (uint32_t)y * sy + 4u * x;
     float qx[5], qy[5], qz[5];  // v along z
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const long long q = (long long)(z0 - 2 + k) * sz + cb;
-        qx[k] = vx[q];
-        qy[k] = vy[q];
-        qz[k] = vz[q];
+        const uint32_t q = c - 2 * sz + k * sz;
+        qx[k] = ldo(vx, q);
+        qy[k] = ldo(vy, q);
+        qz[k] = ldo(vz, q);
     }
-    for (int z = z0; z <= z1; ++z) {
-        const long long c = z * sz + cb, q = c + 2 * sz;
-        qx[4] = vx[q];
-        qy[4] = vy[q];
-        qz[4] = vz[q];
-        const float exx = fd4(vx, c, 1, i0);
+    for (int z = z0; z <= z1; ++z, c += sz) {
+        const uint32_t q = c + 2 * sz;
+        qx[4] = ldo(vx, q);
+        qy[4] = ldo(vy, q);
+        qz[4] = ldo(vz, q);
+        const float exx = fd4(vx, c, 4, i0);
         const float eyy = fd4(vy, c, sy, i1);
         const float ezz = fdq(qz, i2);
-        const float dxy = fd4(vx, c, sy, i1) + fd4(vy, c, 1, i0);
-        const float dxz = fdq(qx, i2) + fd4(vz, c, 1, i0);
+        const float dxy = fd4(vx, c, sy, i1) + fd4(vy, c, 4, i0);
+        const float dxz = fdq(qx, i2) + fd4(vz, c, 4, i0);
         const float dyz = fdq(qy, i2) + fd4(vz, c, sy, i1);
         qshift(qx);
         qshift(qy);
         qshift(qz);
-        const float l = lam[c], m = mu[c];
-        const long long ca = boff + c;
-        const float pxx = d.adj[3][ca], pyy = d.adj[4][ca], pzz = d.adj[5][ca];
+        const float l = ldo(lam, c), m = ldo(mu, c);
+        const float pxx = ldo(G(d.adj[3] + boff), c), pyy = ldo(G(d.adj[4] + boff), c),
+                    pzz = ldo(G(d.adj[5] + boff), c);
         const float trs = ((pxx + pyy) + pzz) / (3.0f * l + 2.0f * m);
         const float im2 = 0.5f / m;
         const float hxx = (pxx - l * trs) * im2, hyy = (pyy - l * trs) * im2,
                     hzz = (pzz - l * trs) * im2;
-        const float hxy = d.adj[6][ca] / m, hxz = d.adj[7][ca] / m, hyz = d.adj[8][ca] / m;
-        d.grad[0][ca] += dt * (trs * ((exx + eyy) + ezz));
-        d.grad[1][ca] += dt * (2.0f * (hxx * exx + hyy * eyy + hzz * ezz) + hxy * dxy +
-                               hxz * dxz + hyz * dyz);
+        const float hxy = ldo(G(d.adj[6] + boff), c) / m, hxz = ldo(G(d.adj[7] + boff), c) / m,
+                    hyz = ldo(G(d.adj[8] + boff), c) / m;
+        gfloat* __restrict__ gl = G(d.grad[0] + boff);
+        gfloat* __restrict__ gm = G(d.grad[1] + boff);
+        sto(gl, c, ldo(gl, c) + dt * (trs * ((exx + eyy) + ezz)));
+        sto(gm, c, ldo(gm, c) + dt * (2.0f * (hxx * exx + hyy * eyy + hzz * ezz) + hxy * dxy +
+                                      hxz * dxz + hyz * dyz));
         if (REV) {
             const float l2m = l + 2.0f * m;
-            float* sgp = d.f[F_SXX] + ca;
-            sgp[0] = sgp[0] - dt * (l2m * exx + l * (eyy + ezz));
-            sgp = d.f[F_SYY] + ca;
-            sgp[0] = sgp[0] - dt * (l2m * eyy + l * (exx + ezz));
-            sgp = d.f[F_SZZ] + ca;
-            sgp[0] = sgp[0] - dt * (l2m * ezz + l * (exx + eyy));
-            sgp = d.f[F_SXY] + ca;
-            sgp[0] = sgp[0] - dt * (m * dxy);
-            sgp = d.f[F_SXZ] + ca;
-            sgp[0] = sgp[0] - dt * (m * dxz);
-            sgp = d.f[F_SYZ] + ca;
-            sgp[0] = sgp[0] - dt * (m * dyz);
+            gfloat* __restrict__ sg = G(d.f[F_SXX] + boff);
+            sto(sg, c, ldo(sg, c) - dt * (l2m * exx + l * (eyy + ezz)));
+            sg = G(d.f[F_SYY] + boff);
+            sto(sg, c, ldo(sg, c) - dt * (l2m * eyy + l * (exx + ezz)));
+            sg = G(d.f[F_SZZ] + boff);
+            sto(sg, c, ldo(sg, c) - dt * (l2m * ezz + l * (exx + eyy)));
+            sg = G(d.f[F_SXY] + boff);
+            sto(sg, c, ldo(sg, c) - dt * (m * dxy));
+            sg = G(d.f[F_SXZ] + boff);
+            sto(sg, c, ldo(sg, c) - dt * (m * dxz));
+            sg = G(d.f[F_SYZ] + boff);
+            sto(sg, c, ldo(sg, c) - dt * (m * dyz));
         }
     }
 }
@@ -645,6 +655,8 @@ extern "C" int b2r_el_create(b2r_ctx* ctx, float dt, const float inv_h[3], b2r_e
     B2R_CHECK(ctx->g.elem_size == 4, B2R_E_UNSUPPORTED, "elastic fields are f32");
     B2R_CHECK(ctx->g.radius == 4, B2R_E_ARG,
               "elastic needs radius 4 (two radius-2 stages per step), got %d", ctx->g.radius);
+    B2R_CHECK(ctx->k.bstride * 4 < (1ll << 32), B2R_E_UNSUPPORTED,
+              "elastic blocks are limited to 4 GiB per field (32-bit offsets)");
     if (int rc = b2r_activate(ctx)) return rc;
     b2r_el_shot* s = new b2r_el_shot();
     s->ctx = ctx;
@@ -655,6 +667,7 @@ extern "C" int b2r_el_create(b2r_ctx* ctx, float dt, const float inv_h[3], b2r_e
     for (int a = 0; a < 3; ++a) s->d.ih[a] = inv_h[a];
     for (int k = 0; k < F_NUM; ++k) {
         if (int rc = b2r_field_alloc(ctx, &s->fields[k])) {
+            for (int j = 0; j < k; ++j) (void)b2r_field_free(ctx, s->fields[j]);
             delete s;
             return rc;
         }

@@ -54,27 +54,32 @@ __global__ __launch_bounds__(HEAT_TX * WY) void heat_step_kernel(const float* __
     const int zlo = max(zs, box.lo[2]), zhi = min(ze, box.hi[2]);
     if (zlo > zhi) return;
 
-    const long long boff = (long long)b * g.bstride + g.xoff;
-    const float* __restrict__ src = in + boff;
-    float* __restrict__ dst = out + boff;
-    const long long slice = g.slice;
-    const int pitch = g.pitch;
+    // the workgroup reads planes zlo-D .. zhi+D: one base pointer at plane
+    // zlo-D, cells addressed by 32-bit byte offsets from it (the launch keeps
+    // zc + 2D planes below 4 GiB)
+    const long long base = (long long)b * g.bstride + g.xoff + (long long)(zlo - D) * g.slice;
+    const gfloat* __restrict__ src = G(in + base);
+    gfloat* __restrict__ dst = G(out + base);
+    const uint32_t sz = 4u * (uint32_t)g.slice;
+    const uint32_t pitch = 4u * (uint32_t)g.pitch;
 
     const int x = x0 + lane;
     const bool xin = x < g.S[0];
     const bool xupd = x >= box.lo[0] && x <= box.hi[0];
     int y[RY];
+    uint32_t yo[RY];  // byte offset of the thread's cell of row j within a plane
     bool yin[RY], yupd[RY];
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
         y[j] = y0 + wy * RY + j;
         yin[j] = y[j] < g.S[1];
         yupd[j] = y[j] >= box.lo[1] && y[j] <= box.hi[1];
+        yo[j] = (uint32_t)y[j] * pitch + 4u * (uint32_t)x;
     }
 
     // halo slots: D rows above/below the tile (TX wide), D columns left/right (TY tall)
     int hl[NSLOT];
-    long long ho[NSLOT];
+    uint32_t ho[NSLOT];
     bool hok[NSLOT];
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) {
@@ -97,29 +102,30 @@ __global__ __launch_bounds__(HEAT_TX * WY) void heat_step_kernel(const float* __
             const int gx = x0 - D + lx, gy = y0 - D + ly;
             hok[s] = gx < g.S[0] && gy < g.S[1];
             hl[s] = ly * LX + lx;
-            ho[s] = (long long)gy * pitch + gx;
+            ho[s] = (uint32_t)gy * pitch + 4u * (uint32_t)gx;
         }
     }
 
     float q[RY][Q];
 #pragma unroll
     for (int p = 0; p < 2 * D; ++p) {
-        const long long pz = (long long)(zlo - D + p) * slice;
 #pragma unroll
         for (int j = 0; j < RY; ++j)
-            q[j][p] = (xin && yin[j]) ? src[pz + (long long)y[j] * pitch + x] : 0.0f;
+            q[j][p] = (xin && yin[j]) ? ldo(src, p * sz + yo[j]) : 0.0f;
     }
 
-    // halo values of the plane after the current one, loaded a plane ahead
-    float hv[NSLOT];
+    // loaded a plane ahead: the halo values of the next plane (hv) and the
+    // tile's values D planes ahead of the next plane (qn), so the wait at the
+    // first barrier finds them already returned
+    float hv[NSLOT], qn[RY];
 #pragma unroll
-    for (int s = 0; s < NSLOT; ++s) hv[s] = hok[s] ? src[(long long)zlo * slice + ho[s]] : 0.0f;
-    for (int z = zlo; z <= zhi; ++z) {
-        const long long pz = (long long)z * slice;
-        const long long pf = pz + (long long)D * slice;
+    for (int s = 0; s < NSLOT; ++s) hv[s] = hok[s] ? ldo(src, D * sz + ho[s]) : 0.0f;
 #pragma unroll
-        for (int j = 0; j < RY; ++j)
-            q[j][2 * D] = (xin && yin[j]) ? src[pf + (long long)y[j] * pitch + x] : 0.0f;
+    for (int j = 0; j < RY; ++j) qn[j] = (xin && yin[j]) ? ldo(src, 2 * D * sz + yo[j]) : 0.0f;
+    uint32_t pz = D * sz;  // plane z
+    for (int z = zlo; z <= zhi; ++z, pz += sz) {
+#pragma unroll
+        for (int j = 0; j < RY; ++j) q[j][2 * D] = qn[j];
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < RY; ++j) lds[(wy * RY + j + D) * LX + lane + D] = q[j][D];
@@ -127,9 +133,14 @@ __global__ __launch_bounds__(HEAT_TX * WY) void heat_step_kernel(const float* __
         for (int s = 0; s < NSLOT; ++s)
             if (hok[s]) lds[hl[s]] = hv[s];
         if (z < zhi) {
+            // planes z+1 and z+1+D: z+1 <= zhi <= S[2]-1-D keeps both in the block
+            const uint32_t pn = pz + (D + 1) * sz;
+#pragma unroll
+            for (int j = 0; j < RY; ++j)
+                if (xin && yin[j]) qn[j] = ldo(src, pn + yo[j]);
 #pragma unroll
             for (int s = 0; s < NSLOT; ++s)
-                if (hok[s]) hv[s] = src[pz + slice + ho[s]];
+                if (hok[s]) hv[s] = ldo(src, pz + sz + ho[s]);
         }
         __syncthreads();
 #pragma unroll
@@ -150,7 +161,7 @@ __global__ __launch_bounds__(HEAT_TX * WY) void heat_step_kernel(const float* __
                 const float lx = lap<D, MODE>(mx, px, uc);
                 const float ly = lap<D, MODE>(my, py, uc);
                 const float lz = lap<D, MODE>(mz, pz_, uc);
-                dst[pz + (long long)y[j] * pitch + x] = heat_combine<MODE>(uc, lx, ly, lz, hk);
+                sto(dst, pz + yo[j], heat_combine<MODE>(uc, lx, ly, lz, hk));
             }
         }
 #pragma unroll
@@ -168,6 +179,9 @@ int b2r_heat_check(b2r_ctx* ctx, const b2r_heat_params* p)
     B2R_CHECK(p->D == ctx->g.radius, B2R_E_ARG, "heat D=%d but context radius=%d", p->D,
               ctx->g.radius);
     B2R_CHECK(p->mode == B2R_HEAT_F32 || p->mode == B2R_HEAT_REF, B2R_E_ARG, "bad mode");
+    // the step kernel addresses 2D + 1 planes or more by 32-bit byte offsets
+    B2R_CHECK(4ull * (unsigned long long)ctx->k.slice * (2 * p->D + 2) <= 0xffffffffull,
+              B2R_E_UNSUPPORTED, "x-y plane too large for the heat kernels");
     if (p->mode == B2R_HEAT_REF)
         B2R_CHECK(p->h2[0] > 0 && p->h2[1] > 0 && p->h2[2] > 0, B2R_E_ARG, "h2 must be > 0");
     return B2R_OK;
@@ -190,6 +204,9 @@ static void launch_heat_ry(b2r_ctx* ctx, const float* in, float* out, const Heat
         if (zc < 4 * D) zc = 4 * D;
     }
     if (zc > nz) zc = nz;
+    // a workgroup addresses its zc + 2D planes by 32-bit byte offsets
+    const long long zmax = (long long)(0xffffffffu / (4ull * (unsigned long long)g.slice)) - 2 * D;
+    if (zc > zmax) zc = (int)zmax;
     const int nzc = ceil_div(nz, zc);
     dim3 grid(ntx * nty * nzc * g.nblk);
     dim3 block(HEAT_TX, WY);
