@@ -294,15 +294,28 @@ __global__ void el_inject_kernel(const ElDev* __restrict__ tab, int stage,
     int bc[3];
     block_coords(d.g, b, bc);
     const int* p = sr.src + 4 * s;
-    int lc[3];
-    for (int a = 0; a < 3; ++a) lc[a] = p[a] - bc[a] * d.g.B[a] + d.g.h;
     const Box box = update_box(d.g, b, 2, stage);
-    for (int a = 0; a < 3; ++a)
-        if (lc[a] < box.lo[a] || lc[a] > box.hi[a]) return;
-    const long long c = cell_index(d.g, b, lc[0], lc[1], lc[2]);
-    const float tb = d.dt * d.f[F_B][c];
+    const float w = sr.wav[(long long)t * sr.nsrc + s];
     float* v = d.f[p[3]];
-    v[c] = v[c] + tb * sr.wav[(long long)t * sr.nsrc + s];
+    // with periodic faces the source cell also has images one period away,
+    // which can lie in this block's halo (or, for a single block along an
+    // axis, in its box twice); the redundant halo stages see every image
+    const int k = d.g.periodic ? 1 : 0;
+    for (int kz = -k; kz <= k; ++kz)
+        for (int ky = -k; ky <= k; ++ky)
+            for (int kx = -k; kx <= k; ++kx) {
+                const int sh[3] = {kx, ky, kz};
+                int lc[3];
+                bool in = true;
+                for (int a = 0; a < 3; ++a) {
+                    lc[a] = p[a] + sh[a] * d.g.nb[a] * d.g.B[a] - bc[a] * d.g.B[a] + d.g.h;
+                    in = in && lc[a] >= box.lo[a] && lc[a] <= box.hi[a];
+                }
+                if (!in) continue;
+                const long long c = cell_index(d.g, b, lc[0], lc[1], lc[2]);
+                const float tb = d.dt * d.f[F_B][c];
+                v[c] = v[c] + tb * w;
+            }
 }
 
 // Receivers sample their component from the owning block after the step.

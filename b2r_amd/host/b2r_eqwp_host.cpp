@@ -60,10 +60,18 @@ void eqwp_cuda_finalize()
 
 double eqwp_gpu_main(int ts, int b2r_R, int Ngx, int Ngy, int Ngz, bool op_flag)
 {
-    (void)ts; (void)Ngx; (void)Ngy; (void)Ngz; (void)op_flag;
+    (void)Ngx; (void)Ngy; (void)Ngz;
     float ms = 0.0f;
     // the reference structure: stress, vx, vy, vz per step, synchronised
     die_on(b2r_el_forward(g_shot, b2r_R, 0, &ms), "eqwp_gpu_main");
+    // eqwp.cu:689-709: with op_flag, the displacement goes to disk (the
+    // reference has this switched off; B2R_OUTPUT switches it on)
+    if (op_flag && (B2R_OUTPUT & 3)) {
+        std::vector<float> h((size_t)ENV_DIM_X * ENV_DIM_Y * ENV_DIM_Z);
+        die_on(b2r_el_get_state(g_shot, -1, h.data()), "displacement download");
+        if (B2R_OUTPUT & 1) write_block_data_bin(ts, h.data(), ENV_DIM_X, ENV_DIM_Y, ENV_DIM_Z);
+        if (B2R_OUTPUT & 2) write_block_data(ts, h.data(), ENV_DIM_X, ENV_DIM_Y, ENV_DIM_Z);
+    }
     return ms / 1000.0;
 }
 

@@ -49,3 +49,17 @@ void eqwp_cuda_finalize();
 double eqwp_gpu_main(int ts, int b2r_R, int Ngx, int Ngy, int Ngz, bool op_flag);
 CELL_DT* eqwp_field(int which);
 b2r_el_shot* eqwp_shot();
+
+// on-disk outputs (eqwp.cu:616-650, heat-3D.hpp:105-127), written to the
+// working directory.  B2R_OUTPUT selects what the drivers write: bit 0
+// displacement-<ts>.bin, bit 1 the text forms (displacement-<ts>.txt for eqwp,
+// one R-<R>-rank-<b>-i-<step>.txt per block for heat); eqwp_gpu_main writes the
+// displacement when op_flag is set.  The reference writes one rank's padded
+// block; one process here holds every block, so the displacement files carry
+// the global owned grid (header ENV_DIM_X, ENV_DIM_Y, ENV_DIM_Z), and
+// print_block_data_step takes the padded block to print (the reference reads
+// its rank's global grid).
+extern int B2R_OUTPUT;
+void write_block_data_bin(int time_step, CELL_DT* p_data, int Ngx, int Ngy, int Ngz);
+void write_block_data(int time_step, CELL_DT* p_data, int Ngx, int Ngy, int Ngz);
+void print_block_data_step(int rank, float step, const CELL_DT* block);

@@ -1,7 +1,7 @@
 // b2r_main.cpp -- the B2R driver of b2r.cpp:125-250 on one GPU.
 //
 //   b2r_main R Bx By Bz [--model heat|gol|eqwp] [--env X Y Z] [--D d]
-//            [--iters N] [--periodic] [--dump FILE]
+//            [--iters N] [--periodic] [--dump FILE] [--output bin|txt|both]
 //
 // Checks the arguments like b2r.cpp:132-167, then runs the time loop of
 // b2r.cpp:179-221: every R steps boundary_updating of the model's fields
@@ -10,6 +10,9 @@
 // --dump writes the final global owned grid (heat/gol: the model field; eqwp:
 // v_y), x fastest, raw cells, for checking the driver end to end; the
 // reference instead redirects its timing report to a file (b2r.cpp:82-97).
+// --output writes the reference's files into the working directory: eqwp
+// displacement-<ts>.bin / .txt after the last step group (eqwp.cu:689-709),
+// heat R-<R>-rank-<b>-i-<iters>.txt per block at the end (heat-3D.hpp:105-127).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -55,6 +58,13 @@ int main(int argc, char* argv[])
             B2R_PERIODIC = 1;
         } else if (!strcmp(argv[i], "--dump") && i + 1 < argc) {
             dump = argv[++i];
+        } else if (!strcmp(argv[i], "--output") && i + 1 < argc) {
+            const char* o = argv[++i];
+            B2R_OUTPUT = !strcmp(o, "bin") ? 1 : !strcmp(o, "txt") ? 2 : !strcmp(o, "both") ? 3 : -1;
+            if (B2R_OUTPUT < 0) {
+                printf("unknown output %s (bin, txt, both)\n", o);
+                return 1;
+            }
         }
     }
     if (model == "gol") {
@@ -143,6 +153,16 @@ int main(int argc, char* argv[])
             return -1;
         }
         fclose(fp);
+    }
+    if (model == "heat" && (B2R_OUTPUT & 2)) {
+        std::vector<CELL_DT> blk((size_t)B2R_BLOCK_SIZE_X * B2R_BLOCK_SIZE_Y * B2R_BLOCK_SIZE_Z);
+        for (int r = 0; r < BLOCK_DIM_X * BLOCK_DIM_Y * BLOCK_DIM_Z; ++r) {
+            if (b2r_block_download(ctx, cur, r, blk.data()) < 0) {
+                fprintf(stderr, "B2R Error: block download: %s\n", b2r_last_error());
+                return -1;
+            }
+            print_block_data_step(r, (float)n_iter, blk.data());
+        }
     }
     if (model == "eqwp") eqwp_cuda_finalize();
     b2r_sync_finalize();

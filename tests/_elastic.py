@@ -59,6 +59,42 @@ def forward(env, lam, mu, b, dt, ih, nsteps, src_xyz=(), src_comp=(), wav=None,
     return out + (_unframe(d),) if disp else out
 
 
+def _wrap(a):
+    a[...] = np.pad(a[2:-2, 2:-2, 2:-2], 2, mode="wrap")
+
+
+def forward_periodic(env, lam, mu, b, dt, ih, nsteps, src_xyz=(), src_comp=(), wav=None,
+                     rec_xyz=(), rec_comp=1):
+    """FP32 forward on a periodic grid: the oracle's stress and velocity stages
+    (orc_el_stress_f32, orc_el_velocity_f32), each after filling the 2-cell frame
+    of the fields it differentiates by periodic wrap; point forces and receivers
+    as orc_el_forward_f32.  Returns (owned state list of 9, receivers nt x nrec)."""
+    gx, gy, gz = env
+    f32 = np.float32
+    st = _state((gz, gy, gx), f32)
+    L, M, B = (np.pad(np.asarray(a, f32), 2, mode="wrap") for a in (lam, mu, b))
+    src_xyz = np.asarray(src_xyz, np.int32).reshape(-1, 3)
+    src_comp = np.asarray(src_comp, np.int32).ravel()
+    rec_xyz = np.asarray(rec_xyz, np.int32).reshape(-1, 3)
+    rec = np.zeros((nsteps, len(rec_xyz)), f32)
+    ihs = np.asarray(ih, f32)
+    dt = f32(dt)
+    for t in range(nsteps):
+        for k in range(3):
+            _wrap(st[k])
+        O.lib.orc_el_stress_f32(_ptrs(st), _p(L), _p(M), gx, gy, gz, _p(ihs), float(dt))
+        for k in range(3, 9):
+            _wrap(st[k])
+        O.lib.orc_el_velocity_f32(_ptrs(st), _p(B), None, gx, gy, gz, _p(ihs), float(dt))
+        for i, (x, y, z) in enumerate(src_xyz):
+            c = (z + 2, y + 2, x + 2)
+            tb = dt * B[c]
+            st[src_comp[i]][c] = st[src_comp[i]][c] + tb * f32(wav[t, i])
+        for r, (x, y, z) in enumerate(rec_xyz):
+            rec[t, r] = st[rec_comp][z + 2, y + 2, x + 2]
+    return [_unframe(a) for a in st], rec
+
+
 def adjoint(env, lam, mu, b, dt, ih, nsteps, src_xyz, src_comp, wav, rec_xyz, rec_comp, res):
     """FP64 gradients (g_lam, g_mu, g_b) of 1/2 |rec - obs|^2 given res = rec - obs."""
     gx, gy, gz = env

@@ -112,3 +112,44 @@ def test_adjoint_gradient_matches_finite_differences(which):
         fd = (_misfit64(P, *mp, obs)[0] - _misfit64(P, *mm, obs)[0]) / (2 * eps)
         an = float((g[which] * dirn).sum())
         assert abs(fd - an) <= 1e-6 * max(abs(fd), abs(an)) + 1e-30, (fd, an)
+
+
+def _periodic_case(env, src, nsteps, seed=5):
+    P = E.small_problem(env=env, seed=seed, nsteps=nsteps)
+    return P, np.array([src], np.int32)
+
+
+def test_periodic_forward_is_translation_equivariant():
+    # a source one cell from the x face: the wave crosses the faces within the run,
+    # so a frame filled any other way than by wrap breaks the equivariance
+    env, nsteps = (16, 14, 12), 10
+    P, src = _periodic_case(env, (1, 7, 6), nsteps)
+    rec = np.array([[0, 0, 0], [15, 13, 11], [3, 7, 6]], np.int32)
+    st, r = E.forward_periodic(env, P["lam"], P["mu"], P["b"], P["dt"], P["ih"], nsteps, src,
+                               [1], P["wav"], rec, 1)
+    assert np.abs(st[1][:, :, -3:]).max() > 0  # reached across the x face
+    sh = (3, -4, 5)  # shift along z, y, x
+
+    def roll(a):
+        return np.roll(a, sh, axis=(0, 1, 2))
+    s2 = (src + np.array([sh[2], sh[1], sh[0]])) % np.array(env)
+    r2 = (rec + np.array([sh[2], sh[1], sh[0]])) % np.array(env)
+    st2, rr2 = E.forward_periodic(env, roll(P["lam"]), roll(P["mu"]), roll(P["b"]), P["dt"],
+                                  P["ih"], nsteps, s2, [1], P["wav"], r2, 1)
+    for k in range(9):
+        assert np.array_equal(roll(st[k]), st2[k]), k
+    assert np.array_equal(r, rr2)
+
+
+def test_periodic_forward_equals_frozen_away_from_faces():
+    # while the wave stays two cells inside the faces, wrap and zero frame agree
+    env, nsteps = (40, 40, 40), 3
+    P, src = _periodic_case(env, (20, 19, 21), nsteps)
+    st, _ = E.forward_periodic(env, P["lam"], P["mu"], P["b"], P["dt"], P["ih"], nsteps, src,
+                               [1], P["wav"])
+    fz, _ = E.forward(env, P["lam"], P["mu"], P["b"], P["dt"], P["ih"], nsteps, src, [1],
+                      P["wav"])
+    assert np.abs(st[1]).max() > 0
+    for k in range(9):
+        assert not np.any(st[k][:2]) and not np.any(st[k][-2:]), k
+        assert np.array_equal(st[k], fz[k]), k

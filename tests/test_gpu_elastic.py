@@ -59,6 +59,35 @@ def test_forward_bitwise_vs_oracle(env, blocks, R, mode):
     assert bits_equal(rec, want_rec)
 
 
+@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("env,blocks,R", [((32, 24, 40), (1, 1, 1), 1),
+                                          ((48, 40, 32), (2, 1, 2), 1),
+                                          ((48, 48, 48), (1, 2, 2), 2)])
+def test_periodic_forward_bitwise_vs_oracle(env, blocks, R, mode):
+    # _PERIODIC_ (b2r_config.h:12) for eqwp: a source two cells from the x face,
+    # so the wave wraps within the run
+    nsteps = 12
+    P = E.small_problem(env=env, seed=7, nsteps=nsteps)
+    src = np.array([[2, env[1] // 2, env[2] // 3]], np.int32)
+    rec = np.array([[env[0] - 1, env[1] // 2, env[2] // 3], [0, 0, 0], [5, 3, env[2] - 1]],
+                   np.int32)
+    say("elastic periodic forward", env, blocks, "R", R, "mode", mode)
+    with A.Context(env, blocks, R, 4, A.BOUNDARY_PERIODIC, 4) as c:
+        with A.ElasticShot(c, float(P["dt"]), P["ih"]) as s:
+            s.set_medium(P["lam"], P["mu"], P["b"])
+            s.set_sources(src, [1], P["wav"])
+            s.set_receivers(rec, 1, nsteps)
+            s.forward(nsteps, mode)
+            got = [s.state(k) for k in range(9)]
+            got_rec = s.receivers()
+    st, want_rec = E.forward_periodic(env, P["lam"], P["mu"], P["b"], P["dt"], P["ih"], nsteps,
+                                      src, [1], P["wav"], rec, 1)
+    assert np.abs(st[1][:, :, -4:]).max() > 0  # wrapped across the x face
+    for k in range(9):
+        assert bits_equal(got[k], st[k]), A.EL_FIELDS[k]
+    assert bits_equal(got_rec, want_rec)
+
+
 def test_forward_displacement_and_split_runs():
     P = E.small_problem(env=(24, 20, 22), nsteps=10)
     with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:

@@ -11,6 +11,7 @@ import pytest
 
 import _elastic as E
 import _oracle as O
+from b2r_amd import output
 
 pytestmark = pytest.mark.gpu
 
@@ -22,8 +23,9 @@ def run(tmp_path, *args):
     out = str(tmp_path / "dump.bin")
     print("b2r_main", *args, flush=True)
     sys.stdout.flush()
+    # the reference's output files go to the working directory
     r = subprocess.run([EXE, *map(str, args), "--dump", out], capture_output=True, text=True,
-                       timeout=120)
+                       timeout=120, cwd=str(tmp_path))
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Iterations with B2R_R" in r.stdout and "GLUP/s" in r.stdout
     return out
@@ -71,3 +73,64 @@ def test_driver_eqwp_matches_oracle(tmp_path):
                       [1] * len(src), wav, np.array([[1, 1, 1]], np.int32), 1)
     assert np.any(st[1])
     assert np.array_equal(got.view(np.uint32), st[1].view(np.uint32))
+
+
+def test_driver_eqwp_periodic_matches_oracle(tmp_path):
+    # _PERIODIC_ for eqwp (b2r_config.h:12): the reference model on a periodic grid
+    env, iters = (40, 40, 40), 14
+    got = np.fromfile(run(tmp_path, 2, 1, 2, 1, "--model", "eqwp", "--env", *env, "--iters",
+                          iters, "--periodic"), np.float32).reshape(env[2], env[1], env[0])
+    gx, gy, gz = env
+    shape = (gz, gy, gx)
+    src = np.array([[x, gy - 7, z] for x in range(gx // 2, gx // 2 + gx // 10)
+                    for z in range(gz // 2, gz // 2 + gz // 10)], np.int32)
+    wav = np.full((iters, len(src)), -10000.0, np.float32)
+    st, _ = E.forward_periodic(env, np.full(shape, 32.0, np.float32),
+                               np.full(shape, 174.87, np.float32),
+                               np.full(shape, np.float32(1.0 / 1800.0), np.float32),
+                               np.float32(0.5), (np.float32(0.1),) * 3, iters, src, [1] * len(src),
+                               wav)
+    assert np.any(st[1][:, :4])  # the wave crossed the y face
+    assert np.array_equal(got.view(np.uint32), st[1].view(np.uint32))
+
+
+def test_driver_eqwp_writes_displacement(tmp_path):
+    # eqwp.cu:689-709 with the write switched on: displacement-<ts>.bin and .txt
+    # after the last step group (ts = its first step), against the oracle
+    env, R, iters = (32, 32, 32), 2, 10
+    run(tmp_path, R, 1, 1, 1, "--model", "eqwp", "--env", *env, "--iters", iters, "--output",
+        "both")
+    got = output.read_displacement_bin(tmp_path / ("displacement-%d.bin" % (iters - R)))
+    gx, gy, gz = env
+    shape = (gz, gy, gx)
+    src = np.array([[x, gy - 7, z] for x in range(gx // 2, gx // 2 + gx // 10)
+                    for z in range(gz // 2, gz // 2 + gz // 10)], np.int32)
+    wav = np.full((iters, len(src)), -10000.0, np.float32)
+    _, _, disp = E.forward(env, np.full(shape, 32.0, np.float32), np.full(shape, 174.87, np.float32),
+                           np.full(shape, np.float32(1.0 / 1800.0), np.float32), np.float32(0.5),
+                           (np.float32(0.1),) * 3, iters, src, [1] * len(src), wav,
+                           np.array([[1, 1, 1]], np.int32), 1, disp=True)
+    assert np.any(disp)
+    assert np.array_equal(got.view(np.uint32), disp.view(np.uint32))
+    txt = output.read_displacement_txt(tmp_path / ("displacement-%d.txt" % (iters - R)), env)
+    # six significant digits: a relative rounding error of at most 5e-6 (compared in
+    # float64: rtol times a float32 subnormal would underflow in float32)
+    assert np.allclose(txt, got.astype(np.float64), rtol=1e-5, atol=0)
+
+
+def test_driver_heat_writes_block_dumps(tmp_path):
+    # heat-3D.hpp:105-127: one padded-block text dump per block, "%0.2f"
+    env, R, blocks, D, iters = (32, 32, 24), 2, (2, 2, 1), 2, 6
+    run(tmp_path, R, *blocks, "--model", "heat", "--env", *env, "--D", D, "--iters", iters,
+        "--output", "txt")
+    u = O.frame(O.uniform((env[2], env[1], env[0]), 0), D)
+    want = O.unframe(O.heat_f32(u, D, 1, (0, 0, 0), 0.1 * 0.1, (1.0, 1.0, 0.25), iters, False), D)
+    h = R * D
+    B = [env[a] // blocks[a] for a in range(3)]
+    for r in range(blocks[0] * blocks[1] * blocks[2]):
+        cx, cy, cz = r % blocks[0], r // blocks[0] % blocks[1], r // (blocks[0] * blocks[1])
+        blk = output.read_block_step_txt(tmp_path / ("R-%d-rank-%d-i-%0.1f.txt" % (R, r, iters)))
+        assert blk.shape == (B[2] + 2 * h, B[1] + 2 * h, B[0] + 2 * h)
+        own = blk[h:h + B[2], h:h + B[1], h:h + B[0]]
+        ref = want[cz * B[2]:(cz + 1) * B[2], cy * B[1]:(cy + 1) * B[1], cx * B[0]:(cx + 1) * B[0]]
+        assert np.allclose(own, ref, rtol=0, atol=0.005 + 1e-6)

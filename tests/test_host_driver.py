@@ -29,3 +29,8 @@ def test_halo_deeper_than_half_block():
 def test_unknown_model():
     r = run(1, 1, 1, 1, "--model", "lbm")
     assert r.returncode == 1 and "unknown model" in r.stdout
+
+
+def test_unknown_output():
+    r = run(1, 1, 1, 1, "--output", "hdf5")
+    assert r.returncode == 1 and "unknown output" in r.stdout
