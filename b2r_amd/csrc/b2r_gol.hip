@@ -221,12 +221,13 @@ __global__ __launch_bounds__(GF_THREADS) void gol_fused_kernel(const uint8_t* __
     const int x = x0 + 16 * (q - 1);
     const int y = y0 - R + r;
     const bool ld_ok = y < g.S[1] && x + 15 >= 0 && x < g.S[0];
-    const uint8_t* __restrict__ col = src + (long long)y * pitch + x;
-    auto load = [&](int z) {
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (ld_ok) v = *reinterpret_cast<const uint4*>(col + (long long)z * slice);
-        return v;
-    };
+    // every thread loads (no branch, so a load in flight is never copied):
+    // threads outside the block read the tile's first quad of the last row,
+    // in bounds and aligned, and use zero instead
+    const uint8_t* __restrict__ col =
+        src + (ld_ok ? (long long)y * pitch + x : (long long)min(y, g.S[1] - 1) * pitch + x0);
+    auto load = [&](int z) { return *reinterpret_cast<const uint4*>(col + (long long)z * slice); };
+    const uint32_t ldm = ld_ok ? ~0u : 0u;
 
     // per-generation boxes (frozen faces) and this thread's update masks
     Box bx[R];
@@ -245,23 +246,20 @@ __global__ __launch_bounds__(GF_THREADS) void gol_fused_kernel(const uint8_t* __
 
     // gen[t][0..2]: generation t at planes (newest-2, newest-1, newest)
     uint4 gen[R][3];
-    // input planes are loaded GF_PD planes ahead of their use
+    // input planes are loaded GF_PD planes ahead of their use into slots that
+    // take turns (the loop is unrolled by GF_PD), so no register copy ever
+    // waits for a load still in flight
     const int zend = zc1 + R;
-    uint4 pf[GF_PD];
-#pragma unroll
-    for (int k = 0; k < GF_PD; ++k)
-        pf[k] = zc0 - R + k < zend ? load(zc0 - R + k) : make_uint4(0, 0, 0, 0);
     int lc = 0;  // generations computed so far (selects the LDS buffer)
-    for (int zin = zc0 - R; zin < zend; ++zin) {
+    auto plane = [&](int zin, uint4& slot) {
         gen[0][0] = gen[0][1];
         gen[0][1] = gen[0][2];
-        gen[0][2] = pf[0];
-#pragma unroll
-        for (int k = 0; k + 1 < GF_PD; ++k) pf[k] = pf[k + 1];
-        if (zin + GF_PD < zend) pf[GF_PD - 1] = load(zin + GF_PD);
+        gen[0][2] = make_uint4(slot.x & ldm, slot.y & ldm, slot.z & ldm, slot.w & ldm);
+        slot = load(min(zin + GF_PD, zend - 1));  // the last reloads are unused
 #pragma unroll
         for (int t = 1; t <= R; ++t) {
-            if (zin < zc0 - R + 2 * t) continue;  // uniform: generation t-1 not yet 3 planes
+            // uniform: generation t-1 not yet 3 planes, or a plane past the chunk
+            if (zin < zc0 - R + 2 * t || zin >= zend) continue;
             const int p = zin - t;
             const uint4 zs = q_add(q_add(gen[t - 1][0], gen[t - 1][1]), gen[t - 1][2]);
             const int bo = (lc & 1) * ZSB + GF_PAD;
@@ -311,6 +309,15 @@ __global__ __launch_bounds__(GF_THREADS) void gol_fused_kernel(const uint8_t* __
                 *reinterpret_cast<uint4*>(dst + (long long)p * slice + (long long)y * pitch + x) = nw;
             }
         }
+    };
+    const int z0 = zc0 - R;
+    uint4 sl[GF_PD];
+#pragma unroll
+    for (int k = 0; k < GF_PD; ++k) sl[k] = load(min(z0 + k, zend - 1));
+    // whole groups of GF_PD planes; planes past zend compute nothing
+    for (int zin = z0; zin < zend; zin += GF_PD) {
+#pragma unroll
+        for (int k = 0; k < GF_PD; ++k) plane(zin + k, sl[k]);
     }
 }
 
