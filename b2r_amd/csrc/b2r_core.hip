@@ -188,6 +188,7 @@ extern "C" int b2r_set_tuning(b2r_ctx* ctx, const b2r_tuning* t)
               "negative z chunk");
     B2R_CHECK(t->heat_wy == 0 || t->heat_wy == 4 || t->heat_wy == 8 || t->heat_wy == 16,
               B2R_E_ARG, "heat_wy must be 0, 4, 8 or 16");
+    B2R_CHECK(t->gol_tile >= 0 && t->gol_tile <= 2, B2R_E_ARG, "gol_tile must be 0, 1 or 2");
     ctx->tune = *t;
     return B2R_OK;
 }

@@ -142,10 +142,11 @@ __global__ __launch_bounds__(256) void gol_step_kernel(const uint8_t* __restrict
 
 // ---------------------------------------------------------------------------
 // R generations in one launch: temporal blocking (B2R's deep halo on chip).
-// A workgroup owns an output tile of 16*GF_TXQ cells x TY = GF_NR-2R rows of
-// one block and marches z through a chunk of planes.  Each thread owns one
-// 16-cell quad position of the tile grown by one quad per side in x and R rows
-// per side in y, and keeps, for every generation t = 0..R-1, the last three
+// A workgroup owns an output tile of 16*TXQ cells x TY = NR-2R rows of one
+// block and marches z through a chunk of planes.  The tile shape (TXQ, NR) is
+// a template parameter, chosen per context by the gol_tile tuning field.  Each
+// thread owns one 16-cell quad position of the tile grown by one quad per side
+// in x and R rows per side in y, and keeps, for every generation t = 0..R-1, the last three
 // planes of its own position in registers.  For every input plane z_in it
 // computes generation t (t = 1..R) of plane p = z_in - t from generation t-1:
 //   zs  = sum of its three planes of generation t-1 (registers, SWAR bytes),
@@ -162,13 +163,10 @@ __global__ __launch_bounds__(256) void gol_step_kernel(const uint8_t* __restrict
 // at a frozen face with their unchanged values, elsewhere with values the
 // next exchange overwrites.
 // ---------------------------------------------------------------------------
-constexpr int GF_TXQ = 8;            // output quads (16 cells) per tile row
-constexpr int GF_NQ = GF_TXQ + 2;    // plus one quad per side
-constexpr int GF_NR = 32;            // rows incl. R halo rows per side
-constexpr int GF_PLANE = GF_NQ * GF_NR;
-constexpr int GF_THREADS = GF_PLANE;  // one quad position per thread
-constexpr int GF_PAD = GF_NQ + 1;     // LDS rows read at +-(NQ+1) stay in bounds
-constexpr int GF_PD = 2;              // input planes in flight per thread
+// Tile shapes: TXQ output quads (16 cells) per tile row, NR rows including the
+// R halo rows per side; gol_tile 0 (default), 1, 2 select them.
+constexpr int GF_SHAPES[3][2] = {{8, 32}, {8, 48}, {16, 56}};
+constexpr int GF_PD = 2;  // input planes in flight per thread
 
 __device__ __forceinline__ uint4 q_add(uint4 a, uint4 b)
 {
@@ -185,13 +183,16 @@ __device__ __forceinline__ uint32_t dword_mask(int x, int lo, int hi)
     return m;
 }
 
-template <int R>
-__global__ __launch_bounds__(GF_THREADS) void gol_fused_kernel(const uint8_t* __restrict__ in,
-                                                        uint8_t* __restrict__ out, KGeo g,
-                                                        int ntx, int nty, int nzc, int zc)
+template <int R, int TXQ, int NR>
+__global__ __launch_bounds__((TXQ + 2) * NR) void gol_fused_kernel(const uint8_t* __restrict__ in,
+                                                                   uint8_t* __restrict__ out,
+                                                                   KGeo g, int ntx, int nty,
+                                                                   int nzc, int zc)
 {
-    constexpr int TY = GF_NR - 2 * R;
-    constexpr int ZSB = GF_PLANE + 2 * GF_PAD;
+    constexpr int GF_NQ = TXQ + 2;         // quads per row: plus one per side
+    constexpr int GF_PAD = GF_NQ + 1;      // LDS rows read at +-(NQ+1) stay in bounds
+    constexpr int TY = NR - 2 * R;
+    constexpr int ZSB = GF_NQ * NR + 2 * GF_PAD;
     __shared__ uint4 zsbuf[2 * ZSB];
     // the x and w dwords of zs again, densely, so that the neighbour-column
     // reads of consecutive lanes hit consecutive banks
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(GF_THREADS) void gol_fused_kernel(const uint8_t* __
     const int zci = tt % nzc;
     const int b = tt / nzc;
     const int h = g.h;
-    const int x0 = h + txi * 16 * GF_TXQ;  // first output cell
+    const int x0 = h + txi * 16 * TXQ;  // first output cell
     const int y0 = h + tyi * TY;
     const int zc0 = h + zci * zc;
     const int zc1 = min(zc0 + zc, h + g.B[2]);
@@ -237,7 +238,7 @@ __global__ __launch_bounds__(GF_THREADS) void gol_fused_kernel(const uint8_t* __
         bx[t - 1] = update_box(g, b, 1, t - 1);
         const int gr = R - t;  // growth of generation t's region
         const Box& B_ = bx[t - 1];
-        clip[t - 1] = x0 - 16 < B_.lo[0] || x0 + 16 * (GF_TXQ + 1) - 1 > B_.hi[0] ||
+        clip[t - 1] = x0 - 16 < B_.lo[0] || x0 + 16 * (TXQ + 1) - 1 > B_.hi[0] ||
                       y0 - gr < B_.lo[1] || y0 + TY + gr - 1 > B_.hi[1] ||
                       zc0 - gr < B_.lo[2] || zc1 + gr - 1 > B_.hi[2];
     }
@@ -326,12 +327,12 @@ static bool gol_fused_ok(const b2r_ctx* ctx, int R)
     return R >= 1 && R <= 4 && (ctx->k.periodic || ctx->k.fmask == 0x3F);
 }
 
-template <int R>
+template <int R, int TXQ, int NR>
 static int gol_fused_launch_t(b2r_ctx* ctx, const void* in, void* out)
 {
-    constexpr int TY = GF_NR - 2 * R;
+    constexpr int TY = NR - 2 * R;
     const KGeo& g = ctx->k;
-    const int ntx = ceil_div(g.B[0], 16 * GF_TXQ);
+    const int ntx = ceil_div(g.B[0], 16 * TXQ);
     const int nty = ceil_div(g.B[1], TY);
     long long tiles = (long long)ntx * nty * g.nblk;
     int zc = ctx->tune.gol_zc;
@@ -344,19 +345,29 @@ static int gol_fused_launch_t(b2r_ctx* ctx, const void* in, void* out)
     if (zc > g.B[2]) zc = g.B[2];
     const int nzc = ceil_div(g.B[2], zc);
     dim3 grid(ntx * nty * nzc * g.nblk);
-    gol_fused_kernel<R><<<grid, GF_THREADS, 0, ctx->stream>>>((const uint8_t*)in, (uint8_t*)out,
-                                                             g, ntx, nty, nzc, zc);
+    gol_fused_kernel<R, TXQ, NR><<<grid, (TXQ + 2) * NR, 0, ctx->stream>>>(
+        (const uint8_t*)in, (uint8_t*)out, g, ntx, nty, nzc, zc);
     B2R_LAUNCH_CHECK();
     return B2R_OK;
+}
+
+template <int R>
+static int gol_fused_launch_r(b2r_ctx* ctx, const void* in, void* out)
+{
+    switch (ctx->tune.gol_tile) {
+    case 1: return gol_fused_launch_t<R, GF_SHAPES[1][0], GF_SHAPES[1][1]>(ctx, in, out);
+    case 2: return gol_fused_launch_t<R, GF_SHAPES[2][0], GF_SHAPES[2][1]>(ctx, in, out);
+    default: return gol_fused_launch_t<R, GF_SHAPES[0][0], GF_SHAPES[0][1]>(ctx, in, out);
+    }
 }
 
 int b2r_gol_fused_launch(b2r_ctx* ctx, const void* in, void* out)
 {
     switch (ctx->g.R) {
-    case 1: return gol_fused_launch_t<1>(ctx, in, out);
-    case 2: return gol_fused_launch_t<2>(ctx, in, out);
-    case 3: return gol_fused_launch_t<3>(ctx, in, out);
-    case 4: return gol_fused_launch_t<4>(ctx, in, out);
+    case 1: return gol_fused_launch_r<1>(ctx, in, out);
+    case 2: return gol_fused_launch_r<2>(ctx, in, out);
+    case 3: return gol_fused_launch_r<3>(ctx, in, out);
+    case 4: return gol_fused_launch_r<4>(ctx, in, out);
     }
     b2r_set_error("fused Game of Life supports R <= 4 (got %d)", ctx->g.R);
     return B2R_E_UNSUPPORTED;

@@ -43,12 +43,17 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
                     }
         }
     } else {
-        for (int zc : zcs) {
-            b2r_tuning t = base;
-            if (kind == 1) t.gol_step_zc = zc;
-            else t.gol_zc = zc;
-            cand.push_back(t);
-        }
+        for (int tile = 0; tile < (kind == 2 ? 3 : 1); ++tile)
+            for (int zc : zcs) {
+                b2r_tuning t = base;
+                if (kind == 1) {
+                    t.gol_step_zc = zc;
+                } else {
+                    t.gol_tile = tile;
+                    t.gol_zc = zc;
+                }
+                cand.push_back(t);
+            }
     }
     const int warm = 2, iters = 6;
     float best_ms = 1e30f;
@@ -82,9 +87,10 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
                     best.heat_wy, best.heat_ry, best.heat_zc);
         } else if (kind == 3) {
             fprintf(f, " heat_D=%d best=heat_zc:%d", p ? p->D : 0, best.heat_zc);
+        } else if (kind == 1) {
+            fprintf(f, " best=gol_step_zc:%d", best.gol_step_zc);
         } else {
-            fprintf(f, " best=%s:%d", kind == 1 ? "gol_step_zc" : "gol_zc",
-                    kind == 1 ? best.gol_step_zc : best.gol_zc);
+            fprintf(f, " best=gol_tile:%d,gol_zc:%d", best.gol_tile, best.gol_zc);
         }
         fprintf(f, " ms_per_launch=%.5f tried=%d\n", best_ms / iters, tried);
         fclose(f);

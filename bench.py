@@ -89,7 +89,16 @@ def gol_bench(A, R, steps, warmup, fused=1, zc=0, record=None):
     return dict(R=R, fused=fused, ms=ms, wall_s=wall, glups=cells * steps / (ms * 1e-3) / 1e9,
                 kernel_ms=kms, gens_per_launch=gens,
                 kernel_gbs_alg=2.0 * cells * gens / (kms * 1e-3) / 1e9,
-                zc=tune["gol_zc"] if fused else tune["gol_step_zc"])
+                zc=tune["gol_zc"] if fused else tune["gol_step_zc"],
+                tile=tune["gol_tile"] if fused else None)
+
+
+# (output quads of 16 cells per tile row, rows incl. halo) of the gol_tile values
+GOL_TILES = {0: (8, 32), 1: (8, 48), 2: (16, 56)}
+
+
+def gol_kernel_name(R, tile):
+    return "gol_fused_kernel<%d, %d, %d>" % ((R,) + GOL_TILES[tile])
 
 
 # ------------------------------------------------------------------ heat
@@ -473,21 +482,21 @@ def profile_kernels(A):
     """A short fixed sequence for rocprofv3 counter runs: a copy kernel of known
     size (1 GiB read + 1 GiB written per launch, the scale for FETCH_SIZE /
     WRITE_SIZE), then 5 launches each of the fused Game of Life kernel (512^3,
-    2x2x2, R = 2 and 4, the z chunks the autotuner picks), the heat delta1 step
-    kernel (768^3, RY = 8, 8-plane chunks) and the elastic stress and velocity
-    kernels (256^3, 64-plane chunks)."""
+    2x2x2, R = 2 and 4, the tile shapes and z chunks the autotuner picks), the
+    heat delta1 step kernel (768^3, 16-wave workgroups, RY = 8, 32-plane chunks)
+    and the elastic stress and velocity kernels (256^3, 64-plane chunks)."""
     nbytes = 1 << 30
     with A.Context((16, 16, 16), (1, 1, 1), 1, 1, 0, 4) as c:
         src, dst = c.dev_alloc(nbytes), c.dev_alloc(nbytes)
         log("stream_copy 1 GiB x3")
         c.stream_copy(dst, src, nbytes, 3)
-    for R, zc in ((2, 32), (4, 64)):
+    for R, tile, zc in ((2, 2, 32), (4, 2, 16)):
         with A.Context((512, 512, 512), (2, 2, 2), R, 1, A.BOUNDARY_PERIODIC, 1) as c:
             a, b = c.field(), c.field()
             c.init_bernoulli(a, 1, int(0.3 * 16777216))
             c.boundary_updating_direct([a])
-            c.set_tuning(gol_zc=zc)
-            log(f"gol_fused_kernel<{R}> x5")
+            c.set_tuning(gol_zc=zc, gol_tile=tile)
+            log(f"{gol_kernel_name(R, tile)} x5")
             c.time_steps(2, a, b, 5)
     prm = A.heat_params(1, A.HEAT_F32, (0.1, 0.1, 0.1))
     with A.Context((768, 768, 768), (1, 1, 1), 1, 1, 0, 4) as c:
@@ -562,8 +571,10 @@ def main():
     log(json.dumps(plain))
     tr = read_traffic()
     traffic = None
-    if tr and f"gol_fused_R{best['R']}_bytes_per_launch" in tr:
-        traffic = float(tr[f"gol_fused_R{best['R']}_bytes_per_launch"])
+    # measured for the same R and tile shape only (the profile sequence fixes them)
+    tkey = f"gol_fused_R{best['R']}_tile{best['tile']}_bytes_per_launch"
+    if tr and tkey in tr:
+        traffic = float(tr[tkey])
 
     also = {"gol_512_R3_fused": {k: r3[k] for k in ("glups", "kernel_ms", "zc")}}
     heat_ref_grid = None
@@ -659,13 +670,13 @@ def main():
         "data": "synthetic (SplitMix64 Bernoulli(0.3), seed 1)",
         "config": {"workload": "gol3d_512^3_u8_periodic_2x2x2", "R": best["R"],
                    "generations": args.steps, "fused_R_generations_per_launch": True,
-                   "gol_zc": best["zc"],
+                   "gol_zc": best["zc"], "gol_tile": best["tile"],
                    "glups_by_R": {str(r["R"]): round(r["glups"], 3) for r in runs},
                    "plain_step_by_step_R1_glups": round(plain["glups"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": f"gol_fused_kernel<{best['R']}>",
+                     "kernel": gol_kernel_name(best["R"], best["tile"]),
                      "kernel_ms": round(best["kernel_ms"], 5),
                      "bytes_per_launch_algorithmic": 2 * cells * best["gens_per_launch"],
                      "note": "2 B/LUP x 512^3 cells x R generations per launch; a fused "

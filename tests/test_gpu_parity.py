@@ -293,12 +293,13 @@ def test_gol_full_size_R_invariance_and_population():
         assert np.array_equal(o, want), key
 
 
+@pytest.mark.parametrize("tile", [0, 1, 2])
 @pytest.mark.parametrize("zc", [4, 9, 64])
-def test_gol_fused_chunking(zc):
-    # z-chunk lengths that do not divide B, and a single chunk
+def test_gol_fused_chunking(zc, tile):
+    # z-chunk lengths that do not divide B, and a single chunk, for every tile shape
     env = (96, 72, 64)
     with A.Context(env, (1, 2, 1), 3, 1, 0, 1) as c:
-        c.set_tuning(gol_zc=zc, gol_step_zc=zc)
+        c.set_tuning(gol_zc=zc, gol_step_zc=zc, gol_tile=tile)
         a, b = c.field(), c.field()
         c.init_bernoulli(a, 5, int(0.3 * 16777216))
         res, _ = c.gol_run(a, b, 9, fused=1)
@@ -323,3 +324,24 @@ def test_heat_tuning_variants(wy, ry, zc):
     g = O.frame(O.uniform((36, 40, 70), 9), D)
     want = O.unframe(O.heat_f32(g, D, 0, k, 0, (1, 1, 1), 6, False), D)
     assert bits_equal(got, want)
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("R", [1, 2, 3, 4])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_gol_fused_tile_shapes(tile, R, periodic):
+    # every tile shape of the fused kernel, blocks narrower and wider than a tile
+    env, gens = (288, 40, 36), 2 * R + 1
+    say("gol fused tile", tile, "R", R, "periodic", periodic)
+    with A.Context(env, (2, 1, 1), R, 1, 1 if periodic else 0, 1) as c:
+        c.set_tuning(gol_tile=tile, gol_zc=11)
+        a, b = c.field(), c.field()
+        c.init_bernoulli(a, 4, int(0.3 * 16777216))
+        res, _ = c.gol_run(a, b, gens, fused=1)
+        got = c.download(res)
+    u = O.bernoulli((env[2], env[1], env[0]), 4, 0.3)
+    if periodic:
+        want = O.gol(u, gens, True)
+    else:
+        want = O.unframe(O.gol(O.frame(u, 1), gens, False), 1)
+    assert np.array_equal(got, want)
