@@ -153,8 +153,9 @@ __global__ __launch_bounds__(HEAT_TX * WY) void heat_step_kernel(const float* __
                 for (int k = 1; k <= D; ++k) {
                     mx[k] = row[-k];
                     px[k] = row[k];
-                    my[k] = row[-k * LX];
-                    py[k] = row[k * LX];
+                    // rows of this thread are in registers (j, k are unrolled)
+                    my[k] = j - k >= 0 ? q[j - k >= 0 ? j - k : 0][D] : row[-k * LX];
+                    py[k] = j + k < RY ? q[j + k < RY ? j + k : 0][D] : row[k * LX];
                     mz[k] = q[j][D - k];
                     pz_[k] = q[j][D + k];
                 }
