@@ -184,7 +184,8 @@ int b2r_unpack(b2r_ctx* ctx, int dim, void* d_field, const void* d_msgs);
 /* One inner B2R step b2r_i of fdm_heat_diffuse_delta{D} over every block:
  * out = step(in) on the region [D*(b2r_i+1), S-1-D*(b2r_i+1)] per axis, clipped
  * at frozen global faces to the owned range (heat-3D.cu:173-235, 250-521,
- * 57-158).  Cells outside the region are not written.  geom.radius must be D. */
+ * 57-158).  Cells outside the region are not written.  geom.radius must be D,
+ * and 2D+2 x-y planes of a block must stay below 4 GiB (32-bit offsets). */
 int b2r_heat_step(b2r_ctx* ctx, const void* d_in, void* d_out,
                   const b2r_heat_params* p, int b2r_i);
 /* nsteps steps of the B2R loop of b2r.cpp:179-221 with heat_3D_gpu_main's R-step
@@ -225,7 +226,10 @@ int b2r_gol_run(b2r_ctx* ctx, void* d_a, void* d_b, int nsteps, int fused,
  *            eqwp.cu:341-477, lambda+2mu / lambda / mu of :460-466)
  *   v     += dt*b*div(sigma)                (eqwp_fd4_vx/vy/vz, :60-331)
  * first derivatives (-1/12 (f[+2]-f[-2]) + 2/3 (f[+1]-f[-1])) * inv_h (:22-23),
- * then point forces v_c += dt*b*f(t) and receiver samples after the step. */
+ * then point forces v_c += dt*b*f(t) (also at a source's periodic images in
+ * halos) and receiver samples after the step.  A block's field must stay below
+ * 4 GiB (B2R_E_UNSUPPORTED otherwise): the stencil kernels address it by
+ * 32-bit byte offsets. */
 typedef struct b2r_el_shot b2r_el_shot;
 int b2r_el_create(b2r_ctx* ctx, float dt, const float inv_h[3], b2r_el_shot** out);
 int b2r_el_destroy(b2r_el_shot* s);
