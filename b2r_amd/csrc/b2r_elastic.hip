@@ -13,7 +13,8 @@
 // D=2, b2r.cpp:61-64, which is too shallow for two radius-2 stages per step).
 // Updates are in place: the stress stage reads v and writes sigma, the
 // velocity stage reads sigma and writes v.  A thread owns one (x, y) column of
-// a 64 x 4 tile and marches z; z neighbours come from register queues.
+// a 64 x 4 tile and marches z; z neighbours come from register queues, x and
+// y neighbours from the current plane staged in LDS with a 2-cell halo.
 //
 // Two forward paths run the same kernels (identical results):
 //   plain: per shot and step, stress + vx + vy + vz launches and a device
@@ -152,18 +153,95 @@ __device__ __forceinline__ void qshift(float (&q)[5])
     q[3] = q[4];
 }
 
+// A plane of fields staged in LDS for the x and y derivatives: the tile's
+// cells plus the 2-cell plus-shaped halo of fd4 -- rows y0-2, y0-1, y0+TY,
+// y0+TY+1 (one cell per thread) and columns x0-2, x0-1, x0+TX, x0+TX+1 (the
+// first 4*TY threads).  Cells outside the block stay 0; no updated cell reads
+// them.
+constexpr int EL_LX = EL_TX + 4, EL_LY = EL_TY + 4;
+struct Halo {
+    int ly, lx;         // this thread's cell in the LDS plane
+    int hry;            // LDS row of its halo-row cell (column lx)
+    int hcy, hcx;       // LDS row and column of its halo-column cell
+    bool in;            // its own cell lies in the block
+    bool hc_any;        // it owns a halo-column cell
+    bool hr_ok, hc_ok;  // the halo cells lie in the block
+    uint32_t hro, hco;  // their byte offsets within a plane
+};
+
+__device__ __forceinline__ Halo el_halo(const KGeo& g, int x, int y, uint32_t sy)
+{
+    Halo h;
+    const int lane = threadIdx.x, ty = threadIdx.y, tid = ty * EL_TX + lane;
+    const int x0 = x - lane, y0 = y - ty;
+    h.ly = ty + 2;
+    h.lx = lane + 2;
+    h.in = x < g.S[0] && y < g.S[1];
+    const int hr = tid / EL_TX;
+    h.hry = hr < 2 ? hr : EL_TY + hr;
+    const int hy = y0 - 2 + h.hry;
+    h.hr_ok = hy >= 0 && hy < g.S[1] && x < g.S[0];
+    h.hro = h.hr_ok ? (uint32_t)hy * sy + 4u * x : 0u;
+    h.hc_any = tid < 4 * EL_TY;
+    const int hcc = tid % 4;
+    h.hcy = tid / 4 + 2;
+    h.hcx = hcc < 2 ? hcc : EL_TX + hcc;
+    const int hx = x0 - 2 + h.hcx, hcy = y0 + tid / 4;
+    h.hc_ok = h.hc_any && hx >= 0 && hx < g.S[0] && hcy < g.S[1];
+    h.hco = h.hc_ok ? (uint32_t)hcy * sy + 4u * hx : 0u;
+    return h;
+}
+
+// the halo values of field f at the plane of byte offset pz
+struct HaloV {
+    float r, c;
+};
+__device__ __forceinline__ HaloV el_halo_load(const gfloat* __restrict__ f, const Halo& h,
+                                              uint32_t pz)
+{
+    HaloV v;
+    v.r = h.hr_ok ? ldo(f, pz + h.hro) : 0.0f;
+    v.c = h.hc_ok ? ldo(f, pz + h.hco) : 0.0f;
+    return v;
+}
+
+__device__ __forceinline__ void el_stage(float (&p)[EL_LY][EL_LX], const Halo& h, float center,
+                                         HaloV hv)
+{
+    p[h.ly][h.lx] = center;
+    p[h.hry][h.lx] = hv.r;
+    if (h.hc_any) p[h.hcy][h.hcx] = hv.c;
+}
+
+// fd4 along x and y of a staged plane at this thread's cell
+__device__ __forceinline__ float fdx(const float (&p)[EL_LY][EL_LX], const Halo& h, float ih)
+{
+    return (EL_C2 * (p[h.ly][h.lx + 2] - p[h.ly][h.lx - 2]) +
+            EL_C1 * (p[h.ly][h.lx + 1] - p[h.ly][h.lx - 1])) * ih;
+}
+__device__ __forceinline__ float fdy(const float (&p)[EL_LY][EL_LX], const Halo& h, float ih)
+{
+    return (EL_C2 * (p[h.ly + 2][h.lx] - p[h.ly - 2][h.lx]) +
+            EL_C1 * (p[h.ly + 1][h.lx] - p[h.ly - 1][h.lx])) * ih;
+}
+
 // ---------------------------------------------------------------- forward
-// stress stage s = 2i: sigma += dt*C:grad(v) over update_box(radius 2, s)
+// stress stage s = 2i: sigma += dt*C:grad(v) over update_box(radius 2, s).
+// v of the current plane is staged in LDS for the x and y derivatives; z
+// neighbours come from register queues.
 __global__ __launch_bounds__(EL_WG) void el_stress_kernel(const ElDev* __restrict__ tab, int nshot,
                                                         int stage)
 {
+    __shared__ float sv[3][EL_LY][EL_LX];
     const Tile T = decode_tile(tab, nshot);
     const ElDev& d = tab[T.shot];
-    if (T.b >= d.g.nblk) return;
+    if (T.b >= d.g.nblk) return;  // padding tile (uniform)
     const KGeo& g = d.g;
     const Box box = update_box(g, T.b, 2, stage);
+    const int z0 = max(T.zs, box.lo[2]), z1 = min(T.ze - 1, box.hi[2]);
+    if (z0 > z1) return;  // uniform
     const int x = T.x, y = T.y;
-    if (x < box.lo[0] || x > box.hi[0] || y < box.lo[1] || y > box.hi[1]) return;
+    const bool upd = x >= box.lo[0] && x <= box.hi[0] && y >= box.lo[1] && y <= box.hi[1];
     const long long boff = (long long)T.b * g.bstride + g.xoff;
     const gfloat* __restrict__ vx = G(d.f[F_VX] + boff);
     const gfloat* __restrict__ vy = G(d.f[F_VY] + boff);
@@ -178,60 +256,75 @@ __global__ __launch_bounds__(EL_WG) void el_stress_kernel(const ElDev* __restric
     gfloat* __restrict__ syz = G(d.f[F_SYZ] + boff);
     const uint32_t sy = 4u * g.pitch, sz = 4u * (uint32_t)g.slice;  // byte strides
     const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
-    const int z0 = max(T.zs, box.lo[2]), z1 = min(T.ze - 1, box.hi[2]);
-    if (z0 > z1) return;
-    uint32_t c = (uint32_t)z0 * sz + (uint32_t)y * sy + 4u * x;
+    const Halo h = el_halo(g, x, y, sy);
+    uint32_t c = (uint32_t)z0 * sz + (h.in ? (uint32_t)y * sy + 4u * x : 0u);
     float qx[5], qy[5], qz[5];  // v along z, planes z-2..z+2
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint32_t q = c - 2 * sz + k * sz;
-        qx[k] = ldo(vx, q);
-        qy[k] = ldo(vy, q);
-        qz[k] = ldo(vz, q);
+        qx[k] = h.in ? ldo(vx, q) : 0.0f;
+        qy[k] = h.in ? ldo(vy, q) : 0.0f;
+        qz[k] = h.in ? ldo(vz, q) : 0.0f;
     }
     for (int z = z0; z <= z1; ++z, c += sz) {
-        qx[4] = ldo(vx, c + 2 * sz);
-        qy[4] = ldo(vy, c + 2 * sz);
-        qz[4] = ldo(vz, c + 2 * sz);
-        const float exx = fd4(vx, c, 4, i0);
-        const float eyy = fd4(vy, c, sy, i1);
-        const float ezz = fdq(qz, i2);
-        const float dxy = fd4(vx, c, sy, i1) + fd4(vy, c, 4, i0);
-        const float dxz = fdq(qx, i2) + fd4(vz, c, 4, i0);
-        const float dyz = fdq(qy, i2) + fd4(vz, c, sy, i1);
+        qx[4] = h.in ? ldo(vx, c + 2 * sz) : 0.0f;
+        qy[4] = h.in ? ldo(vy, c + 2 * sz) : 0.0f;
+        qz[4] = h.in ? ldo(vz, c + 2 * sz) : 0.0f;
+        const uint32_t pz = (uint32_t)z * sz;
+        const HaloV hx = el_halo_load(vx, h, pz), hy = el_halo_load(vy, h, pz),
+                    hz = el_halo_load(vz, h, pz);
+        __syncthreads();  // the previous plane's reads are done
+        el_stage(sv[0], h, qx[2], hx);
+        el_stage(sv[1], h, qy[2], hy);
+        el_stage(sv[2], h, qz[2], hz);
+        __syncthreads();
+        if (upd) {
+            const float exx = fdx(sv[0], h, i0);
+            const float eyy = fdy(sv[1], h, i1);
+            const float ezz = fdq(qz, i2);
+            const float dxy = fdy(sv[0], h, i1) + fdx(sv[1], h, i0);
+            const float dxz = fdq(qx, i2) + fdx(sv[2], h, i0);
+            const float dyz = fdq(qy, i2) + fdy(sv[2], h, i1);
+            const float l = ldo(lam, c), m = ldo(mu, c);
+            const float l2m = l + 2.0f * m;
+            sto(sxx, c, ldo(sxx, c) + dt * (l2m * exx + l * (eyy + ezz)));
+            sto(syy, c, ldo(syy, c) + dt * (l2m * eyy + l * (exx + ezz)));
+            sto(szz, c, ldo(szz, c) + dt * (l2m * ezz + l * (exx + eyy)));
+            sto(sxy, c, ldo(sxy, c) + dt * (m * dxy));
+            sto(sxz, c, ldo(sxz, c) + dt * (m * dxz));
+            sto(syz, c, ldo(syz, c) + dt * (m * dyz));
+        }
         qshift(qx);
         qshift(qy);
         qshift(qz);
-        const float l = ldo(lam, c), m = ldo(mu, c);
-        const float l2m = l + 2.0f * m;
-        sto(sxx, c, ldo(sxx, c) + dt * (l2m * exx + l * (eyy + ezz)));
-        sto(syy, c, ldo(syy, c) + dt * (l2m * eyy + l * (exx + ezz)));
-        sto(szz, c, ldo(szz, c) + dt * (l2m * ezz + l * (exx + eyy)));
-        sto(sxy, c, ldo(sxy, c) + dt * (m * dxy));
-        sto(sxz, c, ldo(sxz, c) + dt * (m * dxz));
-        sto(syz, c, ldo(syz, c) + dt * (m * dyz));
     }
 }
 
-// velocity stage s = 2i+1 for the components in MASK (1 x, 2 y, 4 z)
+// velocity stage s = 2i+1 for the components in MASK (1 x, 2 y, 4 z).
+// The stresses with x or y derivatives are staged in LDS per plane; z
+// neighbours come from register queues rounded exactly as direct loads.
 template <int MASK>
 __global__ __launch_bounds__(EL_WG) void el_velocity_kernel(const ElDev* __restrict__ tab, int nshot,
                                                           int stage)
 {
+    // staged planes: 0 sxx (vx), 1 syy (vy), 2 sxy (vx, vy), 3 sxz (vz), 4 syz (vz)
+    constexpr bool ST[5] = {(MASK & 1) != 0, (MASK & 2) != 0, (MASK & 3) != 0, (MASK & 4) != 0,
+                            (MASK & 4) != 0};
+    __shared__ float sv[5][EL_LY][EL_LX];
     const Tile T = decode_tile(tab, nshot);
     const ElDev& d = tab[T.shot];
-    if (T.b >= d.g.nblk) return;
+    if (T.b >= d.g.nblk) return;  // padding tile (uniform)
     const KGeo& g = d.g;
     const Box box = update_box(g, T.b, 2, stage);
+    const int z0 = max(T.zs, box.lo[2]), z1 = min(T.ze - 1, box.hi[2]);
+    if (z0 > z1) return;  // uniform
     const int x = T.x, y = T.y;
-    if (x < box.lo[0] || x > box.hi[0] || y < box.lo[1] || y > box.hi[1]) return;
+    const bool upd = x >= box.lo[0] && x <= box.hi[0] && y >= box.lo[1] && y <= box.hi[1];
     const long long boff = (long long)T.b * g.bstride + g.xoff;
-    const gfloat* __restrict__ sxx = G(d.f[F_SXX] + boff);
-    const gfloat* __restrict__ syy = G(d.f[F_SYY] + boff);
+    const gfloat* __restrict__ sf[5] = {G(d.f[F_SXX] + boff), G(d.f[F_SYY] + boff),
+                                        G(d.f[F_SXY] + boff), G(d.f[F_SXZ] + boff),
+                                        G(d.f[F_SYZ] + boff)};
     const gfloat* __restrict__ szz = G(d.f[F_SZZ] + boff);
-    const gfloat* __restrict__ sxy = G(d.f[F_SXY] + boff);
-    const gfloat* __restrict__ sxz = G(d.f[F_SXZ] + boff);
-    const gfloat* __restrict__ syz = G(d.f[F_SYZ] + boff);
     const gfloat* __restrict__ bb = G(d.f[F_B] + boff);
     gfloat* __restrict__ vx = G(d.f[F_VX] + boff);
     gfloat* __restrict__ vy = G(d.f[F_VY] + boff);
@@ -239,41 +332,57 @@ __global__ __launch_bounds__(EL_WG) void el_velocity_kernel(const ElDev* __restr
     gfloat* __restrict__ disp = d.disp ? G(d.disp + boff) : nullptr;
     const uint32_t sy = 4u * g.pitch, sz = 4u * (uint32_t)g.slice;  // byte strides
     const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
-    const int z0 = max(T.zs, box.lo[2]), z1 = min(T.ze - 1, box.hi[2]);
-    if (z0 > z1) return;
-    uint32_t c = (uint32_t)z0 * sz + (uint32_t)y * sy + 4u * x;
-    // z-derivative fields (sxz for vx, syz for vy, szz for vz) from register
-    // queues over planes z-2..z+2, rounded exactly as direct loads
+    const Halo h = el_halo(g, x, y, sy);
+    uint32_t c = (uint32_t)z0 * sz + (h.in ? (uint32_t)y * sy + 4u * x : 0u);
+    // z-derivative fields (sxz for vx, syz for vy, szz for vz), planes z-2..z+2
     float qa[5], qb[5], qc[5];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint32_t q = c - 2 * sz + k * sz;
-        if (MASK & 1) qa[k] = ldo(sxz, q);
-        if (MASK & 2) qb[k] = ldo(syz, q);
-        if (MASK & 4) qc[k] = ldo(szz, q);
+        if (MASK & 1) qa[k] = h.in ? ldo(sf[3], q) : 0.0f;
+        if (MASK & 2) qb[k] = h.in ? ldo(sf[4], q) : 0.0f;
+        if (MASK & 4) qc[k] = h.in ? ldo(szz, q) : 0.0f;
     }
     for (int z = z0; z <= z1; ++z, c += sz) {
         const uint32_t q = c + 2 * sz;
-        const float tb = dt * ldo(bb, c);
-        if (MASK & 1) {
-            qa[4] = ldo(sxz, q);
-            const float ax = (fd4(sxx, c, 4, i0) + fd4(sxy, c, sy, i1)) + fdq(qa, i2);
-            qshift(qa);
-            sto(vx, c, ldo(vx, c) + tb * ax);
+        if (MASK & 1) qa[4] = h.in ? ldo(sf[3], q) : 0.0f;
+        if (MASK & 2) qb[4] = h.in ? ldo(sf[4], q) : 0.0f;
+        if (MASK & 4) qc[4] = h.in ? ldo(szz, q) : 0.0f;
+        const uint32_t pz = (uint32_t)z * sz;
+        float cv[5];
+        HaloV hv[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            if (!ST[k]) continue;
+            if (k == 3 && (MASK & 1)) cv[k] = qa[2];
+            else if (k == 4 && (MASK & 2)) cv[k] = qb[2];
+            else cv[k] = h.in ? ldo(sf[k], c) : 0.0f;
+            hv[k] = el_halo_load(sf[k], h, pz);
         }
-        if (MASK & 2) {
-            qb[4] = ldo(syz, q);
-            const float ay = (fd4(sxy, c, 4, i0) + fd4(syy, c, sy, i1)) + fdq(qb, i2);
-            qshift(qb);
-            sto(vy, c, ldo(vy, c) + tb * ay);
-            if (disp) sto(disp, c, ldo(disp, c) + (dt * tb) * ay);
+        __syncthreads();  // the previous plane's reads are done
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            if (ST[k]) el_stage(sv[k], h, cv[k], hv[k]);
+        __syncthreads();
+        if (upd) {
+            const float tb = dt * ldo(bb, c);
+            if (MASK & 1) {
+                const float ax = (fdx(sv[0], h, i0) + fdy(sv[2], h, i1)) + fdq(qa, i2);
+                sto(vx, c, ldo(vx, c) + tb * ax);
+            }
+            if (MASK & 2) {
+                const float ay = (fdx(sv[2], h, i0) + fdy(sv[1], h, i1)) + fdq(qb, i2);
+                sto(vy, c, ldo(vy, c) + tb * ay);
+                if (disp) sto(disp, c, ldo(disp, c) + (dt * tb) * ay);
+            }
+            if (MASK & 4) {
+                const float az = (fdx(sv[3], h, i0) + fdy(sv[4], h, i1)) + fdq(qc, i2);
+                sto(vz, c, ldo(vz, c) + tb * az);
+            }
         }
-        if (MASK & 4) {
-            qc[4] = ldo(szz, q);
-            const float az = (fd4(sxz, c, 4, i0) + fd4(syz, c, sy, i1)) + fdq(qc, i2);
-            qshift(qc);
-            sto(vz, c, ldo(vz, c) + tb * az);
-        }
+        if (MASK & 1) qshift(qa);
+        if (MASK & 2) qshift(qb);
+        if (MASK & 4) qshift(qc);
     }
 }
 
@@ -369,28 +478,27 @@ __global__ void el_tick_kernel(int* tcount, int n, int delta)
 // register queues rounded exactly as direct loads.
 
 // velocity stage: a = M sigma' (rounded as the forward velocity stage);
-// g_b += dt (w/b).a;  REV: v -= dt b a
+// g_b += dt (w/b).a;  REV: v -= dt b a.  The stresses are staged in LDS for
+// the x and y derivatives as in el_velocity_kernel.
 template <bool REV>
 __global__ __launch_bounds__(EL_WG) void el_adj_vel_kernel(const ElDev* __restrict__ tab, int nshot)
 {
+    __shared__ float sv[5][EL_LY][EL_LX];  // sxx, syy, sxy, sxz, syz
     const Tile T = decode_tile(tab, nshot);
     const ElDev& d = tab[T.shot];
-    if (T.b >= d.g.nblk) return;  // padding tile
+    if (T.b >= d.g.nblk) return;  // padding tile (uniform)
     const KGeo& g = d.g;
-    const int h = g.h, x = T.x, y = T.y;
-    if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
-    const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
-    if (z0 > z1) return;
+    const int hh = g.h, x = T.x, y = T.y;
+    const int z0 = max(T.zs, hh), z1 = min(T.ze, hh + g.B[2]) - 1;
+    if (z0 > z1) return;  // uniform
+    const bool upd = x >= hh && x < hh + g.B[0] && y >= hh && y < hh + g.B[1];
     const long long boff = g.xoff;
     const uint32_t sy = 4u * g.pitch, sz = 4u * (uint32_t)g.slice;  // byte strides
     const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
     float* const* sg = REV ? d.f + F_SXX : d.fw1 + F_SXX;
-    const gfloat* __restrict__ sxx = G(sg[0] + boff);
-    const gfloat* __restrict__ syy = G(sg[1] + boff);
+    const gfloat* __restrict__ sf[5] = {G(sg[0] + boff), G(sg[1] + boff), G(sg[3] + boff),
+                                        G(sg[4] + boff), G(sg[5] + boff)};
     const gfloat* __restrict__ szz = G(sg[2] + boff);
-    const gfloat* __restrict__ sxy = G(sg[3] + boff);
-    const gfloat* __restrict__ sxz = G(sg[4] + boff);
-    const gfloat* __restrict__ syz = G(sg[5] + boff);
     const gfloat* __restrict__ b = G(d.f[F_B] + boff);
     const gfloat* __restrict__ wx = G(d.adj[0] + boff);
     const gfloat* __restrict__ wy = G(d.adj[1] + boff);
@@ -399,50 +507,68 @@ __global__ __launch_bounds__(EL_WG) void el_adj_vel_kernel(const ElDev* __restri
     gfloat* __restrict__ vx = G(d.f[F_VX] + boff);
     gfloat* __restrict__ vy = G(d.f[F_VY] + boff);
     gfloat* __restrict__ vz = G(d.f[F_VZ] + boff);
-    uint32_t c = (uint32_t)z0 * sz + (uint32_t)y * sy + 4u * x;
+    const Halo h = el_halo(g, x, y, sy);
+    uint32_t c = (uint32_t)z0 * sz + (h.in ? (uint32_t)y * sy + 4u * x : 0u);
     float qa[5], qb[5], qc[5];  // sxz, syz, szz along z
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint32_t q = c - 2 * sz + k * sz;
-        qa[k] = ldo(sxz, q);
-        qb[k] = ldo(syz, q);
-        qc[k] = ldo(szz, q);
+        qa[k] = h.in ? ldo(sf[3], q) : 0.0f;
+        qb[k] = h.in ? ldo(sf[4], q) : 0.0f;
+        qc[k] = h.in ? ldo(szz, q) : 0.0f;
     }
     for (int z = z0; z <= z1; ++z, c += sz) {
         const uint32_t q = c + 2 * sz;
-        qa[4] = ldo(sxz, q);
-        qb[4] = ldo(syz, q);
-        qc[4] = ldo(szz, q);
-        const float ax = (fd4(sxx, c, 4, i0) + fd4(sxy, c, sy, i1)) + fdq(qa, i2);
-        const float ay = (fd4(sxy, c, 4, i0) + fd4(syy, c, sy, i1)) + fdq(qb, i2);
-        const float az = (fd4(sxz, c, 4, i0) + fd4(syz, c, sy, i1)) + fdq(qc, i2);
+        qa[4] = h.in ? ldo(sf[3], q) : 0.0f;
+        qb[4] = h.in ? ldo(sf[4], q) : 0.0f;
+        qc[4] = h.in ? ldo(szz, q) : 0.0f;
+        const uint32_t pz = (uint32_t)z * sz;
+        float cv[5];
+        HaloV hv[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            cv[k] = k == 3 ? qa[2] : k == 4 ? qb[2] : (h.in ? ldo(sf[k], c) : 0.0f);
+            hv[k] = el_halo_load(sf[k], h, pz);
+        }
+        __syncthreads();  // the previous plane's reads are done
+#pragma unroll
+        for (int k = 0; k < 5; ++k) el_stage(sv[k], h, cv[k], hv[k]);
+        __syncthreads();
+        if (upd) {
+            const float ax = (fdx(sv[0], h, i0) + fdy(sv[2], h, i1)) + fdq(qa, i2);
+            const float ay = (fdx(sv[2], h, i0) + fdy(sv[1], h, i1)) + fdq(qb, i2);
+            const float az = (fdx(sv[3], h, i0) + fdy(sv[4], h, i1)) + fdq(qc, i2);
+            const float bc = ldo(b, c);
+            sto(gb, c,
+                ldo(gb, c) + dt * ((ldo(wx, c) * ax + ldo(wy, c) * ay + ldo(wz, c) * az) / bc));
+            if (REV) {
+                const float tb = dt * bc;
+                sto(vx, c, ldo(vx, c) - tb * ax);
+                sto(vy, c, ldo(vy, c) - tb * ay);
+                sto(vz, c, ldo(vz, c) - tb * az);
+            }
+        }
         qshift(qa);
         qshift(qb);
         qshift(qc);
-        const float bc = ldo(b, c);
-        sto(gb, c, ldo(gb, c) + dt * ((ldo(wx, c) * ax + ldo(wy, c) * ay + ldo(wz, c) * az) / bc));
-        if (REV) {
-            const float tb = dt * bc;
-            sto(vx, c, ldo(vx, c) - tb * ax);
-            sto(vy, c, ldo(vy, c) - tb * ay);
-            sto(vz, c, ldo(vz, c) - tb * az);
-        }
     }
 }
 
 // stress stage: e = E v (rounded as the forward stress stage), sh = C^-1 p;
-// g_lambda, g_mu += ...;  REV: sigma -= dt C e
+// g_lambda, g_mu += ...;  REV: sigma -= dt C e.  v is staged in LDS for the x
+// and y derivatives as in el_stress_kernel.
 template <bool REV>
 __global__ __launch_bounds__(EL_WG) void el_adj_str_kernel(const ElDev* __restrict__ tab, int nshot)
 {
+    __shared__ float sv[3][EL_LY][EL_LX];
     const Tile T = decode_tile(tab, nshot);
     const ElDev& d = tab[T.shot];
-    if (T.b >= d.g.nblk) return;  // padding tile
+    if (T.b >= d.g.nblk) return;  // padding tile (uniform)
     const KGeo& g = d.g;
-    const int h = g.h, x = T.x, y = T.y;
-    if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
-    const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
-    if (z0 > z1) return;
+    const int hh = g.h, x = T.x, y = T.y;
+    const int z0 = max(T.zs, hh), z1 = min(T.ze, hh + g.B[2]) - 1;
+    if (z0 > z1) return;  // uniform
+    const bool upd = x >= hh && x < hh + g.B[0] && y >= hh && y < hh + g.B[1];
     const long long boff = g.xoff;
     const uint32_t sy = 4u * g.pitch, sz = 4u * (uint32_t)g.slice;  // byte strides
     const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
@@ -452,58 +578,70 @@ __global__ __launch_bounds__(EL_WG) void el_adj_str_kernel(const ElDev* __restri
     const gfloat* __restrict__ vz = G(vv[2] + boff);
     const gfloat* __restrict__ lam = G(d.f[F_LAM] + boff);
     const gfloat* __restrict__ mu = G(d.f[F_MU] + boff);
-    uint32_t c = (uint32_t)z0 * sz + (uint32_t)y * sy + 4u * x;
+    const Halo h = el_halo(g, x, y, sy);
+    uint32_t c = (uint32_t)z0 * sz + (h.in ? (uint32_t)y * sy + 4u * x : 0u);
     float qx[5], qy[5], qz[5];  // v along z
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint32_t q = c - 2 * sz + k * sz;
-        qx[k] = ldo(vx, q);
-        qy[k] = ldo(vy, q);
-        qz[k] = ldo(vz, q);
+        qx[k] = h.in ? ldo(vx, q) : 0.0f;
+        qy[k] = h.in ? ldo(vy, q) : 0.0f;
+        qz[k] = h.in ? ldo(vz, q) : 0.0f;
     }
     for (int z = z0; z <= z1; ++z, c += sz) {
         const uint32_t q = c + 2 * sz;
-        qx[4] = ldo(vx, q);
-        qy[4] = ldo(vy, q);
-        qz[4] = ldo(vz, q);
-        const float exx = fd4(vx, c, 4, i0);
-        const float eyy = fd4(vy, c, sy, i1);
-        const float ezz = fdq(qz, i2);
-        const float dxy = fd4(vx, c, sy, i1) + fd4(vy, c, 4, i0);
-        const float dxz = fdq(qx, i2) + fd4(vz, c, 4, i0);
-        const float dyz = fdq(qy, i2) + fd4(vz, c, sy, i1);
+        qx[4] = h.in ? ldo(vx, q) : 0.0f;
+        qy[4] = h.in ? ldo(vy, q) : 0.0f;
+        qz[4] = h.in ? ldo(vz, q) : 0.0f;
+        const uint32_t pz = (uint32_t)z * sz;
+        const HaloV hx = el_halo_load(vx, h, pz), hy = el_halo_load(vy, h, pz),
+                    hz = el_halo_load(vz, h, pz);
+        __syncthreads();  // the previous plane's reads are done
+        el_stage(sv[0], h, qx[2], hx);
+        el_stage(sv[1], h, qy[2], hy);
+        el_stage(sv[2], h, qz[2], hz);
+        __syncthreads();
+        if (upd) {
+            const float exx = fdx(sv[0], h, i0);
+            const float eyy = fdy(sv[1], h, i1);
+            const float ezz = fdq(qz, i2);
+            const float dxy = fdy(sv[0], h, i1) + fdx(sv[1], h, i0);
+            const float dxz = fdq(qx, i2) + fdx(sv[2], h, i0);
+            const float dyz = fdq(qy, i2) + fdy(sv[2], h, i1);
+            const float l = ldo(lam, c), m = ldo(mu, c);
+            const float pxx = ldo(G(d.adj[3] + boff), c), pyy = ldo(G(d.adj[4] + boff), c),
+                        pzz = ldo(G(d.adj[5] + boff), c);
+            const float trs = ((pxx + pyy) + pzz) / (3.0f * l + 2.0f * m);
+            const float im2 = 0.5f / m;
+            const float hxx = (pxx - l * trs) * im2, hyy = (pyy - l * trs) * im2,
+                        hzz = (pzz - l * trs) * im2;
+            const float hxy = ldo(G(d.adj[6] + boff), c) / m,
+                        hxz = ldo(G(d.adj[7] + boff), c) / m,
+                        hyz = ldo(G(d.adj[8] + boff), c) / m;
+            gfloat* __restrict__ gl = G(d.grad[0] + boff);
+            gfloat* __restrict__ gm = G(d.grad[1] + boff);
+            sto(gl, c, ldo(gl, c) + dt * (trs * ((exx + eyy) + ezz)));
+            sto(gm, c, ldo(gm, c) + dt * (2.0f * (hxx * exx + hyy * eyy + hzz * ezz) +
+                                          hxy * dxy + hxz * dxz + hyz * dyz));
+            if (REV) {
+                const float l2m = l + 2.0f * m;
+                gfloat* __restrict__ sg = G(d.f[F_SXX] + boff);
+                sto(sg, c, ldo(sg, c) - dt * (l2m * exx + l * (eyy + ezz)));
+                sg = G(d.f[F_SYY] + boff);
+                sto(sg, c, ldo(sg, c) - dt * (l2m * eyy + l * (exx + ezz)));
+                sg = G(d.f[F_SZZ] + boff);
+                sto(sg, c, ldo(sg, c) - dt * (l2m * ezz + l * (exx + eyy)));
+                sg = G(d.f[F_SXY] + boff);
+                sto(sg, c, ldo(sg, c) - dt * (m * dxy));
+                sg = G(d.f[F_SXZ] + boff);
+                sto(sg, c, ldo(sg, c) - dt * (m * dxz));
+                sg = G(d.f[F_SYZ] + boff);
+                sto(sg, c, ldo(sg, c) - dt * (m * dyz));
+            }
+        }
         qshift(qx);
         qshift(qy);
         qshift(qz);
-        const float l = ldo(lam, c), m = ldo(mu, c);
-        const float pxx = ldo(G(d.adj[3] + boff), c), pyy = ldo(G(d.adj[4] + boff), c),
-                    pzz = ldo(G(d.adj[5] + boff), c);
-        const float trs = ((pxx + pyy) + pzz) / (3.0f * l + 2.0f * m);
-        const float im2 = 0.5f / m;
-        const float hxx = (pxx - l * trs) * im2, hyy = (pyy - l * trs) * im2,
-                    hzz = (pzz - l * trs) * im2;
-        const float hxy = ldo(G(d.adj[6] + boff), c) / m, hxz = ldo(G(d.adj[7] + boff), c) / m,
-                    hyz = ldo(G(d.adj[8] + boff), c) / m;
-        gfloat* __restrict__ gl = G(d.grad[0] + boff);
-        gfloat* __restrict__ gm = G(d.grad[1] + boff);
-        sto(gl, c, ldo(gl, c) + dt * (trs * ((exx + eyy) + ezz)));
-        sto(gm, c, ldo(gm, c) + dt * (2.0f * (hxx * exx + hyy * eyy + hzz * ezz) + hxy * dxy +
-                                      hxz * dxz + hyz * dyz));
-        if (REV) {
-            const float l2m = l + 2.0f * m;
-            gfloat* __restrict__ sg = G(d.f[F_SXX] + boff);
-            sto(sg, c, ldo(sg, c) - dt * (l2m * exx + l * (eyy + ezz)));
-            sg = G(d.f[F_SYY] + boff);
-            sto(sg, c, ldo(sg, c) - dt * (l2m * eyy + l * (exx + ezz)));
-            sg = G(d.f[F_SZZ] + boff);
-            sto(sg, c, ldo(sg, c) - dt * (l2m * ezz + l * (exx + eyy)));
-            sg = G(d.f[F_SXY] + boff);
-            sto(sg, c, ldo(sg, c) - dt * (m * dxy));
-            sg = G(d.f[F_SXZ] + boff);
-            sto(sg, c, ldo(sg, c) - dt * (m * dxz));
-            sg = G(d.f[F_SYZ] + boff);
-            sto(sg, c, ldo(sg, c) - dt * (m * dyz));
-        }
     }
 }
 
