@@ -111,10 +111,11 @@ extern "C" int b2r_create(const b2r_geom* geom, int device, b2r_ctx** out)
     L.nblocks = bd[0] * bd[1] * bd[2];
     L.elem_size = g.elem_size;
     if (g.elem_size == 1) {
-        // byte cells: owned x = h lands on a 16-byte boundary, rows padded to 16
-        // bytes with 3 spare bytes so any dword touching [0,S_x) is in the row.
-        L.x_offset = (16 - h % 16) % 16;
-        L.pitch = (int)round_up(L.x_offset + L.S[0] + 3, 16);
+        // byte cells: owned x = h lands on a 32-byte boundary, rows padded to 32
+        // bytes with 3 spare bytes, so any dword touching [0,S_x) and any
+        // 16- or 32-cell group on the grid of x = h is in the row.
+        L.x_offset = (32 - h % 32) % 32;
+        L.pitch = (int)round_up(L.x_offset + L.S[0] + 3, 32);
     } else {
         L.x_offset = 0;
         L.pitch = L.S[0];
@@ -188,7 +189,7 @@ extern "C" int b2r_set_tuning(b2r_ctx* ctx, const b2r_tuning* t)
               "negative z chunk");
     B2R_CHECK(t->heat_wy == 0 || t->heat_wy == 4 || t->heat_wy == 8 || t->heat_wy == 16,
               B2R_E_ARG, "heat_wy must be 0, 4, 8 or 16");
-    B2R_CHECK(t->gol_tile >= 0 && t->gol_tile <= 2, B2R_E_ARG, "gol_tile must be 0, 1 or 2");
+    B2R_CHECK(t->gol_tile >= 0 && t->gol_tile <= 6, B2R_E_ARG, "gol_tile must be 0..6");
     ctx->tune = *t;
     return B2R_OK;
 }

@@ -163,9 +163,13 @@ __global__ __launch_bounds__(256) void gol_step_kernel(const uint8_t* __restrict
 // at a frozen face with their unchanged values, elsewhere with values the
 // next exchange overwrites.
 // ---------------------------------------------------------------------------
-// Tile shapes: TXQ output quads (16 cells) per tile row, NR rows including the
-// R halo rows per side; gol_tile 0 (default), 1, 2 select them.
+// Tile shapes: gol_tile 1, 2, 3 select this kernel with TXQ output quads (16
+// cells) per tile row and NR rows including the R halo rows per side
+// (GF_SHAPES); gol_tile 4, 5, 6 select the bit-sliced kernel below with NW
+// output words (32 cells) per row and NR rows (GB_SHAPES); 0 (automatic) is
+// the bit-sliced 8 x 64 shape.
 constexpr int GF_SHAPES[3][2] = {{8, 32}, {8, 48}, {16, 56}};
+constexpr int GB_SHAPES[3][2] = {{8, 32}, {8, 64}, {8, 96}};
 constexpr int GF_PD = 2;  // input planes in flight per thread
 
 __device__ __forceinline__ uint4 q_add(uint4 a, uint4 b)
@@ -322,6 +326,218 @@ __global__ __launch_bounds__((TXQ + 2) * NR) void gol_fused_kernel(const uint8_t
     }
 }
 
+// ---------------------------------------------------------------------------
+// Bit-sliced form of the fused kernel (gol_tile 0, 4..6): R generations per launch
+// as above, but a thread owns a 32-cell word of a row, bit i = cell x+i,
+// packed from the byte cells when a plane is loaded and unpacked when
+// generation R is stored.  Per generation and plane:
+//   S   = z-sum of its three planes of generation t-1, two bit planes (0..3);
+//   YZ  = S(y-1) + S + S(y+1) through an LDS plane, four bit planes (0..9);
+//   box = YZ(x-1) + YZ + YZ(x+1) through a second LDS plane, the x
+//         neighbours shifted in across word boundaries, five bit planes;
+//   the rule: alive next iff box == 5, or alive and 3 <= box <= 8.
+// Every adder is built from full adders of two 3-input logic operations
+// (xor3, majority), about 45 operations per 32 cells and generation against
+// about 150 for the byte-lane form.  Rows of byte fields are laid out so that
+// owned x = h sits on a 32-byte boundary and a word never crosses the padded
+// row end.  Cells hold 0 or 1.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t dswap(uint32_t x, int d, uint32_t m)
+{
+    const uint32_t t = ((x >> d) ^ x) & m;
+    return x ^ t ^ (t << d);
+}
+
+// 32 byte cells (a: cells 0..15, b: 16..31) -> one word, bit i = cell i
+__device__ __forceinline__ uint32_t gb_pack(uint4 a, uint4 b)
+{
+    const uint32_t m = 0x01010101u;
+    // byte j of dword k is cell 4k+j; dword k shifted by k puts it at bit
+    // 8j+k, then four delta swaps move bit 8j+k to bit 4k+j (the index bits
+    // (j1 j0 k2 k1 k0) rotate to (k2 k1 k0 j1 j0))
+    uint32_t w = (a.x & m) | ((a.y & m) << 1) | ((a.z & m) << 2) | ((a.w & m) << 3) |
+                 ((b.x & m) << 4) | ((b.y & m) << 5) | ((b.z & m) << 6) | ((b.w & m) << 7);
+    w = dswap(w, 12, 0x0000F0F0u);  // index bits 4 <-> 2
+    w = dswap(w, 6, 0x00CC00CCu);   // 3 <-> 1
+    w = dswap(w, 3, 0x0A0A0A0Au);   // 2 <-> 0
+    w = dswap(w, 1, 0x22222222u);   // 1 <-> 0
+    return w;
+}
+
+__device__ __forceinline__ void gb_unpack(uint32_t w, uint4& a, uint4& b)
+{
+    w = dswap(w, 1, 0x22222222u);
+    w = dswap(w, 3, 0x0A0A0A0Au);
+    w = dswap(w, 6, 0x00CC00CCu);
+    w = dswap(w, 12, 0x0000F0F0u);
+    const uint32_t m = 0x01010101u;
+    a = make_uint4(w & m, (w >> 1) & m, (w >> 2) & m, (w >> 3) & m);
+    b = make_uint4((w >> 4) & m, (w >> 5) & m, (w >> 6) & m, (w >> 7) & m);
+}
+
+__device__ __forceinline__ uint32_t maj3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return (a & b) | (c & (a ^ b));
+}
+
+// bits of the cells x .. x+31 inside [lo, hi]
+__device__ __forceinline__ uint32_t word_mask(int x, int lo, int hi)
+{
+    const int a = max(lo - x, 0), e = min(hi - x, 31);
+    if (a > e) return 0u;
+    const uint32_t upto = e >= 31 ? ~0u : (2u << e) - 1u;
+    return upto & ~((1u << a) - 1u);
+}
+
+template <int R, int NW, int NR>
+__global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* __restrict__ in,
+                                                                 uint8_t* __restrict__ out,
+                                                                 KGeo g, int ntx, int nty,
+                                                                 int nzc, int zc)
+{
+    constexpr int NQ = NW + 2;  // words per row: plus one per side
+    constexpr int NT = NQ * NR;
+    constexpr int TY = NR - 2 * R;
+    __shared__ uint2 sb[NT + 2 * NQ];  // z-sums of a plane, a spare row above and below
+    __shared__ uint4 yb[NT + 2];       // y-sums, a spare word before and after
+
+    const int tid = threadIdx.x;
+    const int r = tid / NQ, q = tid % NQ;
+    int tt = xcd_tile(blockIdx.x, gridDim.x);
+    const int txi = tt % ntx;
+    tt /= ntx;
+    const int tyi = tt % nty;
+    tt /= nty;
+    const int zci = tt % nzc;
+    const int b = tt / nzc;
+    const int h = g.h;
+    const int x0 = h + txi * 32 * NW;  // first output cell
+    const int y0 = h + tyi * TY;
+    const int zc0 = h + zci * zc;
+    const int zc1 = min(zc0 + zc, h + g.B[2]);
+    const long long boff = (long long)b * g.bstride + g.xoff;
+    const uint8_t* __restrict__ src = in + boff;
+    uint8_t* __restrict__ dst = out + boff;
+    const long long slice = g.slice;
+    const int pitch = g.pitch;
+
+    // this thread's word: x = x .. x+31, row y
+    const int x = x0 + 32 * (q - 1);
+    const int y = y0 - R + r;
+    const bool ld_ok = y < g.S[1] && x + 31 >= 0 && x < g.S[0];
+    // every thread loads (no branch): threads outside the block read the
+    // tile's first word of the last row, in bounds and aligned, and use zero
+    const uint8_t* __restrict__ col =
+        src + (ld_ok ? (long long)y * pitch + x : (long long)min(y, g.S[1] - 1) * pitch + x0);
+    const uint32_t ldm = ld_ok ? ~0u : 0u;
+    // a halo word needs only its 16 cells next to the tile (R <= 4): its other
+    // half reloads the same 16 bytes (no extra traffic) and is masked off
+    const int oa = q == 0 ? 16 : 0, ob = q == NQ - 1 ? 0 : 16;
+    const uint32_t ma = q == 0 ? 0u : ~0u, mb = q == NQ - 1 ? 0u : ~0u;
+
+    Box bx[R];
+    bool clip[R];
+    uint32_t xm[R];  // bits inside generation t's box along x
+#pragma unroll
+    for (int t = 1; t <= R; ++t) {
+        bx[t - 1] = update_box(g, b, 1, t - 1);
+        const int gr = R - t;
+        const Box& B_ = bx[t - 1];
+        clip[t - 1] = x0 - 32 < B_.lo[0] || x0 + 32 * (NW + 1) - 1 > B_.hi[0] ||
+                      y0 - gr < B_.lo[1] || y0 + TY + gr - 1 > B_.hi[1] ||
+                      zc0 - gr < B_.lo[2] || zc1 + gr - 1 > B_.hi[2];
+        xm[t - 1] = word_mask(x, B_.lo[0], B_.hi[0]);
+    }
+    const bool out_ok = q >= 1 && q <= NQ - 2 && r >= R && r < R + TY && y < h + g.B[1] &&
+                        x < h + g.B[0];
+
+    uint32_t gen[R][3];  // generation t at planes (newest-2, newest-1, newest)
+    const int zend = zc1 + R;
+    auto plane = [&](int zin, uint4& la, uint4& lb) {
+        gen[0][0] = gen[0][1];
+        gen[0][1] = gen[0][2];
+        gen[0][2] = gb_pack(make_uint4(la.x & ma, la.y & ma, la.z & ma, la.w & ma),
+                            make_uint4(lb.x & mb, lb.y & mb, lb.z & mb, lb.w & mb)) & ldm;
+        {  // the load GF_PD planes ahead (the last ones are unused)
+            const uint8_t* pp = col + (long long)min(zin + GF_PD, zend - 1) * slice;
+            la = *reinterpret_cast<const uint4*>(pp + oa);
+            lb = *reinterpret_cast<const uint4*>(pp + ob);
+        }
+#pragma unroll
+        for (int t = 1; t <= R; ++t) {
+            // uniform: generation t-1 not yet 3 planes, or a plane past the chunk
+            if (zin < zc0 - R + 2 * t || zin >= zend) continue;
+            const int p = zin - t;
+            const uint32_t ga = gen[t - 1][0], self = gen[t - 1][1], gc = gen[t - 1][2];
+            const uint32_t s0 = ga ^ self ^ gc, s1 = maj3(ga, self, gc);
+            sb[tid + NQ] = make_uint2(s0, s1);
+            __syncthreads();
+            const uint2 up = sb[tid], dn = sb[tid + 2 * NQ];
+            // YZ = up + S + dn, four bit planes
+            const uint32_t y0b = up.x ^ s0 ^ dn.x, k1 = maj3(up.x, s0, dn.x);
+            const uint32_t u = up.y ^ s1 ^ dn.y, v = maj3(up.y, s1, dn.y);
+            const uint32_t y1b = u ^ k1, w = u & k1;
+            const uint4 Y = make_uint4(y0b, y1b, v ^ w, v & w);
+            yb[tid + 1] = Y;
+            __syncthreads();
+            const uint4 L = yb[tid], Rg = yb[tid + 2];
+            // x neighbours: bit i of l is cell x+i-1, of rr cell x+i+1
+            const uint4 l = make_uint4(__builtin_amdgcn_alignbit(Y.x, L.x, 31),
+                                       __builtin_amdgcn_alignbit(Y.y, L.y, 31),
+                                       __builtin_amdgcn_alignbit(Y.z, L.z, 31),
+                                       __builtin_amdgcn_alignbit(Y.w, L.w, 31));
+            const uint4 rr = make_uint4(__builtin_amdgcn_alignbit(Rg.x, Y.x, 1),
+                                        __builtin_amdgcn_alignbit(Rg.y, Y.y, 1),
+                                        __builtin_amdgcn_alignbit(Rg.z, Y.z, 1),
+                                        __builtin_amdgcn_alignbit(Rg.w, Y.w, 1));
+            // box = l + Y + rr (each <= 9), five bit planes
+            const uint32_t b0 = l.x ^ Y.x ^ rr.x, c1 = maj3(l.x, Y.x, rr.x);
+            const uint32_t e1 = l.y ^ Y.y ^ rr.y, c2a = maj3(l.y, Y.y, rr.y);
+            const uint32_t b1 = e1 ^ c1, c2b = e1 & c1;
+            const uint32_t e2 = l.z ^ Y.z ^ rr.z, c3a = maj3(l.z, Y.z, rr.z);
+            const uint32_t b2 = e2 ^ c2a ^ c2b, c3b = maj3(e2, c2a, c2b);
+            const uint32_t e3 = l.w ^ Y.w ^ rr.w, c4a = maj3(l.w, Y.w, rr.w);
+            const uint32_t b3 = e3 ^ c3a ^ c3b, c4b = maj3(e3, c3a, c3b);
+            const uint32_t b4 = c4a | c4b;
+            // rule: box == 5, or alive and 3 <= box <= 8
+            const uint32_t le8 = ~b4 & ~(b3 & (b2 | b1 | b0));
+            const uint32_t ge3 = b4 | b3 | b2 | (b1 & b0);
+            const uint32_t eq5 = ~b4 & ~b3 & b2 & ~b1 & b0;
+            uint32_t nw = eq5 | (self & le8 & ge3);
+            if (clip[t - 1]) {
+                const Box& B_ = bx[t - 1];
+                const bool in_yz = y >= B_.lo[1] && y <= B_.hi[1] && p >= B_.lo[2] && p <= B_.hi[2];
+                const uint32_t m = in_yz ? xm[t - 1] : 0u;
+                nw = (nw & m) | (self & ~m);
+            }
+            if (t < R) {
+                gen[t][0] = gen[t][1];
+                gen[t][1] = gen[t][2];
+                gen[t][2] = nw;
+            } else if (out_ok) {
+                uint4 oa, ob;
+                gb_unpack(nw, oa, ob);
+                uint4* o = reinterpret_cast<uint4*>(dst + (long long)p * slice + (long long)y * pitch + x);
+                o[0] = oa;
+                o[1] = ob;
+            }
+        }
+    };
+    const int z0 = zc0 - R;
+    uint4 sl[GF_PD][2];
+#pragma unroll
+    for (int k = 0; k < GF_PD; ++k) {
+        const uint8_t* pp = col + (long long)min(z0 + k, zend - 1) * slice;
+        sl[k][0] = *reinterpret_cast<const uint4*>(pp + oa);
+        sl[k][1] = *reinterpret_cast<const uint4*>(pp + ob);
+    }
+    // whole groups of GF_PD planes; planes past zend compute nothing
+    for (int zin = z0; zin < zend; zin += GF_PD) {
+#pragma unroll
+        for (int k = 0; k < GF_PD; ++k) plane(zin + k, sl[k][0], sl[k][1]);
+    }
+}
+
 static bool gol_fused_ok(const b2r_ctx* ctx, int R)
 {
     return R >= 1 && R <= 4 && (ctx->k.periodic || ctx->k.fmask == 0x3F);
@@ -351,13 +567,39 @@ static int gol_fused_launch_t(b2r_ctx* ctx, const void* in, void* out)
     return B2R_OK;
 }
 
+template <int R, int NW, int NR>
+static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
+{
+    constexpr int TY = NR - 2 * R;
+    const KGeo& g = ctx->k;
+    const int ntx = ceil_div(g.B[0], 32 * NW);
+    const int nty = ceil_div(g.B[1], TY);
+    long long tiles = (long long)ntx * nty * g.nblk;
+    int zc = ctx->tune.gol_zc;
+    if (zc <= 0) {
+        int nzc = (int)((2048 + tiles - 1) / tiles);
+        zc = ceil_div(g.B[2], nzc < 1 ? 1 : nzc);
+        if (zc < 4 * R) zc = 4 * R;
+    }
+    if (zc > g.B[2]) zc = g.B[2];
+    const int nzc = ceil_div(g.B[2], zc);
+    dim3 grid(ntx * nty * nzc * g.nblk);
+    gol_bits_kernel<R, NW, NR><<<grid, (NW + 2) * NR, 0, ctx->stream>>>(
+        (const uint8_t*)in, (uint8_t*)out, g, ntx, nty, nzc, zc);
+    B2R_LAUNCH_CHECK();
+    return B2R_OK;
+}
+
 template <int R>
 static int gol_fused_launch_r(b2r_ctx* ctx, const void* in, void* out)
 {
     switch (ctx->tune.gol_tile) {
-    case 1: return gol_fused_launch_t<R, GF_SHAPES[1][0], GF_SHAPES[1][1]>(ctx, in, out);
-    case 2: return gol_fused_launch_t<R, GF_SHAPES[2][0], GF_SHAPES[2][1]>(ctx, in, out);
-    default: return gol_fused_launch_t<R, GF_SHAPES[0][0], GF_SHAPES[0][1]>(ctx, in, out);
+    case 1: return gol_fused_launch_t<R, GF_SHAPES[0][0], GF_SHAPES[0][1]>(ctx, in, out);
+    case 2: return gol_fused_launch_t<R, GF_SHAPES[1][0], GF_SHAPES[1][1]>(ctx, in, out);
+    case 3: return gol_fused_launch_t<R, GF_SHAPES[2][0], GF_SHAPES[2][1]>(ctx, in, out);
+    case 4: return gol_bits_launch_t<R, GB_SHAPES[0][0], GB_SHAPES[0][1]>(ctx, in, out);
+    case 6: return gol_bits_launch_t<R, GB_SHAPES[2][0], GB_SHAPES[2][1]>(ctx, in, out);
+    default: return gol_bits_launch_t<R, GB_SHAPES[1][0], GB_SHAPES[1][1]>(ctx, in, out);
     }
 }
 

@@ -94,10 +94,14 @@ def gol_bench(A, R, steps, warmup, fused=1, zc=0, record=None):
 
 
 # (output quads of 16 cells per tile row, rows incl. halo) of the gol_tile values
-GOL_TILES = {0: (8, 32), 1: (8, 48), 2: (16, 56)}
+GOL_TILES = {1: (8, 32), 2: (8, 48), 3: (16, 56)}
+# (output words of 32 cells per row, rows incl. halo) of the bit-sliced kernel
+GOL_BIT_TILES = {0: (8, 64), 4: (8, 32), 5: (8, 64), 6: (8, 96)}
 
 
 def gol_kernel_name(R, tile):
+    if tile in GOL_BIT_TILES:
+        return "gol_bits_kernel<%d, %d, %d>" % ((R,) + GOL_BIT_TILES[tile])
     return "gol_fused_kernel<%d, %d, %d>" % ((R,) + GOL_TILES[tile])
 
 
@@ -490,7 +494,7 @@ def profile_kernels(A):
         src, dst = c.dev_alloc(nbytes), c.dev_alloc(nbytes)
         log("stream_copy 1 GiB x3")
         c.stream_copy(dst, src, nbytes, 3)
-    for R, tile, zc in ((2, 2, 32), (4, 2, 16)):
+    for R, tile, zc in ((2, 6, 0), (4, 6, 8)):
         with A.Context((512, 512, 512), (2, 2, 2), R, 1, A.BOUNDARY_PERIODIC, 1) as c:
             a, b = c.field(), c.field()
             c.init_bernoulli(a, 1, int(0.3 * 16777216))

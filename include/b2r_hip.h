@@ -99,8 +99,9 @@ typedef struct b2r_tuning {
     int gol_step_zc;  /* z planes marched per single-step Game of Life workgroup    */
     int el_zc;        /* z planes marched per elastic workgroup                     */
     int heat_wy;      /* waves (thread rows) per heat step workgroup: 4, 8 or 16   */
-    int gol_tile;     /* fused Game of Life tile: 0 = 128 cells x 32 rows incl.
-                         halo rows, 1 = 128 x 48, 2 = 256 x 56                     */
+    int gol_tile;     /* fused Game of Life kernel and tile (cells x rows incl. halo
+                         rows): 1-3 byte-lane 128 x 32, 128 x 48, 256 x 56; 4-6
+                         bit-sliced 256 x 32, 256 x 64, 256 x 96; 0 = 5         */
 } b2r_tuning;
 
 /* ---- library / context ---------------------------------------------------- */

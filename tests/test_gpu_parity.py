@@ -280,20 +280,21 @@ def test_gol_full_size_R_invariance_and_population():
     env = (512, 512, 512)
     outs = {}
     for R in (1, 2, 4):
-        for fused in (0, 1):
-            say("gol 512^3 R", R, "fused", fused)
+        for fused, tile in ((0, 0), (1, 1), (1, 5)):
+            say("gol 512^3 R", R, "fused", fused, "tile", tile)
             with A.Context(env, (2, 2, 2), R, 1, 1, 1) as c:
+                c.set_tuning(gol_tile=tile)
                 a, b = c.field(), c.field()
                 c.init_bernoulli(a, 1, int(0.3 * 16777216))
                 res, _ = c.gol_run(a, b, 8, fused=fused)
-                outs[(R, fused)] = c.download(res)
+                outs[(R, fused, tile)] = c.download(res)
     u = O.bernoulli((512, 512, 512), 1, 0.3)
     want = O.gol(u, 8, periodic=True)
     for key, o in outs.items():
         assert np.array_equal(o, want), key
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("zc", [4, 9, 64])
 def test_gol_fused_chunking(zc, tile):
     # z-chunk lengths that do not divide B, and a single chunk, for every tile shape
@@ -326,11 +327,12 @@ def test_heat_tuning_variants(wy, ry, zc):
     assert bits_equal(got, want)
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("R", [1, 2, 3, 4])
 @pytest.mark.parametrize("periodic", [False, True])
 def test_gol_fused_tile_shapes(tile, R, periodic):
-    # every tile shape of the fused kernel, blocks narrower and wider than a tile
+    # every tile shape of the fused kernels (byte-lane 1-3, bit-sliced 4-6),
+    # blocks narrower and wider than a tile
     env, gens = (288, 40, 36), 2 * R + 1
     say("gol fused tile", tile, "R", R, "periodic", periodic)
     with A.Context(env, (2, 1, 1), R, 1, 1 if periodic else 0, 1) as c:
