@@ -111,11 +111,11 @@ extern "C" int b2r_create(const b2r_geom* geom, int device, b2r_ctx** out)
     L.nblocks = bd[0] * bd[1] * bd[2];
     L.elem_size = g.elem_size;
     if (g.elem_size == 1) {
-        // byte cells: owned x = h lands on a 32-byte boundary, rows padded to 32
+        // byte cells: owned x = h lands on a 16-byte boundary, rows padded to 16
         // bytes with 3 spare bytes, so any dword touching [0,S_x) and any
-        // 16- or 32-cell group on the grid of x = h is in the row.
-        L.x_offset = (32 - h % 32) % 32;
-        L.pitch = (int)round_up(L.x_offset + L.S[0] + 3, 32);
+        // 16-cell group on the grid of x = h is in the row.
+        L.x_offset = (16 - h % 16) % 16;
+        L.pitch = (int)round_up(L.x_offset + L.S[0] + 3, 16);
     } else {
         L.x_offset = 0;
         L.pitch = L.S[0];

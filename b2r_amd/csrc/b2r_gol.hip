@@ -338,9 +338,11 @@ __global__ __launch_bounds__((TXQ + 2) * NR) void gol_fused_kernel(const uint8_t
 //   the rule: alive next iff box == 5, or alive and 3 <= box <= 8.
 // Every adder is built from full adders of two 3-input logic operations
 // (xor3, majority), about 45 operations per 32 cells and generation against
-// about 150 for the byte-lane form.  Rows of byte fields are laid out so that
-// owned x = h sits on a 32-byte boundary and a word never crosses the padded
-// row end.  Cells hold 0 or 1.
+// about 150 for the byte-lane form.  Words lie on the 16-byte grid of x = h,
+// so each half of a word (16 cells) lies inside or outside the padded row as a
+// whole: halo words load only their half next to the tile, a row's last word
+// loads and stores its second half only when that half is in the row.  Cells
+// hold 0 or 1.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t dswap(uint32_t x, int d, uint32_t m)
 {
@@ -430,10 +432,14 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
     const uint8_t* __restrict__ col =
         src + (ld_ok ? (long long)y * pitch + x : (long long)min(y, g.S[1] - 1) * pitch + x0);
     const uint32_t ldm = ld_ok ? ~0u : 0u;
-    // a halo word needs only its 16 cells next to the tile (R <= 4): its other
-    // half reloads the same 16 bytes (no extra traffic) and is masked off
-    const int oa = q == 0 ? 16 : 0, ob = q == NQ - 1 ? 0 : 16;
-    const uint32_t ma = q == 0 ? 0u : ~0u, mb = q == NQ - 1 ? 0u : ~0u;
+    // a halo word needs only its 16 cells next to the tile (R <= 4), and the
+    // second half of a row's last word may lie past the padded row end: such a
+    // half reloads the word's other half (no extra traffic) and is masked off
+    const bool hi_in = x + 32 <= pitch - g.xoff;  // second half inside the row
+    const int oa = q == 0 && ld_ok ? 16 : 0;  // the clamped word reads its first half
+    const bool b_ld = ld_ok && q != NQ - 1 && hi_in;
+    const int ob = b_ld ? 16 : oa;
+    const uint32_t ma = q == 0 ? 0u : ~0u, mb = b_ld ? ~0u : 0u;
 
     Box bx[R];
     bool clip[R];
@@ -515,11 +521,11 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
                 gen[t][1] = gen[t][2];
                 gen[t][2] = nw;
             } else if (out_ok) {
-                uint4 oa, ob;
-                gb_unpack(nw, oa, ob);
+                uint4 lo, hi;
+                gb_unpack(nw, lo, hi);
                 uint4* o = reinterpret_cast<uint4*>(dst + (long long)p * slice + (long long)y * pitch + x);
-                o[0] = oa;
-                o[1] = ob;
+                o[0] = lo;
+                if (hi_in) o[1] = hi;
             }
         }
     };

@@ -10,8 +10,8 @@ def expected_layout(env, blocks, R, radius, esize):
     h = radius * R
     S = [b + 2 * h for b in B]
     if esize == 1:
-        xoff = (32 - h % 32) % 32
-        pitch = -(-(xoff + S[0] + 3) // 32) * 32
+        xoff = (16 - h % 16) % 16
+        pitch = -(-(xoff + S[0] + 3) // 16) * 16
     else:
         xoff, pitch = 0, S[0]
     slice_ = pitch * S[1]
