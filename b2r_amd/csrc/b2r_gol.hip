@@ -337,8 +337,8 @@ __global__ __launch_bounds__((TXQ + 2) * NR) void gol_fused_kernel(const uint8_t
 //         neighbours shifted in across word boundaries, five bit planes;
 //   the rule: alive next iff box == 5, or alive and 3 <= box <= 8.
 // Every adder is built from full adders of two 3-input logic operations
-// (xor3, majority), about 45 operations per 32 cells and generation against
-// about 150 for the byte-lane form.  Words lie on the 16-byte grid of x = h,
+// (xor3, majority); the compiled generation takes about 70 instructions per 32
+// cells against about 150 for the byte-lane form.  Words lie on the 16-byte grid of x = h,
 // so each half of a word (16 cells) lies inside or outside the padded row as a
 // whole: halo words load only their half next to the tile, a row's last word
 // loads and stores its second half only when that half is in the row.  Cells
