@@ -23,13 +23,12 @@
 constexpr int HEAT_TX = 64;
 constexpr int HEAT_WY = 4;  // thread rows (waves) of the fused kernel
 
-template <int D, int MODE, int RY, int WY>
-__global__ __launch_bounds__(HEAT_TX * WY) void heat_step_kernel(const float* __restrict__ in,
-                                                        float* __restrict__ out, KGeo g,
-                                                        HeatK hk, int b2r_i, int ntx, int nty,
-                                                        int nzc, int zc)
+template <int D, int MODE, int RY, int WY, int WX>
+__global__ __launch_bounds__(HEAT_TX * WX * WY) void heat_step_kernel(
+    const float* __restrict__ in, float* __restrict__ out, KGeo g, HeatK hk, int b2r_i, int ntx,
+    int nty, int nzc, int zc)
 {
-    constexpr int TX = HEAT_TX, TY = WY * RY, NT = TX * WY;
+    constexpr int TX = HEAT_TX * WX, TY = WY * RY, NT = TX * WY;
     constexpr int LX = TX + 2 * D, LY = TY + 2 * D;
     constexpr int NHY = 2 * D * TX, NHX = 2 * D * TY, NH = NHY + NHX;
     constexpr int NSLOT = (NH + NT - 1) / NT;
@@ -188,12 +187,12 @@ int b2r_heat_check(b2r_ctx* ctx, const b2r_heat_params* p)
     return B2R_OK;
 }
 
-template <int D, int MODE, int RY, int WY>
+template <int D, int MODE, int RY, int WY, int WX>
 static void launch_heat_ry(b2r_ctx* ctx, const float* in, float* out, const HeatK& hk, int i)
 {
-    constexpr int TY = WY * RY;
+    constexpr int TX = HEAT_TX * WX, TY = WY * RY;
     const KGeo& g = ctx->k;
-    const int ntx = ceil_div(g.S[0] - 2 * D, HEAT_TX);
+    const int ntx = ceil_div(g.S[0] - 2 * D, TX);
     const int nty = ceil_div(g.S[1] - 2 * D, TY);
     const int nz = g.S[2] - 2 * D;
     int zc = ctx->tune.heat_zc;
@@ -210,30 +209,39 @@ static void launch_heat_ry(b2r_ctx* ctx, const float* in, float* out, const Heat
     if (zc > zmax) zc = (int)zmax;
     const int nzc = ceil_div(nz, zc);
     dim3 grid(ntx * nty * nzc * g.nblk);
-    dim3 block(HEAT_TX, WY);
-    heat_step_kernel<D, MODE, RY, WY><<<grid, block, 0, ctx->stream>>>(in, out, g, hk, i, ntx,
-                                                                       nty, nzc, zc);
+    dim3 block(TX, WY);
+    heat_step_kernel<D, MODE, RY, WY, WX><<<grid, block, 0, ctx->stream>>>(in, out, g, hk, i, ntx,
+                                                                           nty, nzc, zc);
 }
 
-template <int D, int MODE, int WY>
+template <int D, int MODE, int WY, int WX>
 static void launch_heat_wy(b2r_ctx* ctx, const float* in, float* out, const HeatK& hk, int i)
 {
     switch (ctx->tune.heat_ry) {
-    case 1: launch_heat_ry<D, MODE, 1, WY>(ctx, in, out, hk, i); break;
-    case 2: launch_heat_ry<D, MODE, 2, WY>(ctx, in, out, hk, i); break;
-    case 8: launch_heat_ry<D, MODE, 8, WY>(ctx, in, out, hk, i); break;
-    default: launch_heat_ry<D, MODE, 4, WY>(ctx, in, out, hk, i); break;
+    case 1: launch_heat_ry<D, MODE, 1, WY, WX>(ctx, in, out, hk, i); break;
+    case 2: launch_heat_ry<D, MODE, 2, WY, WX>(ctx, in, out, hk, i); break;
+    case 8: launch_heat_ry<D, MODE, 8, WY, WX>(ctx, in, out, hk, i); break;
+    default: launch_heat_ry<D, MODE, 4, WY, WX>(ctx, in, out, hk, i); break;
     }
 }
 
-// workgroup = 64 x-lanes x heat_wy waves (4 by default), each thread heat_ry rows
+// workgroup = 64*heat_wx x-lanes (1, 2 or 4 waves across) x heat_wy thread
+// rows (4 by default), each thread heat_ry rows; heat_wx*heat_wy <= 16
 template <int D, int MODE>
 static void launch_heat(b2r_ctx* ctx, const float* in, float* out, const HeatK& hk, int i)
 {
+    const int wx = ctx->tune.heat_wx;
     switch (ctx->tune.heat_wy) {
-    case 8: launch_heat_wy<D, MODE, 8>(ctx, in, out, hk, i); break;
-    case 16: launch_heat_wy<D, MODE, 16>(ctx, in, out, hk, i); break;
-    default: launch_heat_wy<D, MODE, 4>(ctx, in, out, hk, i); break;
+    case 8:
+        if (wx == 2) launch_heat_wy<D, MODE, 8, 2>(ctx, in, out, hk, i);
+        else launch_heat_wy<D, MODE, 8, 1>(ctx, in, out, hk, i);
+        break;
+    case 16: launch_heat_wy<D, MODE, 16, 1>(ctx, in, out, hk, i); break;
+    default:
+        if (wx == 4) launch_heat_wy<D, MODE, 4, 4>(ctx, in, out, hk, i);
+        else if (wx == 2) launch_heat_wy<D, MODE, 4, 2>(ctx, in, out, hk, i);
+        else launch_heat_wy<D, MODE, 4, 1>(ctx, in, out, hk, i);
+        break;
     }
 }
 

@@ -33,14 +33,16 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
             }
         } else {
             for (int wy : wys)
-                for (int ry : rys)
-                    for (int zc : zcs) {
-                        b2r_tuning t = base;
-                        t.heat_ry = ry;
-                        t.heat_wy = wy;
-                        t.heat_zc = zc;
-                        cand.push_back(t);
-                    }
+                for (int wx = 1; wx * wy <= 16; wx *= 2)
+                    for (int ry : rys)
+                        for (int zc : zcs) {
+                            b2r_tuning t = base;
+                            t.heat_ry = ry;
+                            t.heat_wy = wy;
+                            t.heat_wx = wx;
+                            t.heat_zc = zc;
+                            cand.push_back(t);
+                        }
         }
     } else {
         for (int tile = 1; tile <= (kind == 2 ? 6 : 1); ++tile)
@@ -83,8 +85,8 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
         fprintf(f, "kind=%s S=%dx%dx%d blocks=%d R=%d radius=%d", names[kind], ctx->L.S[0],
                 ctx->L.S[1], ctx->L.S[2], ctx->L.nblocks, ctx->g.R, ctx->g.radius);
         if (kind == 0) {
-            fprintf(f, " heat_D=%d best=heat_wy:%d,heat_ry:%d,heat_zc:%d", p ? p->D : 0,
-                    best.heat_wy, best.heat_ry, best.heat_zc);
+            fprintf(f, " heat_D=%d best=heat_wy:%d,heat_wx:%d,heat_ry:%d,heat_zc:%d", p ? p->D : 0,
+                    best.heat_wy, best.heat_wx, best.heat_ry, best.heat_zc);
         } else if (kind == 3) {
             fprintf(f, " heat_D=%d best=heat_zc:%d", p ? p->D : 0, best.heat_zc);
         } else if (kind == 1) {

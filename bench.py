@@ -137,8 +137,8 @@ def heat_bench(A, env, blocks, R, D, k, steps, init, fused=0, record=None, warmu
                 glups=round(cells * steps / (ms * 1e-3) / 1e9, 2),
                 kernel_ms=round(kms, 4), steps_per_launch=per_launch,
                 kernel_gbs_alg=round(8.0 * cells * per_launch / (kms * 1e-3) / 1e9, 1),
-                tuning={"heat_wy": tune["heat_wy"], "heat_ry": tune["heat_ry"],
-                        "heat_zc": tune["heat_zc"]})
+                tuning={"heat_wy": tune["heat_wy"], "heat_wx": tune["heat_wx"],
+                        "heat_ry": tune["heat_ry"], "heat_zc": tune["heat_zc"]})
 
 
 def blob_tables(env, nb=8, sigma=20.0, seed=2):

@@ -309,15 +309,15 @@ def test_gol_fused_chunking(zc, tile):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("wy", [4, 8, 16])
+@pytest.mark.parametrize("wy,wx", [(4, 1), (4, 2), (4, 4), (8, 1), (8, 2), (16, 1)])
 @pytest.mark.parametrize("ry", [1, 2, 4, 8])
 @pytest.mark.parametrize("zc", [0, 5])
-def test_heat_tuning_variants(wy, ry, zc):
+def test_heat_tuning_variants(wy, wx, ry, zc):
     env, D = (70, 40, 36), 3
     k = (0.021, 0.033, 0.017)
     p = A.heat_params(D, 0, k)
     with A.Context(env, (1, 2, 1), 2, D, 0, 4) as c:
-        c.set_tuning(heat_wy=wy, heat_ry=ry, heat_zc=zc)
+        c.set_tuning(heat_wy=wy, heat_wx=wx, heat_ry=ry, heat_zc=zc)
         a, b = c.field(), c.field()
         c.init_uniform(a, 9)
         res, _ = c.heat_run(a, b, p, 6)
