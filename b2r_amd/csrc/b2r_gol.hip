@@ -459,21 +459,29 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
 
     uint32_t gen[R][3];  // generation t at planes (newest-2, newest-1, newest)
     const int zend = zc1 + R;
-    auto plane = [&](int zin, uint4& la, uint4& lb) {
+    // Odd chunks march down, even chunks up, so two neighbouring chunks reach
+    // their shared halo planes at the same time and the second read of them
+    // hits in L2.  Input plane k of the march is zp(k); generation t is
+    // computed t planes behind it.
+    const int nin = zend - (zc0 - R);  // input planes of the chunk
+    const int dz = (zci & 1) ? -1 : 1;
+    const int zfirst = dz > 0 ? zc0 - R : zend - 1;
+    auto zp = [&](int k) { return zfirst + dz * k; };
+    auto plane = [&](int k, uint4& la, uint4& lb) {
         gen[0][0] = gen[0][1];
         gen[0][1] = gen[0][2];
         gen[0][2] = gb_pack(make_uint4(la.x & ma, la.y & ma, la.z & ma, la.w & ma),
                             make_uint4(lb.x & mb, lb.y & mb, lb.z & mb, lb.w & mb)) & ldm;
         {  // the load GF_PD planes ahead (the last ones are unused)
-            const uint8_t* pp = col + (long long)min(zin + GF_PD, zend - 1) * slice;
+            const uint8_t* pp = col + (long long)zp(min(k + GF_PD, nin - 1)) * slice;
             la = *reinterpret_cast<const uint4*>(pp + oa);
             lb = *reinterpret_cast<const uint4*>(pp + ob);
         }
 #pragma unroll
         for (int t = 1; t <= R; ++t) {
             // uniform: generation t-1 not yet 3 planes, or a plane past the chunk
-            if (zin < zc0 - R + 2 * t || zin >= zend) continue;
-            const int p = zin - t;
+            if (k < 2 * t || k >= nin) continue;
+            const int p = zp(k) - dz * t;
             const uint32_t ga = gen[t - 1][0], self = gen[t - 1][1], gc = gen[t - 1][2];
             const uint32_t s0 = ga ^ self ^ gc, s1 = maj3(ga, self, gc);
             sb[tid + NQ] = make_uint2(s0, s1);
@@ -529,18 +537,17 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
             }
         }
     };
-    const int z0 = zc0 - R;
     uint4 sl[GF_PD][2];
 #pragma unroll
     for (int k = 0; k < GF_PD; ++k) {
-        const uint8_t* pp = col + (long long)min(z0 + k, zend - 1) * slice;
+        const uint8_t* pp = col + (long long)zp(min(k, nin - 1)) * slice;
         sl[k][0] = *reinterpret_cast<const uint4*>(pp + oa);
         sl[k][1] = *reinterpret_cast<const uint4*>(pp + ob);
     }
-    // whole groups of GF_PD planes; planes past zend compute nothing
-    for (int zin = z0; zin < zend; zin += GF_PD) {
+    // whole groups of GF_PD planes; planes past the chunk compute nothing
+    for (int k = 0; k < nin; k += GF_PD) {
 #pragma unroll
-        for (int k = 0; k < GF_PD; ++k) plane(zin + k, sl[k][0], sl[k][1]);
+        for (int j = 0; j < GF_PD; ++j) plane(k + j, sl[j][0], sl[j][1]);
     }
 }
 
