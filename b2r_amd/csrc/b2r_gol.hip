@@ -332,9 +332,9 @@ __global__ __launch_bounds__((TXQ + 2) * NR) void gol_fused_kernel(const uint8_t
 // packed from the byte cells when a plane is loaded and unpacked when
 // generation R is stored.  Per generation and plane:
 //   S   = z-sum of its three planes of generation t-1, two bit planes (0..3);
-//   YZ  = S(y-1) + S + S(y+1) through an LDS plane, four bit planes (0..9);
-//   box = YZ(x-1) + YZ + YZ(x+1) through a second LDS plane, the x
-//         neighbours shifted in across word boundaries, five bit planes;
+//   XZ  = S(x-1) + S + S(x+1) through an LDS plane, the x neighbours shifted
+//         in across word boundaries, four bit planes (0..9);
+//   box = XZ(y-1) + XZ + XZ(y+1) through a second LDS plane, five bit planes;
 //   the rule: alive next iff box == 5, or alive and 3 <= box <= 8.
 // Every adder is built from full adders of two 3-input logic operations
 // (xor3, majority); the compiled generation takes about 70 instructions per 32
@@ -400,8 +400,8 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
     constexpr int NQ = NW + 2;  // words per row: plus one per side
     constexpr int NT = NQ * NR;
     constexpr int TY = NR - 2 * R;
-    __shared__ uint2 sb[NT + 2 * NQ];  // z-sums of a plane, a spare row above and below
-    __shared__ uint4 yb[NT + 2];       // y-sums, a spare word before and after
+    __shared__ uint2 sb[NT + 2];       // z-sums of a plane, a spare word before and after
+    __shared__ uint4 xb[NT + 2 * NQ];  // x-sums, a spare row above and below
 
     const int tid = threadIdx.x;
     const int r = tid / NQ, q = tid % NQ;
@@ -484,33 +484,29 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
             const int p = zp(k) - dz * t;
             const uint32_t ga = gen[t - 1][0], self = gen[t - 1][1], gc = gen[t - 1][2];
             const uint32_t s0 = ga ^ self ^ gc, s1 = maj3(ga, self, gc);
-            sb[tid + NQ] = make_uint2(s0, s1);
+            sb[tid + 1] = make_uint2(s0, s1);
             __syncthreads();
-            const uint2 up = sb[tid], dn = sb[tid + 2 * NQ];
-            // YZ = up + S + dn, four bit planes
-            const uint32_t y0b = up.x ^ s0 ^ dn.x, k1 = maj3(up.x, s0, dn.x);
-            const uint32_t u = up.y ^ s1 ^ dn.y, v = maj3(up.y, s1, dn.y);
-            const uint32_t y1b = u ^ k1, w = u & k1;
-            const uint4 Y = make_uint4(y0b, y1b, v ^ w, v & w);
-            yb[tid + 1] = Y;
+            const uint2 L = sb[tid], Rg = sb[tid + 2];
+            // x neighbours of the z-sums: bit i of l is cell x+i-1, of rr cell x+i+1
+            const uint32_t l0 = __builtin_amdgcn_alignbit(s0, L.x, 31),
+                           l1 = __builtin_amdgcn_alignbit(s1, L.y, 31);
+            const uint32_t r0 = __builtin_amdgcn_alignbit(Rg.x, s0, 1),
+                           r1 = __builtin_amdgcn_alignbit(Rg.y, s1, 1);
+            // XZ = l + S + rr, four bit planes (0..9)
+            const uint32_t x0b = l0 ^ s0 ^ r0, k1 = maj3(l0, s0, r0);
+            const uint32_t u = l1 ^ s1 ^ r1, v = maj3(l1, s1, r1);
+            const uint32_t x1b = u ^ k1, w = u & k1;
+            const uint4 X = make_uint4(x0b, x1b, v ^ w, v & w);
+            xb[tid + NQ] = X;
             __syncthreads();
-            const uint4 L = yb[tid], Rg = yb[tid + 2];
-            // x neighbours: bit i of l is cell x+i-1, of rr cell x+i+1
-            const uint4 l = make_uint4(__builtin_amdgcn_alignbit(Y.x, L.x, 31),
-                                       __builtin_amdgcn_alignbit(Y.y, L.y, 31),
-                                       __builtin_amdgcn_alignbit(Y.z, L.z, 31),
-                                       __builtin_amdgcn_alignbit(Y.w, L.w, 31));
-            const uint4 rr = make_uint4(__builtin_amdgcn_alignbit(Rg.x, Y.x, 1),
-                                        __builtin_amdgcn_alignbit(Rg.y, Y.y, 1),
-                                        __builtin_amdgcn_alignbit(Rg.z, Y.z, 1),
-                                        __builtin_amdgcn_alignbit(Rg.w, Y.w, 1));
-            // box = l + Y + rr (each <= 9), five bit planes
-            const uint32_t b0 = l.x ^ Y.x ^ rr.x, c1 = maj3(l.x, Y.x, rr.x);
-            const uint32_t e1 = l.y ^ Y.y ^ rr.y, c2a = maj3(l.y, Y.y, rr.y);
+            const uint4 U = xb[tid], Dn = xb[tid + 2 * NQ];
+            // box = U + X + Dn (each <= 9), five bit planes
+            const uint32_t b0 = U.x ^ X.x ^ Dn.x, c1 = maj3(U.x, X.x, Dn.x);
+            const uint32_t e1 = U.y ^ X.y ^ Dn.y, c2a = maj3(U.y, X.y, Dn.y);
             const uint32_t b1 = e1 ^ c1, c2b = e1 & c1;
-            const uint32_t e2 = l.z ^ Y.z ^ rr.z, c3a = maj3(l.z, Y.z, rr.z);
+            const uint32_t e2 = U.z ^ X.z ^ Dn.z, c3a = maj3(U.z, X.z, Dn.z);
             const uint32_t b2 = e2 ^ c2a ^ c2b, c3b = maj3(e2, c2a, c2b);
-            const uint32_t e3 = l.w ^ Y.w ^ rr.w, c4a = maj3(l.w, Y.w, rr.w);
+            const uint32_t e3 = U.w ^ X.w ^ Dn.w, c4a = maj3(U.w, X.w, Dn.w);
             const uint32_t b3 = e3 ^ c3a ^ c3b, c4b = maj3(e3, c3a, c3b);
             const uint32_t b4 = c4a | c4b;
             // rule: box == 5, or alive and 3 <= box <= 8
