@@ -58,7 +58,8 @@ class Layout(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("heat_ry", C.c_int), ("heat_zc", C.c_int), ("gol_zc", C.c_int),
                 ("gol_step_zc", C.c_int), ("el_zc", C.c_int), ("heat_wy", C.c_int),
-                ("gol_tile", C.c_int), ("heat_wx", C.c_int)]
+                ("gol_tile", C.c_int), ("heat_wx", C.c_int), ("el_kernel", C.c_int),
+                ("el_step_zc", C.c_int)]
 
 
 class HeatParams(C.Structure):
@@ -124,6 +125,7 @@ _SIGS = {
     "b2r_el_gradient_ex": (_i, [_vp, _vp, _i, _i, _i, C.POINTER(C.c_double), _vp, _vp, _vp,
                                 _vp]),
     "b2r_el_autotune": (_i, [_vp, C.c_char_p, C.POINTER(Tuning)]),
+    "b2r_el_autotune_batch": (_i, [C.POINTER(_vp), _i, C.c_char_p]),
     "b2r_el_gradient_batch": (_i, [C.POINTER(_vp), _i, C.POINTER(_vp), _i, _i, _i,
                                    C.POINTER(C.c_double), C.POINTER(_vp), C.POINTER(_vp),
                                    C.POINTER(_vp), _pf]),
@@ -461,6 +463,14 @@ class ElasticShot:
                                       C.byref(m), *[_ptr(a) for a in g],
                                       _ptr(adj) if adjoint_state else None))
         return (m.value, g, adj) if adjoint_state else (m.value, g)
+
+
+def el_autotune_batch(shots, record=None):
+    """Pick the elastic kernel and z chunk for a batch as a whole (resets the
+    shots); returns the tuning now set in every shot's context."""
+    arr = (_vp * len(shots))(*[s.h for s in shots])
+    check(_lib.b2r_el_autotune_batch(arr, len(shots), record.encode() if record else None))
+    return shots[0].ctx.get_tuning()
 
 
 def el_forward_batch(shots, nsteps, mode=1, timed=False):

@@ -185,11 +185,13 @@ extern "C" int b2r_set_tuning(b2r_ctx* ctx, const b2r_tuning* t)
     B2R_CHECK(ctx && t, B2R_E_ARG, "null argument");
     B2R_CHECK(t->heat_ry == 0 || t->heat_ry == 1 || t->heat_ry == 2 || t->heat_ry == 4 ||
               t->heat_ry == 8, B2R_E_ARG, "heat_ry must be 0, 1, 2, 4 or 8");
-    B2R_CHECK(t->heat_zc >= 0 && t->gol_zc >= 0 && t->gol_step_zc >= 0 && t->el_zc >= 0, B2R_E_ARG,
+    B2R_CHECK(t->heat_zc >= 0 && t->gol_zc >= 0 && t->gol_step_zc >= 0 && t->el_zc >= 0 &&
+                  t->el_step_zc >= 0, B2R_E_ARG,
               "negative z chunk");
     B2R_CHECK(t->heat_wy == 0 || t->heat_wy == 4 || t->heat_wy == 8 || t->heat_wy == 16,
               B2R_E_ARG, "heat_wy must be 0, 4, 8 or 16");
     B2R_CHECK(t->gol_tile >= 0 && t->gol_tile <= 6, B2R_E_ARG, "gol_tile must be 0..6");
+    B2R_CHECK(t->el_kernel >= 0 && t->el_kernel <= 2, B2R_E_ARG, "el_kernel must be 0, 1 or 2");
     B2R_CHECK((t->heat_wx == 0 || t->heat_wx == 1 || t->heat_wx == 2 || t->heat_wx == 4) &&
                   (t->heat_wx > 1 ? t->heat_wx : 1) * (t->heat_wy > 0 ? t->heat_wy : 4) <= 16,
               B2R_E_ARG, "heat_wx must be 0, 1, 2 or 4 with heat_wx * heat_wy <= 16");

@@ -52,6 +52,7 @@ struct SrcRec {
 struct ElDev {
     KGeo g;
     float* f[F_NUM];
+    float* nx[9];     // the other state set, written by the fused step kernel
     float* disp;      // optional displacement accumulator (eqwp.cu:231, 234)
     float* adj[9];    // adjoint state
     float* fw0[9];    // forward state before the step being reversed
@@ -82,6 +83,7 @@ struct b2r_el_shot {
     int* d_rec = nullptr;       // x,y,z per receiver
     float* d_recv = nullptr;    // rec_nt x nrec samples
     void* fields[F_NUM];
+    void* pp[9] = {};           // second state set of the fused step (allocated on first use)
 };
 
 constexpr float EL_C2 = -1.f / 12.f;
@@ -101,10 +103,13 @@ __device__ __forceinline__ float fd4(const gfloat* __restrict__ p, uint32_t c, u
 
 // workgroup -> (shot, block, tile) for batched launches.  A device table of
 // n > 1 shots is followed by an int array holding each shot's first tile.
+// Tiles are TX x TY cells; a thread's cell lies OFF cells before the tile's
+// origin plus its thread index (OFF > 0: the workgroup also covers a ring).
 struct Tile {
     int shot, b, x, y, zs, ze;
 };
 
+template <int TX = EL_TX, int TY = EL_TY, int OFF = 0>
 __device__ __forceinline__ Tile decode_tile(const ElDev* __restrict__ tab, int nshot)
 {
     const int gt = blockIdx.x;
@@ -132,8 +137,8 @@ __device__ __forceinline__ Tile decode_tile(const ElDev* __restrict__ tab, int n
     lt /= d.nty;
     const int zci = lt % d.nzc;
     T.b = lt / d.nzc;
-    T.x = d.o[0] + txi * EL_TX + threadIdx.x;
-    T.y = d.o[1] + tyi * EL_TY + threadIdx.y;
+    T.x = d.o[0] + txi * TX + (int)threadIdx.x - OFF;
+    T.y = d.o[1] + tyi * TY + (int)threadIdx.y - OFF;
     T.zs = d.o[2] + zci * d.zc;
     T.ze = min(T.zs + d.zc, d.o[2] + d.w[2]);  // exclusive
     return T;
@@ -386,6 +391,213 @@ __global__ __launch_bounds__(EL_WG) void el_velocity_kernel(const ElDev* __restr
     }
 }
 
+// ---------------------------------------------------------------- fused step
+// One whole step (stress stage 2i, then velocity stage 2i+1) in one launch,
+// from the state in d.f into the other state set d.nx.  A workgroup of
+// 64 x 16 threads, one cell each, covers a 60 x 12 tile of velocity cells and
+// the 2-cell ring around it, and marches z: at plane p it forms the new
+// stresses of all its cells (the ring's are the ones the velocity stencil of
+// the tile needs, recomputed here instead of written and read back), at plane
+// p-2 the new velocities of the tile.  Because a step reads one state set and
+// writes the other, neighbouring workgroups, whose rings overlap this tile,
+// never see a half-updated cell.  Per cell and step 9 state and 3 medium
+// fields are read and 9 written (84 B), against 120 B for the two in-place
+// stages.  The results equal el_stress_kernel followed by
+// el_velocity_kernel<7> bit for bit (same expressions and rounding); cells
+// outside the stress box keep their stresses as there.  New stresses are
+// written on the velocity box only: the later stages of the window read them
+// nowhere else, and the exchange after the window refreshes every halo cell.
+constexpr int EF_TX = 64, EF_TY = 16;                // threads of a fused workgroup
+constexpr int EF_IX = EF_TX - 4, EF_IY = EF_TY - 4;  // its tile of velocity cells
+constexpr int EF_WG = EF_TX * EF_TY;
+constexpr int EF_LX = EF_TX + 4, EF_LY = EF_TY + 4;  // staged v plane with its halo
+
+template <int NY, int NX>
+__device__ __forceinline__ float fdx_at(const float (&p)[NY][NX], int y, int x, float ih)
+{
+    return (EL_C2 * (p[y][x + 2] - p[y][x - 2]) + EL_C1 * (p[y][x + 1] - p[y][x - 1])) * ih;
+}
+template <int NY, int NX>
+__device__ __forceinline__ float fdy_at(const float (&p)[NY][NX], int y, int x, float ih)
+{
+    return (EL_C2 * (p[y + 2][x] - p[y - 2][x]) + EL_C1 * (p[y + 1][x] - p[y - 1][x])) * ih;
+}
+
+__global__ __launch_bounds__(EF_WG) void el_step_kernel(const ElDev* __restrict__ tab, int nshot,
+                                                      int i)
+{
+    // two buffers each, alternating between planes, so one barrier per plane
+    // separates the staging of a plane from the reads of the previous one
+    __shared__ float sv[2][3][EF_LY][EF_LX];  // v at plane p with the 2-cell plus halo
+    __shared__ float ss[2][5][EF_TY][EF_TX];  // new sxx, syy, sxy, sxz, syz at plane p-2
+    const Tile T = decode_tile<EF_IX, EF_IY, 2>(tab, nshot);
+    const ElDev& d = tab[T.shot];
+    if (T.b >= d.g.nblk) return;  // padding tile (uniform)
+    const KGeo& g = d.g;
+    const Box bs = update_box(g, T.b, 2, 2 * i), bv = update_box(g, T.b, 2, 2 * i + 1);
+    const int k0 = max(T.zs, bv.lo[2]), k1 = min(T.ze - 1, bv.hi[2]);
+    if (k0 > k1) return;  // uniform
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * EF_TX + tx;
+    // the tiled region starts at 4 or deeper, so x, y >= 2 and every stencil
+    // of an updated cell stays in the block; planes k0-4 .. k1+4 lie in it too
+    const int x = T.x, y = T.y;
+    const bool in = x < g.S[0] && y < g.S[1];
+    const bool s_xy = x >= bs.lo[0] && x <= bs.hi[0] && y >= bs.lo[1] && y <= bs.hi[1];
+    const bool v_xy = tx >= 2 && tx < EF_TX - 2 && ty >= 2 && ty < EF_TY - 2 && x >= bv.lo[0] &&
+                      x <= bv.hi[0] && y >= bv.lo[1] && y <= bv.hi[1];
+    const long long boff = (long long)T.b * g.bstride + g.xoff;
+    const gfloat* __restrict__ fv[3] = {G(d.f[F_VX] + boff), G(d.f[F_VY] + boff),
+                                        G(d.f[F_VZ] + boff)};
+    const gfloat* __restrict__ fs[6] = {G(d.f[F_SXX] + boff), G(d.f[F_SYY] + boff),
+                                        G(d.f[F_SZZ] + boff), G(d.f[F_SXY] + boff),
+                                        G(d.f[F_SXZ] + boff), G(d.f[F_SYZ] + boff)};
+    const gfloat* __restrict__ lam = G(d.f[F_LAM] + boff);
+    const gfloat* __restrict__ mu = G(d.f[F_MU] + boff);
+    const gfloat* __restrict__ bb = G(d.f[F_B] + boff);
+    gfloat* __restrict__ ov[3] = {G(d.nx[F_VX] + boff), G(d.nx[F_VY] + boff), G(d.nx[F_VZ] + boff)};
+    gfloat* __restrict__ os[6] = {G(d.nx[F_SXX] + boff), G(d.nx[F_SYY] + boff),
+                                  G(d.nx[F_SZZ] + boff), G(d.nx[F_SXY] + boff),
+                                  G(d.nx[F_SXZ] + boff), G(d.nx[F_SYZ] + boff)};
+    gfloat* __restrict__ disp = d.disp ? G(d.disp + boff) : nullptr;
+    const uint32_t sy = 4u * g.pitch, sz = 4u * (uint32_t)g.slice;  // byte strides
+    const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
+    // halo of the staged v plane: waves 0-3 hold rows y0-2, y0-1, y0+16, y0+17
+    // of the workgroup's cells, wave 4 columns x0-2, x0-1, x0+64, x0+65
+    const bool h_any = tid < 4 * EF_TX + 4 * EF_TY;
+    int hly, hlx, hx, hy;
+    if (tid < 4 * EF_TX) {
+        const int r = tid / EF_TX;
+        hly = r < 2 ? r : EF_TY + r;
+        hlx = tx + 2;
+        hx = x;
+        hy = y - ty - 2 + hly;
+    } else {
+        const int j = tid - 4 * EF_TX, cc = j & 3;
+        hlx = cc < 2 ? cc : EF_TX + cc;
+        hly = (j >> 2) + 2;
+        hx = x - tx - 2 + hlx;
+        hy = y - ty - 2 + hly;
+    }
+    const bool h_ok = h_any && hx >= 0 && hx < g.S[0] && hy >= 0 && hy < g.S[1];
+    const uint32_t ho = h_ok ? (uint32_t)hy * sy + 4u * hx : 0u;
+    const uint32_t cxy = in ? (uint32_t)y * sy + 4u * x : 0u;
+    // register queues: v over planes p-2..p+2; new sxz, syz, szz over p-4..p;
+    // new sxx, syy, sxy over p-2..p
+    float vq[3][5], zq[3][5], xq[3][3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) vq[f][j] = ldo(fv[f], (uint32_t)(k0 - 4 + j) * sz + cxy);
+#pragma unroll
+        for (int j = 0; j < 5; ++j) zq[f][j] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) xq[f][j] = 0.0f;
+    }
+    // Every load is issued one plane ahead of its use (v of plane p+3, the
+    // halo, old stresses and medium of plane p+1, b of plane p-1) into one of
+    // two register sets that alternate between planes, so no load in flight
+    // is moved between registers.  Loads are unconditional: a cell outside
+    // the block reads the block's cell (0, 0) of its plane instead (cxy, ho =
+    // 0), a value no updated cell uses.
+    struct Pre {
+        float v[3], h[3], s[6], l, m, b;
+    };
+    auto load = [&](Pre& n, int p) {  // the set consumed at plane p
+        const uint32_t pz = (uint32_t)p * sz, c = pz + cxy;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            n.v[f] = ldo(fv[f], c + 2 * sz);
+            n.h[f] = h_any ? ldo(fv[f], pz + ho) : 0.0f;  // uniform: waves 0-4
+        }
+#pragma unroll
+        for (int f = 0; f < 6; ++f) n.s[f] = ldo(fs[f], c);
+        n.l = ldo(lam, c);
+        n.m = ldo(mu, c);
+        n.b = ldo(bb, c - 2 * sz);
+    };
+    auto plane = [&](int p, const Pre& cur, Pre& nxt, int buf) {
+        load(nxt, min(p + 1, k1 + 2));
+        const uint32_t ck = (uint32_t)(p - 2) * sz + cxy;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) vq[f][4] = cur.v[f];
+        // stage v of plane p and the new stresses of plane k = p-2 (the reads
+        // of this buffer, two planes back, ended before the last barrier)
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            sv[buf][f][ty + 2][tx + 2] = vq[f][2];
+            if (h_any) sv[buf][f][hly][hlx] = cur.h[f];
+        }
+        ss[buf][0][ty][tx] = xq[0][0];
+        ss[buf][1][ty][tx] = xq[1][0];
+        ss[buf][2][ty][tx] = xq[2][0];
+        ss[buf][3][ty][tx] = zq[0][2];
+        ss[buf][4][ty][tx] = zq[1][2];
+        __syncthreads();
+        // new stresses of plane p (el_stress_kernel's expressions), computed
+        // by every thread and kept where the cell is in the stress box
+        const bool upd = s_xy && p >= bs.lo[2] && p <= bs.hi[2];
+        float n[6];
+        {
+            const auto& V = sv[buf];
+            const int ly = ty + 2, lx = tx + 2;
+            const float l = cur.l, m = cur.m;
+            const float exx = fdx_at(V[0], ly, lx, i0);
+            const float eyy = fdy_at(V[1], ly, lx, i1);
+            const float ezz = fdq(vq[2], i2);
+            const float dxy = fdy_at(V[0], ly, lx, i1) + fdx_at(V[1], ly, lx, i0);
+            const float dxz = fdq(vq[0], i2) + fdx_at(V[2], ly, lx, i0);
+            const float dyz = fdq(vq[1], i2) + fdy_at(V[2], ly, lx, i1);
+            const float l2m = l + 2.0f * m;
+            n[0] = cur.s[0] + dt * (l2m * exx + l * (eyy + ezz));
+            n[1] = cur.s[1] + dt * (l2m * eyy + l * (exx + ezz));
+            n[2] = cur.s[2] + dt * (l2m * ezz + l * (exx + eyy));
+            n[3] = cur.s[3] + dt * (m * dxy);
+            n[4] = cur.s[4] + dt * (m * dxz);
+            n[5] = cur.s[5] + dt * (m * dyz);
+        }
+#pragma unroll
+        for (int f = 0; f < 6; ++f) n[f] = upd ? n[f] : cur.s[f];
+        xq[0][2] = n[0];
+        xq[1][2] = n[1];
+        xq[2][2] = n[3];
+        zq[0][4] = n[4];
+        zq[1][4] = n[5];
+        zq[2][4] = n[2];
+        // new velocities of plane k (el_velocity_kernel's expressions)
+        if (p - 2 >= k0 && p - 2 <= k1 && v_xy) {
+            const auto& S = ss[buf];
+            const float tb = dt * cur.b;
+            const float ax = (fdx_at(S[0], ty, tx, i0) + fdy_at(S[2], ty, tx, i1)) + fdq(zq[0], i2);
+            const float ay = (fdx_at(S[2], ty, tx, i0) + fdy_at(S[1], ty, tx, i1)) + fdq(zq[1], i2);
+            const float az = (fdx_at(S[3], ty, tx, i0) + fdy_at(S[4], ty, tx, i1)) + fdq(zq[2], i2);
+            sto(ov[0], ck, vq[0][0] + tb * ax);
+            sto(ov[1], ck, vq[1][0] + tb * ay);
+            sto(ov[2], ck, vq[2][0] + tb * az);
+            if (disp) sto(disp, ck, ldo(disp, ck) + (dt * tb) * ay);
+            sto(os[0], ck, xq[0][0]);
+            sto(os[1], ck, xq[1][0]);
+            sto(os[2], ck, zq[2][2]);
+            sto(os[3], ck, xq[2][0]);
+            sto(os[4], ck, zq[0][2]);
+            sto(os[5], ck, zq[1][2]);
+        }
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            qshift(vq[f]);
+            qshift(zq[f]);
+            xq[f][0] = xq[f][1];
+            xq[f][1] = xq[f][2];
+        }
+    };
+    // planes in pairs; an odd count ends with a plane k1+3 that reloads
+    // plane k1+2 and stores nothing
+    Pre pa, pb;
+    load(pa, k0 - 2);
+    for (int p = k0 - 2; p <= k1 + 2; p += 2) {
+        plane(p, pa, pb, 0);
+        plane(p + 1, pb, pa, 1);
+    }
+}
 
 // Point forces after the velocity stage of inner step i: every block whose
 // stage box holds the source cell adds dt*b*f there, so redundant halo
@@ -758,25 +970,31 @@ __global__ void el_sum_kernel(const double* __restrict__ part, int n, double* __
 }
 
 // ---------------------------------------------------------------- host side
-static void el_tiles(ElDev& d, int zc_want)
+// Tiling of the stage kernels (fused = false: 64 x 4 tiles over the stress
+// boxes) or of the fused step kernel (60 x 12 velocity tiles over the
+// velocity boxes).
+static void el_tiles(ElDev& d, int zc_want, bool fused = false)
 {
     const KGeo& g = d.g;
-    // Stage update boxes lie in [2, S-2); on an axis with one block and both
-    // global faces frozen they are the owned cells [h, h+B) at every stage.
+    // Stage update boxes lie in [2, S-2), velocity boxes in [4, S-4); on an
+    // axis with one block and both global faces frozen they are the owned
+    // cells [h, h+B) at every stage.
     for (int a = 0; a < 3; ++a) {
         const bool owned = g.nb[a] == 1 && !g.periodic && ((g.fmask >> (2 * a)) & 3) == 3;
-        d.o[a] = owned ? g.h : 2;
-        d.w[a] = owned ? g.B[a] : g.S[a] - 4;
+        const int o = fused ? 4 : 2;
+        d.o[a] = owned ? g.h : o;
+        d.w[a] = owned ? g.B[a] : g.S[a] - 2 * o;
     }
-    d.ntx = ceil_div(d.w[0], EL_TX);
-    d.nty = ceil_div(d.w[1], EL_TY);
+    d.ntx = ceil_div(d.w[0], fused ? EF_IX : EL_TX);
+    d.nty = ceil_div(d.w[1], fused ? EF_IY : EL_TY);
     const int nz = d.w[2];
     long long tiles = (long long)d.ntx * d.nty * g.nblk;
     int zc = zc_want;
-    if (zc <= 0) {  // automatic: about 8192 workgroups, at least 8 planes each
-        const int nzc = (int)((8192 + tiles - 1) / tiles);
+    if (zc <= 0) {  // automatic: about 2M threads, at least 8 (fused: 16) planes each
+        const long long want = fused ? 2048 : 8192;
+        const int nzc = (int)((want + tiles - 1) / tiles);
         zc = ceil_div(nz, nzc < 1 ? 1 : nzc);
-        if (zc < 8) zc = 8;
+        if (zc < (fused ? 16 : 8)) zc = fused ? 16 : 8;
     }
     if (zc > nz) zc = nz;
     d.zc = zc;
@@ -835,6 +1053,8 @@ extern "C" int b2r_el_destroy(b2r_el_shot* s)
     b2r_ctx* c = s->ctx;
     (void)hipStreamSynchronize(c->stream);
     for (int k = 0; k < F_NUM; ++k) b2r_field_free(c, s->fields[k]);
+    for (int k = 0; k < 9; ++k)
+        if (s->pp[k]) b2r_field_free(c, s->pp[k]);
     if (s->d.disp) b2r_field_free(c, s->d.disp);
     void* ptrs[] = {s->d_src, s->d_wav, s->d_rec, s->d_recv};
     for (void* p : ptrs)
@@ -976,6 +1196,9 @@ extern "C" int b2r_el_set_state(b2r_el_shot* s, int which, const float* h_global
 struct ElBatch {
     std::vector<b2r_el_shot*> shots;
     ElDev* d_tab = nullptr;   // device copy of the shot table
+    ElDev* d_tab1 = nullptr;  // fused: the table with the two state sets swapped
+    bool fused = false;       // one el_step_kernel launch per step (ping-pong state)
+    int par = 0;              // fused: the state is in the second set (pp)
     int* d_t = nullptr;       // per shot step counters (device)
     int total_tiles = 0;      // workgroups of a batched launch
     int max_src = 0;          // max over shots of sources x blocks
@@ -992,7 +1215,20 @@ static int el_step_launch(ElBatch& bt, int i, bool plain)
 {
     const int nshot = (int)bt.shots.size();
     dim3 blk(EL_TX, EL_TY);
-    if (plain) {
+    if (bt.fused) {
+        // read the set holding the state, write the other, then apply the
+        // point terms to the new state through the swapped table
+        const ElDev* cur = bt.par ? bt.d_tab1 : bt.d_tab;
+        const ElDev* nxt = bt.par ? bt.d_tab : bt.d_tab1;
+        el_step_kernel<<<bt.total_tiles, dim3(EF_TX, EF_TY), 0, bt.stream>>>(cur, nshot, i);
+        if (bt.max_src)
+            el_inject_kernel<<<dim3(ceil_div(bt.max_src, 64), nshot), 64, 0, bt.stream>>>(
+                nxt, 2 * i + 1, bt.d_t);
+        if (bt.max_rec)
+            el_record_kernel<<<dim3(ceil_div(bt.max_rec, 64), nshot), 64, 0, bt.stream>>>(nxt,
+                                                                                         bt.d_t);
+        bt.par ^= 1;
+    } else if (plain) {
         for (int k = 0; k < nshot; ++k) {
             // a one-entry table (tile0 = 0 for plain tables)
             const ElDev* tk = bt.d_tab + k;
@@ -1029,7 +1265,7 @@ static int el_exchange(ElBatch& bt)
         const KGeo& g = s->ctx->k;
         if (g.nblk == 1 && !g.periodic) continue;  // nothing to exchange
         void* f[9];
-        for (int k = 0; k < 9; ++k) f[k] = s->fields[k];
+        for (int k = 0; k < 9; ++k) f[k] = bt.par ? s->pp[k] : s->fields[k];
         // the exchange runs on the batch stream
         hipStream_t keep = s->ctx->stream;
         s->ctx->stream = bt.stream;
@@ -1053,10 +1289,29 @@ static int el_upload_table(const std::vector<ElDev>& tab, hipStream_t st, ElDev*
     return B2R_OK;
 }
 
+// Whether a batched step runs the fused kernel: el_kernel 1, or 0 where no
+// global face is open (an open face keeps cells outside the velocity box in
+// the state, which the fused step does not carry over).
+static int el_fused_choice(b2r_el_shot* const* shots, int nshots, bool* fused)
+{
+    *fused = true;
+    for (int k = 0; k < nshots; ++k) {
+        const b2r_ctx* c = shots[k]->ctx;
+        const bool ok = c->k.periodic || c->k.fmask == 0x3F;
+        B2R_CHECK(ok || c->tune.el_kernel != 1, B2R_E_UNSUPPORTED,
+                  "el_kernel 1 needs every global face frozen or all periodic");
+        if (!ok || c->tune.el_kernel == 2) *fused = false;
+    }
+    return B2R_OK;
+}
+
 // Device table of the shots (their ElDev with sources and receivers) and
 // step counters.  Batched tables number the workgroups of all shots
-// consecutively (tile0); plain tables give every shot tile0 = 0.
-static int el_batch_init(ElBatch& bt, b2r_el_shot* const* shots, int nshots, bool plain)
+// consecutively (tile0); plain tables give every shot tile0 = 0.  A fused
+// batch also gets each shot's second state set and the table with the sets
+// swapped.
+static int el_batch_init(ElBatch& bt, b2r_el_shot* const* shots, int nshots, bool plain,
+                         bool fused = false)
 {
     B2R_CHECK(shots && nshots >= 1 && nshots <= EL_MAX_BATCH, B2R_E_ARG, "1..%d shots",
               EL_MAX_BATCH);
@@ -1072,8 +1327,10 @@ static int el_batch_init(ElBatch& bt, b2r_el_shot* const* shots, int nshots, boo
                   "all shots of a batch must be on one device");
         B2R_CHECK(s->ctx->g.R == R, B2R_E_ARG, "all shots of a batch must share R");
         s->d.g = s->ctx->k;
-        el_tiles(s->d, s->ctx->tune.el_zc);
+        const bool fz = fused && !plain;
+        el_tiles(s->d, fz ? s->ctx->tune.el_step_zc : s->ctx->tune.el_zc, fz);
         s->d.sr = el_srcrec(s);
+        for (int j = 0; j < 9; ++j) s->d.nx[j] = nullptr;
         tab[k] = s->d;
         tab[k].tile0 = plain ? 0 : tiles;
         tiles += s->d.ngrid;
@@ -1086,6 +1343,22 @@ static int el_batch_init(ElBatch& bt, b2r_el_shot* const* shots, int nshots, boo
     // work queued on the shots' own context streams comes first
     for (int k = 0; k < nshots; ++k)
         if (shots[k]->ctx->stream != bt.stream) B2R_HIP(hipStreamSynchronize(shots[k]->ctx->stream));
+    bt.fused = fused && !plain;
+    bt.par = 0;
+    if (bt.fused) {
+        std::vector<ElDev> tab1(tab);
+        for (int k = 0; k < nshots; ++k) {
+            b2r_el_shot* s = shots[k];
+            for (int j = 0; j < 9; ++j) {
+                if (!s->pp[j])
+                    if (int rc = b2r_field_alloc(s->ctx, &s->pp[j])) return rc;
+                tab[k].nx[j] = (float*)s->pp[j];
+                tab1[k].f[j] = (float*)s->pp[j];
+                tab1[k].nx[j] = (float*)s->fields[j];
+            }
+        }
+        if (int rc = el_upload_table(tab1, bt.stream, &bt.d_tab1)) return rc;
+    }
     if (int rc = el_upload_table(tab, bt.stream, &bt.d_tab)) return rc;
     B2R_HIP(hipMalloc(&bt.d_t, sizeof(int) * nshots));
     B2R_HIP(hipMemcpyAsync(bt.d_t, t0.data(), sizeof(int) * nshots, hipMemcpyHostToDevice,
@@ -1093,11 +1366,29 @@ static int el_batch_init(ElBatch& bt, b2r_el_shot* const* shots, int nshots, boo
     return B2R_OK;
 }
 
+// Copies the state between the two sets of a fused batch: to_second = true
+// before the first step (every cell the steps do not write, the frozen frame
+// among them, must be alike in both), false after a run that ended in the
+// second set.
+static int el_fused_copy(ElBatch& bt, bool to_second)
+{
+    for (b2r_el_shot* s : bt.shots) {
+        const size_t bytes = (size_t)s->ctx->L.field_elems * 4;
+        for (int j = 0; j < 9; ++j)
+            B2R_HIP(hipMemcpyAsync(to_second ? s->pp[j] : s->fields[j],
+                                   to_second ? s->fields[j] : s->pp[j], bytes,
+                                   hipMemcpyDeviceToDevice, bt.stream));
+    }
+    return B2R_OK;
+}
+
 static void el_batch_free(ElBatch& bt)
 {
     (void)hipStreamSynchronize(bt.stream);
     if (bt.d_tab) (void)hipFree(bt.d_tab);
+    if (bt.d_tab1) (void)hipFree(bt.d_tab1);
     if (bt.d_t) (void)hipFree(bt.d_t);
+    bt.d_tab1 = nullptr;
     bt.d_tab = nullptr;
     bt.d_t = nullptr;
 }
@@ -1118,8 +1409,13 @@ extern "C" int b2r_el_forward_batch(b2r_el_shot* const* shots, int nshots, int n
 {
     B2R_CHECK(nsteps >= 0 && mode >= 0 && mode <= 2, B2R_E_ARG, "bad nsteps or mode");
     const bool plain = mode == 0, graph = mode == 2;
+    B2R_CHECK(shots && nshots >= 1, B2R_E_ARG, "no shots");
+    for (int k = 0; k < nshots; ++k) B2R_CHECK(shots[k], B2R_E_ARG, "null shot %d", k);
+    bool fused = false;
+    if (!plain)
+        if (int rc = el_fused_choice(shots, nshots, &fused)) return rc;
     ElBatch bt;
-    if (int rc = el_batch_init(bt, shots, nshots, plain)) {
+    if (int rc = el_batch_init(bt, shots, nshots, plain, fused)) {
         el_batch_free(bt);
         return rc;
     }
@@ -1135,7 +1431,8 @@ extern "C" int b2r_el_forward_batch(b2r_el_shot* const* shots, int nshots, int n
     while (chunk < 32 && chunk * 2 <= nsteps) chunk *= 2;
     chunk -= chunk % R;
     if (chunk < R) chunk = R;
-    if (graph && nsteps >= chunk) {
+    // a fused graph must end in the set it starts from: an even step count
+    if (graph && nsteps >= chunk && !(bt.fused && (chunk & 1))) {
         if (hipStreamBeginCapture(bt.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             b2r_set_error("hipStreamBeginCapture failed");
             rc = B2R_E_HIP;
@@ -1152,6 +1449,7 @@ extern "C" int b2r_el_forward_batch(b2r_el_shot* const* shots, int nshots, int n
         }
     }
     if (!rc) rc = b2r_time_begin(c0, ms);
+    if (!rc && bt.fused) rc = el_fused_copy(bt, true);
     int done = 0;
     if (!rc && exec) {
         for (; done + chunk <= nsteps && !rc; done += chunk)
@@ -1165,6 +1463,8 @@ extern "C" int b2r_el_forward_batch(b2r_el_shot* const* shots, int nshots, int n
         rc = el_group(bt, n, plain);
         done += n;
     }
+    // a fused run that ended in the second set copies the state back
+    if (!rc && bt.par) rc = el_fused_copy(bt, false);
     if (!rc) rc = b2r_time_end(c0, ms);
     if (!rc) B2R_HIP(hipStreamSynchronize(bt.stream));
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -1181,48 +1481,128 @@ extern "C" int b2r_el_forward(b2r_el_shot* s, int nsteps, int mode, float* ms)
 }
 
 // Autotuner of the elastic stencil kernels (no counterpart in the reference):
-// times the batched step of this shot (stress + three-component velocity and
-// the point terms) for each z-chunk candidate, keeps the fastest in the
-// shot's context, appends one record line as b2r_autotune does, and leaves
-// the shot reset (zero state, t = 0).
+// times the batched step of this shot (the fused step kernel where the faces
+// allow it and the separate stress and three-component velocity kernels,
+// each with the point terms) for each z-chunk candidate, keeps the fastest in
+// the shot's context, appends one record line as b2r_autotune does, and
+// leaves the shot reset (zero state, t = 0).
 extern "C" int b2r_el_autotune(b2r_el_shot* s, const char* record_path, b2r_tuning* out)
 {
     B2R_CHECK(s, B2R_E_ARG, "null shot");
     b2r_ctx* c = s->ctx;
     const b2r_tuning base = c->tune;
     const int zcs[] = {0, 16, 32, 64, 128, 256};
+    const bool can_fuse = c->k.periodic || c->k.fmask == 0x3F;
     const int warm = 2, iters = 4;
-    float best_ms = 1e30f;
-    int best = base.el_zc, tried = 0;
-    for (int zc : zcs) {
-        c->tune = base;
-        c->tune.el_zc = zc;
-        float ms = 0;
-        int rc = b2r_el_forward(s, warm, 1, nullptr);
-        if (!rc) rc = b2r_el_forward(s, iters, 1, &ms);
-        if (rc) {
+    // best z chunk and time per kernel (index 1 fused, 2 separate)
+    float kms[3] = {1e30f, 1e30f, 1e30f};
+    int kzc[3] = {0, base.el_step_zc, base.el_zc}, tried = 0;
+    for (int kern = can_fuse ? 1 : 2; kern <= 2; ++kern)
+        for (int zc : zcs) {
             c->tune = base;
-            (void)b2r_el_reset(s);
-            return rc;
+            c->tune.el_kernel = kern;
+            (kern == 1 ? c->tune.el_step_zc : c->tune.el_zc) = zc;
+            float ms = 0;
+            int rc = b2r_el_forward(s, warm, 1, nullptr);
+            if (!rc) rc = b2r_el_forward(s, iters, 1, &ms);
+            if (rc) {
+                c->tune = base;
+                (void)b2r_el_reset(s);
+                return rc;
+            }
+            ++tried;
+            if (ms < kms[kern]) {
+                kms[kern] = ms;
+                kzc[kern] = zc;
+            }
         }
-        ++tried;
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = zc;
-        }
-    }
+    const int best_k = kms[1] < kms[2] ? 1 : 2;
+    const float best_ms = kms[best_k];
     c->tune = base;
-    c->tune.el_zc = best;
-    el_tiles(s->d, best);
+    c->tune.el_kernel = best_k;
+    c->tune.el_step_zc = kzc[1];
+    c->tune.el_zc = kzc[2];
+    el_tiles(s->d, kzc[2]);
     if (int rc = b2r_el_reset(s)) return rc;
     if (out) *out = c->tune;
     if (record_path && *record_path) {
         FILE* f = fopen(record_path, "a");
         B2R_CHECK(f, B2R_E_ARG, "cannot open %s", record_path);
-        fprintf(f, "kind=el_step S=%dx%dx%d blocks=%d R=%d radius=%d best=el_zc:%d "
-                   "ms_per_step=%.5f tried=%d\n",
-                c->L.S[0], c->L.S[1], c->L.S[2], c->L.nblocks, c->g.R, c->g.radius, best,
-                best_ms / iters, tried);
+        fprintf(f, "kind=el_step S=%dx%dx%d blocks=%d R=%d radius=%d "
+                   "best=el_kernel:%d,el_zc:%d,el_step_zc:%d ms_per_step=%.5f",
+                c->L.S[0], c->L.S[1], c->L.S[2], c->L.nblocks, c->g.R, c->g.radius, best_k,
+                kzc[2], kzc[1], best_ms / iters);
+        if (can_fuse)
+            fprintf(f, " fused_ms_per_step=%.5f separate_ms_per_step=%.5f", kms[1] / iters,
+                    kms[2] / iters);
+        fprintf(f, " tried=%d\n", tried);
+        fclose(f);
+    }
+    return B2R_OK;
+}
+
+extern "C" int b2r_el_autotune_batch(b2r_el_shot* const* shots, int nshots, const char* record_path)
+{
+    B2R_CHECK(shots && nshots >= 1 && nshots <= EL_MAX_BATCH, B2R_E_ARG, "1..%d shots",
+              EL_MAX_BATCH);
+    std::vector<b2r_tuning> base(nshots);
+    bool can_fuse = true;
+    long long cells = 0;
+    for (int k = 0; k < nshots; ++k) {
+        B2R_CHECK(shots[k], B2R_E_ARG, "null shot %d", k);
+        const b2r_ctx* c = shots[k]->ctx;
+        base[k] = c->tune;
+        can_fuse = can_fuse && (c->k.periodic || c->k.fmask == 0x3F);
+        cells += (long long)c->g.env_x * c->g.env_y * c->g.env_z;
+    }
+    auto restore = [&]() {
+        for (int k = 0; k < nshots; ++k) shots[k]->ctx->tune = base[k];
+    };
+    const int zcs[] = {0, 16, 32, 64, 128};
+    const int warm = 2, iters = 4;
+    float kms[3] = {1e30f, 1e30f, 1e30f};
+    int kzc[3] = {0, 0, 0}, tried = 0;
+    for (int kern = can_fuse ? 1 : 2; kern <= 2; ++kern)
+        for (int zc : zcs) {
+            for (int k = 0; k < nshots; ++k) {
+                b2r_tuning& t = shots[k]->ctx->tune;
+                t = base[k];
+                t.el_kernel = kern;
+                (kern == 1 ? t.el_step_zc : t.el_zc) = zc;
+            }
+            float ms = 0;
+            int rc = b2r_el_forward_batch(shots, nshots, warm, 1, nullptr);
+            if (!rc) rc = b2r_el_forward_batch(shots, nshots, iters, 1, &ms);
+            if (rc) {
+                restore();
+                for (int k = 0; k < nshots; ++k) (void)b2r_el_reset(shots[k]);
+                return rc;
+            }
+            ++tried;
+            if (ms < kms[kern]) {
+                kms[kern] = ms;
+                kzc[kern] = zc;
+            }
+        }
+    const int best_k = kms[1] < kms[2] ? 1 : 2;
+    for (int k = 0; k < nshots; ++k) {
+        b2r_tuning& t = shots[k]->ctx->tune;
+        t = base[k];
+        t.el_kernel = best_k;
+        if (can_fuse) t.el_step_zc = kzc[1];
+        t.el_zc = kzc[2];
+        if (int rc = b2r_el_reset(shots[k])) return rc;
+    }
+    if (record_path && *record_path) {
+        FILE* f = fopen(record_path, "a");
+        B2R_CHECK(f, B2R_E_ARG, "cannot open %s", record_path);
+        fprintf(f, "kind=el_batch shots=%d cells=%lld best=el_kernel:%d,el_zc:%d,el_step_zc:%d "
+                   "ms_per_step=%.5f",
+                nshots, cells, best_k, kzc[2], kzc[1], kms[best_k] / iters);
+        if (can_fuse)
+            fprintf(f, " fused_ms_per_step=%.5f separate_ms_per_step=%.5f", kms[1] / iters,
+                    kms[2] / iters);
+        fprintf(f, " tried=%d\n", tried);
         fclose(f);
     }
     return B2R_OK;

@@ -178,10 +178,11 @@ def elastic_batch_bench(A, W, nshots=16, nsteps=100, record=None):
     cells = sum(e[0] * e[1] * e[2] for e in sizes)
     ctxs, shots = elastic_shots(A, W, sizes, nsteps)
     try:
-        for s in shots:
-            s.autotune(record)
+        t = A.el_autotune_batch(shots, record)
         out = {"shots": nshots, "steps": nsteps, "cells_per_step": cells,
-               "sizes": ["%dx%dx%d" % e for e in sizes]}
+               "sizes": ["%dx%dx%d" % e for e in sizes],
+               "el_kernel": t["el_kernel"],
+               "el_zc": t["el_step_zc" if t["el_kernel"] == 1 else "el_zc"]}
         for s in shots:
             s.reset()
         A.el_forward_batch(shots, 2, mode=2)

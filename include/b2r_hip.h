@@ -104,6 +104,12 @@ typedef struct b2r_tuning {
                          bit-sliced 256 x 32, 256 x 64, 256 x 96; 0 = 5         */
     int heat_wx;      /* waves across x per heat step workgroup: 1, 2 or 4, with
                          heat_wx * heat_wy <= 16 (heat_wy 0 counts as 4)          */
+    int el_kernel;    /* batched elastic step: 1 one fused stress + velocity launch
+                         from a second copy of the state (ping-pong), 2 separate
+                         stress and velocity launches in place; 0 = 1 where every
+                         global face is frozen or all are periodic, else 2       */
+    int el_step_zc;   /* z planes marched per fused elastic step workgroup (el_zc
+                         serves the separate kernels)                           */
 } b2r_tuning;
 
 /* ---- library / context ---------------------------------------------------- */
@@ -306,10 +312,18 @@ int b2r_time_steps(b2r_ctx* ctx, int kind, void* d_a, void* d_b, const b2r_heat_
 int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b, const b2r_heat_params* p,
                  const char* record_path, b2r_tuning* out);
 /* The elastic kernels' autotuner (no counterpart in the reference): times the
- * batched step of one shot for each elastic z-chunk candidate, keeps the
- * fastest (b2r_tuning.el_zc) in the shot's context, appends a record line as
- * above, and leaves the shot reset (zero state, t = 0). */
+ * batched step of one shot with the fused step kernel (where the faces allow
+ * it) and with the separate stress and velocity kernels, for each z-chunk
+ * candidate, keeps each kernel's fastest z chunk (b2r_tuning.el_step_zc,
+ * el_zc) and the faster kernel (el_kernel) in the shot's context, appends a
+ * record line as above, and leaves the shot reset (zero state, t = 0). */
 int b2r_el_autotune(b2r_el_shot* s, const char* record_path, b2r_tuning* out);
+/* The same for a batch as b2r_el_forward_batch runs it (no counterpart in the
+ * reference): each candidate (kernel, z chunk) is set in every shot's context
+ * and timed on the whole batch, whose many workgroups load the GPU
+ * differently from any one shot; the fastest is kept in every context, one
+ * record line is appended, and the shots are left reset. */
+int b2r_el_autotune_batch(b2r_el_shot* const* shots, int nshots, const char* record_path);
 /* Scratch device buffer owned by the context (freed by b2r_destroy). */
 int b2r_dev_alloc(b2r_ctx* ctx, size_t bytes, void** d_ptr);
 int b2r_dev_free(b2r_ctx* ctx, void* d_ptr);

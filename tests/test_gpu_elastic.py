@@ -39,15 +39,23 @@ def _shot(ctx, P, rec_comp=1, nt=None):
     return s
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
+# (mode, el_kernel): the plain loop, and the batched modes with the fused
+# step kernel (1) and with the separate stress and velocity kernels (2)
+MODE_KERNELS = [(0, 0), (1, 1), (1, 2), (2, 1), (2, 2)]
+
+
+@pytest.mark.parametrize("mode,kern", MODE_KERNELS)
 @pytest.mark.parametrize("env,blocks,R", [((48, 40, 48), (1, 1, 1), 1),
                                           ((48, 40, 48), (2, 1, 2), 1),
                                           ((48, 40, 48), (2, 2, 1), 2),
-                                          ((48, 56, 56), (1, 2, 2), 3)])
-def test_forward_bitwise_vs_oracle(env, blocks, R, mode):
+                                          ((48, 56, 56), (1, 2, 2), 3),
+                                          ((130, 50, 36), (1, 1, 1), 1),
+                                          ((136, 56, 40), (2, 2, 1), 2)])
+def test_forward_bitwise_vs_oracle(env, blocks, R, mode, kern):
     P = E.small_problem(env=env, nsteps=13)
-    say("elastic forward", blocks, "R", R, "mode", mode)
+    say("elastic forward", env, blocks, "R", R, "mode", mode, "el_kernel", kern)
     with A.Context(P["env"], blocks, R, 4, A.BOUNDARY_FROZEN, 4) as c:
+        c.set_tuning(el_kernel=kern)
         with _shot(c, P) as s:
             s.forward(P["nsteps"], mode)
             got = [s.state(k) for k in range(9)]
@@ -59,11 +67,11 @@ def test_forward_bitwise_vs_oracle(env, blocks, R, mode):
     assert bits_equal(rec, want_rec)
 
 
-@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("mode,kern", [(0, 0), (2, 1), (2, 2)])
 @pytest.mark.parametrize("env,blocks,R", [((32, 24, 40), (1, 1, 1), 1),
                                           ((48, 40, 32), (2, 1, 2), 1),
                                           ((48, 48, 48), (1, 2, 2), 2)])
-def test_periodic_forward_bitwise_vs_oracle(env, blocks, R, mode):
+def test_periodic_forward_bitwise_vs_oracle(env, blocks, R, mode, kern):
     # _PERIODIC_ (b2r_config.h:12) for eqwp: a source two cells from the x face,
     # so the wave wraps within the run
     nsteps = 12
@@ -71,8 +79,9 @@ def test_periodic_forward_bitwise_vs_oracle(env, blocks, R, mode):
     src = np.array([[2, env[1] // 2, env[2] // 3]], np.int32)
     rec = np.array([[env[0] - 1, env[1] // 2, env[2] // 3], [0, 0, 0], [5, 3, env[2] - 1]],
                    np.int32)
-    say("elastic periodic forward", env, blocks, "R", R, "mode", mode)
+    say("elastic periodic forward", env, blocks, "R", R, "mode", mode, "el_kernel", kern)
     with A.Context(env, blocks, R, 4, A.BOUNDARY_PERIODIC, 4) as c:
+        c.set_tuning(el_kernel=kern)
         with A.ElasticShot(c, float(P["dt"]), P["ih"]) as s:
             s.set_medium(P["lam"], P["mu"], P["b"])
             s.set_sources(src, [1], P["wav"])
@@ -88,13 +97,16 @@ def test_periodic_forward_bitwise_vs_oracle(env, blocks, R, mode):
     assert bits_equal(got_rec, want_rec)
 
 
-def test_forward_displacement_and_split_runs():
+@pytest.mark.parametrize("kern", [1, 2])
+def test_forward_displacement_and_split_runs(kern):
     P = E.small_problem(env=(24, 20, 22), nsteps=10)
     with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
+        c.set_tuning(el_kernel=kern)
         with _shot(c, P) as s:
             s.enable_displacement()
             s.forward(4, 0)
-            s.forward(6, 1)   # continues the clock (sources, receivers)
+            s.forward(3, 1)   # continues the clock (sources, receivers)
+            s.forward(3, 2)   # an odd count: a fused run ends in the second set
             disp = s.state("disp")
             vy = s.state("vy")
             rec = s.receivers()
@@ -103,12 +115,21 @@ def test_forward_displacement_and_split_runs():
     assert bits_equal(vy, st[1]) and bits_equal(disp, d) and bits_equal(rec, want_rec)
 
 
-def test_uneven_batch_matches_single_shots():
+def test_uneven_batch_matches_single_shots(tmp_path):
     envs = [(32, 24, 40), (48, 32, 24), (24, 40, 32)]
     Ps = [E.small_problem(env=e, seed=i, nsteps=9) for i, e in enumerate(envs)]
     ctxs = [A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) for P in Ps]
     shots = [_shot(c, P) for c, P in zip(ctxs, Ps)]
     try:
+        # the batch autotuner sets one choice in every context and resets the shots
+        rec_file = str(tmp_path / "tune.txt")
+        shots[0].forward(2, 1)
+        t = A.el_autotune_batch(shots, rec_file)
+        assert t["el_kernel"] in (1, 2)
+        assert all(c.get_tuning()["el_kernel"] == t["el_kernel"] for c in ctxs)
+        assert not np.any(shots[0].state(1))
+        line = open(rec_file).read()
+        assert "kind=el_batch shots=3" in line and "fused_ms_per_step=" in line
         A.el_forward_batch(shots, 9, mode=2)
         for P, s in zip(Ps, shots):
             _, want = E.forward(P["env"], P["lam"], P["mu"], P["b"], P["dt"], P["ih"], 9,
@@ -323,11 +344,12 @@ def test_elastic_zchunk_tuning_and_autotune(tmp_path):
     P = E.small_problem(env=(40, 36, 44), nsteps=12)
     outs = []
     with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
-        for zc in (0, 5, 16, 64):
-            c.set_tuning(el_zc=zc)
-            with _shot(c, P) as s:
-                s.forward(P["nsteps"], 1)
-                outs.append((s.receivers(), s.state(1)))
+        for kern in (1, 2):
+            for zc in (0, 5, 16, 64):
+                c.set_tuning(el_kernel=kern, el_zc=zc, el_step_zc=zc)
+                with _shot(c, P) as s:
+                    s.forward(P["nsteps"], 1)
+                    outs.append((s.receivers(), s.state(1)))
         for rec, vy in outs[1:]:
             assert bits_equal(rec, outs[0][0]) and bits_equal(vy, outs[0][1])
         rec_file = str(tmp_path / "tune.txt")
@@ -335,11 +357,32 @@ def test_elastic_zchunk_tuning_and_autotune(tmp_path):
             s.forward(3, 1)
             t = s.autotune(rec_file)
             assert t["el_zc"] in (0, 16, 32, 64, 128, 256)
+            assert t["el_step_zc"] in (0, 16, 32, 64, 128, 256) and t["el_kernel"] in (1, 2)
             assert not np.any(s.state(1))
             s.forward(P["nsteps"], 2)
             assert bits_equal(s.receivers(), outs[0][0])
         line = open(rec_file).read()
-        assert "kind=el_step" in line and "best=el_zc:" in line
+        assert "kind=el_step" in line and "best=el_kernel:" in line and ",el_step_zc:" in line
+
+
+def test_fused_kernel_needs_closed_faces():
+    # an open face keeps cells the fused step does not carry over: asking for
+    # it there is an error, the automatic choice takes the separate kernels
+    P = E.small_problem(env=(24, 20, 22), nsteps=4)
+    with A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) as c:
+        c.set_frozen_faces(0x3F & ~1)
+        with _shot(c, P) as s:
+            c.set_tuning(el_kernel=1)
+            with pytest.raises(A.B2RError):
+                s.forward(2, 1)
+            c.set_tuning(el_kernel=0)
+            s.reset()
+            s.forward(4, 2)
+            got = s.receivers()
+            c.set_tuning(el_kernel=2)
+            s.reset()
+            s.forward(4, 2)
+            assert bits_equal(got, s.receivers())
 
 
 def test_gradient_batch_larger_grids_agree_with_plain():
