@@ -158,10 +158,7 @@ __global__ __launch_bounds__(HEAT_TX * WX * WY) void heat_step_kernel(
                     mz[k] = q[j][D - k];
                     pz_[k] = q[j][D + k];
                 }
-                const float lx = lap<D, MODE>(mx, px, uc);
-                const float ly = lap<D, MODE>(my, py, uc);
-                const float lz = lap<D, MODE>(mz, pz_, uc);
-                sto(dst, pz + yo[j], heat_combine<MODE>(uc, lx, ly, lz, hk));
+                sto(dst, pz + yo[j], heat_update<D, MODE>(mx, px, my, py, mz, pz_, uc, hk));
             }
         }
 #pragma unroll
@@ -247,12 +244,7 @@ static void launch_heat(b2r_ctx* ctx, const float* in, float* out, const HeatK& 
 
 int b2r_heat_launch(b2r_ctx* ctx, const void* in, void* out, const b2r_heat_params* p, int i)
 {
-    HeatK hk;
-    for (int a = 0; a < 3; ++a) {
-        hk.k[a] = p->k[a];
-        hk.h2[a] = p->h2[a];
-    }
-    hk.adt = p->adt;
+    const HeatK hk = heat_k(p);
     const float* fi = (const float*)in;
     float* fo = (float*)out;
 #define B2R_HEAT_CASE(DD)                                                              \
@@ -429,10 +421,7 @@ __global__ __launch_bounds__(256) void heat_fused_kernel(const float* __restrict
                         mz[k] = q[t - 1][j][D - k];
                         pz_[k] = q[t - 1][j][D + k];
                     }
-                    const float lx = lap<D, MODE>(mx, px, uc);
-                    const float ly = lap<D, MODE>(my, py, uc);
-                    const float lz = lap<D, MODE>(mz, pz_, uc);
-                    r = heat_combine<MODE>(uc, lx, ly, lz, hk);
+                    r = heat_update<D, MODE>(mx, px, my, py, mz, pz_, uc, hk);
                 }
                 if (t < R) {
 #pragma unroll
@@ -499,12 +488,7 @@ static int heat_fused_mode(b2r_ctx* ctx, const float* in, float* out, const Heat
 
 int b2r_heat_fused_launch(b2r_ctx* ctx, const void* in, void* out, const b2r_heat_params* p)
 {
-    HeatK hk;
-    for (int a = 0; a < 3; ++a) {
-        hk.k[a] = p->k[a];
-        hk.h2[a] = p->h2[a];
-    }
-    hk.adt = p->adt;
+    const HeatK hk = heat_k(p);
     if (p->mode == B2R_HEAT_F32)
         return heat_fused_mode<B2R_HEAT_F32>(ctx, (const float*)in, (float*)out, hk, p->D);
     return heat_fused_mode<B2R_HEAT_REF>(ctx, (const float*)in, (float*)out, hk, p->D);

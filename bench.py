@@ -523,6 +523,45 @@ def profile_kernels(A):
     print(json.dumps({"profile_kernels": "done"}), flush=True)
 
 
+def heat_sections(A, bw, rec=None):
+    """Heat delta1 768^3 (single pass and fused R=2) and delta4 768^3 on 2x2x2
+    blocks (R and fusion tried, best kept), as percent of the STREAM copy bw."""
+    also = {}
+    env = (768, 768, 768)
+    log("heat delta1 768^3 single pass")
+    h1 = heat_bench(A, env, (1, 1, 1), 1, 1, (0.1, 0.1, 0.1), 100,
+                    lambda c, f: c.init_uniform(f, 0), fused=0, record=rec)
+    h1["kernel_pct_of_stream"] = round(100 * h1["kernel_gbs_alg"] / bw, 1)
+    h1["run_pct_of_stream"] = round(100 * h1["glups"] * 8 / bw, 1)
+    also["heat_delta1_768_single_pass"] = h1
+    log(json.dumps(h1))
+    log("heat delta1 768^3 fused R=2")
+    h1f = heat_bench(A, env, (1, 1, 1), 2, 1, (0.1, 0.1, 0.1), 100,
+                     lambda c, f: c.init_uniform(f, 0), fused=1, record=rec)
+    h1f["effective_pct_of_stream"] = round(100 * h1f["glups"] * 8 / bw, 1)
+    also["heat_delta1_768_fused_R2"] = h1f
+    log(json.dumps(h1f))
+    amp, (ex, ey, ez) = blob_tables(env)
+    h4runs = []
+    for R4, f4 in ((1, 0), (1, 1), (2, 0)):
+        log(f"heat delta4 768^3 blobs 2x2x2, 500 steps, R={R4} fused={f4}")
+        h4 = heat_bench(A, env, (2, 2, 2), R4, 4, (0.05, 0.05, 0.05), 500,
+                        lambda c, f: c.init_blobs(f, amp, ex, ey, ez), fused=f4, record=rec)
+        h4["run_pct_of_stream"] = round(100 * h4["glups"] * 8 / bw, 1)
+        h4["kernel_pct_of_stream"] = round(100 * h4["kernel_gbs_alg"] / bw, 1)
+        h4runs.append(h4)
+        log(json.dumps(h4))
+    also["heat_delta4_768_2x2x2"] = max(h4runs, key=lambda r: r["glups"])
+    record_choice(rec, "kind=heat_R S=768x768x768 blocks=2x2x2 heat_D=4",
+                  "%d%s" % (also["heat_delta4_768_2x2x2"]["R"],
+                            "_fused" if also["heat_delta4_768_2x2x2"]["fused"] else ""),
+                  {"%d%s" % (r["R"], "_fused" if r["fused"] else ""): r["glups"]
+                   for r in h4runs})
+    also["heat_delta4_768_2x2x2_tried"] = [
+        {k: r[k] for k in ("R", "fused", "glups", "tuning")} for r in h4runs]
+    return also
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -536,8 +575,8 @@ def main():
                     help="append the autotuner's choices to this file")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="run the fixed kernel sequence used by the rocprofv3 counter runs")
-    ap.add_argument("--only", choices=["elastic-gradient", "elastic-batch"],
-                    help="run one elastic section alone and print its JSON (for profiling)")
+    ap.add_argument("--only", choices=["elastic-gradient", "elastic-batch", "heat"],
+                    help="run one section alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the halo "
@@ -555,6 +594,12 @@ def main():
         return
     if args.only == "elastic-batch":
         print(json.dumps(elastic_batch_bench(A, W, 16, args.elastic_steps)), flush=True)
+        return
+    if args.only == "heat":
+        bw = stream_bw(A)
+        out = heat_sections(A, bw)
+        out["stream_copy_GBs"] = round(bw, 1)
+        print(json.dumps(out), flush=True)
         return
     rec = args.tune_record or None
 
@@ -587,37 +632,7 @@ def main():
         log("stream")
         bw = stream_bw(A)
         also["stream_copy_GBs"] = round(bw, 1)
-        env = (768, 768, 768)
-        log("heat delta1 768^3 single pass")
-        h1 = heat_bench(A, env, (1, 1, 1), 1, 1, (0.1, 0.1, 0.1), 100,
-                        lambda c, f: c.init_uniform(f, 0), fused=0, record=rec)
-        h1["kernel_pct_of_stream"] = round(100 * h1["kernel_gbs_alg"] / bw, 1)
-        h1["run_pct_of_stream"] = round(100 * h1["glups"] * 8 / bw, 1)
-        also["heat_delta1_768_single_pass"] = h1
-        log(json.dumps(h1))
-        log("heat delta1 768^3 fused R=2")
-        h1f = heat_bench(A, env, (1, 1, 1), 2, 1, (0.1, 0.1, 0.1), 100,
-                         lambda c, f: c.init_uniform(f, 0), fused=1, record=rec)
-        h1f["effective_pct_of_stream"] = round(100 * h1f["glups"] * 8 / bw, 1)
-        also["heat_delta1_768_fused_R2"] = h1f
-        log(json.dumps(h1f))
-        amp, (ex, ey, ez) = blob_tables(env)
-        h4runs = []
-        for R4, f4 in ((1, 0), (1, 1), (2, 0)):
-            log(f"heat delta4 768^3 blobs 2x2x2, 500 steps, R={R4} fused={f4}")
-            h4 = heat_bench(A, env, (2, 2, 2), R4, 4, (0.05, 0.05, 0.05), 500,
-                            lambda c, f: c.init_blobs(f, amp, ex, ey, ez), fused=f4, record=rec)
-            h4["run_pct_of_stream"] = round(100 * h4["glups"] * 8 / bw, 1)
-            h4runs.append(h4)
-            log(json.dumps(h4))
-        also["heat_delta4_768_2x2x2"] = max(h4runs, key=lambda r: r["glups"])
-        record_choice(rec, "kind=heat_R S=768x768x768 blocks=2x2x2 heat_D=4",
-                      "%d%s" % (also["heat_delta4_768_2x2x2"]["R"],
-                                "_fused" if also["heat_delta4_768_2x2x2"]["fused"] else ""),
-                      {"%d%s" % (r["R"], "_fused" if r["fused"] else ""): r["glups"]
-                       for r in h4runs})
-        also["heat_delta4_768_2x2x2_tried"] = [
-            {k: r[k] for k in ("R", "fused", "glups", "tuning")} for r in h4runs]
+        also.update(heat_sections(A, bw, rec))
         h0 = heat_bench(A, (128, 128, 128), (1, 1, 1), 1, 1, (0.1, 0.1, 0.1), 100,
                         lambda c, f: c.init_uniform(f, 0))
         also["heat_delta1_128"] = h0

@@ -5,6 +5,8 @@ oracle (a wrong tap, sign, index or loop bound) fails at least one of these."""
 import math
 import os
 
+from fractions import Fraction
+
 import numpy as np
 import pytest
 
@@ -138,30 +140,71 @@ def test_f32_order_close_to_f64(D, mode, periodic):
     assert rel_l2(f32, f64) < 2e-6
 
 
-def test_f32_mode_matches_numpy_one_step():
-    # an independent vectorised restatement of the F32 evaluation order
+def _round_f32(q):
+    """The float32 nearest to the rational q, ties to even (normal range)."""
+    if q == 0:
+        return np.float32(0.0)
+    sign = -1 if q < 0 else 1
+    q = abs(q)
+    e = q.numerator.bit_length() - q.denominator.bit_length()
+    if Fraction(2) ** e > q:
+        e -= 1  # 2**e <= q < 2**(e+1)
+    m = q / Fraction(2) ** (e - 23)  # in [2**23, 2**24)
+    n, r = divmod(m.numerator, m.denominator)
+    if 2 * r > m.denominator or (2 * r == m.denominator and n & 1):
+        n += 1
+    return np.float32(sign * n * 2.0 ** (e - 23))
+
+
+def _fmaf(a, b, c):
+    """a*b + c rounded once to float32, in exact rational arithmetic."""
+    return _round_f32(Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c)))
+
+
+def test_exact_fmaf_helper():
+    # one rounding, not two: 24929 * 673 = 2^24 + 1, a midpoint between float32
+    # neighbours 2^24 and 2^24 + 2; adding 2^-30 puts the exact sum just above
+    # it (rounds up), while rounding to double first lands on the midpoint and
+    # ties to even (down)
+    a, b, c = np.float32(24929), np.float32(673), np.float32(2.0 ** -30)
+    assert _fmaf(a, b, c) == np.float32(2 ** 24 + 2)
+    assert np.float32(float(a) * float(b) + float(c)) == np.float32(2 ** 24)
+    assert _fmaf(a, b, -c) == np.float32(2 ** 24)
+    assert _fmaf(np.float32(3), np.float32(5), np.float32(-15)) == 0
+    assert _fmaf(np.float32(-1.5), np.float32(2), np.float32(0.25)) == np.float32(-2.75)
+
+
+def test_f32_mode_matches_exact_one_step():
+    # an independent restatement of the F32 evaluation order: float32 adds and
+    # products in numpy, each fused multiply-add in exact rational arithmetic
     D = 3
-    owned = O.uniform((9, 10, 11), seed=11)
+    owned = O.uniform((6, 7, 5), seed=11)
     g = O.frame(owned, D)
     k = np.float32([0.021, 0.033, 0.017])
     got = O.heat_f32(g, D, 0, k, 0.0, (1, 1, 1), 1, False)
     c = O.coef_f32(D)
+    w = [[np.float32(k[a] * c[j]) for j in range(D + 1)] for a in range(3)]
+    w0 = np.float32(np.float32(k[0] + k[1]) + k[2]) * c[0]
     u = g
-
-    def sh(a, axis, s):
-        return np.roll(a, -s, axis=axis)
-
-    def L(axis):
-        acc = c[D] * (sh(u, axis, -D) + sh(u, axis, D))
-        for j in range(D - 1, 0, -1):
-            acc = acc + c[j] * (sh(u, axis, -j) + sh(u, axis, j))
-        return acc + c[0] * u
-
-    lx, ly, lz = L(2), L(1), L(0)
-    t = k[0] * lx
-    t = t + k[1] * ly
-    t = t + k[2] * lz
-    want = u + t
+    want = u.copy()
+    nz, ny, nx = u.shape
+    for z in range(D, nz - D):
+        for y in range(D, ny - D):
+            for x in range(D, nx - D):
+                def pair(a, j):
+                    if a == 0:
+                        return np.float32(u[z, y, x - j] + u[z, y, x + j])
+                    if a == 1:
+                        return np.float32(u[z, y - j, x] + u[z, y + j, x])
+                    return np.float32(u[z - j, y, x] + u[z + j, y, x])
+                t = np.float32(w[0][D] * pair(0, D))
+                t = _fmaf(w[1][D], pair(1, D), t)
+                t = _fmaf(w[2][D], pair(2, D), t)
+                for j in range(D - 1, 0, -1):
+                    for a in range(3):
+                        t = _fmaf(w[a][j], pair(a, j), t)
+                t = _fmaf(w0, u[z, y, x], t)
+                want[z, y, x] = np.float32(u[z, y, x] + t)
     s = slice(D, -D)
     assert np.array_equal(got[s, s, s].view(np.uint32), want[s, s, s].view(np.uint32))
 
