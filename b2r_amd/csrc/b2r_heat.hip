@@ -113,12 +113,19 @@ __global__ __launch_bounds__(HEAT_TX * WX * WY) void heat_step_kernel(
             q[j][p] = (xin && yin[j]) ? ldo(src, p * sz + yo[j]) : 0.0f;
     }
 
-    // loaded a plane ahead: the halo values of the next plane (hv) and the
-    // tile's values D planes ahead of the next plane (qn), so the wait at the
-    // first barrier finds them already returned
-    float hv[NSLOT], qn[RY];
+    // The tile's values are loaded D planes ahead of the next plane (qn, for
+    // the z-queue).  For D >= 2 so are the halo values (a queue hq of HA+1
+    // planes): a halo row is its neighbour tile's own row, which that tile
+    // loads D planes ahead, so both requests reach L2 at about the same time
+    // instead of D planes apart (about 1 MB of plane data per XCD per plane at
+    // D=4, which the 4 MB L2 does not keep for 4 planes).  At D=1 the halo is
+    // loaded one plane ahead.
+    constexpr int HA = D >= 2 ? D : 0;
+    float hq[NSLOT][HA + 1], qn[RY];
 #pragma unroll
-    for (int s = 0; s < NSLOT; ++s) hv[s] = hok[s] ? ldo(src, D * sz + ho[s]) : 0.0f;
+    for (int s = 0; s < NSLOT; ++s)
+#pragma unroll
+        for (int k = 0; k <= HA; ++k) hq[s][k] = hok[s] ? ldo(src, (D + k) * sz + ho[s]) : 0.0f;
 #pragma unroll
     for (int j = 0; j < RY; ++j) qn[j] = (xin && yin[j]) ? ldo(src, 2 * D * sz + yo[j]) : 0.0f;
     uint32_t pz = D * sz;  // plane z
@@ -129,17 +136,20 @@ __global__ __launch_bounds__(HEAT_TX * WX * WY) void heat_step_kernel(
 #pragma unroll
         for (int j = 0; j < RY; ++j) lds[(wy * RY + j + D) * LX + lane + D] = q[j][D];
 #pragma unroll
-        for (int s = 0; s < NSLOT; ++s)
-            if (hok[s]) lds[hl[s]] = hv[s];
+        for (int s = 0; s < NSLOT; ++s) {
+            if (hok[s]) lds[hl[s]] = hq[s][0];
+#pragma unroll
+            for (int k = 0; k < HA; ++k) hq[s][k] = hq[s][k + 1];
+        }
         if (z < zhi) {
-            // planes z+1 and z+1+D: z+1 <= zhi <= S[2]-1-D keeps both in the block
+            // planes z+1+HA and z+1+D: z+1 <= zhi <= S[2]-1-D keeps both in the block
             const uint32_t pn = pz + (D + 1) * sz;
 #pragma unroll
             for (int j = 0; j < RY; ++j)
                 if (xin && yin[j]) qn[j] = ldo(src, pn + yo[j]);
 #pragma unroll
             for (int s = 0; s < NSLOT; ++s)
-                if (hok[s]) hv[s] = ldo(src, pz + sz + ho[s]);
+                if (hok[s]) hq[s][HA] = ldo(src, pz + (HA + 1) * sz + ho[s]);
         }
         __syncthreads();
 #pragma unroll
