@@ -338,7 +338,15 @@ __global__ __launch_bounds__((TXQ + 2) * NR) void gol_fused_kernel(const uint8_t
 //   the rule: alive next iff box == 5, or alive and 3 <= box <= 8.
 // Every adder is built from full adders of two 3-input logic operations
 // (xor3, majority); the compiled generation takes about 70 instructions per 32
-// cells against about 150 for the byte-lane form.  Words lie on the 16-byte grid of x = h,
+// cells against about 150 for the byte-lane form.
+// With pull set (B_x a multiple of 32), the words just outside a block's
+// owned x range -- the one ending at x = h and the one starting at x = h+B --
+// are loaded from the x-neighbour's row at the same (y, z) (its cells x+B or
+// x-B, owned cells there or, in its y/z halo, ones the Y and Z exchange phases
+// filled), so the X phase of the exchange is not needed: that phase moves
+// 1-4 bytes per row, one cache line each, and cost a quarter of the R=4
+// window.  At a frozen face (no neighbour) the word comes from the block's
+// own frozen halo.  Words lie on the 16-byte grid of x = h,
 // so each half of a word (16 cells) lies inside or outside the padded row as a
 // whole: halo words load only their half next to the tile, a row's last word
 // loads and stores its second half only when that half is in the row.  Cells
@@ -395,7 +403,7 @@ template <int R, int NW, int NR>
 __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* __restrict__ in,
                                                                  uint8_t* __restrict__ out,
                                                                  KGeo g, int ntx, int nty,
-                                                                 int nzc, int zc)
+                                                                 int nzc, int zc, int pull)
 {
     constexpr int NQ = NW + 2;  // words per row: plus one per side
     constexpr int NT = NQ * NR;
@@ -405,13 +413,19 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
 
     const int tid = threadIdx.x;
     const int r = tid / NQ, q = tid % NQ;
+    // workgroup order: x tile, then the block's x coordinate, then y tile, z
+    // chunk and the block's y, z coordinates, so the tiles of x-neighbouring
+    // blocks over the same rows run together on one XCD and a pulled halo word
+    // is found in the L2 their owner filled
     int tt = xcd_tile(blockIdx.x, gridDim.x);
     const int txi = tt % ntx;
     tt /= ntx;
+    const int bxi = tt % g.nb[0];
+    tt /= g.nb[0];
     const int tyi = tt % nty;
     tt /= nty;
     const int zci = tt % nzc;
-    const int b = tt / nzc;
+    const int b = tt / nzc * g.nb[0] + bxi;
     const int h = g.h;
     const int x0 = h + txi * 32 * NW;  // first output cell
     const int y0 = h + tyi * TY;
@@ -431,6 +445,13 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
     // tile's first word of the last row, in bounds and aligned, and use zero
     const uint8_t* __restrict__ col =
         src + (ld_ok ? (long long)y * pitch + x : (long long)min(y, g.S[1] - 1) * pitch + x0);
+    if (pull && ld_ok && (x == h - 32 || x == h + g.B[0])) {
+        const int side = x == h - 32 ? 0 : 1;
+        const int nb = neighbour_block(g, b, 0, side);
+        if (nb >= 0)
+            col = in + (long long)nb * g.bstride + g.xoff + (long long)y * pitch +
+                  (side ? x - g.B[0] : x + g.B[0]);
+    }
     const uint32_t ldm = ld_ok ? ~0u : 0u;
     // a halo word needs only its 16 cells next to the tile (R <= 4), and the
     // second half of a row's last word may lie past the padded row end: such a
@@ -547,6 +568,16 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
     }
 }
 
+// The bit-sliced kernel reads the x halo from the x-neighbours (pull above)
+// when the block's owned x range is a whole number of 32-cell words.
+static int gol_bits_pull(const KGeo& g) { return g.B[0] % 32 == 0; }
+
+int b2r_gol_fused_reads_x_halo(const b2r_ctx* ctx)
+{
+    const int t = ctx->tune.gol_tile;
+    return (t == 0 || t >= 4) && gol_bits_pull(ctx->k);
+}
+
 static bool gol_fused_ok(const b2r_ctx* ctx, int R)
 {
     return R >= 1 && R <= 4 && (ctx->k.periodic || ctx->k.fmask == 0x3F);
@@ -581,6 +612,7 @@ static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
 {
     constexpr int TY = NR - 2 * R;
     const KGeo& g = ctx->k;
+    const int pull = gol_bits_pull(g);
     const int ntx = ceil_div(g.B[0], 32 * NW);
     const int nty = ceil_div(g.B[1], TY);
     long long tiles = (long long)ntx * nty * g.nblk;
@@ -594,7 +626,7 @@ static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
     const int nzc = ceil_div(g.B[2], zc);
     dim3 grid(ntx * nty * nzc * g.nblk);
     gol_bits_kernel<R, NW, NR><<<grid, (NW + 2) * NR, 0, ctx->stream>>>(
-        (const uint8_t*)in, (uint8_t*)out, g, ntx, nty, nzc, zc);
+        (const uint8_t*)in, (uint8_t*)out, g, ntx, nty, nzc, zc, pull);
     B2R_LAUNCH_CHECK();
     return B2R_OK;
 }
@@ -687,9 +719,12 @@ extern "C" int b2r_gol_run(b2r_ctx* ctx, void* d_a, void* d_b, int nsteps, int f
     const int R = ctx->g.R;
     const bool use_fused = fused && gol_fused_ok(ctx, R);
     for (int s = 0; s < nsteps; s += R) {
-        if (int rc = b2r_direct_exchange(ctx, &cur, 1)) return rc;
         const int n = nsteps - s < R ? nsteps - s : R;
-        if (use_fused && n == R) {
+        const bool fuse = use_fused && n == R;
+        // a bit-sliced launch reads the x halo from the neighbours itself
+        const int first = fuse && b2r_gol_fused_reads_x_halo(ctx) ? 1 : 0;
+        if (int rc = b2r_direct_exchange(ctx, &cur, 1, first)) return rc;
+        if (fuse) {
             if (int rc = b2r_gol_fused_launch(ctx, cur, nxt)) return rc;
             void* t = cur;
             cur = nxt;

@@ -4,7 +4,10 @@
 // 16, heat-3D.cu:8, eqwp.cu:25).  For one kernel kind on the context's actual
 // geometry it times every candidate shape with back-to-back launches between
 // HIP events (b2r_time_steps), keeps the fastest in the context and appends one
-// line per search to a plain-text record:
+// line per search to a plain-text record.  The fused Game of Life candidates
+// are timed with the exchange that precedes each launch in b2r_gol_run, since
+// the bit-sliced kernel reads the x halo itself and needs no X phase while the
+// byte-lane kernel does (kernel time alone would not show that difference):
 //   kind=<k> S=<Sx>x<Sy>x<Sz> blocks=<n> R=<R> radius=<r> best=<field>:<v>,... ms=<t> tried=<n>
 #include "b2r_common.h"
 
@@ -14,6 +17,21 @@
 
 extern "C" int b2r_time_steps(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
                               const b2r_heat_params* p, int iters, float* ms);
+
+// iters windows of exchange + fused Game of Life launch, as b2r_gol_run runs them
+static int gol_windows(b2r_ctx* ctx, void* d_a, void* d_b, int iters, float* ms)
+{
+    if (int rc = b2r_activate(ctx)) return rc;
+    if (int rc = b2r_time_begin(ctx, ms)) return rc;
+    const int first = b2r_gol_fused_reads_x_halo(ctx) ? 1 : 0;
+    for (int it = 0; it < iters; ++it) {
+        void* in = (it & 1) ? d_b : d_a;
+        void* out = (it & 1) ? d_a : d_b;
+        if (int rc = b2r_direct_exchange(ctx, &in, 1, first)) return rc;
+        if (int rc = b2r_gol_fused_launch(ctx, in, out)) return rc;
+    }
+    return b2r_time_end(ctx, ms);
+}
 
 extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
                             const b2r_heat_params* p, const char* record_path, b2r_tuning* out)
@@ -64,8 +82,14 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
     for (const b2r_tuning& t : cand) {
         ctx->tune = t;
         float ms = 0;
-        int rc = b2r_time_steps(ctx, kind, d_a, d_b, p, warm, &ms);
-        if (!rc) rc = b2r_time_steps(ctx, kind, d_a, d_b, p, iters, &ms);
+        int rc;
+        if (kind == 2) {
+            rc = gol_windows(ctx, d_a, d_b, warm, &ms);
+            if (!rc) rc = gol_windows(ctx, d_a, d_b, iters, &ms);
+        } else {
+            rc = b2r_time_steps(ctx, kind, d_a, d_b, p, warm, &ms);
+            if (!rc) rc = b2r_time_steps(ctx, kind, d_a, d_b, p, iters, &ms);
+        }
         if (rc) {
             ctx->tune = base;
             return rc;
@@ -94,7 +118,8 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
         } else {
             fprintf(f, " best=gol_tile:%d,gol_zc:%d", best.gol_tile, best.gol_zc);
         }
-        fprintf(f, " ms_per_launch=%.5f tried=%d\n", best_ms / iters, tried);
+        fprintf(f, " %s=%.5f tried=%d\n", kind == 2 ? "ms_per_window" : "ms_per_launch",
+                best_ms / iters, tried);
         fclose(f);
     }
     return B2R_OK;

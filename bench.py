@@ -488,8 +488,10 @@ def profile_kernels(A):
     size (1 GiB read + 1 GiB written per launch, the scale for FETCH_SIZE /
     WRITE_SIZE), then 5 launches each of the fused Game of Life kernel (512^3,
     2x2x2, R = 2 and 4, the tile shapes and z chunks the autotuner picks), the
-    heat delta1 step kernel (768^3, 16-wave workgroups, RY = 8, 32-plane chunks)
-    and the elastic stress and velocity kernels (256^3, 64-plane chunks)."""
+    heat delta1 step kernel (768^3, 16-wave workgroups, RY = 8, 32-plane chunks),
+    the heat delta4 step kernel (768^3 on 2x2x2 blocks, 8x2 waves, RY = 4,
+    64-plane chunks), the elastic stress and velocity kernels (256^3, 64-plane
+    chunks) and the fused elastic step kernel (256^3, automatic z chunk)."""
     nbytes = 1 << 30
     with A.Context((16, 16, 16), (1, 1, 1), 1, 1, 0, 4) as c:
         src, dst = c.dev_alloc(nbytes), c.dev_alloc(nbytes)
@@ -510,14 +512,24 @@ def profile_kernels(A):
         c.set_tuning(heat_wy=16, heat_ry=8, heat_zc=32)
         log("heat_step_kernel<1> x5")
         c.time_steps(0, a, b, 5, prm)
+    prm4 = A.heat_params(4, A.HEAT_F32, (0.05, 0.05, 0.05))
+    with A.Context((768, 768, 768), (2, 2, 2), 1, 4, 0, 4) as c:
+        a, b = c.field(), c.field()
+        c.init_uniform(a, 0)
+        c.set_tuning(heat_wy=8, heat_wx=2, heat_ry=4, heat_zc=64)
+        log("heat_step_kernel<4> x5")
+        c.time_steps(0, a, b, 5, prm4)
     from b2r_amd import workloads as W
     env = (256, 256, 256)
     vp, vs, rho = W.layered_medium(env, 0)
     with A.Context(env, (1, 1, 1), 1, 4, 0, 4) as c:
-        c.set_tuning(el_zc=64)
+        c.set_tuning(el_zc=64, el_kernel=2)
         s = A.ElasticShot(c, W.cfl_dt(float(vp.max()), 10.0), (0.1, 0.1, 0.1))
         s.set_medium(*W.lame(vp, vs, rho))
         log("elastic stress + velocity x5")
+        s.forward(5, 1)
+        c.set_tuning(el_kernel=1, el_step_zc=0)
+        log("elastic fused step x5")
         s.forward(5, 1)
         s.close()
     print(json.dumps({"profile_kernels": "done"}), flush=True)
