@@ -1480,6 +1480,18 @@ extern "C" int b2r_el_forward(b2r_el_shot* s, int nsteps, int mode, float* ms)
     return b2r_el_forward_batch(&s, 1, nsteps, mode, ms);
 }
 
+// Device time of `steps` batched steps (mode 1) without the fixed cost of a
+// run: a fused run copies the state into the second set first, once per call,
+// which a tuning run of a few steps would otherwise charge to every step.
+static int el_time_batch(b2r_el_shot* const* shots, int nshots, int steps, float* ms)
+{
+    float m0 = 0.0f, m1 = 0.0f;
+    if (int rc = b2r_el_forward_batch(shots, nshots, 0, 1, &m0)) return rc;
+    if (int rc = b2r_el_forward_batch(shots, nshots, steps, 1, &m1)) return rc;
+    *ms = m1 > m0 ? m1 - m0 : 0.0f;
+    return B2R_OK;
+}
+
 // Autotuner of the elastic stencil kernels (no counterpart in the reference):
 // times the batched step of this shot (the fused step kernel where the faces
 // allow it and the separate stress and three-component velocity kernels,
@@ -1493,7 +1505,7 @@ extern "C" int b2r_el_autotune(b2r_el_shot* s, const char* record_path, b2r_tuni
     const b2r_tuning base = c->tune;
     const int zcs[] = {0, 16, 32, 64, 128, 256};
     const bool can_fuse = c->k.periodic || c->k.fmask == 0x3F;
-    const int warm = 2, iters = 4;
+    const int warm = 2, iters = 6;
     // best z chunk and time per kernel (index 1 fused, 2 separate)
     float kms[3] = {1e30f, 1e30f, 1e30f};
     int kzc[3] = {0, base.el_step_zc, base.el_zc}, tried = 0;
@@ -1504,7 +1516,7 @@ extern "C" int b2r_el_autotune(b2r_el_shot* s, const char* record_path, b2r_tuni
             (kern == 1 ? c->tune.el_step_zc : c->tune.el_zc) = zc;
             float ms = 0;
             int rc = b2r_el_forward(s, warm, 1, nullptr);
-            if (!rc) rc = b2r_el_forward(s, iters, 1, &ms);
+            if (!rc) rc = el_time_batch(&s, 1, iters, &ms);
             if (rc) {
                 c->tune = base;
                 (void)b2r_el_reset(s);
@@ -1559,7 +1571,7 @@ extern "C" int b2r_el_autotune_batch(b2r_el_shot* const* shots, int nshots, cons
         for (int k = 0; k < nshots; ++k) shots[k]->ctx->tune = base[k];
     };
     const int zcs[] = {0, 16, 32, 64, 128};
-    const int warm = 2, iters = 4;
+    const int warm = 2, iters = 6;
     float kms[3] = {1e30f, 1e30f, 1e30f};
     int kzc[3] = {0, 0, 0}, tried = 0;
     for (int kern = can_fuse ? 1 : 2; kern <= 2; ++kern)
@@ -1572,7 +1584,7 @@ extern "C" int b2r_el_autotune_batch(b2r_el_shot* const* shots, int nshots, cons
             }
             float ms = 0;
             int rc = b2r_el_forward_batch(shots, nshots, warm, 1, nullptr);
-            if (!rc) rc = b2r_el_forward_batch(shots, nshots, iters, 1, &ms);
+            if (!rc) rc = el_time_batch(shots, nshots, iters, &ms);
             if (rc) {
                 restore();
                 for (int k = 0; k < nshots; ++k) (void)b2r_el_reset(shots[k]);

@@ -192,6 +192,14 @@ static inline float lap_f32(const float* c, int D, float uc, const float* m, con
     return acc + c[0] * uc;
 }
 
+/* F32-mode L_a: the taps of lap_f32 with each product fused with its sum. */
+static inline float lap_fma(const float* c, int D, float uc, const float* m, const float* p)
+{
+    float acc = c[D] * (m[D] + p[D]);
+    for (int j = D - 1; j >= 1; --j) acc = fmaf(c[j], m[j] + p[j], acc);
+    return fmaf(c[0], uc, acc);
+}
+
 /* Reference form of L_a (heat-3D.cpp:58-60 for D=1, heat-3D.cu:316-318 for D>1). */
 static inline float lap_ref(const float* c, int D, float uc, const float* m, const float* p)
 {
@@ -200,10 +208,9 @@ static inline float lap_ref(const float* c, int D, float uc, const float* m, con
 }
 
 /* FP32 heat with exactly the GPU kernels' arithmetic.
- * mode 0 (F32): u' = u + t, t = sum over taps j = D..1 (x, y, z within a tap)
- *               of w[a][j]*(m_aj + p_aj) by fused multiply-adds, then + w0*u,
- *               with w[a][j] = k_a*c_j and w0 = ((kx + ky) + kz)*c_0 in float
- *               (the update u + kx*Lx + ky*Ly + kz*Lz with the weights folded)
+ * mode 0 (F32): L_a = c_D*(m_D+p_D) + ... + c_1*(m_1+p_1) + c_0*u (outermost
+ *               first) and t = kx*Lx + ky*Ly + kz*Lz, each product fused with
+ *               its sum (fmaf, one rounding), then u' = u + t
  * mode 1 (REF): u' = (float)((double)u + adt*(((double)Lx/h2x + (double)Ly/h2y)
  *                                              + (double)Lz/h2z)) */
 void orc_heat_f32(float* u, int gx, int gy, int gz, int D, int mode, const float* k,
@@ -211,11 +218,8 @@ void orc_heat_f32(float* u, int gx, int gy, int gz, int D, int mode, const float
 {
     grid_t g = mkgrid(gx, gy, gz, D, periodic);
     long long n = grid_elems(&g);
-    float c[5], w[3][5], w0;
+    float c[5];
     coef_f32(D, c);
-    for (int a = 0; a < 3; ++a)
-        for (int j = 0; j <= D; ++j) w[a][j] = k[a] * c[j];
-    w0 = ((k[0] + k[1]) + k[2]) * c[0];
     float* v = (float*)malloc(sizeof(float) * n);
     memcpy(v, u, sizeof(float) * n);
     float* src = u;
@@ -234,15 +238,9 @@ void orc_heat_f32(float* u, int gx, int gy, int gz, int D, int mode, const float
                     }
                     float r;
                     if (mode == 0) {
-                        float t = w[0][D] * (mx[D] + px[D]);
-                        t = fmaf(w[1][D], my[D] + py[D], t);
-                        t = fmaf(w[2][D], mz[D] + pz[D], t);
-                        for (int j = D - 1; j >= 1; --j) {
-                            t = fmaf(w[0][j], mx[j] + px[j], t);
-                            t = fmaf(w[1][j], my[j] + py[j], t);
-                            t = fmaf(w[2][j], mz[j] + pz[j], t);
-                        }
-                        t = fmaf(w0, uc, t);
+                        float t = k[0] * lap_fma(c, D, uc, mx, px);
+                        t = fmaf(k[1], lap_fma(c, D, uc, my, py), t);
+                        t = fmaf(k[2], lap_fma(c, D, uc, mz, pz), t);
                         r = uc + t;
                     } else {
                         float lx = lap_ref(c, D, uc, mx, px);

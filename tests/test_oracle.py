@@ -183,8 +183,6 @@ def test_f32_mode_matches_exact_one_step():
     k = np.float32([0.021, 0.033, 0.017])
     got = O.heat_f32(g, D, 0, k, 0.0, (1, 1, 1), 1, False)
     c = O.coef_f32(D)
-    w = [[np.float32(k[a] * c[j]) for j in range(D + 1)] for a in range(3)]
-    w0 = np.float32(np.float32(k[0] + k[1]) + k[2]) * c[0]
     u = g
     want = u.copy()
     nz, ny, nx = u.shape
@@ -197,13 +195,15 @@ def test_f32_mode_matches_exact_one_step():
                     if a == 1:
                         return np.float32(u[z, y - j, x] + u[z, y + j, x])
                     return np.float32(u[z - j, y, x] + u[z + j, y, x])
-                t = np.float32(w[0][D] * pair(0, D))
-                t = _fmaf(w[1][D], pair(1, D), t)
-                t = _fmaf(w[2][D], pair(2, D), t)
-                for j in range(D - 1, 0, -1):
-                    for a in range(3):
-                        t = _fmaf(w[a][j], pair(a, j), t)
-                t = _fmaf(w0, u[z, y, x], t)
+
+                def lap(a):
+                    acc = np.float32(c[D] * pair(a, D))
+                    for j in range(D - 1, 0, -1):
+                        acc = _fmaf(c[j], pair(a, j), acc)
+                    return _fmaf(c[0], u[z, y, x], acc)
+                t = np.float32(k[0] * lap(0))
+                t = _fmaf(k[1], lap(1), t)
+                t = _fmaf(k[2], lap(2), t)
                 want[z, y, x] = np.float32(u[z, y, x] + t)
     s = slice(D, -D)
     assert np.array_equal(got[s, s, s].view(np.uint32), want[s, s, s].view(np.uint32))
