@@ -76,19 +76,17 @@ struct b2r_ctx {
 
 int b2r_activate(b2r_ctx* ctx);  // hipSetDevice(ctx->device)
 int b2r_check_field(const b2r_ctx* ctx, const void* f);
-// Ordered X, Y, Z halo exchange inside each field; first_dim 1 leaves out the
-// X phase for a consumer that reads the x halo from the neighbouring blocks
-// itself (the x-halo bytes the Y and Z phases then copy are not read).
-int b2r_direct_exchange(b2r_ctx* ctx, void* const* fields, int nfields, int first_dim = 0);
+// Ordered X, Y, Z halo exchange inside each field.
+int b2r_direct_exchange(b2r_ctx* ctx, void* const* fields, int nfields);
 int b2r_time_begin(b2r_ctx* ctx, float* ms);
 int b2r_time_end(b2r_ctx* ctx, float* ms);
 int b2r_heat_check(b2r_ctx* ctx, const b2r_heat_params* p);
 int b2r_heat_launch(b2r_ctx* ctx, const void* in, void* out, const b2r_heat_params* p, int i);
 int b2r_gol_launch(b2r_ctx* ctx, const void* in, void* out, int i);
 int b2r_gol_fused_launch(b2r_ctx* ctx, const void* in, void* out);
-// 1: a fused Game of Life launch with the context's tuning reads the x halo
-// from the neighbouring blocks itself (the exchange before it needs no X phase)
-int b2r_gol_fused_reads_x_halo(const b2r_ctx* ctx);
+// 1: a fused Game of Life launch with the context's tuning reads its halo
+// from the neighbouring blocks itself (no exchange is needed before it)
+int b2r_gol_fused_reads_halo(const b2r_ctx* ctx);
 int b2r_heat_fused_launch(b2r_ctx* ctx, const void* in, void* out, const b2r_heat_params* p);
 
 // ---------------------------------------------------------------------------

@@ -749,7 +749,7 @@ extern "C" int b2r_boundary_updating(b2r_ctx* ctx, int time_step, void* d_field)
     return B2R_OK;
 }
 
-int b2r_direct_exchange(b2r_ctx* ctx, void* const* fields, int nfields, int first_dim)
+int b2r_direct_exchange(b2r_ctx* ctx, void* const* fields, int nfields)
 {
     const KGeo& g = ctx->k;
     const long long es = ctx->g.elem_size;
@@ -758,7 +758,7 @@ int b2r_direct_exchange(b2r_ctx* ctx, void* const* fields, int nfields, int firs
         int n = nfields - base < 16 ? nfields - base : 16;
         FieldList fl;
         for (int i = 0; i < n; ++i) fl.p[i] = fields[base + i];
-        for (int d = first_dim; d < 3; ++d) {
+        for (int d = 0; d < 3; ++d) {
             int V;
             long long rows;
             if (d == 0) {

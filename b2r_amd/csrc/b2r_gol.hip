@@ -339,14 +339,14 @@ __global__ __launch_bounds__((TXQ + 2) * NR) void gol_fused_kernel(const uint8_t
 // Every adder is built from full adders of two 3-input logic operations
 // (xor3, majority); the compiled generation takes about 70 instructions per 32
 // cells against about 150 for the byte-lane form.
-// With pull set (B_x a multiple of 32), the words just outside a block's
-// owned x range -- the one ending at x = h and the one starting at x = h+B --
-// are loaded from the x-neighbour's row at the same (y, z) (its cells x+B or
-// x-B, owned cells there or, in its y/z halo, ones the Y and Z exchange phases
-// filled), so the X phase of the exchange is not needed: that phase moves
-// 1-4 bytes per row, one cache line each, and cost a quarter of the R=4
-// window.  At a frozen face (no neighbour) the word comes from the block's
-// own frozen halo.  Words lie on the 16-byte grid of x = h,
+// With pull set (B_x a multiple of 32), a word outside the block's owned
+// cells is loaded from the neighbouring block that owns it, so no exchange is
+// needed before the launch (its X phase alone moved 1-4 bytes per row, one
+// cache line each, a quarter of the R=4 window).  The source is resolved as
+// the ordered X, Y, Z exchange would have filled that halo cell: along z
+// first, then y, then x, moving to the neighbour across each face the word
+// lies beyond and stopping at a frozen face (no neighbour), where the cell
+// keeps its own value.  Words lie on the 16-byte grid of x = h,
 // so each half of a word (16 cells) lies inside or outside the padded row as a
 // whole: halo words load only their half next to the tile, a row's last word
 // loads and stores its second half only when that half is in the row.  Cells
@@ -416,7 +416,9 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
     // workgroup order: x tile, then the block's x coordinate, then y tile, z
     // chunk and the block's y, z coordinates, so the tiles of x-neighbouring
     // blocks over the same rows run together on one XCD and a pulled halo word
-    // is found in the L2 their owner filled
+    // is found in the L2 their owner filled (counting y tiles and z chunks
+    // across blocks as well, so that y and z neighbours across a block face
+    // also run together, was measured slower: 4589 against 4727 GLUP/s at R=4)
     int tt = xcd_tile(blockIdx.x, gridDim.x);
     const int txi = tt % ntx;
     tt /= ntx;
@@ -445,13 +447,50 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
     // tile's first word of the last row, in bounds and aligned, and use zero
     const uint8_t* __restrict__ col =
         src + (ld_ok ? (long long)y * pitch + x : (long long)min(y, g.S[1] - 1) * pitch + x0);
-    if (pull && ld_ok && (x == h - 32 || x == h + g.B[0])) {
-        const int side = x == h - 32 ? 0 : 1;
-        const int nb = neighbour_block(g, b, 0, side);
-        if (nb >= 0)
-            col = in + (long long)nb * g.bstride + g.xoff + (long long)y * pitch +
-                  (side ? x - g.B[0] : x + g.B[0]);
+    // pull: col is the word's source for planes of the owned z range (y, then
+    // x resolved); planes of the z halo add a uniform offset into the z
+    // neighbour (the y, x resolution is the same there), or read the own block
+    // (colown) at a frozen z face
+    const uint8_t* __restrict__ colown = col;
+    long long zoff[2] = {0, 0};
+    bool znb[2] = {false, false};
+    if (pull) {
+        if (ld_ok) {
+            int bb = b, yy = y, xx = x;
+            const int oy = y < h ? -1 : (y >= h + g.B[1] ? 1 : 0);
+            bool go = true;
+            if (oy != 0) {
+                const int nb = neighbour_block(g, bb, 1, oy > 0);
+                if (nb >= 0) {
+                    bb = nb;
+                    yy = y - oy * g.B[1];
+                } else {
+                    go = false;
+                }
+            }
+            const int ox = x + 32 <= h ? -1 : (x >= h + g.B[0] ? 1 : 0);
+            if (go && ox != 0) {
+                const int nb = neighbour_block(g, bb, 0, ox > 0);
+                if (nb >= 0) {
+                    bb = nb;
+                    xx = x - ox * g.B[0];
+                }
+            }
+            col = in + (long long)bb * g.bstride + g.xoff + (long long)yy * pitch + xx;
+        }
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int nb = neighbour_block(g, b, 2, side);
+            znb[side] = nb >= 0;
+            zoff[side] = (long long)(nb - b) * g.bstride + (side ? -1ll : 1ll) * g.B[2] * slice;
+        }
     }
+    // the source column of plane p (p uniform over the workgroup)
+    auto colp = [&](int p) -> const uint8_t* {
+        if (pull && p < h) return znb[0] ? col + zoff[0] : colown;
+        if (pull && p >= h + g.B[2]) return znb[1] ? col + zoff[1] : colown;
+        return col;
+    };
     const uint32_t ldm = ld_ok ? ~0u : 0u;
     // a halo word needs only its 16 cells next to the tile (R <= 4), and the
     // second half of a row's last word may lie past the padded row end: such a
@@ -494,7 +533,8 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
         gen[0][2] = gb_pack(make_uint4(la.x & ma, la.y & ma, la.z & ma, la.w & ma),
                             make_uint4(lb.x & mb, lb.y & mb, lb.z & mb, lb.w & mb)) & ldm;
         {  // the load GF_PD planes ahead (the last ones are unused)
-            const uint8_t* pp = col + (long long)zp(min(k + GF_PD, nin - 1)) * slice;
+            const int zz = zp(min(k + GF_PD, nin - 1));
+            const uint8_t* pp = colp(zz) + (long long)zz * slice;
             la = *reinterpret_cast<const uint4*>(pp + oa);
             lb = *reinterpret_cast<const uint4*>(pp + ob);
         }
@@ -557,7 +597,8 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
     uint4 sl[GF_PD][2];
 #pragma unroll
     for (int k = 0; k < GF_PD; ++k) {
-        const uint8_t* pp = col + (long long)zp(min(k, nin - 1)) * slice;
+        const int zz = zp(min(k, nin - 1));
+        const uint8_t* pp = colp(zz) + (long long)zz * slice;
         sl[k][0] = *reinterpret_cast<const uint4*>(pp + oa);
         sl[k][1] = *reinterpret_cast<const uint4*>(pp + ob);
     }
@@ -568,11 +609,11 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
     }
 }
 
-// The bit-sliced kernel reads the x halo from the x-neighbours (pull above)
-// when the block's owned x range is a whole number of 32-cell words.
+// The bit-sliced kernel reads its halo from the neighbouring blocks (pull
+// above) when the block's owned x range is a whole number of 32-cell words.
 static int gol_bits_pull(const KGeo& g) { return g.B[0] % 32 == 0; }
 
-int b2r_gol_fused_reads_x_halo(const b2r_ctx* ctx)
+int b2r_gol_fused_reads_halo(const b2r_ctx* ctx)
 {
     const int t = ctx->tune.gol_tile;
     return (t == 0 || t >= 4) && gol_bits_pull(ctx->k);
@@ -721,9 +762,9 @@ extern "C" int b2r_gol_run(b2r_ctx* ctx, void* d_a, void* d_b, int nsteps, int f
     for (int s = 0; s < nsteps; s += R) {
         const int n = nsteps - s < R ? nsteps - s : R;
         const bool fuse = use_fused && n == R;
-        // a bit-sliced launch reads the x halo from the neighbours itself
-        const int first = fuse && b2r_gol_fused_reads_x_halo(ctx) ? 1 : 0;
-        if (int rc = b2r_direct_exchange(ctx, &cur, 1, first)) return rc;
+        // a bit-sliced launch reads its halo from the neighbours itself
+        if (!(fuse && b2r_gol_fused_reads_halo(ctx)))
+            if (int rc = b2r_direct_exchange(ctx, &cur, 1)) return rc;
         if (fuse) {
             if (int rc = b2r_gol_fused_launch(ctx, cur, nxt)) return rc;
             void* t = cur;

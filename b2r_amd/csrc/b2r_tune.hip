@@ -6,8 +6,8 @@
 // HIP events (b2r_time_steps), keeps the fastest in the context and appends one
 // line per search to a plain-text record.  The fused Game of Life candidates
 // are timed with the exchange that precedes each launch in b2r_gol_run, since
-// the bit-sliced kernel reads the x halo itself and needs no X phase while the
-// byte-lane kernel does (kernel time alone would not show that difference):
+// the bit-sliced kernel reads its halo from the neighbouring blocks and needs
+// none while the byte-lane kernel does (kernel time alone would not show it):
 //   kind=<k> S=<Sx>x<Sy>x<Sz> blocks=<n> R=<R> radius=<r> best=<field>:<v>,... ms=<t> tried=<n>
 #include "b2r_common.h"
 
@@ -23,11 +23,12 @@ static int gol_windows(b2r_ctx* ctx, void* d_a, void* d_b, int iters, float* ms)
 {
     if (int rc = b2r_activate(ctx)) return rc;
     if (int rc = b2r_time_begin(ctx, ms)) return rc;
-    const int first = b2r_gol_fused_reads_x_halo(ctx) ? 1 : 0;
+    const bool pulls = b2r_gol_fused_reads_halo(ctx);
     for (int it = 0; it < iters; ++it) {
         void* in = (it & 1) ? d_b : d_a;
         void* out = (it & 1) ? d_a : d_b;
-        if (int rc = b2r_direct_exchange(ctx, &in, 1, first)) return rc;
+        if (!pulls)
+            if (int rc = b2r_direct_exchange(ctx, &in, 1)) return rc;
         if (int rc = b2r_gol_fused_launch(ctx, in, out)) return rc;
     }
     return b2r_time_end(ctx, ms);
