@@ -309,6 +309,33 @@ def test_gol_fused_chunking(zc, tile):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("tile", [6, 3])
+@pytest.mark.parametrize("R", [1, 2, 4])
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("env,blocks", [((64, 48, 40), (2, 3, 2)), ((64, 30, 28), (1, 2, 2)),
+                                        ((96, 26, 24), (3, 1, 1)), ((128, 20, 36), (2, 2, 3))])
+def test_gol_bits_halo_from_neighbours(env, blocks, periodic, R, tile):
+    # block x ranges of whole 32-cell words: the bit-sliced kernel (tile 6)
+    # reads every halo word from the owning neighbour, resolved z, y, x as the
+    # ordered exchange fills it, and no exchange runs between its launches;
+    # the byte-lane kernel (tile 3) runs with the exchange.  Single-block axes
+    # wrap onto themselves when periodic; frozen faces keep their own halo.
+    gens = 3 * R + 1
+    say("gol pull", env, blocks, "R", R, "periodic", periodic, "tile", tile)
+    with A.Context(env, blocks, R, 1, 1 if periodic else 0, 1) as c:
+        c.set_tuning(gol_tile=tile, gol_zc=0)
+        a, b = c.field(), c.field()
+        c.init_bernoulli(a, 8, int(0.3 * 16777216))
+        res, _ = c.gol_run(a, b, gens, fused=1)
+        got = c.download(res)
+    u = O.bernoulli((env[2], env[1], env[0]), 8, 0.3)
+    if periodic:
+        want = O.gol(u, gens, True)
+    else:
+        want = O.unframe(O.gol(O.frame(u, 1), gens, False), 1)
+    assert np.array_equal(got, want)
+
+
 @pytest.mark.parametrize("wy,wx", [(4, 1), (4, 2), (4, 4), (8, 1), (8, 2), (16, 1)])
 @pytest.mark.parametrize("ry", [1, 2, 4, 8])
 @pytest.mark.parametrize("zc", [0, 5])
