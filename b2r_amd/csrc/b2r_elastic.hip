@@ -1848,6 +1848,16 @@ static int el_gradient_plain(b2r_el_shot* s, const float* h_obs, int nsteps, int
 // time-reversible) together with its adjoint, the state reset from the
 // checkpoint at every segment boundary; no per-step storage.  graph != 0
 // replays whole segments from HIP graphs (step counters on the device).
+// Device memory el_gradient_rev takes for one shot: checkpoints, adjoint
+// state and gradients, observations and residuals, FP64 partial sums.
+static size_t el_gradient_bytes(const b2r_el_shot* s, int nsteps, int ckpt)
+{
+    const int ncp = (nsteps + ckpt - 1) / ckpt;
+    const size_t fe = (size_t)s->ctx->L.field_elems;
+    const long long nobs = (long long)s->nrec * s->rec_nt;
+    return ((size_t)9 * (ncp - 1) + 12) * fe * 4 + (size_t)nobs * 8 + 8 + 257 * sizeof(double);
+}
+
 static int el_gradient_rev(b2r_el_shot* const* shots, int nshots, const float* const* h_obs,
                            int nsteps, int ckpt, bool graph, int flags, double* misfit,
                            float* const* h_glam, float* const* h_gmu, float* const* h_gb,
@@ -1887,8 +1897,7 @@ static int el_gradient_rev(b2r_el_shot* const* shots, int nshots, const float* c
         b2r_el_shot* s = shots[k];
         const size_t fe = (size_t)s->ctx->L.field_elems;
         const long long nobs = (long long)s->nrec * s->rec_nt;
-        const size_t nf = (size_t)9 * (ncp - 1) + 12;
-        const size_t bytes = nf * fe * 4 + (size_t)nobs * 8 + 8 + 257 * sizeof(double);
+        const size_t bytes = el_gradient_bytes(s, nsteps, ckpt);
         if (hipMalloc(&mem[k], bytes) != hipSuccess) {
             mem[k] = nullptr;
             b2r_set_error("gradient: device allocation of %zu MB for shot %d failed", bytes >> 20, k);
@@ -2044,9 +2053,40 @@ extern "C" int b2r_el_gradient_batch(b2r_el_shot* const* shots, int nshots,
     B2R_CHECK(shots && h_obs && misfit && h_glam && h_gmu && h_gb && nshots >= 1 &&
                   mode >= 0 && mode <= 2 && nsteps >= 1 && ckpt >= 1,
               B2R_E_ARG, "bad argument");
-    if (mode > 0)
-        return el_gradient_rev(shots, nshots, h_obs, nsteps, ckpt, mode == 2, 0, misfit, h_glam,
-                               h_gmu, h_gb, nullptr, ms);
+    if (mode > 0) {
+        // consecutive groups of shots whose gradient memory fits the budget
+        for (int k = 0; k < nshots; ++k) B2R_CHECK(shots[k], B2R_E_ARG, "null shot %d", k);
+        if (int rc = b2r_activate(shots[0]->ctx)) return rc;
+        size_t budget = (size_t)shots[0]->ctx->tune.el_grad_mb << 20;
+        if (budget == 0) {
+            size_t free_b = 0, total_b = 0;
+            B2R_HIP(hipMemGetInfo(&free_b, &total_b));
+            budget = free_b / 20 * 19;
+        }
+        float total_ms = 0.0f;
+        for (int k0 = 0; k0 < nshots;) {
+            int k1 = k0;
+            size_t used = 0;
+            while (k1 < nshots) {
+                const size_t need = el_gradient_bytes(shots[k1], nsteps, ckpt);
+                if (k1 > k0 && used + need > budget) break;
+                B2R_CHECK(need <= budget, B2R_E_NOMEM,
+                          "gradient: shot %d needs %zu MB, more than the %zu MB allowed", k1,
+                          need >> 20, budget >> 20);
+                used += need;
+                ++k1;
+            }
+            float gms = 0.0f;
+            if (int rc = el_gradient_rev(shots + k0, k1 - k0, h_obs + k0, nsteps, ckpt, mode == 2, 0,
+                                         misfit + k0, h_glam + k0, h_gmu + k0, h_gb + k0, nullptr,
+                                         ms ? &gms : nullptr))
+                return rc;
+            total_ms += gms;
+            k0 = k1;
+        }
+        if (ms) *ms = total_ms;
+        return B2R_OK;
+    }
     // plain: one shot after the other, timed as a whole
     if (int rc = b2r_activate(shots[0]->ctx)) return rc;
     if (int rc = b2r_time_begin(shots[0]->ctx, ms)) return rc;

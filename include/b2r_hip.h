@@ -110,6 +110,9 @@ typedef struct b2r_tuning {
                          global face is frozen or all are periodic, else 2       */
     int el_step_zc;   /* z planes marched per fused elastic step workgroup (el_zc
                          serves the separate kernels)                           */
+    int el_grad_mb;   /* device memory the batched gradient may take for its
+                         checkpoints and adjoint state, MB; the shots run in
+                         groups that fit; 0 = the free memory less 5 %          */
 } b2r_tuning;
 
 /* ---- library / context ---------------------------------------------------- */
@@ -289,7 +292,11 @@ int b2r_el_gradient_ex(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt,
  * h_obs[k] and outputs misfit[k], h_glam[k], h_gmu[k], h_gb[k].  mode 0: the
  * plain per-shot loop (flags bit 1 above, one shot after the other); mode 1:
  * all shots per launch, reverse sweep as b2r_el_gradient; mode 2: mode 1 with
- * every checkpoint segment replayed from a HIP graph.  *ms: device time. */
+ * every checkpoint segment replayed from a HIP graph.  Modes 1 and 2 run the
+ * shots in consecutive groups whose checkpoints and adjoint state fit the
+ * memory b2r_tuning.el_grad_mb allows (16 shots of 256^3 over 1500 steps with
+ * checkpoints every 50 need about 320 GB); a single shot that does not fit is
+ * an error.  *ms: device time, summed over the groups. */
 int b2r_el_gradient_batch(b2r_el_shot* const* shots, int nshots, const float* const* h_obs,
                           int nsteps, int ckpt, int mode, double* misfit, float* const* h_glam,
                           float* const* h_gmu, float* const* h_gb, float* ms);

@@ -248,6 +248,39 @@ def test_gradient_batch_uneven_shots(mode):
             c.close()
 
 
+@pytest.mark.parametrize("mb", [2, 4, 6, 0])
+def test_gradient_batch_memory_groups(mb):
+    # a memory budget smaller than the batch runs the shots in groups: the
+    # results equal the one-group batch bit for bit; a budget below one shot
+    # is an error
+    sizes = [(20, 18, 16), (24, 16, 20), (16, 22, 18), (18, 18, 18)]
+    probs = [E.small_problem(env=e, nsteps=26, seed=13 + i) for i, e in enumerate(sizes)]
+    obs = [_obs(P) for P in probs]
+    ctxs = [A.Context(P["env"], (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4) for P in probs]
+    shots = [_shot(c, P) for c, P in zip(ctxs, probs)]
+    try:
+        mis, grads, _ = A.el_gradient_batch(shots, obs, 26, 7, 2)
+        # 2.6-3.2 MB per shot here: 2 MB fits none, 4 MB one shot per group,
+        # 6 MB two, 0 the free memory (one group)
+        ctxs[0].set_tuning(el_grad_mb=mb)
+        say("gradient memory budget", mb, "MB")
+        if mb == 2:
+            with pytest.raises(A.B2RError):
+                A.el_gradient_batch(shots, obs, 26, 7, 2)
+            return
+        mis2, grads2, ms = A.el_gradient_batch(shots, obs, 26, 7, 2, timed=True)
+        assert ms > 0
+        for k in range(len(shots)):
+            assert mis2[k] == mis[k]
+            for i in range(3):
+                assert bits_equal(grads2[k][i], grads[k][i])
+    finally:
+        for s in shots:
+            s.close()
+        for c in ctxs:
+            c.close()
+
+
 def test_gradient_finite_differences():
     P = E.small_problem(env=(18, 16, 14), nsteps=24)
     obs = _obs(P)
