@@ -497,7 +497,7 @@ def profile_kernels(A):
         src, dst = c.dev_alloc(nbytes), c.dev_alloc(nbytes)
         log("stream_copy 1 GiB x3")
         c.stream_copy(dst, src, nbytes, 3)
-    for R, tile, zc in ((2, 6, 0), (4, 6, 8)):
+    for R, tile, zc in ((2, 6, 0), (4, 4, 32)):
         with A.Context((512, 512, 512), (2, 2, 2), R, 1, A.BOUNDARY_PERIODIC, 1) as c:
             a, b = c.field(), c.field()
             c.init_bernoulli(a, 1, int(0.3 * 16777216))
