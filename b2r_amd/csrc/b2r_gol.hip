@@ -165,11 +165,11 @@ __global__ __launch_bounds__(256) void gol_step_kernel(const uint8_t* __restrict
 // ---------------------------------------------------------------------------
 // Tile shapes: gol_tile 1, 2, 3 select this kernel with TXQ output quads (16
 // cells) per tile row and NR rows including the R halo rows per side
-// (GF_SHAPES); gol_tile 4, 5, 6, 7 select the bit-sliced kernel below with NW
+// (GF_SHAPES); gol_tile 4-8 select the bit-sliced kernel below with NW
 // output words (32 cells) per row and NR rows (GB_SHAPES); 0 (automatic) is
 // the bit-sliced 8 x 64 shape.
 constexpr int GF_SHAPES[3][2] = {{8, 32}, {8, 48}, {16, 56}};
-constexpr int GB_SHAPES[4][2] = {{8, 32}, {8, 64}, {8, 96}, {8, 40}};
+constexpr int GB_SHAPES[5][2] = {{8, 32}, {8, 64}, {8, 96}, {8, 40}, {8, 72}};
 constexpr int GF_PD = 2;  // input planes in flight per thread
 
 __device__ __forceinline__ uint4 q_add(uint4 a, uint4 b)
@@ -327,7 +327,7 @@ __global__ __launch_bounds__((TXQ + 2) * NR) void gol_fused_kernel(const uint8_t
 }
 
 // ---------------------------------------------------------------------------
-// Bit-sliced form of the fused kernel (gol_tile 0, 4..7): R generations per launch
+// Bit-sliced form of the fused kernel (gol_tile 0, 4..8): R generations per launch
 // as above, but a thread owns a 32-cell word of a row, bit i = cell x+i,
 // packed from the byte cells when a plane is loaded and unpacked when
 // generation R is stored.  Per generation and plane:
@@ -682,6 +682,7 @@ static int gol_fused_launch_r(b2r_ctx* ctx, const void* in, void* out)
     case 4: return gol_bits_launch_t<R, GB_SHAPES[0][0], GB_SHAPES[0][1]>(ctx, in, out);
     case 6: return gol_bits_launch_t<R, GB_SHAPES[2][0], GB_SHAPES[2][1]>(ctx, in, out);
     case 7: return gol_bits_launch_t<R, GB_SHAPES[3][0], GB_SHAPES[3][1]>(ctx, in, out);
+    case 8: return gol_bits_launch_t<R, GB_SHAPES[4][0], GB_SHAPES[4][1]>(ctx, in, out);
     default: return gol_bits_launch_t<R, GB_SHAPES[1][0], GB_SHAPES[1][1]>(ctx, in, out);
     }
 }

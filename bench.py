@@ -96,7 +96,7 @@ def gol_bench(A, R, steps, warmup, fused=1, zc=0, record=None):
 # (output quads of 16 cells per tile row, rows incl. halo) of the gol_tile values
 GOL_TILES = {1: (8, 32), 2: (8, 48), 3: (16, 56)}
 # (output words of 32 cells per row, rows incl. halo) of the bit-sliced kernel
-GOL_BIT_TILES = {0: (8, 64), 4: (8, 32), 5: (8, 64), 6: (8, 96), 7: (8, 40)}
+GOL_BIT_TILES = {0: (8, 64), 4: (8, 32), 5: (8, 64), 6: (8, 96), 7: (8, 40), 8: (8, 72)}
 
 
 def gol_kernel_name(R, tile):
@@ -497,7 +497,7 @@ def profile_kernels(A):
         src, dst = c.dev_alloc(nbytes), c.dev_alloc(nbytes)
         log("stream_copy 1 GiB x3")
         c.stream_copy(dst, src, nbytes, 3)
-    for R, tile, zc in ((2, 6, 0), (4, 7, 32)):
+    for R, tile, zc in ((2, 8, 16), (4, 8, 16)):
         with A.Context((512, 512, 512), (2, 2, 2), R, 1, A.BOUNDARY_PERIODIC, 1) as c:
             a, b = c.field(), c.field()
             c.init_bernoulli(a, 1, int(0.3 * 16777216))
