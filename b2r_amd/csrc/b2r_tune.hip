@@ -41,11 +41,14 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
     const b2r_tuning base = ctx->tune;
     std::vector<b2r_tuning> cand;
     const int zcs[] = {0, 8, 16, 32, 64, 128, 256};
+    // heat also tries chunk lengths between the powers of two, whose workgroup
+    // counts can fill the CUs' slots in whole rounds where no power of two does
+    const int zcs_heat[] = {0, 8, 16, 24, 32, 48, 64, 96, 128, 192, 256};
     if (kind == 0 || kind == 3) {
         const int rys[] = {1, 2, 4, 8};
         const int wys[] = {4, 8, 16};
         if (kind == 3) {  // the fused kernel's tile is fixed; only its z chunk varies
-            for (int zc : zcs) {
+            for (int zc : zcs_heat) {
                 b2r_tuning t = base;
                 t.heat_zc = zc;
                 cand.push_back(t);
@@ -54,7 +57,7 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
             for (int wy : wys)
                 for (int wx = 1; wx * wy <= 16; wx *= 2)
                     for (int ry : rys)
-                        for (int zc : zcs) {
+                        for (int zc : zcs_heat) {
                             b2r_tuning t = base;
                             t.heat_ry = ry;
                             t.heat_wy = wy;
