@@ -1503,7 +1503,7 @@ extern "C" int b2r_el_autotune(b2r_el_shot* s, const char* record_path, b2r_tuni
     B2R_CHECK(s, B2R_E_ARG, "null shot");
     b2r_ctx* c = s->ctx;
     const b2r_tuning base = c->tune;
-    const int zcs[] = {0, 16, 32, 64, 128, 256};
+    const int zcs[] = {0, 16, 24, 32, 48, 64, 96, 128, 256};
     const bool can_fuse = c->k.periodic || c->k.fmask == 0x3F;
     const int warm = 2, iters = 6;
     // best z chunk and time per kernel (index 1 fused, 2 separate)
@@ -1570,7 +1570,7 @@ extern "C" int b2r_el_autotune_batch(b2r_el_shot* const* shots, int nshots, cons
     auto restore = [&]() {
         for (int k = 0; k < nshots; ++k) shots[k]->ctx->tune = base[k];
     };
-    const int zcs[] = {0, 16, 32, 64, 128};
+    const int zcs[] = {0, 16, 24, 32, 48, 64, 96, 128};
     const int warm = 2, iters = 6;
     float kms[3] = {1e30f, 1e30f, 1e30f};
     int kzc[3] = {0, 0, 0}, tried = 0;
