@@ -59,7 +59,7 @@ class Tuning(C.Structure):
     _fields_ = [("heat_ry", C.c_int), ("heat_zc", C.c_int), ("gol_zc", C.c_int),
                 ("gol_step_zc", C.c_int), ("el_zc", C.c_int), ("heat_wy", C.c_int),
                 ("gol_tile", C.c_int), ("heat_wx", C.c_int), ("el_kernel", C.c_int),
-                ("el_step_zc", C.c_int), ("el_grad_mb", C.c_int)]
+                ("el_step_zc", C.c_int), ("el_grad_mb", C.c_int), ("gol_bytes", C.c_int)]
 
 
 class HeatParams(C.Structure):

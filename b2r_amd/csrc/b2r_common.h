@@ -72,6 +72,8 @@ struct b2r_ctx {
     void* msgbuf;             // boundary_updating message buffers
     size_t msgbuf_bytes;
     b2r_tuning tune;          // launch shapes (0 = automatic)
+    void* gol_pk;             // two bit-packed Game of Life fields (b2r_dev_alloc)
+    size_t gol_pk_bytes;
 };
 
 int b2r_activate(b2r_ctx* ctx);  // hipSetDevice(ctx->device)
@@ -83,10 +85,18 @@ int b2r_time_end(b2r_ctx* ctx, float* ms);
 int b2r_heat_check(b2r_ctx* ctx, const b2r_heat_params* p);
 int b2r_heat_launch(b2r_ctx* ctx, const void* in, void* out, const b2r_heat_params* p, int i);
 int b2r_gol_launch(b2r_ctx* ctx, const void* in, void* out, int i);
-int b2r_gol_fused_launch(b2r_ctx* ctx, const void* in, void* out);
+// packed: in and out are bit-packed fields (b2r_gol_pack_pair)
+int b2r_gol_fused_launch(b2r_ctx* ctx, const void* in, void* out, int packed);
 // 1: a fused Game of Life launch with the context's tuning reads its halo
 // from the neighbouring blocks itself (no exchange is needed before it)
 int b2r_gol_fused_reads_halo(const b2r_ctx* ctx);
+// 1: fused Game of Life runs keep their fields bit-packed between launches
+int b2r_gol_fused_packed(const b2r_ctx* ctx);
+// The context's two packed fields (allocated on first use), both packed from
+// the byte field `from`, halo included; b2r_gol_unpack writes the owned cells
+// of a packed field into a byte field.
+int b2r_gol_pack_pair(b2r_ctx* ctx, const void* from, void** p0, void** p1);
+int b2r_gol_unpack(b2r_ctx* ctx, const void* packed, void* to);
 int b2r_heat_fused_launch(b2r_ctx* ctx, const void* in, void* out, const b2r_heat_params* p);
 
 // ---------------------------------------------------------------------------

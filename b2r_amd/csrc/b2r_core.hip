@@ -100,6 +100,8 @@ extern "C" int b2r_create(const b2r_geom* geom, int device, b2r_ctx** out)
     c->device = device;
     c->msgbuf = nullptr;
     c->msgbuf_bytes = 0;
+    c->gol_pk = nullptr;
+    c->gol_pk_bytes = 0;
     memset(&c->tune, 0, sizeof(c->tune));
     b2r_layout& L = c->L;
     memset(&L, 0, sizeof(L));
@@ -193,6 +195,7 @@ extern "C" int b2r_set_tuning(b2r_ctx* ctx, const b2r_tuning* t)
     B2R_CHECK(t->gol_tile >= 0 && t->gol_tile <= 8, B2R_E_ARG, "gol_tile must be 0..8");
     B2R_CHECK(t->el_kernel >= 0 && t->el_kernel <= 2, B2R_E_ARG, "el_kernel must be 0, 1 or 2");
     B2R_CHECK(t->el_grad_mb >= 0, B2R_E_ARG, "el_grad_mb must be >= 0");
+    B2R_CHECK(t->gol_bytes == 0 || t->gol_bytes == 1, B2R_E_ARG, "gol_bytes must be 0 or 1");
     B2R_CHECK((t->heat_wx == 0 || t->heat_wx == 1 || t->heat_wx == 2 || t->heat_wx == 4) &&
                   (t->heat_wx > 1 ? t->heat_wx : 1) * (t->heat_wy > 0 ? t->heat_wy : 4) <= 16,
               B2R_E_ARG, "heat_wx must be 0, 1, 2 or 4 with heat_wx * heat_wy <= 16");
@@ -879,7 +882,9 @@ extern "C" int b2r_memcpy_d2h(b2r_ctx* ctx, void* h_dst, const void* d_src, size
 // Back-to-back launches of one kernel between two events on the context
 // stream, alternating the two buffers.  Inner-step kernels cycle b2r_i over
 // 0..R-1 so the average covers a whole B2R period; fused kernels run R steps
-// per launch.
+// per launch.  Fused Game of Life launches that b2r_gol_run makes on packed
+// fields alternate the context's packed pair, packed from d_a before the
+// timed region.
 extern "C" int b2r_time_steps(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
                               const b2r_heat_params* p, int iters, float* ms)
 {
@@ -888,6 +893,9 @@ extern "C" int b2r_time_steps(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
     if (int rc = b2r_check_field(ctx, d_b)) return rc;
     B2R_CHECK((kind != 0 && kind != 3) || p, B2R_E_ARG, "heat timing needs parameters");
     if (int rc = b2r_activate(ctx)) return rc;
+    const int packed = kind == 2 && b2r_gol_fused_packed(ctx);
+    if (packed)
+        if (int rc = b2r_gol_pack_pair(ctx, d_a, &d_a, &d_b)) return rc;
     if (int rc = b2r_time_begin(ctx, ms)) return rc;
     for (int it = 0; it < iters; ++it) {
         const void* in = (it & 1) ? d_b : d_a;
@@ -897,7 +905,7 @@ extern "C" int b2r_time_steps(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
         switch (kind) {
         case 0: rc = b2r_heat_launch(ctx, in, out, p, i); break;
         case 1: rc = b2r_gol_launch(ctx, in, out, i); break;
-        case 2: rc = b2r_gol_fused_launch(ctx, in, out); break;
+        case 2: rc = b2r_gol_fused_launch(ctx, in, out, packed); break;
         default: rc = b2r_heat_fused_launch(ctx, in, out, p); break;
         }
         if (rc) return rc;

@@ -399,11 +399,21 @@ __device__ __forceinline__ uint32_t word_mask(int x, int lo, int hi)
     return upto & ~((1u << a) - 1u);
 }
 
-template <int R, int NW, int NR>
+// Packed fields (PK): a block's row is PW = B_x/32 + 2 words, word w holding
+// cells x = h-32+32w .. h+32w-1 (bit i = cell x+i), planes of S_y rows, blocks
+// of S_z planes; the kernel then loads and stores words instead of packing
+// and unpacking byte cells.  PGeo carries the byte strides of either form.
+struct PGeo {
+    long long rb, sb, bb;  // bytes per row, plane and block
+    int pw;                // words per packed row (PK only)
+};
+
+template <int R, int NW, int NR, bool PK = false>
 __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* __restrict__ in,
                                                                  uint8_t* __restrict__ out,
                                                                  KGeo g, int ntx, int nty,
-                                                                 int nzc, int zc, int pull)
+                                                                 int nzc, int zc, int pull,
+                                                                 PGeo pg)
 {
     constexpr int NQ = NW + 2;  // words per row: plus one per side
     constexpr int NT = NQ * NR;
@@ -433,20 +443,24 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
     const int y0 = h + tyi * TY;
     const int zc0 = h + zci * zc;
     const int zc1 = min(zc0 + zc, h + g.B[2]);
-    const long long boff = (long long)b * g.bstride + g.xoff;
+    // byte strides and the byte offset of the word at cell xx within a row
+    const long long pitch = PK ? pg.rb : g.pitch, slice = PK ? pg.sb : g.slice;
+    const long long bstr = PK ? pg.bb : g.bstride, xo = PK ? 0 : g.xoff;
+    auto xbyte = [&](int xx) -> long long { return PK ? 4ll * ((xx - h) / 32 + 1) : xx; };
+    const long long boff = (long long)b * bstr + xo;
     const uint8_t* __restrict__ src = in + boff;
     uint8_t* __restrict__ dst = out + boff;
-    const long long slice = g.slice;
-    const int pitch = g.pitch;
 
     // this thread's word: x = x .. x+31, row y
     const int x = x0 + 32 * (q - 1);
     const int y = y0 - R + r;
-    const bool ld_ok = y < g.S[1] && x + 31 >= 0 && x < g.S[0];
+    const bool ld_ok = PK ? y < g.S[1] && (x - h) / 32 + 1 < pg.pw
+                          : y < g.S[1] && x + 31 >= 0 && x < g.S[0];
     // every thread loads (no branch): threads outside the block read the
     // tile's first word of the last row, in bounds and aligned, and use zero
     const uint8_t* __restrict__ col =
-        src + (ld_ok ? (long long)y * pitch + x : (long long)min(y, g.S[1] - 1) * pitch + x0);
+        src + (ld_ok ? (long long)y * pitch + xbyte(x)
+                     : (long long)min(y, g.S[1] - 1) * pitch + xbyte(x0));
     // pull: col is the word's source for planes of the owned z range (y, then
     // x resolved); planes of the z halo add a uniform offset into the z
     // neighbour (the y, x resolution is the same there), or read the own block
@@ -476,13 +490,13 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
                     xx = x - ox * g.B[0];
                 }
             }
-            col = in + (long long)bb * g.bstride + g.xoff + (long long)yy * pitch + xx;
+            col = in + (long long)bb * bstr + xo + (long long)yy * pitch + xbyte(xx);
         }
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             const int nb = neighbour_block(g, b, 2, side);
             znb[side] = nb >= 0;
-            zoff[side] = (long long)(nb - b) * g.bstride + (side ? -1ll : 1ll) * g.B[2] * slice;
+            zoff[side] = (long long)(nb - b) * bstr + (side ? -1ll : 1ll) * g.B[2] * slice;
         }
     }
     // the source column of plane p (p uniform over the workgroup)
@@ -530,13 +544,20 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
     auto plane = [&](int k, uint4& la, uint4& lb) {
         gen[0][0] = gen[0][1];
         gen[0][1] = gen[0][2];
-        gen[0][2] = gb_pack(make_uint4(la.x & ma, la.y & ma, la.z & ma, la.w & ma),
-                            make_uint4(lb.x & mb, lb.y & mb, lb.z & mb, lb.w & mb)) & ldm;
+        if (PK)
+            gen[0][2] = la.x & ldm;
+        else
+            gen[0][2] = gb_pack(make_uint4(la.x & ma, la.y & ma, la.z & ma, la.w & ma),
+                                make_uint4(lb.x & mb, lb.y & mb, lb.z & mb, lb.w & mb)) & ldm;
         {  // the load GF_PD planes ahead (the last ones are unused)
             const int zz = zp(min(k + GF_PD, nin - 1));
             const uint8_t* pp = colp(zz) + (long long)zz * slice;
-            la = *reinterpret_cast<const uint4*>(pp + oa);
-            lb = *reinterpret_cast<const uint4*>(pp + ob);
+            if (PK) {
+                la.x = *reinterpret_cast<const uint32_t*>(pp);
+            } else {
+                la = *reinterpret_cast<const uint4*>(pp + oa);
+                lb = *reinterpret_cast<const uint4*>(pp + ob);
+            }
         }
 #pragma unroll
         for (int t = 1; t <= R; ++t) {
@@ -586,27 +607,89 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
                 gen[t][1] = gen[t][2];
                 gen[t][2] = nw;
             } else if (out_ok) {
-                uint4 lo, hi;
-                gb_unpack(nw, lo, hi);
-                uint4* o = reinterpret_cast<uint4*>(dst + (long long)p * slice + (long long)y * pitch + x);
-                o[0] = lo;
-                if (hi_in) o[1] = hi;
+                if (PK) {
+                    *reinterpret_cast<uint32_t*>(dst + (long long)p * slice + (long long)y * pitch +
+                                                 xbyte(x)) = nw;
+                } else {
+                    uint4 lo, hi;
+                    gb_unpack(nw, lo, hi);
+                    uint4* o = reinterpret_cast<uint4*>(dst + (long long)p * slice +
+                                                        (long long)y * pitch + x);
+                    o[0] = lo;
+                    if (hi_in) o[1] = hi;
+                }
             }
         }
     };
-    uint4 sl[GF_PD][2];
+    uint4 sl[GF_PD][2] = {};
 #pragma unroll
     for (int k = 0; k < GF_PD; ++k) {
         const int zz = zp(min(k, nin - 1));
         const uint8_t* pp = colp(zz) + (long long)zz * slice;
-        sl[k][0] = *reinterpret_cast<const uint4*>(pp + oa);
-        sl[k][1] = *reinterpret_cast<const uint4*>(pp + ob);
+        if (PK) {
+            sl[k][0].x = *reinterpret_cast<const uint32_t*>(pp);
+        } else {
+            sl[k][0] = *reinterpret_cast<const uint4*>(pp + oa);
+            sl[k][1] = *reinterpret_cast<const uint4*>(pp + ob);
+        }
     }
     // whole groups of GF_PD planes; planes past the chunk compute nothing
     for (int k = 0; k < nin; k += GF_PD) {
 #pragma unroll
         for (int j = 0; j < GF_PD; ++j) plane(k + j, sl[j][0], sl[j][1]);
     }
+}
+
+// Byte field -> two packed fields: every word of every row, halo included;
+// cells outside the block (x < 0 or x >= S_x) are 0.
+__global__ __launch_bounds__(256) void gol_pack_kernel(const uint8_t* __restrict__ in,
+                                                       uint32_t* __restrict__ out0,
+                                                       uint32_t* __restrict__ out1, KGeo g, int pw)
+{
+    const long long rows = (long long)g.nblk * g.S[2] * g.S[1];
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * pw) return;
+    const int w = (int)(i % pw);
+    const long long row = i / pw;  // (block, z, y)
+    const int y = (int)(row % g.S[1]);
+    const long long bz = row / g.S[1];
+    const int z = (int)(bz % g.S[2]);
+    const int b = (int)(bz / g.S[2]);
+    const int x = g.h - 32 + 32 * w;
+    const uint8_t* rp = in + (long long)b * g.bstride + g.xoff + (long long)z * g.slice +
+                        (long long)y * g.pitch;
+    // halves of 16 cells at 16-byte aligned row offsets (xoff + h is a
+    // multiple of 16), each loaded where it lies inside the padded row
+    const int ra = g.xoff + x;
+    uint4 a = make_uint4(0, 0, 0, 0), c = make_uint4(0, 0, 0, 0);
+    if (ra >= 0 && ra + 16 <= g.pitch) a = *reinterpret_cast<const uint4*>(rp + x);
+    if (ra + 16 >= 0 && ra + 32 <= g.pitch) c = *reinterpret_cast<const uint4*>(rp + x + 16);
+    const uint32_t word = gb_pack(a, c) & word_mask(x, 0, g.S[0] - 1);
+    out0[i] = word;
+    out1[i] = word;
+}
+
+// Packed field -> byte field: the owned cells of every block.
+__global__ __launch_bounds__(256) void gol_unpack_kernel(const uint32_t* __restrict__ in,
+                                                         uint8_t* __restrict__ out, KGeo g, int pw)
+{
+    const int nw = g.B[0] / 32;
+    const long long rows = (long long)g.nblk * g.B[2] * g.B[1];
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * nw) return;
+    const int w = (int)(i % nw) + 1;
+    const long long row = i / nw;  // (block, owned z, owned y)
+    const int y = g.h + (int)(row % g.B[1]);
+    const long long bz = row / g.B[1];
+    const int z = g.h + (int)(bz % g.B[2]);
+    const int b = (int)(bz / g.B[2]);
+    const uint32_t word = in[(((long long)b * g.S[2] + z) * g.S[1] + y) * pw + w];
+    uint4 lo, hi;
+    gb_unpack(word, lo, hi);
+    uint4* o = reinterpret_cast<uint4*>(out + (long long)b * g.bstride + g.xoff + (long long)z * g.slice +
+                                        (long long)y * g.pitch + g.h + 32 * (w - 1));
+    o[0] = lo;
+    o[1] = hi;
 }
 
 // The bit-sliced kernel reads its halo from the neighbouring blocks (pull
@@ -648,7 +731,20 @@ static int gol_fused_launch_t(b2r_ctx* ctx, const void* in, void* out)
     return B2R_OK;
 }
 
-template <int R, int NW, int NR>
+// words per packed row of a block: B_x/32 owned words and one per side
+static int gol_pk_words(const KGeo& g) { return g.B[0] / 32 + 2; }
+
+static PGeo gol_pgeo(const KGeo& g)
+{
+    PGeo pg;
+    pg.pw = gol_pk_words(g);
+    pg.rb = 4ll * pg.pw;
+    pg.sb = pg.rb * g.S[1];
+    pg.bb = pg.sb * g.S[2];
+    return pg;
+}
+
+template <int R, int NW, int NR, bool PK>
 static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
 {
     constexpr int TY = NR - 2 * R;
@@ -666,34 +762,84 @@ static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
     if (zc > g.B[2]) zc = g.B[2];
     const int nzc = ceil_div(g.B[2], zc);
     dim3 grid(ntx * nty * nzc * g.nblk);
-    gol_bits_kernel<R, NW, NR><<<grid, (NW + 2) * NR, 0, ctx->stream>>>(
-        (const uint8_t*)in, (uint8_t*)out, g, ntx, nty, nzc, zc, pull);
+    gol_bits_kernel<R, NW, NR, PK><<<grid, (NW + 2) * NR, 0, ctx->stream>>>(
+        (const uint8_t*)in, (uint8_t*)out, g, ntx, nty, nzc, zc, pull,
+        PK ? gol_pgeo(g) : PGeo{});
     B2R_LAUNCH_CHECK();
     return B2R_OK;
 }
 
-template <int R>
-static int gol_fused_launch_r(b2r_ctx* ctx, const void* in, void* out)
+template <int R, bool PK>
+static int gol_bits_launch_r(b2r_ctx* ctx, const void* in, void* out)
 {
+    switch (ctx->tune.gol_tile) {
+    case 4: return gol_bits_launch_t<R, GB_SHAPES[0][0], GB_SHAPES[0][1], PK>(ctx, in, out);
+    case 6: return gol_bits_launch_t<R, GB_SHAPES[2][0], GB_SHAPES[2][1], PK>(ctx, in, out);
+    case 7: return gol_bits_launch_t<R, GB_SHAPES[3][0], GB_SHAPES[3][1], PK>(ctx, in, out);
+    case 8: return gol_bits_launch_t<R, GB_SHAPES[4][0], GB_SHAPES[4][1], PK>(ctx, in, out);
+    default: return gol_bits_launch_t<R, GB_SHAPES[1][0], GB_SHAPES[1][1], PK>(ctx, in, out);
+    }
+}
+
+template <int R>
+static int gol_fused_launch_r(b2r_ctx* ctx, const void* in, void* out, bool packed)
+{
+    if (packed) return gol_bits_launch_r<R, true>(ctx, in, out);
     switch (ctx->tune.gol_tile) {
     case 1: return gol_fused_launch_t<R, GF_SHAPES[0][0], GF_SHAPES[0][1]>(ctx, in, out);
     case 2: return gol_fused_launch_t<R, GF_SHAPES[1][0], GF_SHAPES[1][1]>(ctx, in, out);
     case 3: return gol_fused_launch_t<R, GF_SHAPES[2][0], GF_SHAPES[2][1]>(ctx, in, out);
-    case 4: return gol_bits_launch_t<R, GB_SHAPES[0][0], GB_SHAPES[0][1]>(ctx, in, out);
-    case 6: return gol_bits_launch_t<R, GB_SHAPES[2][0], GB_SHAPES[2][1]>(ctx, in, out);
-    case 7: return gol_bits_launch_t<R, GB_SHAPES[3][0], GB_SHAPES[3][1]>(ctx, in, out);
-    case 8: return gol_bits_launch_t<R, GB_SHAPES[4][0], GB_SHAPES[4][1]>(ctx, in, out);
-    default: return gol_bits_launch_t<R, GB_SHAPES[1][0], GB_SHAPES[1][1]>(ctx, in, out);
+    default: return gol_bits_launch_r<R, false>(ctx, in, out);
     }
 }
 
-int b2r_gol_fused_launch(b2r_ctx* ctx, const void* in, void* out)
+int b2r_gol_fused_packed(const b2r_ctx* ctx)
+{
+    return b2r_gol_fused_reads_halo(ctx) && !ctx->tune.gol_bytes;
+}
+
+int b2r_gol_pack_pair(b2r_ctx* ctx, const void* from, void** p0, void** p1)
+{
+    B2R_CHECK(b2r_gol_fused_packed(ctx), B2R_E_UNSUPPORTED,
+              "packed Game of Life fields need the bit-sliced kernel and B_x %% 32 == 0");
+    const KGeo& g = ctx->k;
+    const PGeo pg = gol_pgeo(g);
+    const size_t half = ((size_t)pg.bb * g.nblk + 255) & ~(size_t)255;
+    if (ctx->gol_pk_bytes < 2 * half) {
+        if (ctx->gol_pk) {
+            if (int rc = b2r_dev_free(ctx, ctx->gol_pk)) return rc;
+            ctx->gol_pk = nullptr;
+            ctx->gol_pk_bytes = 0;
+        }
+        if (int rc = b2r_dev_alloc(ctx, 2 * half, &ctx->gol_pk)) return rc;
+        ctx->gol_pk_bytes = 2 * half;
+    }
+    *p0 = ctx->gol_pk;
+    *p1 = (uint8_t*)ctx->gol_pk + half;
+    const long long words = (long long)g.nblk * g.S[2] * g.S[1] * pg.pw;
+    gol_pack_kernel<<<(unsigned)ceil_div(words, 256), 256, 0, ctx->stream>>>(
+        (const uint8_t*)from, (uint32_t*)*p0, (uint32_t*)*p1, g, pg.pw);
+    B2R_LAUNCH_CHECK();
+    return B2R_OK;
+}
+
+int b2r_gol_unpack(b2r_ctx* ctx, const void* packed, void* to)
+{
+    const KGeo& g = ctx->k;
+    const long long words = (long long)g.nblk * g.B[2] * g.B[1] * (g.B[0] / 32);
+    gol_unpack_kernel<<<(unsigned)ceil_div(words, 256), 256, 0, ctx->stream>>>(
+        (const uint32_t*)packed, (uint8_t*)to, g, gol_pk_words(g));
+    B2R_LAUNCH_CHECK();
+    return B2R_OK;
+}
+
+int b2r_gol_fused_launch(b2r_ctx* ctx, const void* in, void* out, int packed)
 {
     switch (ctx->g.R) {
-    case 1: return gol_fused_launch_r<1>(ctx, in, out);
-    case 2: return gol_fused_launch_r<2>(ctx, in, out);
-    case 3: return gol_fused_launch_r<3>(ctx, in, out);
-    case 4: return gol_fused_launch_r<4>(ctx, in, out);
+    case 1: return gol_fused_launch_r<1>(ctx, in, out, packed);
+    case 2: return gol_fused_launch_r<2>(ctx, in, out, packed);
+    case 3: return gol_fused_launch_r<3>(ctx, in, out, packed);
+    case 4: return gol_fused_launch_r<4>(ctx, in, out, packed);
     }
     b2r_set_error("fused Game of Life supports R <= 4 (got %d)", ctx->g.R);
     return B2R_E_UNSUPPORTED;
@@ -761,14 +907,30 @@ extern "C" int b2r_gol_run(b2r_ctx* ctx, void* d_a, void* d_b, int nsteps, int f
     if (int rc = b2r_time_begin(ctx, ms)) return rc;
     const int R = ctx->g.R;
     const bool use_fused = fused && gol_fused_ok(ctx, R);
-    for (int s = 0; s < nsteps; s += R) {
+    int s = 0;
+    if (use_fused && nsteps >= R && b2r_gol_fused_packed(ctx)) {
+        // whole windows on bit-packed copies of the field: packed once here,
+        // unpacked into the field holding the result after the last window
+        void* pk[2];
+        if (int rc = b2r_gol_pack_pair(ctx, cur, &pk[0], &pk[1])) return rc;
+        const int nwin = nsteps / R;
+        for (int w = 0; w < nwin; ++w) {
+            if (int rc = b2r_gol_fused_launch(ctx, pk[w & 1], pk[(w + 1) & 1], 1)) return rc;
+            void* t = cur;
+            cur = nxt;
+            nxt = t;
+        }
+        if (int rc = b2r_gol_unpack(ctx, pk[nwin & 1], cur)) return rc;
+        s = nwin * R;
+    }
+    for (; s < nsteps; s += R) {
         const int n = nsteps - s < R ? nsteps - s : R;
         const bool fuse = use_fused && n == R;
         // a bit-sliced launch reads its halo from the neighbours itself
         if (!(fuse && b2r_gol_fused_reads_halo(ctx)))
             if (int rc = b2r_direct_exchange(ctx, &cur, 1)) return rc;
         if (fuse) {
-            if (int rc = b2r_gol_fused_launch(ctx, cur, nxt)) return rc;
+            if (int rc = b2r_gol_fused_launch(ctx, cur, nxt, 0)) return rc;
             void* t = cur;
             cur = nxt;
             nxt = t;

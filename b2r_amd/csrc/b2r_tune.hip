@@ -18,10 +18,15 @@
 extern "C" int b2r_time_steps(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
                               const b2r_heat_params* p, int iters, float* ms);
 
-// iters windows of exchange + fused Game of Life launch, as b2r_gol_run runs them
+// iters windows of exchange + fused Game of Life launch, as b2r_gol_run runs
+// them (on the context's packed pair, packed from d_a first, where it keeps
+// the fields packed)
 static int gol_windows(b2r_ctx* ctx, void* d_a, void* d_b, int iters, float* ms)
 {
     if (int rc = b2r_activate(ctx)) return rc;
+    const int packed = b2r_gol_fused_packed(ctx);
+    if (packed)
+        if (int rc = b2r_gol_pack_pair(ctx, d_a, &d_a, &d_b)) return rc;
     if (int rc = b2r_time_begin(ctx, ms)) return rc;
     const bool pulls = b2r_gol_fused_reads_halo(ctx);
     for (int it = 0; it < iters; ++it) {
@@ -29,7 +34,7 @@ static int gol_windows(b2r_ctx* ctx, void* d_a, void* d_b, int iters, float* ms)
         void* out = (it & 1) ? d_a : d_b;
         if (!pulls)
             if (int rc = b2r_direct_exchange(ctx, &in, 1)) return rc;
-        if (int rc = b2r_gol_fused_launch(ctx, in, out)) return rc;
+        if (int rc = b2r_gol_fused_launch(ctx, in, out, packed)) return rc;
     }
     return b2r_time_end(ctx, ms);
 }

@@ -114,6 +114,10 @@ typedef struct b2r_tuning {
     int el_grad_mb;   /* device memory the batched gradient may take for its
                          checkpoints and adjoint state, MB; the shots run in
                          groups that fit; 0 = the free memory less 5 %          */
+    int gol_bytes;    /* fused bit-sliced Game of Life runs with B_x % 32 == 0:
+                         0 keep the fields bit-packed (1 bit per cell) between
+                         launches, packed at the start of b2r_gol_run and
+                         unpacked at its end; 1 byte fields throughout        */
 } b2r_tuning;
 
 /* ---- library / context ---------------------------------------------------- */

@@ -309,21 +309,27 @@ def test_gol_fused_chunking(zc, tile):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("gol_bytes", [0, 1])
 @pytest.mark.parametrize("tile", [6, 7, 8, 3])
 @pytest.mark.parametrize("R", [1, 2, 4])
 @pytest.mark.parametrize("periodic", [False, True])
 @pytest.mark.parametrize("env,blocks", [((64, 48, 40), (2, 3, 2)), ((64, 30, 28), (1, 2, 2)),
                                         ((96, 26, 24), (3, 1, 1)), ((128, 20, 36), (2, 2, 3))])
-def test_gol_bits_halo_from_neighbours(env, blocks, periodic, R, tile):
+def test_gol_bits_halo_from_neighbours(env, blocks, periodic, R, tile, gol_bytes):
     # block x ranges of whole 32-cell words: the bit-sliced kernel (tile 6)
     # reads every halo word from the owning neighbour, resolved z, y, x as the
     # ordered exchange fills it, and no exchange runs between its launches;
     # the byte-lane kernel (tile 3) runs with the exchange.  Single-block axes
     # wrap onto themselves when periodic; frozen faces keep their own halo.
+    # gol_bytes 0 runs the bit-sliced windows on bit-packed fields, then the
+    # last generation (3R + 1 is not a whole number of windows) step by step
+    # on the unpacked byte field.
+    if tile == 3 and gol_bytes:
+        pytest.skip("the byte-lane kernel has no packed form")
     gens = 3 * R + 1
-    say("gol pull", env, blocks, "R", R, "periodic", periodic, "tile", tile)
+    say("gol pull", env, blocks, "R", R, "periodic", periodic, "tile", tile, "bytes", gol_bytes)
     with A.Context(env, blocks, R, 1, 1 if periodic else 0, 1) as c:
-        c.set_tuning(gol_tile=tile, gol_zc=0)
+        c.set_tuning(gol_tile=tile, gol_zc=0, gol_bytes=gol_bytes)
         a, b = c.field(), c.field()
         c.init_bernoulli(a, 8, int(0.3 * 16777216))
         res, _ = c.gol_run(a, b, gens, fused=1)
@@ -334,6 +340,39 @@ def test_gol_bits_halo_from_neighbours(env, blocks, periodic, R, tile):
     else:
         want = O.unframe(O.gol(O.frame(u, 1), gens, False), 1)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_gol_packed_runs_in_sequence(periodic):
+    # Two runs on the same fields and the fields of another context in
+    # between: each run packs its input again (the packed pair is the
+    # context's), whole windows only, then windows plus single steps, and the
+    # timing helper's packed launches leave the byte fields untouched.
+    env, blocks, R = (128, 40, 36), (2, 2, 1), 4
+    u = O.bernoulli((env[2], env[1], env[0]), 12, 0.3)
+    with A.Context(env, blocks, R, 1, 1 if periodic else 0, 1) as c, \
+            A.Context((64, 32, 32), (1, 1, 1), 2, 1, 1, 1) as c2:
+        c.set_tuning(gol_tile=8, gol_zc=0, gol_bytes=0)
+        a, b = c.field(), c.field()
+        c.upload(a, u)
+        say("gol packed run 1, periodic", periodic)
+        res, _ = c.gol_run(a, b, 8, fused=1)
+        x, y = c2.field(), c2.field()
+        c2.init_bernoulli(x, 3, int(0.3 * 16777216))
+        c2.gol_run(x, y, 6, fused=1)
+        dst = a if res == b else b
+        say("gol packed run 2")
+        res2, _ = c.gol_run(res, dst, 11, fused=1)
+        got = c.download(res2)
+        before = c.download(a), c.download(b)
+        c.time_steps(2, a, b, 3)
+        after = c.download(a), c.download(b)
+    if periodic:
+        want = O.gol(u, 19, True)
+    else:
+        want = O.unframe(O.gol(O.frame(u, 1), 19, False), 1)
+    assert np.array_equal(got, want)
+    assert all(np.array_equal(p, q) for p, q in zip(before, after))
 
 
 @pytest.mark.parametrize("wy,wx", [(4, 1), (4, 2), (4, 4), (8, 1), (8, 2), (16, 1)])
