@@ -86,11 +86,14 @@ def gol_bench(A, R, steps, warmup, fused=1, zc=0, record=None):
         tune = c.get_tuning()
     cells = 512 ** 3
     gens = R if fused else 1
+    # 256-cell block rows are whole words: the bit-sliced kernel reads its halo
+    # from the neighbours, and runs on packed fields unless gol_bytes is set
+    packed = bool(fused and tune["gol_tile"] in GOL_BIT_TILES and not tune["gol_bytes"])
     return dict(R=R, fused=fused, ms=ms, wall_s=wall, glups=cells * steps / (ms * 1e-3) / 1e9,
                 kernel_ms=kms, gens_per_launch=gens,
                 kernel_gbs_alg=2.0 * cells * gens / (kms * 1e-3) / 1e9,
                 zc=tune["gol_zc"] if fused else tune["gol_step_zc"],
-                tile=tune["gol_tile"] if fused else None)
+                tile=tune["gol_tile"] if fused else None, packed=packed)
 
 
 # (output quads of 16 cells per tile row, rows incl. halo) of the gol_tile values
@@ -99,9 +102,10 @@ GOL_TILES = {1: (8, 32), 2: (8, 48), 3: (16, 56)}
 GOL_BIT_TILES = {0: (8, 64), 4: (8, 32), 5: (8, 64), 6: (8, 96), 7: (8, 40), 8: (8, 72)}
 
 
-def gol_kernel_name(R, tile):
+def gol_kernel_name(R, tile, packed=True):
     if tile in GOL_BIT_TILES:
-        return "gol_bits_kernel<%d, %d, %d>" % ((R,) + GOL_BIT_TILES[tile])
+        return "gol_bits_kernel<%d, %d, %d, %s>" % ((R,) + GOL_BIT_TILES[tile] +
+                                                    ("true" if packed else "false",))
     return "gol_fused_kernel<%d, %d, %d>" % ((R,) + GOL_TILES[tile])
 
 
@@ -491,19 +495,21 @@ def profile_kernels(A):
     heat delta1 step kernel (768^3, 16-wave workgroups, RY = 8, 32-plane chunks),
     the heat delta4 step kernel (768^3 on 2x2x2 blocks, 8x2 waves, RY = 4,
     64-plane chunks), the elastic stress and velocity kernels (256^3, 64-plane
-    chunks) and the fused elastic step kernel (256^3, automatic z chunk)."""
+    chunks) and the fused elastic step kernel (256^3, automatic z chunk).  The
+    Game of Life launches run on bit-packed fields, as the runs make them, and
+    once more at R = 4 on byte fields (gol_bytes = 1)."""
     nbytes = 1 << 30
     with A.Context((16, 16, 16), (1, 1, 1), 1, 1, 0, 4) as c:
         src, dst = c.dev_alloc(nbytes), c.dev_alloc(nbytes)
         log("stream_copy 1 GiB x3")
         c.stream_copy(dst, src, nbytes, 3)
-    for R, tile, zc in ((2, 8, 16), (4, 8, 16)):
+    for R, tile, zc, byte_fields in ((2, 8, 16, 0), (4, 8, 16, 0), (4, 8, 16, 1)):
         with A.Context((512, 512, 512), (2, 2, 2), R, 1, A.BOUNDARY_PERIODIC, 1) as c:
             a, b = c.field(), c.field()
             c.init_bernoulli(a, 1, int(0.3 * 16777216))
             c.boundary_updating_direct([a])
-            c.set_tuning(gol_zc=zc, gol_tile=tile)
-            log(f"{gol_kernel_name(R, tile)} x5")
+            c.set_tuning(gol_zc=zc, gol_tile=tile, gol_bytes=byte_fields)
+            log(f"{gol_kernel_name(R, tile, not byte_fields)} x5")
             c.time_steps(2, a, b, 5)
     prm = A.heat_params(1, A.HEAT_F32, (0.1, 0.1, 0.1))
     with A.Context((768, 768, 768), (1, 1, 1), 1, 1, 0, 4) as c:
@@ -634,8 +640,10 @@ def main():
     log(json.dumps(plain))
     tr = read_traffic()
     traffic = None
-    # measured for the same R and tile shape only (the profile sequence fixes them)
-    tkey = f"gol_fused_R{best['R']}_tile{best['tile']}_bytes_per_launch"
+    # measured for the same R, tile shape and field form only (the profile
+    # sequence fixes them)
+    form = "packed" if best["packed"] else "bytes"
+    tkey = f"gol_fused_R{best['R']}_tile{best['tile']}_{form}_bytes_per_launch"
     if tr and tkey in tr:
         traffic = float(tr[tkey])
 
@@ -704,20 +712,23 @@ def main():
         "config": {"workload": "gol3d_512^3_u8_periodic_2x2x2", "R": best["R"],
                    "generations": args.steps, "fused_R_generations_per_launch": True,
                    "gol_zc": best["zc"], "gol_tile": best["tile"],
+                   "fields_bit_packed_between_launches": best["packed"],
                    "glups_by_R": {str(r["R"]): round(r["glups"], 3) for r in runs},
                    "plain_step_by_step_R1_glups": round(plain["glups"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": gol_kernel_name(best["R"], best["tile"]),
+                     "kernel": gol_kernel_name(best["R"], best["tile"], best["packed"]),
                      "kernel_ms": round(best["kernel_ms"], 5),
                      "bytes_per_launch_algorithmic": 2 * cells * best["gens_per_launch"],
-                     "note": "2 B/LUP x 512^3 cells x R generations per launch; a fused "
-                             "launch reads and writes each cell once per R generations, "
-                             "so frac can exceed 1",
+                     "note": "2 B/LUP x 512^3 cells x R generations per launch (a u8 "
+                             "cell read and written per generation); a fused launch "
+                             "reads and writes each cell once per R generations, and "
+                             "on bit-packed fields as one bit, so frac can exceed 1 "
+                             "and the kernel is bound by its VALU issue rate",
                      # one generation per launch: the R=1 fused launch, and the
                      # plain step kernel of the reference's structure
-                     "single_pass_kernel": gol_kernel_name(1, r1["tile"]),
+                     "single_pass_kernel": gol_kernel_name(1, r1["tile"], r1["packed"]),
                      "single_pass_achieved": round(r1["kernel_gbs_alg"], 1),
                      "single_pass_frac": round(r1["kernel_gbs_alg"] / HBM_PEAK_GBS, 4),
                      "plain_step_kernel": "gol_step_kernel",
