@@ -165,11 +165,12 @@ __global__ __launch_bounds__(256) void gol_step_kernel(const uint8_t* __restrict
 // ---------------------------------------------------------------------------
 // Tile shapes: gol_tile 1, 2, 3 select this kernel with TXQ output quads (16
 // cells) per tile row and NR rows including the R halo rows per side
-// (GF_SHAPES); gol_tile 4-8 select the bit-sliced kernel below with NW
-// output words (32 cells) per row and NR rows (GB_SHAPES); 0 (automatic) is
-// the bit-sliced 8 x 64 shape.
+// (GF_SHAPES); gol_tile 4-10 select the bit-sliced kernel below with NW
+// output words (32 cells) per row, NR rows and RT rows per thread
+// (GB_SHAPES); 0 (automatic) is the bit-sliced 8 x 64 shape.
 constexpr int GF_SHAPES[3][2] = {{8, 32}, {8, 48}, {16, 56}};
-constexpr int GB_SHAPES[5][2] = {{8, 32}, {8, 64}, {8, 96}, {8, 40}, {8, 72}};
+constexpr int GB_SHAPES[7][3] = {{8, 32, 1}, {8, 64, 1}, {8, 96, 1}, {8, 40, 1},
+                                 {8, 72, 1}, {8, 72, 2}, {8, 136, 2}};
 constexpr int GF_PD = 2;  // input planes in flight per thread
 
 __device__ __forceinline__ uint4 q_add(uint4 a, uint4 b)
@@ -408,21 +409,25 @@ struct PGeo {
     int pw;                // words per packed row (PK only)
 };
 
-template <int R, int NW, int NR, bool PK = false>
-__global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* __restrict__ in,
-                                                                 uint8_t* __restrict__ out,
-                                                                 KGeo g, int ntx, int nty,
-                                                                 int nzc, int zc, int pull,
-                                                                 PGeo pg)
+template <int R, int NW, int NR, bool PK = false, int RT = 1>
+__global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint8_t* __restrict__ in,
+                                                                      uint8_t* __restrict__ out,
+                                                                      KGeo g, int ntx, int nty,
+                                                                      int nzc, int zc, int pull,
+                                                                      PGeo pg)
 {
     constexpr int NQ = NW + 2;  // words per row: plus one per side
     constexpr int NT = NQ * NR;
     constexpr int TY = NR - 2 * R;
+    // a thread owns RT rows of the tile, NR/RT apart: twice the work between
+    // the two barriers of a generation at RT = 2, for half as many waves
+    static_assert(NR % RT == 0, "rows per thread must divide the tile rows");
+    constexpr int NTH = NT / RT;       // threads
     __shared__ uint2 sb[NT + 2];       // z-sums of a plane, a spare word before and after
     __shared__ uint4 xb[NT + 2 * NQ];  // x-sums, a spare row above and below
 
     const int tid = threadIdx.x;
-    const int r = tid / NQ, q = tid % NQ;
+    const int q = tid % NQ;
     // workgroup order: x tile, then the block's x coordinate, then y tile, z
     // chunk and the block's y, z coordinates, so the tiles of x-neighbouring
     // blocks over the same rows run together on one XCD and a pulled halo word
@@ -451,33 +456,41 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
     const uint8_t* __restrict__ src = in + boff;
     uint8_t* __restrict__ dst = out + boff;
 
-    // this thread's word: x = x .. x+31, row y
+    // this thread's words: x = x .. x+31, rows y[j]
     const int x = x0 + 32 * (q - 1);
-    const int y = y0 - R + r;
-    const bool ld_ok = PK ? y < g.S[1] && (x - h) / 32 + 1 < pg.pw
-                          : y < g.S[1] && x + 31 >= 0 && x < g.S[0];
-    // every thread loads (no branch): threads outside the block read the
-    // tile's first word of the last row, in bounds and aligned, and use zero
-    const uint8_t* __restrict__ col =
-        src + (ld_ok ? (long long)y * pitch + xbyte(x)
-                     : (long long)min(y, g.S[1] - 1) * pitch + xbyte(x0));
+    int y[RT];
+    bool ld_ok[RT];
+    const uint8_t* __restrict__ col[RT];
+    const uint8_t* __restrict__ colown[RT];
+#pragma unroll
+    for (int j = 0; j < RT; ++j) {
+        y[j] = y0 - R + tid / NQ + j * (NR / RT);
+        ld_ok[j] = PK ? y[j] < g.S[1] && (x - h) / 32 + 1 < pg.pw
+                      : y[j] < g.S[1] && x + 31 >= 0 && x < g.S[0];
+        // every thread loads (no branch): threads outside the block read the
+        // tile's first word of the last row, in bounds and aligned, and use zero
+        col[j] = src + (ld_ok[j] ? (long long)y[j] * pitch + xbyte(x)
+                                 : (long long)min(y[j], g.S[1] - 1) * pitch + xbyte(x0));
+        colown[j] = col[j];
+    }
     // pull: col is the word's source for planes of the owned z range (y, then
     // x resolved); planes of the z halo add a uniform offset into the z
     // neighbour (the y, x resolution is the same there), or read the own block
     // (colown) at a frozen z face
-    const uint8_t* __restrict__ colown = col;
     long long zoff[2] = {0, 0};
     bool znb[2] = {false, false};
     if (pull) {
-        if (ld_ok) {
-            int bb = b, yy = y, xx = x;
-            const int oy = y < h ? -1 : (y >= h + g.B[1] ? 1 : 0);
+#pragma unroll
+        for (int j = 0; j < RT; ++j) {
+            if (!ld_ok[j]) continue;
+            int bb = b, yy = y[j], xx = x;
+            const int oy = y[j] < h ? -1 : (y[j] >= h + g.B[1] ? 1 : 0);
             bool go = true;
             if (oy != 0) {
                 const int nb = neighbour_block(g, bb, 1, oy > 0);
                 if (nb >= 0) {
                     bb = nb;
-                    yy = y - oy * g.B[1];
+                    yy = y[j] - oy * g.B[1];
                 } else {
                     go = false;
                 }
@@ -490,7 +503,7 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
                     xx = x - ox * g.B[0];
                 }
             }
-            col = in + (long long)bb * bstr + xo + (long long)yy * pitch + xbyte(xx);
+            col[j] = in + (long long)bb * bstr + xo + (long long)yy * pitch + xbyte(xx);
         }
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
@@ -499,21 +512,31 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
             zoff[side] = (long long)(nb - b) * bstr + (side ? -1ll : 1ll) * g.B[2] * slice;
         }
     }
-    // the source column of plane p (p uniform over the workgroup)
-    auto colp = [&](int p) -> const uint8_t* {
-        if (pull && p < h) return znb[0] ? col + zoff[0] : colown;
-        if (pull && p >= h + g.B[2]) return znb[1] ? col + zoff[1] : colown;
-        return col;
+    // the source column of row j at plane p (p uniform over the workgroup)
+    auto colp = [&](int j, int p) -> const uint8_t* {
+        if (pull && p < h) return znb[0] ? col[j] + zoff[0] : colown[j];
+        if (pull && p >= h + g.B[2]) return znb[1] ? col[j] + zoff[1] : colown[j];
+        return col[j];
     };
-    const uint32_t ldm = ld_ok ? ~0u : 0u;
     // a halo word needs only its 16 cells next to the tile (R <= 4), and the
     // second half of a row's last word may lie past the padded row end: such a
     // half reloads the word's other half (no extra traffic) and is masked off
     const bool hi_in = x + 32 <= pitch - g.xoff;  // second half inside the row
-    const int oa = q == 0 && ld_ok ? 16 : 0;  // the clamped word reads its first half
-    const bool b_ld = ld_ok && q != NQ - 1 && hi_in;
-    const int ob = b_ld ? 16 : oa;
-    const uint32_t ma = q == 0 ? 0u : ~0u, mb = b_ld ? ~0u : 0u;
+    uint32_t ldm[RT], ma[RT], mb[RT];
+    int oa[RT], ob[RT];
+    bool out_ok[RT];
+#pragma unroll
+    for (int j = 0; j < RT; ++j) {
+        ldm[j] = ld_ok[j] ? ~0u : 0u;
+        oa[j] = q == 0 && ld_ok[j] ? 16 : 0;  // the clamped word reads its first half
+        const bool b_ld = ld_ok[j] && q != NQ - 1 && hi_in;
+        ob[j] = b_ld ? 16 : oa[j];
+        ma[j] = q == 0 ? 0u : ~0u;
+        mb[j] = b_ld ? ~0u : 0u;
+        const int r = y[j] - (y0 - R);
+        out_ok[j] = q >= 1 && q <= NQ - 2 && r >= R && r < R + TY && y[j] < h + g.B[1] &&
+                    x < h + g.B[0];
+    }
 
     Box bx[R];
     bool clip[R];
@@ -528,10 +551,8 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
                       zc0 - gr < B_.lo[2] || zc1 + gr - 1 > B_.hi[2];
         xm[t - 1] = word_mask(x, B_.lo[0], B_.hi[0]);
     }
-    const bool out_ok = q >= 1 && q <= NQ - 2 && r >= R && r < R + TY && y < h + g.B[1] &&
-                        x < h + g.B[0];
 
-    uint32_t gen[R][3];  // generation t at planes (newest-2, newest-1, newest)
+    uint32_t gen[RT][R][3];  // generation t at planes (newest-2, newest-1, newest)
     const int zend = zc1 + R;
     // Odd chunks march down, even chunks up, so two neighbouring chunks reach
     // their shared halo planes at the same time and the second read of them
@@ -541,22 +562,27 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
     const int dz = (zci & 1) ? -1 : 1;
     const int zfirst = dz > 0 ? zc0 - R : zend - 1;
     auto zp = [&](int k) { return zfirst + dz * k; };
-    auto plane = [&](int k, uint4& la, uint4& lb) {
-        gen[0][0] = gen[0][1];
-        gen[0][1] = gen[0][2];
-        if (PK)
-            gen[0][2] = la.x & ldm;
-        else
-            gen[0][2] = gb_pack(make_uint4(la.x & ma, la.y & ma, la.z & ma, la.w & ma),
-                                make_uint4(lb.x & mb, lb.y & mb, lb.z & mb, lb.w & mb)) & ldm;
-        {  // the load GF_PD planes ahead (the last ones are unused)
+    auto plane = [&](int k, uint4 (&ld)[RT][2]) {
+#pragma unroll
+        for (int j = 0; j < RT; ++j) {
+            uint4& la = ld[j][0];
+            uint4& lb = ld[j][1];
+            gen[j][0][0] = gen[j][0][1];
+            gen[j][0][1] = gen[j][0][2];
+            if (PK)
+                gen[j][0][2] = la.x & ldm[j];
+            else
+                gen[j][0][2] = gb_pack(make_uint4(la.x & ma[j], la.y & ma[j], la.z & ma[j], la.w & ma[j]),
+                                       make_uint4(lb.x & mb[j], lb.y & mb[j], lb.z & mb[j], lb.w & mb[j])) &
+                               ldm[j];
+            // the load GF_PD planes ahead (the last ones are unused)
             const int zz = zp(min(k + GF_PD, nin - 1));
-            const uint8_t* pp = colp(zz) + (long long)zz * slice;
+            const uint8_t* pp = colp(j, zz) + (long long)zz * slice;
             if (PK) {
                 la.x = *reinterpret_cast<const uint32_t*>(pp);
             } else {
-                la = *reinterpret_cast<const uint4*>(pp + oa);
-                lb = *reinterpret_cast<const uint4*>(pp + ob);
+                la = *reinterpret_cast<const uint4*>(pp + oa[j]);
+                lb = *reinterpret_cast<const uint4*>(pp + ob[j]);
             }
         }
 #pragma unroll
@@ -564,79 +590,98 @@ __global__ __launch_bounds__((NW + 2) * NR) void gol_bits_kernel(const uint8_t* 
             // uniform: generation t-1 not yet 3 planes, or a plane past the chunk
             if (k < 2 * t || k >= nin) continue;
             const int p = zp(k) - dz * t;
-            const uint32_t ga = gen[t - 1][0], self = gen[t - 1][1], gc = gen[t - 1][2];
-            const uint32_t s0 = ga ^ self ^ gc, s1 = maj3(ga, self, gc);
-            sb[tid + 1] = make_uint2(s0, s1);
-            __syncthreads();
-            const uint2 L = sb[tid], Rg = sb[tid + 2];
-            // x neighbours of the z-sums: bit i of l is cell x+i-1, of rr cell x+i+1
-            const uint32_t l0 = __builtin_amdgcn_alignbit(s0, L.x, 31),
-                           l1 = __builtin_amdgcn_alignbit(s1, L.y, 31);
-            const uint32_t r0 = __builtin_amdgcn_alignbit(Rg.x, s0, 1),
-                           r1 = __builtin_amdgcn_alignbit(Rg.y, s1, 1);
-            // XZ = l + S + rr, four bit planes (0..9)
-            const uint32_t x0b = l0 ^ s0 ^ r0, k1 = maj3(l0, s0, r0);
-            const uint32_t u = l1 ^ s1 ^ r1, v = maj3(l1, s1, r1);
-            const uint32_t x1b = u ^ k1, w = u & k1;
-            const uint4 X = make_uint4(x0b, x1b, v ^ w, v & w);
-            xb[tid + NQ] = X;
-            __syncthreads();
-            const uint4 U = xb[tid], Dn = xb[tid + 2 * NQ];
-            // box = U + X + Dn (each <= 9), five bit planes
-            const uint32_t b0 = U.x ^ X.x ^ Dn.x, c1 = maj3(U.x, X.x, Dn.x);
-            const uint32_t e1 = U.y ^ X.y ^ Dn.y, c2a = maj3(U.y, X.y, Dn.y);
-            const uint32_t b1 = e1 ^ c1, c2b = e1 & c1;
-            const uint32_t e2 = U.z ^ X.z ^ Dn.z, c3a = maj3(U.z, X.z, Dn.z);
-            const uint32_t b2 = e2 ^ c2a ^ c2b, c3b = maj3(e2, c2a, c2b);
-            const uint32_t e3 = U.w ^ X.w ^ Dn.w, c4a = maj3(U.w, X.w, Dn.w);
-            const uint32_t b3 = e3 ^ c3a ^ c3b, c4b = maj3(e3, c3a, c3b);
-            const uint32_t b4 = c4a | c4b;
-            // rule: box == 5, or alive and 3 <= box <= 8
-            const uint32_t le8 = ~b4 & ~(b3 & (b2 | b1 | b0));
-            const uint32_t ge3 = b4 | b3 | b2 | (b1 & b0);
-            const uint32_t eq5 = ~b4 & ~b3 & b2 & ~b1 & b0;
-            uint32_t nw = eq5 | (self & le8 & ge3);
-            if (clip[t - 1]) {
-                const Box& B_ = bx[t - 1];
-                const bool in_yz = y >= B_.lo[1] && y <= B_.hi[1] && p >= B_.lo[2] && p <= B_.hi[2];
-                const uint32_t m = in_yz ? xm[t - 1] : 0u;
-                nw = (nw & m) | (self & ~m);
+            uint32_t s0[RT], s1[RT];
+#pragma unroll
+            for (int j = 0; j < RT; ++j) {
+                const uint32_t ga = gen[j][t - 1][0], self = gen[j][t - 1][1], gc = gen[j][t - 1][2];
+                s0[j] = ga ^ self ^ gc;
+                s1[j] = maj3(ga, self, gc);
+                sb[tid + j * NTH + 1] = make_uint2(s0[j], s1[j]);
             }
-            if (t < R) {
-                gen[t][0] = gen[t][1];
-                gen[t][1] = gen[t][2];
-                gen[t][2] = nw;
-            } else if (out_ok) {
-                if (PK) {
-                    *reinterpret_cast<uint32_t*>(dst + (long long)p * slice + (long long)y * pitch +
-                                                 xbyte(x)) = nw;
-                } else {
-                    uint4 lo, hi;
-                    gb_unpack(nw, lo, hi);
-                    uint4* o = reinterpret_cast<uint4*>(dst + (long long)p * slice +
-                                                        (long long)y * pitch + x);
-                    o[0] = lo;
-                    if (hi_in) o[1] = hi;
+            __syncthreads();
+            uint4 X[RT];
+#pragma unroll
+            for (int j = 0; j < RT; ++j) {
+                const int pos = tid + j * NTH;
+                const uint2 L = sb[pos], Rg = sb[pos + 2];
+                // x neighbours of the z-sums: bit i of l is cell x+i-1, of rr cell x+i+1
+                const uint32_t l0 = __builtin_amdgcn_alignbit(s0[j], L.x, 31),
+                               l1 = __builtin_amdgcn_alignbit(s1[j], L.y, 31);
+                const uint32_t r0 = __builtin_amdgcn_alignbit(Rg.x, s0[j], 1),
+                               r1 = __builtin_amdgcn_alignbit(Rg.y, s1[j], 1);
+                // XZ = l + S + rr, four bit planes (0..9)
+                const uint32_t x0b = l0 ^ s0[j] ^ r0, k1 = maj3(l0, s0[j], r0);
+                const uint32_t u = l1 ^ s1[j] ^ r1, v = maj3(l1, s1[j], r1);
+                const uint32_t x1b = u ^ k1, w = u & k1;
+                X[j] = make_uint4(x0b, x1b, v ^ w, v & w);
+                xb[pos + NQ] = X[j];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < RT; ++j) {
+                const int pos = tid + j * NTH;
+                const uint32_t self = gen[j][t - 1][1];
+                const uint4 U = xb[pos], Dn = xb[pos + 2 * NQ];
+                // box = U + X + Dn (each <= 9), five bit planes
+                const uint32_t b0 = U.x ^ X[j].x ^ Dn.x, c1 = maj3(U.x, X[j].x, Dn.x);
+                const uint32_t e1 = U.y ^ X[j].y ^ Dn.y, c2a = maj3(U.y, X[j].y, Dn.y);
+                const uint32_t b1 = e1 ^ c1, c2b = e1 & c1;
+                const uint32_t e2 = U.z ^ X[j].z ^ Dn.z, c3a = maj3(U.z, X[j].z, Dn.z);
+                const uint32_t b2 = e2 ^ c2a ^ c2b, c3b = maj3(e2, c2a, c2b);
+                const uint32_t e3 = U.w ^ X[j].w ^ Dn.w, c4a = maj3(U.w, X[j].w, Dn.w);
+                const uint32_t b3 = e3 ^ c3a ^ c3b, c4b = maj3(e3, c3a, c3b);
+                const uint32_t b4 = c4a | c4b;
+                // rule: box == 5, or alive and 3 <= box <= 8
+                const uint32_t le8 = ~b4 & ~(b3 & (b2 | b1 | b0));
+                const uint32_t ge3 = b4 | b3 | b2 | (b1 & b0);
+                const uint32_t eq5 = ~b4 & ~b3 & b2 & ~b1 & b0;
+                uint32_t nw = eq5 | (self & le8 & ge3);
+                if (clip[t - 1]) {
+                    const Box& B_ = bx[t - 1];
+                    const bool in_yz = y[j] >= B_.lo[1] && y[j] <= B_.hi[1] && p >= B_.lo[2] &&
+                                       p <= B_.hi[2];
+                    const uint32_t m = in_yz ? xm[t - 1] : 0u;
+                    nw = (nw & m) | (self & ~m);
+                }
+                if (t < R) {
+                    gen[j][t][0] = gen[j][t][1];
+                    gen[j][t][1] = gen[j][t][2];
+                    gen[j][t][2] = nw;
+                } else if (out_ok[j]) {
+                    if (PK) {
+                        *reinterpret_cast<uint32_t*>(dst + (long long)p * slice +
+                                                     (long long)y[j] * pitch + xbyte(x)) = nw;
+                    } else {
+                        uint4 lo, hi;
+                        gb_unpack(nw, lo, hi);
+                        uint4* o = reinterpret_cast<uint4*>(dst + (long long)p * slice +
+                                                            (long long)y[j] * pitch + x);
+                        o[0] = lo;
+                        if (hi_in) o[1] = hi;
+                    }
                 }
             }
         }
     };
-    uint4 sl[GF_PD][2] = {};
+    uint4 sl[GF_PD][RT][2] = {};
 #pragma unroll
     for (int k = 0; k < GF_PD; ++k) {
         const int zz = zp(min(k, nin - 1));
-        const uint8_t* pp = colp(zz) + (long long)zz * slice;
-        if (PK) {
-            sl[k][0].x = *reinterpret_cast<const uint32_t*>(pp);
-        } else {
-            sl[k][0] = *reinterpret_cast<const uint4*>(pp + oa);
-            sl[k][1] = *reinterpret_cast<const uint4*>(pp + ob);
+#pragma unroll
+        for (int j = 0; j < RT; ++j) {
+            const uint8_t* pp = colp(j, zz) + (long long)zz * slice;
+            if (PK) {
+                sl[k][j][0].x = *reinterpret_cast<const uint32_t*>(pp);
+            } else {
+                sl[k][j][0] = *reinterpret_cast<const uint4*>(pp + oa[j]);
+                sl[k][j][1] = *reinterpret_cast<const uint4*>(pp + ob[j]);
+            }
         }
     }
     // whole groups of GF_PD planes; planes past the chunk compute nothing
     for (int k = 0; k < nin; k += GF_PD) {
 #pragma unroll
-        for (int j = 0; j < GF_PD; ++j) plane(k + j, sl[j][0], sl[j][1]);
+        for (int jp = 0; jp < GF_PD; ++jp) plane(k + jp, sl[jp]);
     }
 }
 
@@ -744,7 +789,7 @@ static PGeo gol_pgeo(const KGeo& g)
     return pg;
 }
 
-template <int R, int NW, int NR, bool PK>
+template <int R, int NW, int NR, int RT, bool PK>
 static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
 {
     constexpr int TY = NR - 2 * R;
@@ -762,7 +807,7 @@ static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
     if (zc > g.B[2]) zc = g.B[2];
     const int nzc = ceil_div(g.B[2], zc);
     dim3 grid(ntx * nty * nzc * g.nblk);
-    gol_bits_kernel<R, NW, NR, PK><<<grid, (NW + 2) * NR, 0, ctx->stream>>>(
+    gol_bits_kernel<R, NW, NR, PK, RT><<<grid, (NW + 2) * NR / RT, 0, ctx->stream>>>(
         (const uint8_t*)in, (uint8_t*)out, g, ntx, nty, nzc, zc, pull,
         PK ? gol_pgeo(g) : PGeo{});
     B2R_LAUNCH_CHECK();
@@ -772,13 +817,18 @@ static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
 template <int R, bool PK>
 static int gol_bits_launch_r(b2r_ctx* ctx, const void* in, void* out)
 {
+#define GB_LAUNCH(i) \
+    gol_bits_launch_t<R, GB_SHAPES[i][0], GB_SHAPES[i][1], GB_SHAPES[i][2], PK>(ctx, in, out)
     switch (ctx->tune.gol_tile) {
-    case 4: return gol_bits_launch_t<R, GB_SHAPES[0][0], GB_SHAPES[0][1], PK>(ctx, in, out);
-    case 6: return gol_bits_launch_t<R, GB_SHAPES[2][0], GB_SHAPES[2][1], PK>(ctx, in, out);
-    case 7: return gol_bits_launch_t<R, GB_SHAPES[3][0], GB_SHAPES[3][1], PK>(ctx, in, out);
-    case 8: return gol_bits_launch_t<R, GB_SHAPES[4][0], GB_SHAPES[4][1], PK>(ctx, in, out);
-    default: return gol_bits_launch_t<R, GB_SHAPES[1][0], GB_SHAPES[1][1], PK>(ctx, in, out);
+    case 4: return GB_LAUNCH(0);
+    case 6: return GB_LAUNCH(2);
+    case 7: return GB_LAUNCH(3);
+    case 8: return GB_LAUNCH(4);
+    case 9: return GB_LAUNCH(5);
+    case 10: return GB_LAUNCH(6);
+    default: return GB_LAUNCH(1);
     }
+#undef GB_LAUNCH
 }
 
 template <int R>

@@ -99,13 +99,16 @@ def gol_bench(A, R, steps, warmup, fused=1, zc=0, record=None):
 # (output quads of 16 cells per tile row, rows incl. halo) of the gol_tile values
 GOL_TILES = {1: (8, 32), 2: (8, 48), 3: (16, 56)}
 # (output words of 32 cells per row, rows incl. halo) of the bit-sliced kernel
-GOL_BIT_TILES = {0: (8, 64), 4: (8, 32), 5: (8, 64), 6: (8, 96), 7: (8, 40), 8: (8, 72)}
+# and rows per thread
+GOL_BIT_TILES = {0: (8, 64, 1), 4: (8, 32, 1), 5: (8, 64, 1), 6: (8, 96, 1), 7: (8, 40, 1),
+                 8: (8, 72, 1), 9: (8, 72, 2), 10: (8, 136, 2)}
 
 
 def gol_kernel_name(R, tile, packed=True):
     if tile in GOL_BIT_TILES:
-        return "gol_bits_kernel<%d, %d, %d, %s>" % ((R,) + GOL_BIT_TILES[tile] +
-                                                    ("true" if packed else "false",))
+        nw, nr, rt = GOL_BIT_TILES[tile]
+        return "gol_bits_kernel<%d, %d, %d, %s, %d>" % (R, nw, nr, "true" if packed else "false",
+                                                        rt)
     return "gol_fused_kernel<%d, %d, %d>" % ((R,) + GOL_TILES[tile])
 
 

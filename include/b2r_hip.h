@@ -102,7 +102,8 @@ typedef struct b2r_tuning {
     int gol_tile;     /* fused Game of Life kernel and tile (cells x rows incl. halo
                          rows): 1-3 byte-lane 128 x 32, 128 x 48, 256 x 56; 4-8
                          bit-sliced 256 x 32, 256 x 64, 256 x 96, 256 x 40,
-                         256 x 72; 0 = 5                                        */
+                         256 x 72; 9, 10 bit-sliced 256 x 72, 256 x 136 with
+                         two rows per thread; 0 = 5                             */
     int heat_wx;      /* waves across x per heat step workgroup: 1, 2 or 4, with
                          heat_wx * heat_wy <= 16 (heat_wy 0 counts as 4)          */
     int el_kernel;    /* batched elastic step: 1 one fused stress + velocity launch

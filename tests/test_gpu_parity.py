@@ -294,7 +294,7 @@ def test_gol_full_size_R_invariance_and_population():
         assert np.array_equal(o, want), key
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 @pytest.mark.parametrize("zc", [4, 9, 64])
 def test_gol_fused_chunking(zc, tile):
     # z-chunk lengths that do not divide B, and a single chunk, for every tile shape
@@ -310,7 +310,7 @@ def test_gol_fused_chunking(zc, tile):
 
 
 @pytest.mark.parametrize("gol_bytes", [0, 1])
-@pytest.mark.parametrize("tile", [6, 7, 8, 3])
+@pytest.mark.parametrize("tile", [6, 7, 8, 9, 10, 3])
 @pytest.mark.parametrize("R", [1, 2, 4])
 @pytest.mark.parametrize("periodic", [False, True])
 @pytest.mark.parametrize("env,blocks", [((64, 48, 40), (2, 3, 2)), ((64, 30, 28), (1, 2, 2)),
@@ -393,7 +393,7 @@ def test_heat_tuning_variants(wy, wx, ry, zc):
     assert bits_equal(got, want)
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 @pytest.mark.parametrize("R", [1, 2, 3, 4])
 @pytest.mark.parametrize("periodic", [False, True])
 def test_gol_fused_tile_shapes(tile, R, periodic):
