@@ -232,6 +232,17 @@ int b2r_heat_gradient(b2r_ctx* ctx, const void* d_u0, const void* d_target,
  * alive & n>=8 -> 0; otherwise unchanged; gol-3D.cu:83-99) over the inner-step
  * region of b2r_i (gol-3D.cu:50, 58).  u8 cells in {0,1}; geom.radius must be 1. */
 int b2r_gol_step(b2r_ctx* ctx, const void* d_in, void* d_out, int b2r_i);
+/* nsteps generations of every block: d_b := d_a first (gol-3D.cu:128-129), then
+ * per window of R generations the exchange and R b2r_gol_step launches
+ * alternating the two fields (the loop of gol-3D.cu:138-145 plus the driver's
+ * exchange, b2r.cpp:179-221), or with fused = 1 (periodic, or every face
+ * frozen; R <= 4) one R-generation launch per window.  Where the bit-sliced
+ * kernel runs (B_x % 32 == 0, gol_tile 0 or 4-10) no exchange runs: it reads
+ * its halo from the neighbouring blocks, and unless gol_bytes is set the
+ * whole windows run on the context's two bit-packed fields (allocated on the
+ * first such call), the result's owned cells then written back to the byte
+ * field.  *result_in_b says which field holds the result; *ms, when not NULL,
+ * the device time between events around the loop. */
 int b2r_gol_run(b2r_ctx* ctx, void* d_a, void* d_b, int nsteps, int fused,
                 int* result_in_b, float* ms);
 
