@@ -103,6 +103,8 @@ _SIGS = {
     "b2r_heat_run": (_i, [_vp, _vp, _vp, C.POINTER(HeatParams), _i, _i, _pi, _pf]),
     "b2r_heat_gradient": (_i, [_vp, _vp, _vp, C.POINTER(HeatParams), _i, _i,
                                C.POINTER(C.c_double), _vp, C.POINTER(C.c_double)]),
+    "b2r_heat_cpu_step": (_i, [C.POINTER(HeatParams), _vp, _vp, _i, _i, _i, _i, _i]),
+    "b2r_heat_run_hybrid": (_i, [_vp, _vp, _vp, C.POINTER(HeatParams), _i, _i, _i, _pi, _pf]),
     "b2r_gol_step": (_i, [_vp, _vp, _vp, _i]),
     "b2r_gol_run": (_i, [_vp, _vp, _vp, _i, _i, _pi, _pf]),
     "b2r_stream_copy": (_i, [_vp, _vp, _vp, C.c_size_t, _i, _pf]),
@@ -324,6 +326,16 @@ class Context:
                                 C.byref(ms) if timed else None))
         return (fb if rib.value else fa), (ms.value if timed else None)
 
+    def heat_run_hybrid(self, fa, fb, params: HeatParams, nsteps, cpu_planes, nthreads=0,
+                        timed=False):
+        """b2r_heat_run with the top cpu_planes z planes stepped on the host
+        cores; returns (result field, wall ms or None)."""
+        rib = C.c_int(0)
+        ms = C.c_float(0.0)
+        check(_lib.b2r_heat_run_hybrid(self.h, fa, fb, C.byref(params), nsteps, cpu_planes,
+                                       nthreads, C.byref(rib), C.byref(ms) if timed else None))
+        return (fb if rib.value else fa), (ms.value if timed else None)
+
     def heat_gradient(self, u0, target, params: HeatParams, nsteps, ckpt, grad_u0):
         """J = 1/2 |u_N - target|^2 over owned cells; writes dJ/du0 into the
         field grad_u0 and returns (J, dJ/dk as 3 floats)."""
@@ -364,6 +376,18 @@ def heat_params(D, mode=HEAT_F32, k=(0.1, 0.1, 0.1), adt=0.1 * 0.1, h2=(1.0, 1.0
         p.h2[a] = h2[a]
     p.adt = adt
     return p
+
+
+def heat_cpu_step(params: HeatParams, grid: np.ndarray, b2r_i=0, nthreads=0) -> np.ndarray:
+    """One inner step of delta-D on the host cores (b2r_heat_cpu_step) over a
+    dense padded (z, y, x) grid; returns the new grid (cells outside the
+    step's region copied from the input)."""
+    a = np.ascontiguousarray(grid, np.float32)
+    out = a.copy()
+    nz, ny, nx = a.shape
+    check(_lib.b2r_heat_cpu_step(C.byref(params), _ptr(a), _ptr(out), nx, ny, nz, b2r_i,
+                                 nthreads))
+    return out
 
 
 # ------------------------------------------------------------------ elastic

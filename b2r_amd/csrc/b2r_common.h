@@ -51,8 +51,12 @@ struct KGeo {
     int S[3];           // padded block size
     int nb[3];          // blocks per axis
     int h;              // halo depth
-    int periodic;       // 1: global faces wrap
+    int periodic;       // bit a: the global faces of axis a wrap (x 1, y 2, z 4)
     int fmask;          // bit 2*axis+side: that global face is frozen (clip to owned)
+    int two_d;          // 1: one z plane (env_z = 1, _ENV_3D_ = 0 of b2r_config.h:13),
+                        // its z halo a frozen zero frame that no exchange touches
+    int zclip;          // > 0: update boxes end below global plane zclip - r*(i+1)
+                        // (the GPU part of a hybrid run, whose z halo ends there)
     int pitch;          // elements per row
     int xoff;           // elements before x = 0 in a row
     long long slice;    // elements per plane
@@ -124,16 +128,32 @@ __host__ __device__ inline Box update_box(const KGeo& g, int b, int r, int i)
     block_coords(g, b, c);
     Box bx;
     for (int a = 0; a < 3; ++a) {
-        bool lo_edge = !g.periodic && c[a] == 0 && ((g.fmask >> (2 * a)) & 1);
-        bool hi_edge = !g.periodic && c[a] == g.nb[a] - 1 && ((g.fmask >> (2 * a + 1)) & 1);
+        const bool per = (g.periodic >> a) & 1;
+        bool lo_edge = !per && c[a] == 0 && ((g.fmask >> (2 * a)) & 1);
+        bool hi_edge = !per && c[a] == g.nb[a] - 1 && ((g.fmask >> (2 * a + 1)) & 1);
         int lo = r * (i + 1), hi = g.S[a] - 1 - r * (i + 1);
         bx.lo[a] = (lo_edge && g.h > lo) ? g.h : lo;
         bx.hi[a] = (hi_edge && g.h + g.B[a] - 1 < hi) ? g.h + g.B[a] - 1 : hi;
+    }
+    if (g.zclip > 0) {
+        // global plane zclip - 1 - r*(i+1) in this block's local coordinates
+        const int hz = g.zclip - 1 - r * (i + 1) - c[2] * g.B[2] + g.h;
+        if (hz < bx.hi[2]) bx.hi[2] = hz;
     }
     return bx;
 }
 
 static inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// Every global face is either periodic or frozen (none open): the condition of
+// the fused kernels, whose redundant halo stages need halos that the exchange
+// refreshes or that never change.
+__host__ __device__ inline bool closed_faces(const KGeo& g)
+{
+    for (int a = 0; a < 3; ++a)
+        if (!((g.periodic >> a) & 1) && ((g.fmask >> (2 * a)) & 3) != 3) return false;
+    return true;
+}
 
 // Neighbour block of b along dim on side 0 (minus) or 1 (plus), -1 if the
 // block sits on a frozen global face (b2r_sync_flag, b2r_halo_sync.cu:140-183;
@@ -144,7 +164,7 @@ __host__ __device__ inline int neighbour_block(const KGeo& g, int b, int dim, in
     block_coords(g, b, c);
     c[dim] += side ? 1 : -1;
     if (c[dim] < 0 || c[dim] >= g.nb[dim]) {
-        if (!g.periodic) return -1;
+        if (!((g.periodic >> dim) & 1)) return -1;
         c[dim] = (c[dim] + g.nb[dim]) % g.nb[dim];
     }
     return (c[2] * g.nb[1] + c[1]) * g.nb[0] + c[0];

@@ -84,9 +84,16 @@ extern "C" int b2r_create(const b2r_geom* geom, int device, b2r_ctx** out)
     }
     int B[3] = {env[0] / bd[0], env[1] / bd[1], env[2] / bd[2]};
     int h = g.radius * g.R;
+    // env_z = 1: the 2D environment of _ENV_3D_ = 0 (b2r_config.h:13).  The
+    // reference then skips the z check and has no z halo (b2r.cpp:53-67,
+    // 142-146); here the one plane keeps a z frame of h planes that stays
+    // zero (the z faces are frozen, never periodic, and never exchanged), so
+    // every z difference of the 3D kernels vanishes at the plane.
+    const bool two_d = env[2] == 1;
+    B2R_CHECK(!two_d || bd[2] == 1, B2R_E_GEOM, "a 2D grid (env_z = 1) has BLOCK_DIM_Z = 1");
     int bmin = B[0] < B[1] ? B[0] : B[1];
-    bmin = bmin < B[2] ? bmin : B[2];
-    // b2r.cpp:54-59: 1 <= R*D <= min(B)/2
+    if (!two_d) bmin = bmin < B[2] ? bmin : B[2];
+    // b2r.cpp:54-59: 1 <= R*D <= min(B)/2 (over x and y only in 2D)
     B2R_CHECK(h <= bmin / 2, B2R_E_GEOM,
               "should meet 1 <= R*radius <= min(B)/2 (R*radius=%d, min(B)=%d)", h, bmin);
 
@@ -133,8 +140,10 @@ extern "C" int b2r_create(const b2r_geom* geom, int device, b2r_ctx** out)
         k.nb[a] = bd[a];
     }
     k.h = h;
-    k.periodic = g.boundary == B2R_BOUNDARY_PERIODIC;
-    k.fmask = k.periodic ? 0 : 0x3F;
+    k.two_d = two_d ? 1 : 0;
+    k.zclip = 0;
+    k.periodic = g.boundary == B2R_BOUNDARY_PERIODIC ? (two_d ? 3 : 7) : 0;
+    k.fmask = k.periodic ? (two_d ? 0x30 : 0) : 0x3F;
     k.pitch = L.pitch;
     k.xoff = L.x_offset;
     k.slice = L.slice;
@@ -214,8 +223,11 @@ extern "C" int b2r_set_frozen_faces(b2r_ctx* ctx, int mask)
 {
     B2R_CHECK(ctx, B2R_E_ARG, "null context");
     B2R_CHECK(mask >= 0 && mask <= 0x3F, B2R_E_ARG, "mask 0x%x outside 6 bits", mask);
-    B2R_CHECK(!ctx->k.periodic || mask == 0, B2R_E_ARG, "a periodic grid has no global faces");
-    ctx->k.fmask = mask;
+    for (int a = 0; a < 3; ++a)
+        B2R_CHECK(!((ctx->k.periodic >> a) & 1) || !((mask >> (2 * a)) & 3), B2R_E_ARG,
+                  "axis %d is periodic: it has no global faces", a);
+    // the z frame of a 2D grid stays frozen
+    ctx->k.fmask = ctx->k.two_d ? (mask | 0x30) : mask;
     return B2R_OK;
 }
 
@@ -746,7 +758,8 @@ extern "C" int b2r_boundary_updating(b2r_ctx* ctx, int time_step, void* d_field)
         B2R_HIP(hipMalloc(&ctx->msgbuf, need));
         ctx->msgbuf_bytes = need;
     }
-    for (int d = 0; d < 3; ++d) {
+    // a 2D grid has no Z phase (b2r_halo_sync.cu:170, 489)
+    for (int d = 0; d < (ctx->k.two_d ? 2 : 3); ++d) {
         if (int rc = b2r_pack(ctx, d, d_field, ctx->msgbuf)) return rc;
         if (int rc = b2r_unpack(ctx, d, d_field, ctx->msgbuf)) return rc;
     }
@@ -762,7 +775,7 @@ int b2r_direct_exchange(b2r_ctx* ctx, void* const* fields, int nfields)
         int n = nfields - base < 16 ? nfields - base : 16;
         FieldList fl;
         for (int i = 0; i < n; ++i) fl.p[i] = fields[base + i];
-        for (int d = 0; d < 3; ++d) {
+        for (int d = 0; d < (g.two_d ? 2 : 3); ++d) {
             int V;
             long long rows;
             if (d == 0) {

@@ -621,10 +621,10 @@ __global__ void el_inject_kernel(const ElDev* __restrict__ tab, int stage,
     // with periodic faces the source cell also has images one period away,
     // which can lie in this block's halo (or, for a single block along an
     // axis, in its box twice); the redundant halo stages see every image
-    const int k = d.g.periodic ? 1 : 0;
-    for (int kz = -k; kz <= k; ++kz)
-        for (int ky = -k; ky <= k; ++ky)
-            for (int kx = -k; kx <= k; ++kx) {
+    const int px = d.g.periodic & 1, py = (d.g.periodic >> 1) & 1, pz = (d.g.periodic >> 2) & 1;
+    for (int kz = -pz; kz <= pz; ++kz)
+        for (int ky = -py; ky <= py; ++ky)
+            for (int kx = -px; kx <= px; ++kx) {
                 const int sh[3] = {kx, ky, kz};
                 int lc[3];
                 bool in = true;
@@ -636,6 +636,9 @@ __global__ void el_inject_kernel(const ElDev* __restrict__ tab, int stage,
                 const long long c = cell_index(d.g, b, lc[0], lc[1], lc[2]);
                 const float tb = d.dt * d.f[F_B][c];
                 v[c] = v[c] + tb * w;
+                // a y force also enters the displacement accumulator, as
+                // Dt*Dt*inv_rho*(div sigma + f.y) does (eqwp.cu:229-235)
+                if (d.disp && p[3] == F_VY) d.disp[c] = d.disp[c] + (d.dt * tb) * w;
             }
 }
 
@@ -980,7 +983,7 @@ static void el_tiles(ElDev& d, int zc_want, bool fused = false)
     // axis with one block and both global faces frozen they are the owned
     // cells [h, h+B) at every stage.
     for (int a = 0; a < 3; ++a) {
-        const bool owned = g.nb[a] == 1 && !g.periodic && ((g.fmask >> (2 * a)) & 3) == 3;
+        const bool owned = g.nb[a] == 1 && !((g.periodic >> a) & 1) && ((g.fmask >> (2 * a)) & 3) == 3;
         const int o = fused ? 4 : 2;
         d.o[a] = owned ? g.h : o;
         d.w[a] = owned ? g.B[a] : g.S[a] - 2 * o;
@@ -1297,9 +1300,9 @@ static int el_fused_choice(b2r_el_shot* const* shots, int nshots, bool* fused)
     *fused = true;
     for (int k = 0; k < nshots; ++k) {
         const b2r_ctx* c = shots[k]->ctx;
-        const bool ok = c->k.periodic || c->k.fmask == 0x3F;
+        const bool ok = closed_faces(c->k);
         B2R_CHECK(ok || c->tune.el_kernel != 1, B2R_E_UNSUPPORTED,
-                  "el_kernel 1 needs every global face frozen or all periodic");
+                  "el_kernel 1 needs every global face frozen or periodic");
         if (!ok || c->tune.el_kernel == 2) *fused = false;
     }
     return B2R_OK;
@@ -1504,7 +1507,7 @@ extern "C" int b2r_el_autotune(b2r_el_shot* s, const char* record_path, b2r_tuni
     b2r_ctx* c = s->ctx;
     const b2r_tuning base = c->tune;
     const int zcs[] = {0, 16, 24, 32, 48, 64, 96, 128, 256};
-    const bool can_fuse = c->k.periodic || c->k.fmask == 0x3F;
+    const bool can_fuse = closed_faces(c->k);
     const int warm = 2, iters = 6;
     // best z chunk and time per kernel (index 1 fused, 2 separate)
     float kms[3] = {1e30f, 1e30f, 1e30f};
@@ -1564,7 +1567,7 @@ extern "C" int b2r_el_autotune_batch(b2r_el_shot* const* shots, int nshots, cons
         B2R_CHECK(shots[k], B2R_E_ARG, "null shot %d", k);
         const b2r_ctx* c = shots[k]->ctx;
         base[k] = c->tune;
-        can_fuse = can_fuse && (c->k.periodic || c->k.fmask == 0x3F);
+        can_fuse = can_fuse && closed_faces(c->k);
         cells += (long long)c->g.env_x * c->g.env_y * c->g.env_z;
     }
     auto restore = [&]() {

@@ -749,7 +749,7 @@ int b2r_gol_fused_reads_halo(const b2r_ctx* ctx)
 
 static bool gol_fused_ok(const b2r_ctx* ctx, int R)
 {
-    return R >= 1 && R <= 4 && (ctx->k.periodic || ctx->k.fmask == 0x3F);
+    return R >= 1 && R <= 4 && closed_faces(ctx->k);
 }
 
 template <int R, int TXQ, int NR>

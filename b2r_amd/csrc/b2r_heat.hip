@@ -191,6 +191,12 @@ int b2r_heat_check(b2r_ctx* ctx, const b2r_heat_params* p)
               B2R_E_UNSUPPORTED, "x-y plane too large for the heat kernels");
     if (p->mode == B2R_HEAT_REF)
         B2R_CHECK(p->h2[0] > 0 && p->h2[1] > 0 && p->h2[2] > 0, B2R_E_ARG, "h2 must be > 0");
+    // a 2D grid has no z term: its z frame is zero, so the step must not see
+    // a z difference (F32 mode with k_z = 0; the reference expression of
+    // heat-3D.cpp:57-60 always has one)
+    if (ctx->k.two_d)
+        B2R_CHECK(p->mode == B2R_HEAT_F32 && p->k[2] == 0.0f, B2R_E_UNSUPPORTED,
+                  "a 2D grid (env_z = 1) runs F32 mode with k[2] = 0");
     return B2R_OK;
 }
 
@@ -450,7 +456,7 @@ __global__ __launch_bounds__(256) void heat_fused_kernel(const float* __restrict
 static bool heat_fused_ok(const b2r_ctx* ctx, int D)
 {
     const int R = ctx->g.R;
-    return R * D <= 4 && (ctx->k.periodic || ctx->k.fmask == 0x3F);
+    return R * D <= 4 && closed_faces(ctx->k);
 }
 
 template <int D, int MODE, int R>

@@ -171,7 +171,7 @@ extern "C" int b2r_heat_gradient(b2r_ctx* ctx, const void* d_u0, const void* d_t
               "bad argument (nsteps=%d, ckpt=%d)", nsteps, ckpt);
     for (const void* f : {d_u0, d_target, (const void*)d_grad_u0})
         if (int rc = b2r_check_field(ctx, f)) return rc;
-    B2R_CHECK(ctx->k.periodic || ctx->k.fmask == 0x3F, B2R_E_UNSUPPORTED,
+    B2R_CHECK(closed_faces(ctx->k), B2R_E_UNSUPPORTED,
               "the heat adjoint needs periodic or frozen global faces");
     if (int rc = b2r_activate(ctx)) return rc;
     const KGeo& g = ctx->k;

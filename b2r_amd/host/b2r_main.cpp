@@ -24,6 +24,25 @@
 
 typedef std::chrono::steady_clock clk;
 
+// b2r.cpp:47-67: the block geometry from the environment and the block
+// counts, then b2r_sync_init (which checks 1 <= R*D <= min(B)/2 and exits
+// with the library's message).  A 2D environment (ENV_DIM_Z = 1, _ENV_3D_ 0)
+// has one block along z and B2R_BLOCK_SIZE_Z = 1 (b2r.cpp:63-67, 142-146).
+static void simu_env_init()
+{
+    if (ENV_DIM_Z == 1) {
+        B2R_ENV_3D = 0;
+        BLOCK_DIM_Z = 1;
+    }
+    B2R_B_X = ENV_DIM_X / BLOCK_DIM_X;
+    B2R_B_Y = ENV_DIM_Y / BLOCK_DIM_Y;
+    B2R_B_Z = ENV_DIM_Z / BLOCK_DIM_Z;
+    B2R_BLOCK_SIZE_X = B2R_B_X + 2 * B2R_D * B2R_R;
+    B2R_BLOCK_SIZE_Y = B2R_B_Y + 2 * B2R_D * B2R_R;
+    B2R_BLOCK_SIZE_Z = B2R_ENV_3D ? B2R_B_Z + 2 * B2R_D * B2R_R : 1;
+    b2r_sync_init();
+}
+
 static double since(clk::time_point t0)
 {
     return std::chrono::duration<double>(clk::now() - t0).count();
@@ -83,8 +102,12 @@ int main(int argc, char* argv[])
                "BLOCK_DIM_(X,Y,Z) separately!\n");
         return 1;
     }
-    b2r_sync_init();  // validates R*D <= min(B)/2 and exits like b2r.cpp:54-59
+    if (ENV_DIM_Z == 1) {  // the 2D environment (b2r.cpp:142-146)
+        B2R_ENV_3D = 0;
+        BLOCK_DIM_Z = 1;
+    }
     for (int r = 0; r < BLOCK_DIM_X * BLOCK_DIM_Y * BLOCK_DIM_Z; ++r) b2r_sync_flag(r);
+    simu_env_init();
     b2r_ctx* ctx = b2r_host_ctx();
 
     CELL_DT* cur = nullptr;
@@ -155,13 +178,18 @@ int main(int argc, char* argv[])
         fclose(fp);
     }
     if (model == "heat" && (B2R_OUTPUT & 2)) {
-        std::vector<CELL_DT> blk((size_t)B2R_BLOCK_SIZE_X * B2R_BLOCK_SIZE_Y * B2R_BLOCK_SIZE_Z);
+        // the library's padded block; a 2D block prints its one plane (the
+        // library keeps a zero z frame around it)
+        b2r_layout L;
+        b2r_get_layout(ctx, &L);
+        const size_t plane = (size_t)L.S[0] * L.S[1];
+        std::vector<CELL_DT> blk(plane * L.S[2]);
         for (int r = 0; r < BLOCK_DIM_X * BLOCK_DIM_Y * BLOCK_DIM_Z; ++r) {
             if (b2r_block_download(ctx, cur, r, blk.data()) < 0) {
                 fprintf(stderr, "B2R Error: block download: %s\n", b2r_last_error());
                 return -1;
             }
-            print_block_data_step(r, (float)n_iter, blk.data());
+            print_block_data_step(r, (float)n_iter, blk.data() + (B2R_ENV_3D ? 0 : plane * L.halo));
         }
     }
     if (model == "eqwp") eqwp_cuda_finalize();

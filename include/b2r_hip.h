@@ -227,6 +227,26 @@ int b2r_heat_gradient(b2r_ctx* ctx, const void* d_u0, const void* d_target,
                       const b2r_heat_params* p, int nsteps, int ckpt, double* misfit,
                       void* d_grad_u0, double* grad_k);
 
+/* One inner step b2r_i of delta-D on the host cores over one dense padded
+ * Ngx*Ngy*Ngz host grid (x fastest): out = step(in) on [D(b2r_i+1),
+ * N-1-D(b2r_i+1)] per axis, the arithmetic of the device kernels term for term
+ * (either mode), so a host step equals a device step bit for bit.  Replaces
+ * cpu_fdm_heat_diffuse1..4 (heat-3D.cpp:48-134; their delta2-4 bounds and taps,
+ * SURVEY defect D3, are not reproduced); nthreads <= 0 takes every hardware
+ * thread.  No device, no context. */
+int b2r_heat_cpu_step(const b2r_heat_params* p, const float* h_in, float* h_out, int Ngx,
+                      int Ngy, int Ngz, int b2r_i, int nthreads);
+/* b2r_heat_run with the host cores taking part: they own the top cpu_planes
+ * owned z planes of the global grid (R*D <= cpu_planes <= env_z - R*D), the
+ * GPU the rest; every R steps the two exchange joint layers of R*D planes over
+ * PCIe and then take R inner steps at once (the CPU+GPU scheme of the B2R paper:
+ * the "joint layer" of heat-3D.cpp:47, 55 and heat_3D_gpu_diffsync_main,
+ * declared at heat-3D.hpp:15 and never defined there).  Equal bit for bit to
+ * b2r_heat_run without fusion; the slab's planes are written back into the
+ * result field.  3D grids with every global face frozen; *ms is wall time. */
+int b2r_heat_run_hybrid(b2r_ctx* ctx, void* d_a, void* d_b, const b2r_heat_params* p,
+                        int nsteps, int cpu_planes, int nthreads, int* result_in_b, float* ms);
+
 /* ---- 3D Game of Life (gol-3D.cu) ------------------------------------------ */
 /* One generation (26-neighbour count n; alive & n<=1 -> 0; dead & n==5 -> 1;
  * alive & n>=8 -> 0; otherwise unchanged; gol-3D.cu:83-99) over the inner-step
@@ -266,7 +286,8 @@ int b2r_el_field(const b2r_el_shot* s, int which, void** d_field);
 /* medium on the global owned grid (x fastest); halos filled by exchange */
 int b2r_el_set_medium(b2r_el_shot* s, const float* lam, const float* mu, const float* b);
 int b2r_el_reset(b2r_el_shot* s);  /* zero state, t = 0 */
-/* the displacement accumulator of eqwp_fd4_vy (eqwp.cu:231, 234): d += dt*dt*b*(div sigma)_y */
+/* the displacement accumulator of eqwp_fd4_vy (eqwp.cu:229-235):
+ * d += dt*dt*b*((div sigma)_y + f_y), f_y the point forces in component 1 */
 int b2r_el_enable_displacement(b2r_el_shot* s);
 /* nsrc point forces at global cells xyz[3i..], component comp[i] (0..2), values
  * wav[t*nsrc + i] for steps t < nt (the force patch of eqwp.cu:561-566 is one

@@ -118,39 +118,42 @@ static void coef_f64(int D, double c[5])
 void orc_heat_coef_f32(int D, float* c) { coef_f32(D, c); }
 void orc_heat_coef_f64(int D, double* c) { coef_f64(D, c); }
 
-/* Index of global cell (x,y,z) in the oracle grid.  Frozen: the grid carries a
- * frame of width D, so it is (gx+2D)*(gy+2D)*(gz+2D) and x in [-D, gx+D).
- * Periodic: gx*gy*gz with wrap. */
+/* Index of global cell (x,y,z) in the oracle grid.  periodic is 0 (frozen), 1
+ * (every axis wraps) or a per-axis mask (bit a: axis a wraps; 3 is a 2D grid
+ * of one z plane that wraps x and y).  A frozen axis carries a frame of width
+ * D (n + 2D cells, coordinate in [-D, n+D)); a periodic axis has n cells and
+ * wraps. */
 typedef struct {
-    int gx, gy, gz, D, periodic;
-    long long nx, ny;
+    int gx, gy, gz, D, periodic;  /* periodic: the per-axis mask */
+    long long nx, ny, nz;
 } grid_t;
+
+static inline int wrap_or_frame(int v, int n, int D, int per)
+{
+    return per ? ((v % n) + n) % n : v + D;
+}
 
 static inline long long gidx(const grid_t* g, int x, int y, int z)
 {
-    if (g->periodic) {
-        x = ((x % g->gx) + g->gx) % g->gx;
-        y = ((y % g->gy) + g->gy) % g->gy;
-        z = ((z % g->gz) + g->gz) % g->gz;
-        return ((long long)z * g->ny + y) * g->nx + x;
-    }
-    return ((long long)(z + g->D) * g->ny + (y + g->D)) * g->nx + (x + g->D);
+    const int pm = g->periodic;
+    x = wrap_or_frame(x, g->gx, g->D, pm & 1);
+    y = wrap_or_frame(y, g->gy, g->D, (pm >> 1) & 1);
+    z = wrap_or_frame(z, g->gz, g->D, (pm >> 2) & 1);
+    return ((long long)z * g->ny + y) * g->nx + x;
 }
 
 static grid_t mkgrid(int gx, int gy, int gz, int D, int periodic)
 {
     grid_t g;
-    g.gx = gx; g.gy = gy; g.gz = gz; g.D = D; g.periodic = periodic;
-    g.nx = periodic ? gx : gx + 2 * D;
-    g.ny = periodic ? gy : gy + 2 * D;
+    const int pm = periodic == 1 ? 7 : periodic;
+    g.gx = gx; g.gy = gy; g.gz = gz; g.D = D; g.periodic = pm;
+    g.nx = (pm & 1) ? gx : gx + 2 * D;
+    g.ny = (pm & 2) ? gy : gy + 2 * D;
+    g.nz = (pm & 4) ? gz : gz + 2 * D;
     return g;
 }
 
-static long long grid_elems(const grid_t* g)
-{
-    long long nz = g->periodic ? g->gz : g->gz + 2 * g->D;
-    return g->nx * g->ny * nz;
-}
+static long long grid_elems(const grid_t* g) { return g->nx * g->ny * g->nz; }
 
 /* FP64 heat: u' = u + sum_a k_a * (c0 u + sum_j c_j (u[-j] + u[+j])). */
 void orc_heat_f64(double* u, int gx, int gy, int gz, int D, const double* k,
@@ -405,13 +408,13 @@ void orc_gol(uint8_t* u, int gx, int gy, int gz, int nsteps, int periodic)
     /* t[3*i + (d+1)] = storage coordinate of owned coordinate i+d */
     for (int i = 0; i < gx; ++i)
         for (int d = -1; d <= 1; ++d)
-            tx[3 * i + d + 1] = periodic ? ((i + d) % gx + gx) % gx : i + d + 1;
+            tx[3 * i + d + 1] = wrap_or_frame(i + d, gx, 1, g.periodic & 1);
     for (int i = 0; i < gy; ++i)
         for (int d = -1; d <= 1; ++d)
-            ty[3 * i + d + 1] = periodic ? ((i + d) % gy + gy) % gy : i + d + 1;
+            ty[3 * i + d + 1] = wrap_or_frame(i + d, gy, 1, (g.periodic >> 1) & 1);
     for (int i = 0; i < gz; ++i)
         for (int d = -1; d <= 1; ++d)
-            tz[3 * i + d + 1] = periodic ? ((i + d) % gz + gz) % gz : i + d + 1;
+            tz[3 * i + d + 1] = wrap_or_frame(i + d, gz, 1, (g.periodic >> 2) & 1);
     uint8_t* v = (uint8_t*)malloc(n);
     memcpy(v, u, n);
     uint8_t* src = u;
@@ -466,13 +469,15 @@ static inline char* cell(const olay_t* L, char* f, int b, int x, int y, int z)
     return f + e * L->esize;
 }
 
-/* Neighbour block of b along dim on side (0 minus, 1 plus); -1 if none. */
+/* Neighbour block of b along dim on side (0 minus, 1 plus); -1 if none.
+ * periodic: bit a set = the global faces of axis a wrap (a 2D grid, one z
+ * plane, wraps x and y only; b2r_config.h:12-13). */
 static int nbr(const olay_t* L, int b, int dim, int side)
 {
     int c[3] = {b % L->nb[0], (b / L->nb[0]) % L->nb[1], b / (L->nb[0] * L->nb[1])};
     c[dim] += side ? 1 : -1;
     if (c[dim] < 0 || c[dim] >= L->nb[dim]) {
-        if (!L->periodic) return -1;
+        if (!((L->periodic >> dim) & 1)) return -1;
         c[dim] = (c[dim] + L->nb[dim]) % L->nb[dim];
     }
     return (c[2] * L->nb[1] + c[1]) * L->nb[0] + c[0];

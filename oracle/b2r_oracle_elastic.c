@@ -131,7 +131,11 @@ void orc_el_forward_f32(float* const* st, const float* lam, const float* mu, con
         for (int i = 0; i < nsrc; ++i) {
             long long c = at(&g, src_xyz[3 * i], src_xyz[3 * i + 1], src_xyz[3 * i + 2]);
             float tb = dt * b[c];
-            st[src_comp[i]][c] = st[src_comp[i]][c] + tb * wav[(long long)t * nsrc + i];
+            float w = wav[(long long)t * nsrc + i];
+            st[src_comp[i]][c] = st[src_comp[i]][c] + tb * w;
+            /* the y force enters the displacement with the divergence,
+             * Dt*Dt*inv_rho*(div sigma + f.y) (eqwp.cu:229-235) */
+            if (disp && src_comp[i] == VY) disp[c] = disp[c] + (dt * tb) * w;
         }
         for (int r = 0; r < nrec; ++r) {
             long long c = at(&g, rec_xyz[3 * r], rec_xyz[3 * r + 1], rec_xyz[3 * r + 2]);

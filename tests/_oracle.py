@@ -117,12 +117,25 @@ def unframe(a: np.ndarray, D: int):
     return a[D:-D, D:-D, D:-D]
 
 
+def _owned(shape, D, periodic):
+    """(gz, gy, gx) of an oracle grid of `shape` (z, y, x): a frozen axis
+    carries a frame of D cells per side, a periodic one none (periodic: bool,
+    or the per-axis mask of periodic_mask)."""
+    pm = periodic_mask(periodic)
+    return tuple(s if (pm >> a) & 1 else s - 2 * D for s, a in zip(shape, (2, 1, 0)))
+
+
+def _cflag(periodic):
+    pm = periodic_mask(periodic)
+    return 1 if pm == 7 else pm
+
+
 def heat_f64(u, D, k, nsteps, periodic):
     """u: framed (frozen) or plain (periodic) float64 grid, modified copy returned."""
     u = np.ascontiguousarray(u, np.float64).copy()
-    gz, gy, gx = u.shape if periodic else tuple(s - 2 * D for s in u.shape)
+    gz, gy, gx = _owned(u.shape, D, periodic)
     kk = np.asarray(k, np.float64)
-    lib.orc_heat_f64(_p(u), gx, gy, gz, D, _p(kk), nsteps, int(periodic))
+    lib.orc_heat_f64(_p(u), gx, gy, gz, D, _p(kk), nsteps, _cflag(periodic))
     return u
 
 
@@ -144,17 +157,17 @@ def heat_grad_f64(u0, tgt, D, k, nsteps, periodic):
 
 def heat_f32(u, D, mode, k, adt, h2, nsteps, periodic):
     u = np.ascontiguousarray(u, np.float32).copy()
-    gz, gy, gx = u.shape if periodic else tuple(s - 2 * D for s in u.shape)
+    gz, gy, gx = _owned(u.shape, D, periodic)
     kk = np.asarray(k, np.float32)
     hh = np.asarray(h2, np.float64)
-    lib.orc_heat_f32(_p(u), gx, gy, gz, D, mode, _p(kk), adt, _p(hh), nsteps, int(periodic))
+    lib.orc_heat_f32(_p(u), gx, gy, gz, D, mode, _p(kk), adt, _p(hh), nsteps, _cflag(periodic))
     return u
 
 
 def gol(u, nsteps, periodic):
     u = np.ascontiguousarray(u, np.uint8).copy()
-    gz, gy, gx = u.shape if periodic else tuple(s - 2 for s in u.shape)
-    lib.orc_gol(_p(u), gx, gy, gz, nsteps, int(periodic))
+    gz, gy, gx = _owned(u.shape, 1, periodic)
+    lib.orc_gol(_p(u), gx, gy, gz, nsteps, _cflag(periodic))
     return u
 
 
@@ -190,12 +203,20 @@ def reference_block_step(h1, b2r_i):
 
 
 # ---------------------------------------------------------------- exchange
+def periodic_mask(periodic):
+    """0/False: no axis wraps; 1/True: every axis (7); otherwise the per-axis
+    mask itself (bit a: axis a wraps; a 2D grid wraps x and y only, 3), as the
+    oracle's C functions read it."""
+    pm = int(periodic)
+    return 7 if pm == 1 else pm
+
+
 def layout_args(lay, nb, esize, periodic):
     nb_ = np.asarray(nb, np.int32)
     B = np.asarray(list(lay.B), np.int32)
     S = np.asarray(list(lay.S), np.int32)
     return nb_, B, S, (lay.halo, lay.x_offset, lay.pitch, lay.slice, lay.block_stride, esize,
-                       int(periodic))
+                       periodic_mask(periodic))
 
 
 def exchange(raw, lay, nb, esize, periodic):

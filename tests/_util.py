@@ -76,6 +76,8 @@ def expected_halo_ids(lay, blocks, periodic, poison):
     dimension-ordered exchange) would also keep poison."""
     B, h = lay.B, lay.halo
     G = [B[a] * blocks[a] for a in range(3)]
+    # 0/1 (or bool): no axis or every axis wraps; otherwise a per-axis mask
+    pm = 7 if int(periodic) == 1 else int(periodic)
     out = []
     for bz in range(blocks[2]):
         for by in range(blocks[1]):
@@ -85,7 +87,7 @@ def expected_halo_ids(lay, blocks, periodic, poison):
                 valid = []
                 for a in range(3):
                     g = np.arange(lay.S[a]) - h + c[a] * B[a]
-                    if periodic:
+                    if (pm >> a) & 1:
                         valid.append(np.ones_like(g, bool))
                         g = g % G[a]
                     else:

@@ -9,7 +9,7 @@ import sys
 import numpy as np
 import pytest
 
-import _elastic as E
+import _eqwp as Q
 import _oracle as O
 from b2r_amd import output
 
@@ -57,20 +57,13 @@ def test_driver_gol_matches_oracle(tmp_path, periodic):
 
 
 def test_driver_eqwp_matches_oracle(tmp_path):
-    # the reference model of eqwp.cu: constant medium, y-force patch for 50 steps
+    # the reference model of eqwp.cu: constant medium, the y-force patch of
+    # every block (its padded coordinates, B2R_D = 4 here), forced while the
+    # window start is below 50
     env, iters = (40, 40, 40), 12
     got = np.fromfile(run(tmp_path, 1, 2, 1, 1, "--model", "eqwp", "--env", *env, "--iters",
                           iters), np.float32).reshape(env[2], env[1], env[0])
-    gx, gy, gz = env
-    shape = (gz, gy, gx)
-    lam = np.full(shape, 32.0, np.float32)
-    mu = np.full(shape, 174.87, np.float32)
-    b = np.full(shape, np.float32(1.0 / 1800.0), np.float32)
-    src = np.array([[x, gy - 7, z] for x in range(gx // 2, gx // 2 + gx // 10)
-                    for z in range(gz // 2, gz // 2 + gz // 10)], np.int32)
-    wav = np.full((iters, len(src)), -10000.0, np.float32)
-    st, _ = E.forward(env, lam, mu, b, np.float32(0.5), (np.float32(0.1),) * 3, iters, src,
-                      [1] * len(src), wav, np.array([[1, 1, 1]], np.int32), 1)
+    st, _ = Q.forward(env, (2, 1, 1), 1, 4, iters)
     assert np.any(st[1])
     assert np.array_equal(got.view(np.uint32), st[1].view(np.uint32))
 
@@ -80,42 +73,37 @@ def test_driver_eqwp_periodic_matches_oracle(tmp_path):
     env, iters = (40, 40, 40), 14
     got = np.fromfile(run(tmp_path, 2, 1, 2, 1, "--model", "eqwp", "--env", *env, "--iters",
                           iters, "--periodic"), np.float32).reshape(env[2], env[1], env[0])
-    gx, gy, gz = env
-    shape = (gz, gy, gx)
-    src = np.array([[x, gy - 7, z] for x in range(gx // 2, gx // 2 + gx // 10)
-                    for z in range(gz // 2, gz // 2 + gz // 10)], np.int32)
-    wav = np.full((iters, len(src)), -10000.0, np.float32)
-    st, _ = E.forward_periodic(env, np.full(shape, 32.0, np.float32),
-                               np.full(shape, 174.87, np.float32),
-                               np.full(shape, np.float32(1.0 / 1800.0), np.float32),
-                               np.float32(0.5), (np.float32(0.1),) * 3, iters, src, [1] * len(src),
-                               wav)
+    st, _ = Q.forward(env, (1, 2, 1), 2, 4, iters, periodic=True)
     assert np.any(st[1][:, :4])  # the wave crossed the y face
     assert np.array_equal(got.view(np.uint32), st[1].view(np.uint32))
 
 
 def test_driver_eqwp_writes_displacement(tmp_path):
     # eqwp.cu:689-709 with the write switched on: displacement-<ts>.bin and .txt
-    # after the last step group (ts = its first step), against the oracle
+    # after the last step group (ts = its first step), against the oracle; the
+    # force enters the displacement at the patch (eqwp.cu:229-235)
     env, R, iters = (32, 32, 32), 2, 10
     run(tmp_path, R, 1, 1, 1, "--model", "eqwp", "--env", *env, "--iters", iters, "--output",
         "both")
     got = output.read_displacement_bin(tmp_path / ("displacement-%d.bin" % (iters - R)))
-    gx, gy, gz = env
-    shape = (gz, gy, gx)
-    src = np.array([[x, gy - 7, z] for x in range(gx // 2, gx // 2 + gx // 10)
-                    for z in range(gz // 2, gz // 2 + gz // 10)], np.int32)
-    wav = np.full((iters, len(src)), -10000.0, np.float32)
-    _, _, disp = E.forward(env, np.full(shape, 32.0, np.float32), np.full(shape, 174.87, np.float32),
-                           np.full(shape, np.float32(1.0 / 1800.0), np.float32), np.float32(0.5),
-                           (np.float32(0.1),) * 3, iters, src, [1] * len(src), wav,
-                           np.array([[1, 1, 1]], np.int32), 1, disp=True)
+    _, _, disp = Q.forward(env, (1, 1, 1), R, 4, iters, disp=True)
     assert np.any(disp)
     assert np.array_equal(got.view(np.uint32), disp.view(np.uint32))
     txt = output.read_displacement_txt(tmp_path / ("displacement-%d.txt" % (iters - R)), env)
     # six significant digits: a relative rounding error of at most 5e-6 (compared in
     # float64: rtol times a float32 subnormal would underflow in float32)
     assert np.allclose(txt, got.astype(np.float64), rtol=1e-5, atol=0)
+
+
+def test_driver_eqwp_force_schedule_follows_window_starts(tmp_path):
+    # eqwp.cu:135-139 forces every inner step of a window that starts before
+    # FRC_TIME = 50: with R = 4 steps 48..51 are forced too (52 steps)
+    env, R, iters = (32, 32, 32), 4, 56
+    assert Q.forced_steps(R) == 52
+    got = np.fromfile(run(tmp_path, R, 1, 1, 1, "--model", "eqwp", "--env", *env, "--iters",
+                          iters), np.float32).reshape(env[2], env[1], env[0])
+    st, _ = Q.forward(env, (1, 1, 1), R, 4, iters)
+    assert np.array_equal(got.view(np.uint32), st[1].view(np.uint32))
 
 
 def test_driver_heat_writes_block_dumps(tmp_path):
@@ -134,3 +122,29 @@ def test_driver_heat_writes_block_dumps(tmp_path):
         own = blk[h:h + B[2], h:h + B[1], h:h + B[0]]
         ref = want[cz * B[2]:(cz + 1) * B[2], cy * B[1]:(cy + 1) * B[1], cx * B[0]:(cx + 1) * B[0]]
         assert np.allclose(own, ref, rtol=0, atol=0.005 + 1e-6)
+
+
+@pytest.mark.parametrize("model,periodic", [("heat", False), ("gol", True), ("gol", False)])
+def test_driver_2d_matches_oracle(tmp_path, model, periodic):
+    # ENV_DIM_Z = 1: the 2D environment (_ENV_3D_ = 0, b2r.cpp:63-67, 142-146);
+    # heat runs F32 with k = Alpha*Dt/h^2 on x and y and no z term
+    env, R, iters = (32, 32, 1), 2, 7
+    args = [R, 2, 2, 1, "--model", model, "--env", *env, "--iters", iters]
+    if periodic:
+        args.append("--periodic")
+    if model == "heat":
+        args += ["--D", 1]
+    raw = np.fromfile(run(tmp_path, *args), np.float32 if model == "heat" else np.uint8)
+    got = raw.reshape(1, env[1], env[0])
+    if model == "heat":
+        u = O.frame(O.uniform((1, env[1], env[0]), 0), 1)
+        k = (np.float32(0.01), np.float32(0.01), 0.0)
+        want = O.unframe(O.heat_f32(u, 1, 0, k, 0.0, (1, 1, 1), iters, False), 1)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    else:
+        u = O.bernoulli((1, env[1], env[0]), 1, 0.3)
+        if periodic:
+            want = O.gol(np.pad(u, ((1, 1), (0, 0), (0, 0))), iters, 3)[1:-1]
+        else:
+            want = O.gol(O.frame(u, 1), iters, False)[1:-1, 1:-1, 1:-1]
+        assert np.array_equal(got, want)
