@@ -59,7 +59,8 @@ class Tuning(C.Structure):
     _fields_ = [("heat_ry", C.c_int), ("heat_zc", C.c_int), ("gol_zc", C.c_int),
                 ("gol_step_zc", C.c_int), ("el_zc", C.c_int), ("heat_wy", C.c_int),
                 ("gol_tile", C.c_int), ("heat_wx", C.c_int), ("el_kernel", C.c_int),
-                ("el_step_zc", C.c_int), ("el_grad_mb", C.c_int), ("gol_bytes", C.c_int)]
+                ("el_step_zc", C.c_int), ("el_grad_mb", C.c_int), ("gol_bytes", C.c_int),
+                ("gol_exchange", C.c_int)]
 
 
 class HeatParams(C.Structure):
@@ -131,6 +132,7 @@ _SIGS = {
     "b2r_el_gradient_batch": (_i, [C.POINTER(_vp), _i, C.POINTER(_vp), _i, _i, _i,
                                    C.POINTER(C.c_double), C.POINTER(_vp), C.POINTER(_vp),
                                    C.POINTER(_vp), _pf]),
+    "b2r_el_gradient_plan": (_i, [C.POINTER(_vp), _i, _i, _i, _pi, C.POINTER(C.c_size_t)]),
     "b2r_autotune": (_i, [_vp, _i, _vp, _vp, C.POINTER(HeatParams), C.c_char_p,
                           C.POINTER(Tuning)]),
     "b2r_dev_alloc": (_i, [_vp, C.c_size_t, C.POINTER(_vp)]),
@@ -522,3 +524,12 @@ def el_gradient_batch(shots, obs, nsteps, ckpt, mode=1, timed=False):
     check(_lib.b2r_el_gradient_batch(arr, n, po, nsteps, ckpt, mode, mis, pl, pm, pb,
                                      C.byref(ms) if timed else None))
     return list(mis), grads, (ms.value if timed else None)
+
+
+def el_gradient_plan(shots, nsteps, ckpt):
+    """(groups, device bytes of the largest group) of a batched gradient."""
+    arr = (_vp * len(shots))(*[s.h for s in shots])
+    n = C.c_int(0)
+    big = C.c_size_t(0)
+    check(_lib.b2r_el_gradient_plan(arr, len(shots), nsteps, ckpt, C.byref(n), C.byref(big)))
+    return n.value, big.value

@@ -205,6 +205,8 @@ extern "C" int b2r_set_tuning(b2r_ctx* ctx, const b2r_tuning* t)
     B2R_CHECK(t->el_kernel >= 0 && t->el_kernel <= 2, B2R_E_ARG, "el_kernel must be 0, 1 or 2");
     B2R_CHECK(t->el_grad_mb >= 0, B2R_E_ARG, "el_grad_mb must be >= 0");
     B2R_CHECK(t->gol_bytes == 0 || t->gol_bytes == 1, B2R_E_ARG, "gol_bytes must be 0 or 1");
+    B2R_CHECK(t->gol_exchange == 0 || t->gol_exchange == 1, B2R_E_ARG,
+              "gol_exchange must be 0 or 1");
     B2R_CHECK((t->heat_wx == 0 || t->heat_wx == 1 || t->heat_wx == 2 || t->heat_wx == 4) &&
                   (t->heat_wx > 1 ? t->heat_wx : 1) * (t->heat_wy > 0 ? t->heat_wy : 4) <= 16,
               B2R_E_ARG, "heat_wx must be 0, 1, 2 or 4 with heat_wx * heat_wy <= 16");

@@ -2048,6 +2048,50 @@ extern "C" int b2r_el_gradient(b2r_el_shot* s, const float* h_obs, int nsteps, i
     return b2r_el_gradient_ex(s, h_obs, nsteps, ckpt, 0, misfit, h_glam, h_gmu, h_gb, nullptr);
 }
 
+// The device-memory budget of the batched gradient (b2r_tuning.el_grad_mb of
+// the first shot, 0 = the free memory less 5 %).
+static int el_grad_budget(b2r_el_shot* const* shots, size_t* budget)
+{
+    if (int rc = b2r_activate(shots[0]->ctx)) return rc;
+    *budget = (size_t)shots[0]->ctx->tune.el_grad_mb << 20;
+    if (*budget == 0) {
+        size_t free_b = 0, total_b = 0;
+        B2R_HIP(hipMemGetInfo(&free_b, &total_b));
+        *budget = free_b / 20 * 19;
+    }
+    return B2R_OK;
+}
+
+extern "C" int b2r_el_gradient_plan(b2r_el_shot* const* shots, int nshots, int nsteps, int ckpt,
+                                    int* ngroups, size_t* bytes_max_group)
+{
+    B2R_CHECK(shots && nshots >= 1 && nsteps >= 1 && ckpt >= 1 && ngroups, B2R_E_ARG,
+              "bad argument");
+    for (int k = 0; k < nshots; ++k) B2R_CHECK(shots[k], B2R_E_ARG, "null shot %d", k);
+    size_t budget = 0;
+    if (int rc = el_grad_budget(shots, &budget)) return rc;
+    int groups = 0;
+    size_t big = 0;
+    for (int k0 = 0; k0 < nshots; ++groups) {
+        size_t used = 0;
+        int k1 = k0;
+        while (k1 < nshots) {
+            const size_t need = el_gradient_bytes(shots[k1], nsteps, ckpt);
+            if (k1 > k0 && used + need > budget) break;
+            B2R_CHECK(need <= budget, B2R_E_NOMEM,
+                      "gradient: shot %d needs %zu MB, more than the %zu MB allowed", k1,
+                      need >> 20, budget >> 20);
+            used += need;
+            ++k1;
+        }
+        big = used > big ? used : big;
+        k0 = k1;
+    }
+    *ngroups = groups;
+    if (bytes_max_group) *bytes_max_group = big;
+    return B2R_OK;
+}
+
 extern "C" int b2r_el_gradient_batch(b2r_el_shot* const* shots, int nshots,
                                      const float* const* h_obs, int nsteps, int ckpt, int mode,
                                      double* misfit, float* const* h_glam, float* const* h_gmu,
@@ -2059,13 +2103,8 @@ extern "C" int b2r_el_gradient_batch(b2r_el_shot* const* shots, int nshots,
     if (mode > 0) {
         // consecutive groups of shots whose gradient memory fits the budget
         for (int k = 0; k < nshots; ++k) B2R_CHECK(shots[k], B2R_E_ARG, "null shot %d", k);
-        if (int rc = b2r_activate(shots[0]->ctx)) return rc;
-        size_t budget = (size_t)shots[0]->ctx->tune.el_grad_mb << 20;
-        if (budget == 0) {
-            size_t free_b = 0, total_b = 0;
-            B2R_HIP(hipMemGetInfo(&free_b, &total_b));
-            budget = free_b / 20 * 19;
-        }
+        size_t budget = 0;
+        if (int rc = el_grad_budget(shots, &budget)) return rc;
         float total_ms = 0.0f;
         for (int k0 = 0; k0 < nshots;) {
             int k1 = k0;

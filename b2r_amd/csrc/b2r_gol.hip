@@ -744,7 +744,7 @@ static int gol_bits_pull(const KGeo& g) { return g.B[0] % 32 == 0; }
 int b2r_gol_fused_reads_halo(const b2r_ctx* ctx)
 {
     const int t = ctx->tune.gol_tile;
-    return (t == 0 || t >= 4) && gol_bits_pull(ctx->k);
+    return (t == 0 || t >= 4) && gol_bits_pull(ctx->k) && !ctx->tune.gol_exchange;
 }
 
 static bool gol_fused_ok(const b2r_ctx* ctx, int R)
@@ -794,7 +794,8 @@ static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
 {
     constexpr int TY = NR - 2 * R;
     const KGeo& g = ctx->k;
-    const int pull = gol_bits_pull(g);
+    // with gol_exchange the run exchanges first and the kernel reads its own halo
+    const int pull = gol_bits_pull(g) && !ctx->tune.gol_exchange;
     const int ntx = ceil_div(g.B[0], 32 * NW);
     const int nty = ceil_div(g.B[1], TY);
     long long tiles = (long long)ntx * nty * g.nblk;
@@ -976,9 +977,13 @@ extern "C" int b2r_gol_run(b2r_ctx* ctx, void* d_a, void* d_b, int nsteps, int f
     for (; s < nsteps; s += R) {
         const int n = nsteps - s < R ? nsteps - s : R;
         const bool fuse = use_fused && n == R;
-        // a bit-sliced launch reads its halo from the neighbours itself
-        if (!(fuse && b2r_gol_fused_reads_halo(ctx)))
+        // a bit-sliced launch reads its halo from the neighbours itself;
+        // gol_exchange runs the pack/unpack exchange of boundary_updating
+        if (fuse && ctx->tune.gol_exchange) {
+            if (int rc = b2r_boundary_updating(ctx, s, cur)) return rc;
+        } else if (!(fuse && b2r_gol_fused_reads_halo(ctx))) {
             if (int rc = b2r_direct_exchange(ctx, &cur, 1)) return rc;
+        }
         if (fuse) {
             if (int rc = b2r_gol_fused_launch(ctx, cur, nxt, 0)) return rc;
             void* t = cur;

@@ -32,8 +32,11 @@ static int gol_windows(b2r_ctx* ctx, void* d_a, void* d_b, int iters, float* ms)
     for (int it = 0; it < iters; ++it) {
         void* in = (it & 1) ? d_b : d_a;
         void* out = (it & 1) ? d_a : d_b;
-        if (!pulls)
+        if (ctx->tune.gol_exchange) {
+            if (int rc = b2r_boundary_updating(ctx, it, in)) return rc;
+        } else if (!pulls) {
             if (int rc = b2r_direct_exchange(ctx, &in, 1)) return rc;
+        }
         if (int rc = b2r_gol_fused_launch(ctx, in, out, packed)) return rc;
     }
     return b2r_time_end(ctx, ms);

@@ -25,9 +25,48 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baselines' OpenMP threads stay next to each other (read when the
+# oracle's OpenMP runtime starts, so set before it is loaded)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+# the CPUs this process may run on, read before an OpenMP runtime binds the
+# main thread to one of them
+try:
+    CPUS_AVAILABLE = len(os.sched_getaffinity(0))
+except (AttributeError, OSError):
+    CPUS_AVAILABLE = None
 
 METRIC = "GLUP/s & % HBM roofline, 1 MI355X; batched+adjoint vs plain at stated FP64 err"
 HBM_PEAK_GBS = 8000.0
+# VALU issue: one wave64 vector instruction per cycle per CU (4 SIMDs, 4 cycles
+# per wave64 instruction each), 256 CUs at 2.4 GHz, in G wave instructions/s
+VALU_PEAK_GINST = 256 * 2.4
+
+
+def host_cpu():
+    """The host CPU as the baselines see it: model, physical cores (distinct
+    (physical id, core id) pairs of /proc/cpuinfo), logical CPUs, the CPUs this
+    process may run on, and the OpenMP binding."""
+    model, cores, logical = None, set(), 0
+    phys = core = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "processor":
+                logical += 1
+            elif k == "model name" and model is None:
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+                cores.add((phys, core))
+    except OSError:
+        pass
+    return {"cpu_model": model, "physical_cores_host": len(cores) or None,
+            "logical_cpus_host": logical or None, "cpus_available": CPUS_AVAILABLE,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
 
 
 def log(*a):
@@ -59,14 +98,18 @@ def record_choice(path, what, best, tried):
 
 
 # ------------------------------------------------------------------ Game of Life
-def gol_bench(A, R, steps, warmup, fused=1, zc=0, record=None):
+def gol_bench(A, R, steps, warmup, fused=1, zc=0, record=None, exchange=False):
     """steps generations of 512^3; kernel_ms = one launch of the dominant kernel
-    (the fused R-generation kernel, or the single-generation kernel)."""
+    (the fused R-generation kernel, or the single-generation kernel).  exchange:
+    every window runs the pack/unpack boundary_updating on byte fields and the
+    kernel reads its own halo (gol_exchange = 1)."""
     env = (512, 512, 512)
     with A.Context(env, (2, 2, 2), R, 1, A.BOUNDARY_PERIODIC, 1) as c:
         a, b = c.field(), c.field()
         c.init_bernoulli(a, 1, int(0.3 * 16777216))
         c.boundary_updating_direct([a])
+        if exchange:
+            c.set_tuning(gol_exchange=1)
         kind = 2 if fused else 1
         if zc:
             c.set_tuning(gol_zc=zc)
@@ -88,7 +131,8 @@ def gol_bench(A, R, steps, warmup, fused=1, zc=0, record=None):
     gens = R if fused else 1
     # 256-cell block rows are whole words: the bit-sliced kernel reads its halo
     # from the neighbours, and runs on packed fields unless gol_bytes is set
-    packed = bool(fused and tune["gol_tile"] in GOL_BIT_TILES and not tune["gol_bytes"])
+    packed = bool(fused and tune["gol_tile"] in GOL_BIT_TILES and not tune["gol_bytes"] and
+                  not tune["gol_exchange"])
     return dict(R=R, fused=fused, ms=ms, wall_s=wall, glups=cells * steps / (ms * 1e-3) / 1e9,
                 kernel_ms=kms, gens_per_launch=gens,
                 kernel_gbs_alg=2.0 * cells * gens / (kms * 1e-3) / 1e9,
@@ -203,7 +247,9 @@ def elastic_batch_bench(A, W, nshots=16, nsteps=100, record=None):
             ms_p += s.forward(nsteps, 0, timed=True)
         wall_p = time.perf_counter() - t0
         out.update(batched_graph_ms=round(ms_b, 2), plain_loop_ms=round(ms_p, 2),
-                   plain_loop_wall_s=round(wall_p, 2),
+                   plain_loop_wall_s=round(wall_p, 2), stated_steps=2000,
+                   batched_ms_per_step=round(ms_b / nsteps, 4),
+                   plain_ms_per_step=round(ms_p / nsteps, 4),
                    batched_glups=round(cells * nsteps / (ms_b * 1e-3) / 1e9, 2),
                    plain_glups=round(cells * nsteps / (ms_p * 1e-3) / 1e9, 2),
                    speedup_batched_vs_plain=round(ms_p / ms_b, 3),
@@ -238,8 +284,13 @@ def elastic_gradient_bench(A, W, nshots=8, nsteps=300, ckpt=50, record=None,
         obs = [s.receivers() for s in shots]
         for s in shots:
             s.set_medium(*true_m)
+        groups, gbytes = A.el_gradient_plan(shots, nsteps, ckpt)
+        g1500, g1500b = A.el_gradient_plan(shots, 1500, ckpt)
         out = {"shots": nshots, "grid": "256^3", "steps": nsteps, "ckpt": ckpt,
-               "receivers": "64x64 surface, v_y"}
+               "receivers": "64x64 surface, v_y", "stated_steps": 1500,
+               "groups": groups, "group_max_GB": round(gbytes / 1e9, 1),
+               "groups_at_stated_steps": g1500, "group_max_GB_at_stated_steps":
+                   round(g1500b / 1e9, 1)}
         res = {}
         for mode, name in modes:
             t0 = time.perf_counter()
@@ -249,6 +300,7 @@ def elastic_gradient_bench(A, W, nshots=8, nsteps=300, ckpt=50, record=None,
             out[name + "_ms"] = round(ms, 1)
             out[name + "_wall_s"] = round(wall, 2)
             out[name + "_shot_glups"] = round(nshots * cells * nsteps / (ms * 1e-3) / 1e9, 3)
+            out[name + "_ms_per_shot_step"] = round(ms / (nshots * nsteps), 4)
             log(f"  gradient {name}: {ms:.1f} ms")
         out["speedup_batched_vs_plain"] = round(out["plain_loop_ms"] / out["batched_graph_ms"], 3)
         mb, gb_ = res["batched_graph"]
@@ -491,55 +543,62 @@ def elastic_checks(A, W):
 
 
 def profile_kernels(A):
-    """A short fixed sequence for rocprofv3 counter runs: a copy kernel of known
-    size (1 GiB read + 1 GiB written per launch, the scale for FETCH_SIZE /
-    WRITE_SIZE), then 5 launches each of the fused Game of Life kernel (512^3,
-    2x2x2, R = 2 and 4, the tile shapes and z chunks the autotuner picks), the
-    heat delta1 step kernel (768^3, 16-wave workgroups, RY = 8, 32-plane chunks),
-    the heat delta4 step kernel (768^3 on 2x2x2 blocks, 8x2 waves, RY = 4,
-    64-plane chunks), the elastic stress and velocity kernels (256^3, 64-plane
-    chunks) and the fused elastic step kernel (256^3, automatic z chunk).  The
-    Game of Life launches run on bit-packed fields, as the runs make them, and
-    once more at R = 4 on byte fields (gol_bytes = 1)."""
+    """A short fixed sequence for rocprofv3 counter runs, each kernel at the
+    launch shape the bench runs it with: a copy kernel of known size (1 GiB
+    read + 1 GiB written per launch, the scale for FETCH_SIZE / WRITE_SIZE),
+    then, after the same autotuning the bench does, 5 launches each of the
+    fused Game of Life kernel (512^3, 2x2x2, periodic, R = 1, 2, 4, on the
+    bit-packed fields the runs use), the heat delta1 step kernel (768^3), the
+    heat delta4 step kernel (768^3 on 2x2x2 blocks) and the elastic step
+    (256^3, the kernel and z chunk b2r_el_autotune picks, then the other
+    kernel).  The tuner's own launches come first, so the last 5 dispatches
+    of each kernel are the profiled ones; each choice is logged as a JSON line
+    on stderr with the kernel's name."""
+    from b2r_amd import workloads as W
     nbytes = 1 << 30
     with A.Context((16, 16, 16), (1, 1, 1), 1, 1, 0, 4) as c:
         src, dst = c.dev_alloc(nbytes), c.dev_alloc(nbytes)
         log("stream_copy 1 GiB x3")
         c.stream_copy(dst, src, nbytes, 3)
-    for R, tile, zc, byte_fields in ((2, 8, 16, 0), (4, 8, 16, 0), (4, 8, 16, 1)):
+    for R in (1, 2, 4):
         with A.Context((512, 512, 512), (2, 2, 2), R, 1, A.BOUNDARY_PERIODIC, 1) as c:
             a, b = c.field(), c.field()
             c.init_bernoulli(a, 1, int(0.3 * 16777216))
             c.boundary_updating_direct([a])
-            c.set_tuning(gol_zc=zc, gol_tile=tile, gol_bytes=byte_fields)
-            log(f"{gol_kernel_name(R, tile, not byte_fields)} x5")
+            t = c.autotune(2, a, b, None, None)
+            c.init_bernoulli(a, 1, int(0.3 * 16777216))
+            packed = t["gol_tile"] in GOL_BIT_TILES and not t["gol_bytes"]
+            log(json.dumps({"profile": "gol_R%d" % R,
+                            "kernel": gol_kernel_name(R, t["gol_tile"], packed),
+                            "gol_tile": t["gol_tile"], "gol_zc": t["gol_zc"],
+                            "packed": packed, "launches": 5}))
             c.time_steps(2, a, b, 5)
-    prm = A.heat_params(1, A.HEAT_F32, (0.1, 0.1, 0.1))
-    with A.Context((768, 768, 768), (1, 1, 1), 1, 1, 0, 4) as c:
-        a, b = c.field(), c.field()
-        c.init_uniform(a, 0)
-        c.set_tuning(heat_wy=16, heat_ry=8, heat_zc=32)
-        log("heat_step_kernel<1> x5")
-        c.time_steps(0, a, b, 5, prm)
-    prm4 = A.heat_params(4, A.HEAT_F32, (0.05, 0.05, 0.05))
-    with A.Context((768, 768, 768), (2, 2, 2), 1, 4, 0, 4) as c:
-        a, b = c.field(), c.field()
-        c.init_uniform(a, 0)
-        c.set_tuning(heat_wy=8, heat_wx=2, heat_ry=4, heat_zc=64)
-        log("heat_step_kernel<4> x5")
-        c.time_steps(0, a, b, 5, prm4)
-    from b2r_amd import workloads as W
+    for name, env, blocks, D, k in (("heat_D1_768", (768, 768, 768), (1, 1, 1), 1, 0.1),
+                                    ("heat_D4_768_2x2x2", (768, 768, 768), (2, 2, 2), 4, 0.05)):
+        prm = A.heat_params(D, A.HEAT_F32, (k, k, k))
+        with A.Context(env, blocks, 1, D, 0, 4) as c:
+            a, b = c.field(), c.field()
+            c.init_uniform(a, 0)
+            c.heat_run(a, b, prm, 2)
+            t = c.autotune(0, a, b, prm, None)
+            log(json.dumps({"profile": name, "kernel": "heat_step_kernel<%d, 0, %d, %d, %d>" % (
+                D, t["heat_ry"], t["heat_wy"], t["heat_wx"]), "heat_zc": t["heat_zc"],
+                "launches": 5}))
+            c.time_steps(0, a, b, 5, prm)
     env = (256, 256, 256)
     vp, vs, rho = W.layered_medium(env, 0)
     with A.Context(env, (1, 1, 1), 1, 4, 0, 4) as c:
-        c.set_tuning(el_zc=64, el_kernel=2)
         s = A.ElasticShot(c, W.cfl_dt(float(vp.max()), 10.0), (0.1, 0.1, 0.1))
         s.set_medium(*W.lame(vp, vs, rho))
-        log("elastic stress + velocity x5")
-        s.forward(5, 1)
-        c.set_tuning(el_kernel=1, el_step_zc=0)
-        log("elastic fused step x5")
-        s.forward(5, 1)
+        t = s.autotune(None)
+        first = t["el_kernel"]
+        for kern in (first, 3 - first):
+            c.set_tuning(el_kernel=kern)
+            log(json.dumps({"profile": "elastic_256", "kernel": "el_step_kernel" if kern == 1
+                            else "el_stress_kernel + el_velocity_kernel<7>",
+                            "el_zc": t["el_zc"], "el_step_zc": t["el_step_zc"],
+                            "chosen": kern == first, "steps": 5}))
+            s.forward(5, 1)
         s.close()
     print(json.dumps({"profile_kernels": "done"}), flush=True)
 
@@ -596,8 +655,12 @@ def main():
                     help="append the autotuner's choices to this file")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="run the fixed kernel sequence used by the rocprofv3 counter runs")
-    ap.add_argument("--only", choices=["elastic-gradient", "elastic-batch", "heat"],
-                    help="run one section alone and print its JSON (for profiling)")
+    ap.add_argument("--only", choices=["elastic-gradient", "elastic-batch", "heat",
+                                       "elastic-full"],
+                    help="run one section alone and print its JSON (for profiling); "
+                         "elastic-full: the 16-shot forward over 2000 steps and the 16-shot "
+                         "256^3 gradient over 1500 steps (checkpoints every 50), batched "
+                         "against the plain per-shot loop")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the halo "
@@ -615,6 +678,16 @@ def main():
         return
     if args.only == "elastic-batch":
         print(json.dumps(elastic_batch_bench(A, W, 16, args.elastic_steps)), flush=True)
+        return
+    if args.only == "elastic-full":
+        log("elastic 16-shot batch, 2000 steps")
+        fwd = elastic_batch_bench(A, W, 16, 2000)
+        log(json.dumps(fwd))
+        log("elastic gradient, 16 shots of 256^3, 1500 steps")
+        grad = elastic_gradient_bench(A, W, 16, 1500, 50,
+                                      modes=((2, "batched_graph"), (0, "plain_loop")))
+        print(json.dumps({"elastic_16_shots_2000_steps": fwd,
+                          "elastic_gradient_16x256_1500_steps": grad}), flush=True)
         return
     if args.only == "heat":
         bw = stream_bw(A)
@@ -637,18 +710,25 @@ def main():
     r3 = gol_bench(A, 3, args.steps, args.warmup, fused=1, record=rec)
     log(json.dumps(r3))
     cells = 512 ** 3
-    achieved = best["kernel_gbs_alg"]
     log("gol 512^3 R=1 step-by-step (plain)")
     plain = gol_bench(A, 1, args.steps, args.warmup, fused=0)
     log(json.dumps(plain))
-    tr = read_traffic()
-    traffic = None
-    # measured for the same R, tile shape and field form only (the profile
-    # sequence fixes them)
+    tr = read_traffic() or {}
+    # counters of the same R, tile shape and field form (the profile sequence
+    # runs the autotuner's choices); null where they were taken at another one
     form = "packed" if best["packed"] else "bytes"
-    tkey = f"gol_fused_R{best['R']}_tile{best['tile']}_{form}_bytes_per_launch"
-    if tr and tkey in tr:
-        traffic = float(tr[tkey])
+    tpre = f"gol_fused_R{best['R']}_tile{best['tile']}_{form}"
+    traffic = float(tr[tpre + "_bytes_per_launch"]) if tpre + "_bytes_per_launch" in tr else None
+    valu = (float(tr[tpre + "_valu_insts_per_launch"])
+            if tpre + "_valu_insts_per_launch" in tr else None)
+    kern_s = best["kernel_ms"] * 1e-3
+    valu_ach = valu / kern_s / 1e9 if valu else None
+    hbm_ach = traffic / kern_s / 1e9 if traffic else None
+    log("gol 512^3 R=1,2,4 fused with the pack/unpack exchange every window")
+    xruns = []
+    for R in (1, 2, 4):
+        xruns.append(gol_bench(A, R, args.steps, args.warmup, fused=1, record=rec, exchange=True))
+        log(json.dumps(xruns[-1]))
 
     also = {"gol_512_R3_fused": {k: r3[k] for k in ("glups", "kernel_ms", "zc")}}
     heat_ref_grid = None
@@ -681,6 +761,7 @@ def main():
     # checker of the GPU results
     log("cpu baseline")
     cpu, grid_cpu, gens = cpu_gol_baseline(args.cpu_budget)
+    cpu.update(host_cpu())
     acc = {"gol_512_vs_oracle": gol_bitexact_check(A, grid_cpu, gens, best["R"])}
     del grid_cpu
     log(json.dumps(acc))
@@ -717,26 +798,44 @@ def main():
                    "gol_zc": best["zc"], "gol_tile": best["tile"],
                    "fields_bit_packed_between_launches": best["packed"],
                    "glups_by_R": {str(r["R"]): round(r["glups"], 3) for r in runs},
-                   "plain_step_by_step_R1_glups": round(plain["glups"], 3)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                   "plain_step_by_step_R1_glups": round(plain["glups"], 3),
+                   # the same run with the reference's scheme: boundary_updating
+                   # (pack/unpack, byte fields) before every R-generation launch
+                   "with_pack_unpack_exchange": {
+                       "glups_by_R": {str(r["R"]): round(r["glups"], 3) for r in xruns},
+                       "kernel_ms_by_R": {str(r["R"]): round(r["kernel_ms"], 5) for r in xruns},
+                       "window_ms_by_R": {str(r["R"]): round(r["ms"] / (args.steps / r["R"]), 5)
+                                          for r in xruns},
+                       "tile_by_R": {str(r["R"]): r["tile"] for r in xruns}}},
+        # The dominant kernel issues VALU instructions at close to the CU rate
+        # and moves far fewer bytes than HBM could: its bound is the VALU
+        # issue rate.  achieved = SQ_INSTS_VALU of one launch (rocprofv3
+        # counters of this template and grid, profiles/traffic.txt) / the
+        # launch time measured here; traffic = FETCH_SIZE + WRITE_SIZE of
+        # one launch, whose rate against HBM is hbm_frac.
+        "roofline": {"bound": "valu",
+                     "achieved": round(valu_ach, 1) if valu_ach else None,
+                     "peak": VALU_PEAK_GINST, "unit": "Ginst/s",
+                     "frac": round(valu_ach / VALU_PEAK_GINST, 4) if valu_ach else None,
                      "traffic": traffic,
                      "kernel": gol_kernel_name(best["R"], best["tile"], best["packed"]),
                      "kernel_ms": round(best["kernel_ms"], 5),
-                     "bytes_per_launch_algorithmic": 2 * cells * best["gens_per_launch"],
-                     "note": "2 B/LUP x 512^3 cells x R generations per launch (a u8 "
-                             "cell read and written per generation); a fused launch "
-                             "reads and writes each cell once per R generations, and "
-                             "on bit-packed fields as one bit, so frac can exceed 1 "
-                             "and the kernel is bound by its VALU issue rate",
+                     "valu_insts_per_launch": valu,
+                     "hbm_achieved_GBs": round(hbm_ach, 1) if hbm_ach else None,
+                     "hbm_frac": round(hbm_ach / HBM_PEAK_GBS, 4) if hbm_ach else None,
+                     "bytes_per_launch_algorithmic_u8": 2 * cells * best["gens_per_launch"],
+                     "note": "VALU issue peak = 1 wave64 instruction per cycle per CU x "
+                             "256 CUs x 2.4 GHz; the packed fields hold 1 bit per cell, so "
+                             "the u8 algorithmic bytes (2 B per cell-generation) exceed what "
+                             "the launch moves (traffic)",
                      # one generation per launch: the R=1 fused launch, and the
-                     # plain step kernel of the reference's structure
+                     # plain step kernel of the reference's structure, as
+                     # algorithmic u8 bytes against HBM
                      "single_pass_kernel": gol_kernel_name(1, r1["tile"], r1["packed"]),
-                     "single_pass_achieved": round(r1["kernel_gbs_alg"], 1),
-                     "single_pass_frac": round(r1["kernel_gbs_alg"] / HBM_PEAK_GBS, 4),
+                     "single_pass_u8_GBs": round(r1["kernel_gbs_alg"], 1),
                      "plain_step_kernel": "gol_step_kernel",
-                     "plain_step_achieved": round(plain["kernel_gbs_alg"], 1),
-                     "plain_step_frac": round(plain["kernel_gbs_alg"] / HBM_PEAK_GBS, 4)},
+                     "plain_step_u8_GBs": round(plain["kernel_gbs_alg"], 1),
+                     "plain_step_hbm_frac": round(plain["kernel_gbs_alg"] / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
         "accuracy": acc,
         "also": also,

@@ -119,6 +119,12 @@ typedef struct b2r_tuning {
                          0 keep the fields bit-packed (1 bit per cell) between
                          launches, packed at the start of b2r_gol_run and
                          unpacked at its end; 1 byte fields throughout        */
+    int gol_exchange; /* 1: fused Game of Life runs exchange the halos with
+                         b2r_boundary_updating (pack/unpack) before every
+                         R-generation launch, on byte fields, and the kernel
+                         reads its own block's halo (the reference's scheme,
+                         b2r.cpp:179-221); 0: the bit-sliced kernel reads its
+                         halo from the neighbouring blocks, no exchange       */
 } b2r_tuning;
 
 /* ---- library / context ---------------------------------------------------- */
@@ -338,6 +344,13 @@ int b2r_el_gradient_ex(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt,
 int b2r_el_gradient_batch(b2r_el_shot* const* shots, int nshots, const float* const* h_obs,
                           int nsteps, int ckpt, int mode, double* misfit, float* const* h_glam,
                           float* const* h_gmu, float* const* h_gb, float* ms);
+
+/* How b2r_el_gradient_batch (modes 1, 2) would split these shots: the number
+ * of consecutive groups whose checkpoints and adjoint state fit the memory
+ * b2r_tuning.el_grad_mb allows, and the device bytes of the largest group.
+ * No counterpart in the reference. */
+int b2r_el_gradient_plan(b2r_el_shot* const* shots, int nshots, int nsteps, int ckpt,
+                         int* ngroups, size_t* bytes_max_group);
 
 /* ---- measurement helpers ---------------------------------------------------- */
 /* iters device-to-device copies of bytes (a STREAM-copy kernel): *ms total. */

@@ -172,3 +172,46 @@ def test_sync_flag_abi_3x2x2(periodic):
                 want.append(1 if cc > 0 else (-1 if periodic else 0))
                 want.append(1 if cc < n - 1 else (-1 if periodic else 0))
             assert c.sync_flag(rank) == want, rank
+
+
+# ------------------------------------------------------------------ GoL with the exchange
+@pytest.mark.parametrize("R,periodic", [(1, True), (2, True), (4, True), (4, False), (3, True)])
+def test_gol_fused_with_pack_unpack_exchange(R, periodic):
+    # gol_exchange = 1: boundary_updating (pack/unpack) before every
+    # R-generation launch, the bit-sliced kernel reading its own halo
+    env = (64, 40, 48)
+    say("gol exchange", R, periodic)
+    with A.Context(env, (2, 2, 2), R, 1, int(periodic), 1) as c:
+        c.set_tuning(gol_exchange=1)
+        a, b = c.field(), c.field()
+        c.init_bernoulli(a, 1, int(0.3 * 16777216))
+        res, _ = c.gol_run(a, b, 12, fused=1)
+        got = c.download(res)
+    u = O.bernoulli((env[2], env[1], env[0]), 1, 0.3)
+    want = O.gol(u, 12, True) if periodic else O.gol(O.frame(u, 1), 12, False)[1:-1, 1:-1, 1:-1]
+    assert np.array_equal(got, want)
+
+
+def test_el_gradient_plan_groups():
+    # the groups of the batched gradient follow the memory budget
+    env = (24, 20, 16)
+    ctxs, shots = [], []
+    try:
+        for _ in range(4):
+            c = A.Context(env, (1, 1, 1), 1, 4, A.BOUNDARY_FROZEN, 4)
+            s = A.ElasticShot(c, 1e-3, (0.1, 0.1, 0.1))
+            s.set_receivers([[3, 3, 3]], 1, 200)
+            ctxs.append(c)
+            shots.append(s)
+        n, big = A.el_gradient_plan(shots, 200, 10)
+        assert n == 1 and big > 0
+        per_shot = big // 4
+        for c in ctxs:
+            c.set_tuning(el_grad_mb=max(1, (2 * per_shot + (1 << 20) - 1) >> 20))
+        n2, big2 = A.el_gradient_plan(shots, 200, 10)
+        assert n2 >= 2 and big2 <= (max(1, (2 * per_shot + (1 << 20) - 1) >> 20) << 20)
+    finally:
+        for s in shots:
+            s.close()
+        for c in ctxs:
+            c.close()
