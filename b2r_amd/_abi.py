@@ -97,6 +97,7 @@ _SIGS = {
     "b2r_field_init_blobs": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "b2r_boundary_updating": (_i, [_vp, _i, _vp]),
     "b2r_boundary_updating_direct": (_i, [_vp, _i, C.POINTER(_vp), _i]),
+    "b2r_boundary_updating_noalloc": (_i, [_vp, _i, _vp]),
     "b2r_msg_elems": (C.c_longlong, [_vp, _i]),
     "b2r_pack": (_i, [_vp, _i, _vp, _vp]),
     "b2r_unpack": (_i, [_vp, _i, _vp, _vp]),
@@ -286,6 +287,9 @@ class Context:
     # -- exchange ---------------------------------------------------------------
     def boundary_updating(self, f, ts=0):
         check(_lib.b2r_boundary_updating(self.h, ts, f))
+
+    def boundary_updating_noalloc(self, f, ts=0):
+        check(_lib.b2r_boundary_updating_noalloc(self.h, ts, f))
 
     def boundary_updating_direct(self, fields, ts=0):
         arr = (_vp * len(fields))(*fields)

@@ -13,6 +13,7 @@
 
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 static thread_local char g_err[512] = "";
@@ -59,6 +60,32 @@ int b2r_time_end(b2r_ctx* ctx, float* ms)
 }
 
 // ---------------------------------------------------------------------------
+// scratch of b2r_boundary_updating_noalloc: one array per GPU, its address
+// taken once per device (b2r_create) and kept here; the array's name is a
+// host-side stand-in, so host code passes the kept address, never the name
+// ---------------------------------------------------------------------------
+constexpr size_t B2R_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[B2R_SCRATCH_BYTES];
+constexpr int B2R_MAX_DEVICES = 64;
+static void* g_scratch_ptr[B2R_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// With `device` current: its scratch address, taken on the first call.
+static int scratch_setup(int device)
+{
+    B2R_CHECK(device >= 0 && device < B2R_MAX_DEVICES, B2R_E_ARG, "device %d beyond %d", device,
+              B2R_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return B2R_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    B2R_CHECK(e == hipSuccess && ptr, B2R_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s",
+              device, hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return B2R_OK;
+}
+
+// ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
 static long long round_up(long long a, long long m) { return (a + m - 1) / m * m; }
@@ -101,6 +128,7 @@ extern "C" int b2r_create(const b2r_geom* geom, int device, b2r_ctx** out)
     B2R_HIP(hipGetDeviceCount(&ndev));
     B2R_CHECK(device >= 0 && device < ndev, B2R_E_ARG, "device %d of %d", device, ndev);
     B2R_HIP(hipSetDevice(device));
+    if (int rc = scratch_setup(device)) return rc;
 
     b2r_ctx* c = new b2r_ctx();
     c->g = g;
@@ -764,6 +792,40 @@ extern "C" int b2r_boundary_updating(b2r_ctx* ctx, int time_step, void* d_field)
     for (int d = 0; d < (ctx->k.two_d ? 2 : 3); ++d) {
         if (int rc = b2r_pack(ctx, d, d_field, ctx->msgbuf)) return rc;
         if (int rc = b2r_unpack(ctx, d, d_field, ctx->msgbuf)) return rc;
+    }
+    return B2R_OK;
+}
+
+// Bytes of the message buffer of one exchange direction (all blocks, both
+// sides), the largest over the directions the exchange runs.
+static size_t msg_bytes_max(const b2r_ctx* ctx)
+{
+    size_t need = 0;
+    for (int d = 0; d < (ctx->k.two_d ? 2 : 3); ++d) {
+        size_t s = (size_t)msg_elems_k(ctx->k, d) * 2 * ctx->k.nblk * ctx->g.elem_size;
+        need = s > need ? s : need;
+    }
+    return need;
+}
+
+extern "C" int b2r_boundary_updating_noalloc(b2r_ctx* ctx, int time_step, void* d_field)
+{
+    (void)time_step;
+    B2R_CHECK(ctx, B2R_E_ARG, "null context");
+    if (int rc = b2r_check_field(ctx, d_field)) return rc;
+    const size_t need = msg_bytes_max(ctx);
+    B2R_CHECK(need <= B2R_SCRATCH_BYTES, B2R_E_ARG,
+              "boundary_updating_noalloc: the messages of one direction need %zu bytes, more "
+              "than the %zu-byte scratch (use b2r_boundary_updating)",
+              need, B2R_SCRATCH_BYTES);
+    if (int rc = b2r_activate(ctx)) return rc;
+    void* msgs = g_scratch_ptr[ctx->device];
+    B2R_CHECK(msgs, B2R_E_HIP, "no scratch address for device %d", ctx->device);
+    // every message element the unpack reads, the pack of the same direction
+    // has just written: nothing needs clearing
+    for (int d = 0; d < (ctx->k.two_d ? 2 : 3); ++d) {
+        if (int rc = b2r_pack(ctx, d, d_field, msgs)) return rc;
+        if (int rc = b2r_unpack(ctx, d, d_field, msgs)) return rc;
     }
     return B2R_OK;
 }

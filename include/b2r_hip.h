@@ -190,6 +190,16 @@ int b2r_field_init_blobs(b2r_ctx* ctx, void* d_field, int nb, const double* amp,
  * S_X*h*B_Z, Z: [z][y][x] over S_X*S_Y*h).  Replaces boundary_updating
  * (b2r_halo_sync.cu:442-508); time_step only labels the exchange (MPI tag). */
 int b2r_boundary_updating(b2r_ctx* ctx, int time_step, void* d_field);
+/* b2r_boundary_updating without any allocation: the message buffers live in a
+ * fixed 64 MiB device array of the library (one per GPU, its address taken
+ * once per device by b2r_create).  No counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Fails with
+ * B2R_E_ARG when one direction's messages (2 * nblocks * the largest
+ * b2r_msg_elems * elem_size bytes) exceed 64 MiB.  Every call on one GPU
+ * shares the array: calls that overlap on one GPU (other streams, other
+ * contexts, other threads) race.  Result equal to b2r_boundary_updating. */
+int b2r_boundary_updating_noalloc(b2r_ctx* ctx, int time_step, void* d_field);
 /* Same result with each halo cell copied straight from its source block (no
  * message buffers): the device-direct path of boundary_updating_direct
  * (b2r_halo_sync.cu:389-431).  Several fields in one launch per direction. */
