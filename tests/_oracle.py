@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+# B2R_ORACLE_LIB selects another build (the sanitizer build, for one)
+ORACLE_SO = os.environ.get("B2R_ORACLE_LIB") or os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libheat3d_ref.so")
 
 _vp = C.c_void_p

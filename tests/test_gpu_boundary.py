@@ -19,7 +19,7 @@ import _oracle as O
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HOST = os.path.join(ROOT, "b2r_amd", "libb2r_host.so")
+HOST = os.environ.get("B2R_HOST_LIB") or os.path.join(ROOT, "b2r_amd", "libb2r_host.so")
 A = None
 
 

@@ -4,7 +4,7 @@ import os
 import subprocess
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-EXE = os.path.join(ROOT, "b2r_amd", "b2r_main")
+EXE = os.environ.get("B2R_MAIN") or os.path.join(ROOT, "b2r_amd", "b2r_main")
 
 
 def run(*args):

@@ -10,7 +10,7 @@ import pytest
 from b2r_amd import output
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HOST_SO = os.path.join(ROOT, "b2r_amd", "libb2r_host.so")
+HOST_SO = os.environ.get("B2R_HOST_LIB") or os.path.join(ROOT, "b2r_amd", "libb2r_host.so")
 
 
 @pytest.fixture(scope="module")

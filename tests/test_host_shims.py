@@ -16,7 +16,7 @@ from conftest import has_gpu
 from b2r_amd import _abi as A
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HOST = os.path.join(ROOT, "b2r_amd", "libb2r_host.so")
+HOST = os.environ.get("B2R_HOST_LIB") or os.path.join(ROOT, "b2r_amd", "libb2r_host.so")
 
 
 def host_lib():
