@@ -849,6 +849,23 @@ int b2r_gol_fused_packed(const b2r_ctx* ctx)
     return b2r_gol_fused_reads_halo(ctx) && !ctx->tune.gol_bytes;
 }
 
+// gens (1..4, at most R) generations of the bit-sliced kernel on packed
+// fields.  A run's last, partial window (gens < R) runs here as well: the
+// halo it needs (gens cells deep) is pulled from the neighbours' owned cells
+// as in a whole window, and its update boxes, those of the window's first
+// stages, are the widest ones.
+static int gol_bits_packed_launch(b2r_ctx* ctx, const void* in, void* out, int gens)
+{
+    switch (gens) {
+    case 1: return gol_bits_launch_r<1, true>(ctx, in, out);
+    case 2: return gol_bits_launch_r<2, true>(ctx, in, out);
+    case 3: return gol_bits_launch_r<3, true>(ctx, in, out);
+    case 4: return gol_bits_launch_r<4, true>(ctx, in, out);
+    }
+    b2r_set_error("bit-sliced Game of Life runs 1..4 generations per launch (got %d)", gens);
+    return B2R_E_UNSUPPORTED;
+}
+
 int b2r_gol_pack_pair(b2r_ctx* ctx, const void* from, void** p0, void** p1)
 {
     B2R_CHECK(b2r_gol_fused_packed(ctx), B2R_E_UNSUPPORTED,
@@ -959,20 +976,23 @@ extern "C" int b2r_gol_run(b2r_ctx* ctx, void* d_a, void* d_b, int nsteps, int f
     const int R = ctx->g.R;
     const bool use_fused = fused && gol_fused_ok(ctx, R);
     int s = 0;
-    if (use_fused && nsteps >= R && b2r_gol_fused_packed(ctx)) {
-        // whole windows on bit-packed copies of the field: packed once here,
-        // unpacked into the field holding the result after the last window
+    if (use_fused && nsteps >= 1 && b2r_gol_fused_packed(ctx)) {
+        // windows on bit-packed copies of the field: packed once here, the
+        // whole windows, then the remaining nsteps mod R generations in one
+        // more launch, unpacked into the field holding the result at the end
         void* pk[2];
         if (int rc = b2r_gol_pack_pair(ctx, cur, &pk[0], &pk[1])) return rc;
-        const int nwin = nsteps / R;
-        for (int w = 0; w < nwin; ++w) {
-            if (int rc = b2r_gol_fused_launch(ctx, pk[w & 1], pk[(w + 1) & 1], 1)) return rc;
+        const int nwin = nsteps / R, tail = nsteps - nwin * R;
+        const int nlaunch = nwin + (tail > 0);
+        for (int w = 0; w < nlaunch; ++w) {
+            const int gens = w < nwin ? R : tail;
+            if (int rc = gol_bits_packed_launch(ctx, pk[w & 1], pk[(w + 1) & 1], gens)) return rc;
             void* t = cur;
             cur = nxt;
             nxt = t;
         }
-        if (int rc = b2r_gol_unpack(ctx, pk[nwin & 1], cur)) return rc;
-        s = nwin * R;
+        if (int rc = b2r_gol_unpack(ctx, pk[nlaunch & 1], cur)) return rc;
+        s = nsteps;
     }
     for (; s < nsteps; s += R) {
         const int n = nsteps - s < R ? nsteps - s : R;

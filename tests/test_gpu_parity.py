@@ -321,9 +321,10 @@ def test_gol_bits_halo_from_neighbours(env, blocks, periodic, R, tile, gol_bytes
     # ordered exchange fills it, and no exchange runs between its launches;
     # the byte-lane kernel (tile 3) runs with the exchange.  Single-block axes
     # wrap onto themselves when periodic; frozen faces keep their own halo.
-    # gol_bytes 0 runs the bit-sliced windows on bit-packed fields, then the
-    # last generation (3R + 1 is not a whole number of windows) step by step
-    # on the unpacked byte field.
+    # gol_bytes 0 runs the bit-sliced windows on bit-packed fields, the last
+    # generation (3R + 1 is not a whole number of windows) as a one-generation
+    # launch of the same kernel, still packed; gol_bytes 1 runs that
+    # generation step by step on the byte field after an exchange.
     if tile == 3 and gol_bytes:
         pytest.skip("the byte-lane kernel has no packed form")
     gens = 3 * R + 1
