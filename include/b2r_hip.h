@@ -275,10 +275,11 @@ int b2r_gol_step(b2r_ctx* ctx, const void* d_in, void* d_out, int b2r_i);
  * frozen; R <= 4) one R-generation launch per window.  Where the bit-sliced
  * kernel runs (B_x % 32 == 0, gol_tile 0 or 4-10) no exchange runs: it reads
  * its halo from the neighbouring blocks, and unless gol_bytes is set the
- * whole windows run on the context's two bit-packed fields (allocated on the
- * first such call), the result's owned cells then written back to the byte
- * field.  *result_in_b says which field holds the result; *ms, when not NULL,
- * the device time between events around the loop. */
+ * windows run on the context's two bit-packed fields (allocated on the first
+ * such call), nsteps mod R generations at the end as one more, shorter
+ * launch, the result's owned cells then written back to the byte field.
+ * *result_in_b says which field holds the result; *ms, when not NULL, the
+ * device time between events around the loop. */
 int b2r_gol_run(b2r_ctx* ctx, void* d_a, void* d_b, int nsteps, int fused,
                 int* result_in_b, float* ms);
 
