@@ -10,7 +10,8 @@
 // each thread owns RY consecutive rows of one x column and marches z through a
 // chunk of planes.  The z neighbours live in a register queue of 2D+1 values per
 // row; the current plane, with a D-wide x/y halo, is staged in LDS so the x and y
-// neighbours are LDS reads.  Loads of the z-front are coalesced 256-byte rows.
+// neighbours are LDS reads (two LDS planes alternate, one barrier per plane).
+// Loads of the z-front are coalesced 256-byte rows.
 //
 // Arithmetic (compiled without FP contraction, so CPU and GPU round alike):
 //   F32: u' = u + ((kx*Lx + ky*Ly) + kz*Lz)
@@ -33,7 +34,10 @@ __global__ __launch_bounds__(HEAT_TX * WX * WY) void heat_step_kernel(
     constexpr int NHY = 2 * D * TX, NHX = 2 * D * TY, NH = NHY + NHX;
     constexpr int NSLOT = (NH + NT - 1) / NT;
     constexpr int Q = 2 * D + 1;
-    __shared__ float lds[LY * LX];
+    // two staged planes, alternating: the plane written at z was last read at
+    // z-2, before every thread passed the barrier of z-1, so one barrier per
+    // plane separates the writes from the reads
+    __shared__ float lds2[2][LY * LX];
 
     const int lane = threadIdx.x, wy = threadIdx.y, tid = wy * TX + lane;
     int t = xcd_tile(blockIdx.x, gridDim.x);
@@ -128,11 +132,11 @@ __global__ __launch_bounds__(HEAT_TX * WX * WY) void heat_step_kernel(
         for (int k = 0; k <= HA; ++k) hq[s][k] = hok[s] ? ldo(src, (D + k) * sz + ho[s]) : 0.0f;
 #pragma unroll
     for (int j = 0; j < RY; ++j) qn[j] = (xin && yin[j]) ? ldo(src, 2 * D * sz + yo[j]) : 0.0f;
-    uint32_t pz = D * sz;  // plane z
-    for (int z = zlo; z <= zhi; ++z, pz += sz) {
+    // one plane of the march; lds is lds2[0] or lds2[1] as a constant of each
+    // call, so the LDS addresses stay loop-invariant
+    auto plane = [&](const int z, const uint32_t pz, float* const lds) {
 #pragma unroll
         for (int j = 0; j < RY; ++j) q[j][2 * D] = qn[j];
-        __syncthreads();
 #pragma unroll
         for (int j = 0; j < RY; ++j) lds[(wy * RY + j + D) * LX + lane + D] = q[j][D];
 #pragma unroll
@@ -175,7 +179,15 @@ __global__ __launch_bounds__(HEAT_TX * WX * WY) void heat_step_kernel(
         for (int j = 0; j < RY; ++j)
 #pragma unroll
             for (int p = 0; p < 2 * D; ++p) q[j][p] = q[j][p + 1];
+    };
+    // planes in pairs (the buffer of plane z is (z - zlo) & 1), an odd one last
+    int z = zlo;
+    uint32_t pz = D * sz;  // byte offset of plane z
+    for (; z + 1 <= zhi; z += 2, pz += 2 * sz) {
+        plane(z, pz, lds2[0]);
+        plane(z + 1, pz + sz, lds2[1]);
     }
+    if (z <= zhi) plane(z, pz, lds2[0]);
 }
 
 int b2r_heat_check(b2r_ctx* ctx, const b2r_heat_params* p)
