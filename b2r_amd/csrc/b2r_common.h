@@ -97,10 +97,9 @@ int b2r_gol_fused_reads_halo(const b2r_ctx* ctx);
 // 1: fused Game of Life runs keep their fields bit-packed between launches
 int b2r_gol_fused_packed(const b2r_ctx* ctx);
 // The context's two packed fields (allocated on first use), both packed from
-// the byte field `from`, halo included; b2r_gol_unpack writes the owned cells
-// of a packed field into a byte field.
+// the byte field `from`, halo included (from = NULL: allocated only, contents
+// unspecified).
 int b2r_gol_pack_pair(b2r_ctx* ctx, const void* from, void** p0, void** p1);
-int b2r_gol_unpack(b2r_ctx* ctx, const void* packed, void* to);
 int b2r_heat_fused_launch(b2r_ctx* ctx, const void* in, void* out, const b2r_heat_params* p);
 
 // ---------------------------------------------------------------------------

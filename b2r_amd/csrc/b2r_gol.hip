@@ -400,22 +400,27 @@ __device__ __forceinline__ uint32_t word_mask(int x, int lo, int hi)
     return upto & ~((1u << a) - 1u);
 }
 
-// Packed fields (PK): a block's row is PW = B_x/32 + 2 words, word w holding
+// Packed fields: a block's row is PW = B_x/32 + 2 words, word w holding
 // cells x = h-32+32w .. h+32w-1 (bit i = cell x+i), planes of S_y rows, blocks
 // of S_z planes; the kernel then loads and stores words instead of packing
-// and unpacking byte cells.  PGeo carries the byte strides of either form.
+// and unpacking byte cells.  PGeo carries the byte strides of the packed form.
+// IO bit 0: the input is packed (else byte cells); bit 1: the output is
+// packed.  IO 3 runs between packed fields, IO 0 between byte fields; IO 2
+// packs the byte input as it loads it (a run's first window) and IO 1
+// unpacks as it stores (its last), saving the separate pack and unpack passes.
 struct PGeo {
     long long rb, sb, bb;  // bytes per row, plane and block
-    int pw;                // words per packed row (PK only)
+    int pw;                // words per packed row
 };
 
-template <int R, int NW, int NR, bool PK = false, int RT = 1>
+template <int R, int NW, int NR, int IO = 0, int RT = 1>
 __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint8_t* __restrict__ in,
                                                                       uint8_t* __restrict__ out,
                                                                       KGeo g, int ntx, int nty,
                                                                       int nzc, int zc, int pull,
                                                                       PGeo pg)
 {
+    constexpr bool PKI = (IO & 1) != 0, PKO = (IO & 2) != 0;
     constexpr int NQ = NW + 2;  // words per row: plus one per side
     constexpr int NT = NQ * NR;
     constexpr int TY = NR - 2 * R;
@@ -448,13 +453,15 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
     const int y0 = h + tyi * TY;
     const int zc0 = h + zci * zc;
     const int zc1 = min(zc0 + zc, h + g.B[2]);
-    // byte strides and the byte offset of the word at cell xx within a row
-    const long long pitch = PK ? pg.rb : g.pitch, slice = PK ? pg.sb : g.slice;
-    const long long bstr = PK ? pg.bb : g.bstride, xo = PK ? 0 : g.xoff;
-    auto xbyte = [&](int xx) -> long long { return PK ? 4ll * ((xx - h) / 32 + 1) : xx; };
-    const long long boff = (long long)b * bstr + xo;
-    const uint8_t* __restrict__ src = in + boff;
-    uint8_t* __restrict__ dst = out + boff;
+    // byte strides and the byte offset of the word at cell xx within a row,
+    // of the input and of the output form
+    const long long pitch = PKI ? pg.rb : g.pitch, slice = PKI ? pg.sb : g.slice;
+    const long long bstr = PKI ? pg.bb : g.bstride, xo = PKI ? 0 : g.xoff;
+    auto xbyte = [&](int xx) -> long long { return PKI ? 4ll * ((xx - h) / 32 + 1) : xx; };
+    const long long pitch_o = PKO ? pg.rb : g.pitch, slice_o = PKO ? pg.sb : g.slice;
+    const long long boff_o = (long long)b * (PKO ? pg.bb : g.bstride) + (PKO ? 0 : g.xoff);
+    const uint8_t* __restrict__ src = in + (long long)b * bstr + xo;
+    uint8_t* __restrict__ dst = out + boff_o;
 
     // this thread's words: x = x .. x+31, rows y[j]
     const int x = x0 + 32 * (q - 1);
@@ -465,7 +472,7 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
 #pragma unroll
     for (int j = 0; j < RT; ++j) {
         y[j] = y0 - R + tid / NQ + j * (NR / RT);
-        ld_ok[j] = PK ? y[j] < g.S[1] && (x - h) / 32 + 1 < pg.pw
+        ld_ok[j] = PKI ? y[j] < g.S[1] && (x - h) / 32 + 1 < pg.pw
                       : y[j] < g.S[1] && x + 31 >= 0 && x < g.S[0];
         // every thread loads (no branch): threads outside the block read the
         // tile's first word of the last row, in bounds and aligned, and use zero
@@ -521,7 +528,7 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
     // a halo word needs only its 16 cells next to the tile (R <= 4), and the
     // second half of a row's last word may lie past the padded row end: such a
     // half reloads the word's other half (no extra traffic) and is masked off
-    const bool hi_in = x + 32 <= pitch - g.xoff;  // second half inside the row
+    const bool hi_in = x + 32 <= g.pitch - g.xoff;  // second half inside the byte row
     uint32_t ldm[RT], ma[RT], mb[RT];
     int oa[RT], ob[RT];
     bool out_ok[RT];
@@ -569,7 +576,7 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
             uint4& lb = ld[j][1];
             gen[j][0][0] = gen[j][0][1];
             gen[j][0][1] = gen[j][0][2];
-            if (PK)
+            if (PKI)
                 gen[j][0][2] = la.x & ldm[j];
             else
                 gen[j][0][2] = gb_pack(make_uint4(la.x & ma[j], la.y & ma[j], la.z & ma[j], la.w & ma[j]),
@@ -578,7 +585,7 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
             // the load GF_PD planes ahead (the last ones are unused)
             const int zz = zp(min(k + GF_PD, nin - 1));
             const uint8_t* pp = colp(j, zz) + (long long)zz * slice;
-            if (PK) {
+            if (PKI) {
                 la.x = *reinterpret_cast<const uint32_t*>(pp);
             } else {
                 la = *reinterpret_cast<const uint4*>(pp + oa[j]);
@@ -648,14 +655,15 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
                     gen[j][t][1] = gen[j][t][2];
                     gen[j][t][2] = nw;
                 } else if (out_ok[j]) {
-                    if (PK) {
-                        *reinterpret_cast<uint32_t*>(dst + (long long)p * slice +
-                                                     (long long)y[j] * pitch + xbyte(x)) = nw;
+                    if (PKO) {
+                        *reinterpret_cast<uint32_t*>(dst + (long long)p * slice_o +
+                                                     (long long)y[j] * pitch_o +
+                                                     4ll * ((x - h) / 32 + 1)) = nw;
                     } else {
                         uint4 lo, hi;
                         gb_unpack(nw, lo, hi);
-                        uint4* o = reinterpret_cast<uint4*>(dst + (long long)p * slice +
-                                                            (long long)y[j] * pitch + x);
+                        uint4* o = reinterpret_cast<uint4*>(dst + (long long)p * slice_o +
+                                                            (long long)y[j] * pitch_o + x);
                         o[0] = lo;
                         if (hi_in) o[1] = hi;
                     }
@@ -670,7 +678,7 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
 #pragma unroll
         for (int j = 0; j < RT; ++j) {
             const uint8_t* pp = colp(j, zz) + (long long)zz * slice;
-            if (PK) {
+            if (PKI) {
                 sl[k][j][0].x = *reinterpret_cast<const uint32_t*>(pp);
             } else {
                 sl[k][j][0] = *reinterpret_cast<const uint4*>(pp + oa[j]);
@@ -714,28 +722,6 @@ __global__ __launch_bounds__(256) void gol_pack_kernel(const uint8_t* __restrict
     out1[i] = word;
 }
 
-// Packed field -> byte field: the owned cells of every block.
-__global__ __launch_bounds__(256) void gol_unpack_kernel(const uint32_t* __restrict__ in,
-                                                         uint8_t* __restrict__ out, KGeo g, int pw)
-{
-    const int nw = g.B[0] / 32;
-    const long long rows = (long long)g.nblk * g.B[2] * g.B[1];
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rows * nw) return;
-    const int w = (int)(i % nw) + 1;
-    const long long row = i / nw;  // (block, owned z, owned y)
-    const int y = g.h + (int)(row % g.B[1]);
-    const long long bz = row / g.B[1];
-    const int z = g.h + (int)(bz % g.B[2]);
-    const int b = (int)(bz / g.B[2]);
-    const uint32_t word = in[(((long long)b * g.S[2] + z) * g.S[1] + y) * pw + w];
-    uint4 lo, hi;
-    gb_unpack(word, lo, hi);
-    uint4* o = reinterpret_cast<uint4*>(out + (long long)b * g.bstride + g.xoff + (long long)z * g.slice +
-                                        (long long)y * g.pitch + g.h + 32 * (w - 1));
-    o[0] = lo;
-    o[1] = hi;
-}
 
 // The bit-sliced kernel reads its halo from the neighbouring blocks (pull
 // above) when the block's owned x range is a whole number of 32-cell words.
@@ -789,7 +775,7 @@ static PGeo gol_pgeo(const KGeo& g)
     return pg;
 }
 
-template <int R, int NW, int NR, int RT, bool PK>
+template <int R, int NW, int NR, int RT, int IO>
 static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
 {
     constexpr int TY = NR - 2 * R;
@@ -808,18 +794,18 @@ static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
     if (zc > g.B[2]) zc = g.B[2];
     const int nzc = ceil_div(g.B[2], zc);
     dim3 grid(ntx * nty * nzc * g.nblk);
-    gol_bits_kernel<R, NW, NR, PK, RT><<<grid, (NW + 2) * NR / RT, 0, ctx->stream>>>(
+    gol_bits_kernel<R, NW, NR, IO, RT><<<grid, (NW + 2) * NR / RT, 0, ctx->stream>>>(
         (const uint8_t*)in, (uint8_t*)out, g, ntx, nty, nzc, zc, pull,
-        PK ? gol_pgeo(g) : PGeo{});
+        IO ? gol_pgeo(g) : PGeo{});
     B2R_LAUNCH_CHECK();
     return B2R_OK;
 }
 
-template <int R, bool PK>
+template <int R, int IO>
 static int gol_bits_launch_r(b2r_ctx* ctx, const void* in, void* out)
 {
 #define GB_LAUNCH(i) \
-    gol_bits_launch_t<R, GB_SHAPES[i][0], GB_SHAPES[i][1], GB_SHAPES[i][2], PK>(ctx, in, out)
+    gol_bits_launch_t<R, GB_SHAPES[i][0], GB_SHAPES[i][1], GB_SHAPES[i][2], IO>(ctx, in, out)
     switch (ctx->tune.gol_tile) {
     case 4: return GB_LAUNCH(0);
     case 6: return GB_LAUNCH(2);
@@ -835,12 +821,12 @@ static int gol_bits_launch_r(b2r_ctx* ctx, const void* in, void* out)
 template <int R>
 static int gol_fused_launch_r(b2r_ctx* ctx, const void* in, void* out, bool packed)
 {
-    if (packed) return gol_bits_launch_r<R, true>(ctx, in, out);
+    if (packed) return gol_bits_launch_r<R, 3>(ctx, in, out);
     switch (ctx->tune.gol_tile) {
     case 1: return gol_fused_launch_t<R, GF_SHAPES[0][0], GF_SHAPES[0][1]>(ctx, in, out);
     case 2: return gol_fused_launch_t<R, GF_SHAPES[1][0], GF_SHAPES[1][1]>(ctx, in, out);
     case 3: return gol_fused_launch_t<R, GF_SHAPES[2][0], GF_SHAPES[2][1]>(ctx, in, out);
-    default: return gol_bits_launch_r<R, false>(ctx, in, out);
+    default: return gol_bits_launch_r<R, 0>(ctx, in, out);
     }
 }
 
@@ -849,18 +835,29 @@ int b2r_gol_fused_packed(const b2r_ctx* ctx)
     return b2r_gol_fused_reads_halo(ctx) && !ctx->tune.gol_bytes;
 }
 
-// gens (1..4, at most R) generations of the bit-sliced kernel on packed
-// fields.  A run's last, partial window (gens < R) runs here as well: the
+// gens (1..4, at most R) generations of the bit-sliced kernel, io as the
+// kernel's IO.  A run's last, partial window (gens < R) runs here as well: the
 // halo it needs (gens cells deep) is pulled from the neighbours' owned cells
 // as in a whole window, and its update boxes, those of the window's first
 // stages, are the widest ones.
-static int gol_bits_packed_launch(b2r_ctx* ctx, const void* in, void* out, int gens)
+template <int R>
+static int gol_bits_io_launch_r(b2r_ctx* ctx, const void* in, void* out, int io)
+{
+    switch (io) {
+    case 0: return gol_bits_launch_r<R, 0>(ctx, in, out);
+    case 1: return gol_bits_launch_r<R, 1>(ctx, in, out);
+    case 2: return gol_bits_launch_r<R, 2>(ctx, in, out);
+    default: return gol_bits_launch_r<R, 3>(ctx, in, out);
+    }
+}
+
+static int gol_bits_packed_launch(b2r_ctx* ctx, const void* in, void* out, int gens, int io)
 {
     switch (gens) {
-    case 1: return gol_bits_launch_r<1, true>(ctx, in, out);
-    case 2: return gol_bits_launch_r<2, true>(ctx, in, out);
-    case 3: return gol_bits_launch_r<3, true>(ctx, in, out);
-    case 4: return gol_bits_launch_r<4, true>(ctx, in, out);
+    case 1: return gol_bits_io_launch_r<1>(ctx, in, out, io);
+    case 2: return gol_bits_io_launch_r<2>(ctx, in, out, io);
+    case 3: return gol_bits_io_launch_r<3>(ctx, in, out, io);
+    case 4: return gol_bits_io_launch_r<4>(ctx, in, out, io);
     }
     b2r_set_error("bit-sliced Game of Life runs 1..4 generations per launch (got %d)", gens);
     return B2R_E_UNSUPPORTED;
@@ -884,19 +881,10 @@ int b2r_gol_pack_pair(b2r_ctx* ctx, const void* from, void** p0, void** p1)
     }
     *p0 = ctx->gol_pk;
     *p1 = (uint8_t*)ctx->gol_pk + half;
+    if (!from) return B2R_OK;  // the pair only, contents left as they are
     const long long words = (long long)g.nblk * g.S[2] * g.S[1] * pg.pw;
     gol_pack_kernel<<<(unsigned)ceil_div(words, 256), 256, 0, ctx->stream>>>(
         (const uint8_t*)from, (uint32_t*)*p0, (uint32_t*)*p1, g, pg.pw);
-    B2R_LAUNCH_CHECK();
-    return B2R_OK;
-}
-
-int b2r_gol_unpack(b2r_ctx* ctx, const void* packed, void* to)
-{
-    const KGeo& g = ctx->k;
-    const long long words = (long long)g.nblk * g.B[2] * g.B[1] * (g.B[0] / 32);
-    gol_unpack_kernel<<<(unsigned)ceil_div(words, 256), 256, 0, ctx->stream>>>(
-        (const uint32_t*)packed, (uint8_t*)to, g, gol_pk_words(g));
     B2R_LAUNCH_CHECK();
     return B2R_OK;
 }
@@ -977,21 +965,31 @@ extern "C" int b2r_gol_run(b2r_ctx* ctx, void* d_a, void* d_b, int nsteps, int f
     const bool use_fused = fused && gol_fused_ok(ctx, R);
     int s = 0;
     if (use_fused && nsteps >= 1 && b2r_gol_fused_packed(ctx)) {
-        // windows on bit-packed copies of the field: packed once here, the
-        // whole windows, then the remaining nsteps mod R generations in one
-        // more launch, unpacked into the field holding the result at the end
-        void* pk[2];
-        if (int rc = b2r_gol_pack_pair(ctx, cur, &pk[0], &pk[1])) return rc;
+        // The windows run on bit-packed copies of the field: the whole
+        // windows, then the remaining nsteps mod R generations in one more
+        // launch.  The last launch unpacks as it stores, into the byte field
+        // that holds the result.  Where no face is frozen the first launch
+        // packs its byte input as it loads it (then no packed halo word is
+        // ever read: each comes from the neighbour that owns it); otherwise a
+        // pack pass fills both packed fields first, the frozen frame included.
         const int nwin = nsteps / R, tail = nsteps - nwin * R;
         const int nlaunch = nwin + (tail > 0);
+        const bool pack_in_first = ctx->k.fmask == 0;
+        void* const result = (nlaunch & 1) ? d_b : d_a;
+        void* pk[2];
+        if (int rc = b2r_gol_pack_pair(ctx, pack_in_first ? nullptr : cur, &pk[0], &pk[1]))
+            return rc;
         for (int w = 0; w < nlaunch; ++w) {
             const int gens = w < nwin ? R : tail;
-            if (int rc = gol_bits_packed_launch(ctx, pk[w & 1], pk[(w + 1) & 1], gens)) return rc;
+            const bool bytes_in = w == 0 && pack_in_first, bytes_out = w == nlaunch - 1;
+            const int io = (bytes_in ? 0 : 1) | (bytes_out ? 0 : 2);
+            const void* in = bytes_in ? d_a : pk[w & 1];
+            void* out = bytes_out ? result : pk[(w + 1) & 1];
+            if (int rc = gol_bits_packed_launch(ctx, in, out, gens, io)) return rc;
             void* t = cur;
             cur = nxt;
             nxt = t;
         }
-        if (int rc = b2r_gol_unpack(ctx, pk[nlaunch & 1], cur)) return rc;
         s = nsteps;
     }
     for (; s < nsteps; s += R) {

@@ -376,6 +376,29 @@ def test_gol_packed_runs_in_sequence(periodic):
     assert all(np.array_equal(p, q) for p, q in zip(before, after))
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("R,gens", [(4, 1), (4, 3), (4, 4), (4, 5), (2, 3), (1, 1), (1, 2)])
+def test_gol_packed_first_and_last_launch(periodic, R, gens):
+    # Runs of one and two launches on packed fields: the last launch unpacks
+    # as it stores; with no frozen face the first packs as it loads (a single
+    # launch then goes from byte field to byte field), with frozen faces a
+    # pack pass fills the packed frame first.
+    env, blocks = (128, 40, 36), (2, 2, 1)
+    u = O.bernoulli((env[2], env[1], env[0]), 21, 0.3)
+    say("gol packed first/last launch, R", R, "gens", gens, "periodic", periodic)
+    with A.Context(env, blocks, R, 1, 1 if periodic else 0, 1) as c:
+        c.set_tuning(gol_tile=8, gol_zc=0, gol_bytes=0)
+        a, b = c.field(), c.field()
+        c.upload(a, u)
+        res, _ = c.gol_run(a, b, gens, fused=1)
+        got = c.download(res)
+    if periodic:
+        want = O.gol(u, gens, True)
+    else:
+        want = O.unframe(O.gol(O.frame(u, 1), gens, False), 1)
+    assert np.array_equal(got, want)
+
+
 @pytest.mark.parametrize("wy,wx", [(4, 1), (4, 2), (4, 4), (8, 1), (8, 2), (16, 1)])
 @pytest.mark.parametrize("ry", [1, 2, 4, 8])
 @pytest.mark.parametrize("zc", [0, 5])
