@@ -149,10 +149,12 @@ GOL_BIT_TILES = {0: (8, 64, 1), 4: (8, 32, 1), 5: (8, 64, 1), 6: (8, 96, 1), 7: 
 
 
 def gol_kernel_name(R, tile, packed=True):
+    """The template of the run's middle launches: IO 3 packed to packed (the
+    first launch of a run packs as it loads, IO 2, the last unpacks as it
+    stores, IO 1), IO 0 on byte fields."""
     if tile in GOL_BIT_TILES:
         nw, nr, rt = GOL_BIT_TILES[tile]
-        return "gol_bits_kernel<%d, %d, %d, %s, %d>" % (R, nw, nr, "true" if packed else "false",
-                                                        rt)
+        return "gol_bits_kernel<%d, %d, %d, %d, %d>" % (R, nw, nr, 3 if packed else 0, rt)
     return "gol_fused_kernel<%d, %d, %d>" % ((R,) + GOL_TILES[tile])
 
 
