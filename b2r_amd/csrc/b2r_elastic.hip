@@ -668,9 +668,9 @@ __global__ void el_tick_kernel(int* tcount, int n, int delta)
 }
 
 // ---------------------------------------------------------------- adjoint
-// Adjoint of one forward step (single block, all global faces frozen, owned
-// cells = update region).  With E the strain-rate operator and M the force
-// divergence (M = -E^T on the zero-framed grid, D^T = -D), a step is
+// Adjoint of one forward step at R = 1, every global face frozen or periodic.
+// With E the strain-rate operator and M the force divergence (M = -E^T on the
+// zero-framed or the periodic grid, D^T = -D), a step is
 //   sigma' = sigma + dt C E v,   v' = v + dt B M sigma'      (B = diag(b))
 // and its transpose, in the variables w = B vh and p = C sh (vh, sh the
 // adjoints of v and sigma), is the same scheme run with -dt:
@@ -688,6 +688,13 @@ __global__ void el_tick_kernel(int* tcount, int n, int delta)
 //     v = v' - dt b M sigma' - dt b f(t) at the sources, then
 //     sigma = sigma' - dt C E v.  Nothing is stored between checkpoints; the
 //     checkpoint reset every ckpt steps bounds the rounding drift.
+// The kernels update the boxes of the forward's stages: the velocity stage's
+// gradient and reconstruction run on the stress stage's box (the owned cells
+// and, toward neighbouring blocks, a 2-cell ring, whose rebuilt velocities the
+// stress stage's stencil reads), the stress stage's on the velocity stage's
+// (the owned cells); gradients are accumulated on owned cells only.  With
+// several blocks or periodic faces the reverse sweep exchanges the state and
+// the adjoint before every step, as the forward exchanges before it.
 // Both kernels write cell-local values only and read with a stencil only
 // fields they do not write, so they run in place; z neighbours come from
 // register queues rounded exactly as direct loads.
@@ -704,10 +711,15 @@ __global__ __launch_bounds__(EL_WG) void el_adj_vel_kernel(const ElDev* __restri
     if (T.b >= d.g.nblk) return;  // padding tile (uniform)
     const KGeo& g = d.g;
     const int hh = g.h, x = T.x, y = T.y;
-    const int z0 = max(T.zs, hh), z1 = min(T.ze, hh + g.B[2]) - 1;
+    // the box of the forward's stress stage: the owned cells and a 2-cell ring
+    // toward neighbouring blocks (where REV rebuilds the velocities that the
+    // stress stage's stencil reads); the gradient takes owned cells only
+    const Box box = update_box(g, T.b, 2, 0);
+    const int z0 = max(T.zs, box.lo[2]), z1 = min(T.ze - 1, box.hi[2]);
     if (z0 > z1) return;  // uniform
-    const bool upd = x >= hh && x < hh + g.B[0] && y >= hh && y < hh + g.B[1];
-    const long long boff = g.xoff;
+    const bool upd = x >= box.lo[0] && x <= box.hi[0] && y >= box.lo[1] && y <= box.hi[1];
+    const bool own_xy = x >= hh && x < hh + g.B[0] && y >= hh && y < hh + g.B[1];
+    const long long boff = (long long)T.b * g.bstride + g.xoff;
     const uint32_t sy = 4u * g.pitch, sz = 4u * (uint32_t)g.slice;  // byte strides
     const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
     float* const* sg = REV ? d.f + F_SXX : d.fw1 + F_SXX;
@@ -754,8 +766,9 @@ __global__ __launch_bounds__(EL_WG) void el_adj_vel_kernel(const ElDev* __restri
             const float ay = (fdx(sv[2], h, i0) + fdy(sv[1], h, i1)) + fdq(qb, i2);
             const float az = (fdx(sv[3], h, i0) + fdy(sv[4], h, i1)) + fdq(qc, i2);
             const float bc = ldo(b, c);
-            sto(gb, c,
-                ldo(gb, c) + dt * ((ldo(wx, c) * ax + ldo(wy, c) * ay + ldo(wz, c) * az) / bc));
+            if (own_xy && z >= hh && z < hh + g.B[2])
+                sto(gb, c,
+                    ldo(gb, c) + dt * ((ldo(wx, c) * ax + ldo(wy, c) * ay + ldo(wz, c) * az) / bc));
             if (REV) {
                 const float tb = dt * bc;
                 sto(vx, c, ldo(vx, c) - tb * ax);
@@ -780,11 +793,13 @@ __global__ __launch_bounds__(EL_WG) void el_adj_str_kernel(const ElDev* __restri
     const ElDev& d = tab[T.shot];
     if (T.b >= d.g.nblk) return;  // padding tile (uniform)
     const KGeo& g = d.g;
-    const int hh = g.h, x = T.x, y = T.y;
-    const int z0 = max(T.zs, hh), z1 = min(T.ze, hh + g.B[2]) - 1;
+    const int x = T.x, y = T.y;
+    // the box of the forward's velocity stage: at R = 1 the owned cells
+    const Box box = update_box(g, T.b, 2, 1);
+    const int z0 = max(T.zs, box.lo[2]), z1 = min(T.ze - 1, box.hi[2]);
     if (z0 > z1) return;  // uniform
-    const bool upd = x >= hh && x < hh + g.B[0] && y >= hh && y < hh + g.B[1];
-    const long long boff = g.xoff;
+    const bool upd = x >= box.lo[0] && x <= box.hi[0] && y >= box.lo[1] && y <= box.hi[1];
+    const long long boff = (long long)T.b * g.bstride + g.xoff;
     const uint32_t sy = 4u * g.pitch, sz = 4u * (uint32_t)g.slice;  // byte strides
     const float dt = d.dt, i0 = d.ih[0], i1 = d.ih[1], i2 = d.ih[2];
     float* const* vv = REV ? d.f : d.fw0;
@@ -864,6 +879,19 @@ __global__ __launch_bounds__(EL_WG) void el_adj_str_kernel(const ElDev* __restri
 // phase 0: receivers feed the adjoint, vh += res, i.e. w += b res;
 // phase 1: sources feed g_b += dt (w/b) f(t) and, REV, leave the state
 // (v -= dt b f(t)).
+// Each point is applied in the block that owns it; the exchange that follows
+// refreshes the halo copies elsewhere.
+__device__ __forceinline__ long long el_owner_cell(const KGeo& g, const int* p)
+{
+    int bc[3], lc[3];
+    for (int a = 0; a < 3; ++a) {
+        bc[a] = p[a] / g.B[a];
+        lc[a] = p[a] - bc[a] * g.B[a] + g.h;
+    }
+    const int b = (bc[2] * g.nb[1] + bc[1]) * g.nb[0] + bc[0];
+    return cell_index(g, b, lc[0], lc[1], lc[2]);
+}
+
 __global__ void el_adj_point_kernel(const ElDev* __restrict__ tab, const int* __restrict__ tcount,
                                     int phase, int rev)
 {
@@ -874,14 +902,13 @@ __global__ void el_adj_point_kernel(const ElDev* __restrict__ tab, const int* __
     const KGeo& g = d.g;
     if (phase == 0) {
         if (i >= sr.nrec || t < 0 || t >= sr.rec_nt) return;
-        const int* p = sr.rec + 3 * i;
-        const long long c = cell_index(g, 0, p[0] + g.h, p[1] + g.h, p[2] + g.h);
+        const long long c = el_owner_cell(g, sr.rec + 3 * i);
         d.adj[sr.rec_comp][c] += d.f[F_B][c] * d.res[(long long)t * sr.nrec + i];
         return;
     }
     if (i >= sr.nsrc || t < 0 || t >= sr.src_nt) return;
     const int* p = sr.src + 4 * i;
-    const long long c = cell_index(g, 0, p[0] + g.h, p[1] + g.h, p[2] + g.h);
+    const long long c = el_owner_cell(g, p);
     const float wv = sr.wav[(long long)t * sr.nsrc + i];
     const float bc = d.f[F_B][c];
     d.grad[2][c] += d.dt * ((d.adj[p[3]][c] / bc) * wv);
@@ -903,7 +930,7 @@ __global__ __launch_bounds__(EL_WG) void el_adj_unscale_kernel(const ElDev* __re
     if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
     const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
     for (int z = z0; z <= z1; ++z) {
-        const long long c = g.xoff + z * g.slice + (long long)y * g.pitch + x;
+        const long long c = (long long)T.b * g.bstride + g.xoff + z * g.slice + (long long)y * g.pitch + x;
         const float l = d.f[F_LAM][c], m = d.f[F_MU][c], bc = d.f[F_B][c];
         for (int k = 0; k < 3; ++k) d.adj[k][c] = d.adj[k][c] / bc;
         const float pxx = d.adj[3][c], pyy = d.adj[4][c], pzz = d.adj[5][c];
@@ -933,11 +960,12 @@ __global__ __launch_bounds__(EL_WG) void el_ckpt_kernel(const ElDev* __restrict_
     if (x < h || x >= h + g.B[0] || y < h || y >= h + g.B[1]) return;
     float* slot = d.cpt + (long long)(ts / ckpt - 1) * 9 * d.fstride;
     const int z0 = max(T.zs, h), z1 = min(T.ze, h + g.B[2]) - 1;
+    const long long boff = (long long)T.b * g.bstride + g.xoff;
     for (int k = 0; k < 9; ++k) {
         float* st = d.f[k];
         float* cp = slot + (long long)k * d.fstride;
         for (int z = z0; z <= z1; ++z) {
-            const long long c = g.xoff + z * g.slice + (long long)y * g.pitch + x;
+            const long long c = boff + z * g.slice + (long long)y * g.pitch + x;
             if (restore) st[c] = cp[c];
             else cp[c] = st[c];
         }
@@ -1671,12 +1699,19 @@ static ElDev el_adj_entry(const ElDev& d)
     return a;
 }
 
-static int el_grad_check(b2r_el_shot* s, int nsteps)
+// The reverse sweep (rev) runs at R = 1 with every global face frozen or
+// periodic, on any block decomposition; the plain per-shot loop on one block
+// with frozen faces.
+static bool el_grad_exchanges(const b2r_ctx* c) { return c->k.nblk > 1 || c->k.periodic; }
+
+static int el_grad_check(b2r_el_shot* s, int nsteps, bool rev)
 {
     B2R_CHECK(s, B2R_E_ARG, "null shot");
     b2r_ctx* c = s->ctx;
-    B2R_CHECK(c->k.nblk == 1 && c->g.R == 1 && !c->k.periodic && c->k.fmask == 0x3F,
-              B2R_E_UNSUPPORTED, "the adjoint runs on one block, R=1, frozen faces");
+    B2R_CHECK(c->g.R == 1 && closed_faces(c->k), B2R_E_UNSUPPORTED,
+              "the adjoint runs at R=1 with every global face frozen or periodic");
+    B2R_CHECK(rev || (!el_grad_exchanges(c) && c->k.fmask == 0x3F), B2R_E_UNSUPPORTED,
+              "the plain per-shot gradient runs on one block with frozen faces");
     B2R_CHECK(s->rec_nt >= nsteps && s->nrec > 0, B2R_E_ARG, "receivers must cover nsteps");
     B2R_CHECK(s->d.disp == nullptr, B2R_E_UNSUPPORTED, "disable the displacement output");
     return B2R_OK;
@@ -1727,7 +1762,7 @@ static int el_gradient_plain(b2r_el_shot* s, const float* h_obs, int nsteps, int
                              double* misfit, float* h_glam, float* h_gmu, float* h_gb,
                              float* h_adj0)
 {
-    if (int rc = el_grad_check(s, nsteps)) return rc;
+    if (int rc = el_grad_check(s, nsteps, false)) return rc;
     b2r_ctx* c = s->ctx;
     if (int rc = b2r_activate(c)) return rc;
     const size_t fe = (size_t)c->L.field_elems, fb = fe * 4;
@@ -1871,7 +1906,7 @@ static int el_gradient_rev(b2r_el_shot* const* shots, int nshots, const float* c
               B2R_E_ARG, "bad argument");
     B2R_CHECK(!h_adj0 || nshots == 1, B2R_E_ARG, "the adjoint state is returned for one shot");
     for (int k = 0; k < nshots; ++k) {
-        if (int rc = el_grad_check(shots[k], nsteps)) return rc;
+        if (int rc = el_grad_check(shots[k], nsteps, true)) return rc;
         B2R_CHECK(h_obs[k] && h_glam[k] && h_gmu[k] && h_gb[k], B2R_E_ARG, "null array, shot %d", k);
     }
     b2r_ctx* c0 = shots[0]->ctx;
@@ -1953,6 +1988,16 @@ static int el_gradient_rev(b2r_el_shot* const* shots, int nshots, const float* c
             if (bt.max_src)
                 el_adj_point_kernel<<<dim3(ceil_div(bt.max_src, 64), nshots), 64, 0, st>>>(
                     bt.d_tab, bt.d_t, 1, 1);
+            // halos of the state after the step and of the adjoint, from their owners
+            if (int r = el_exchange(bt)) return r;
+            for (b2r_el_shot* s : bt.shots) {
+                if (!el_grad_exchanges(s->ctx)) continue;
+                hipStream_t keep = s->ctx->stream;
+                s->ctx->stream = st;
+                const int r = b2r_direct_exchange(s->ctx, (void* const*)s->d.adj, 9);
+                s->ctx->stream = keep;
+                if (r) return r;
+            }
             el_adj_vel_kernel<true><<<bt.total_tiles, blk, 0, st>>>(bt.d_tab, nshots);
             el_stress_kernel<<<bt.total_tiles, blk, 0, st>>>(d_atab, nshots, 0);
             el_adj_str_kernel<true><<<bt.total_tiles, blk, 0, st>>>(bt.d_tab, nshots);

@@ -332,15 +332,16 @@ int b2r_el_misfit(b2r_el_shot* s, const float* h_obs, double* misfit);
  * sweep undoes each step (the velocity-stress scheme is time-reversible)
  * together with its adjoint and resets the state from the checkpoint at every
  * segment boundary, so nothing is stored between checkpoints:
- * (ceil(nsteps/ckpt) - 1) * 9 + 12 extra fields.  One block, R = 1, frozen
- * faces; sources at distinct cells. */
+ * (ceil(nsteps/ckpt) - 1) * 9 + 12 extra fields.  R = 1, every global face
+ * frozen or periodic, any block decomposition (the state and the adjoint are
+ * exchanged before every reversed step); sources at distinct cells. */
 int b2r_el_gradient(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt, double* misfit,
                     float* h_glam, float* h_gmu, float* h_gb);
 /* flags bit 0: start from the current state (not zero); bit 1: the plain
  * algorithm (forward in the reference's per-step structure, each segment
  * recomputed with all its states kept, ckpt + 1 + ceil(nsteps/ckpt) state
- * copies); h_adj0 (optional, 9 global grids): the adjoint state at the start
- * = d misfit / d initial state */
+ * copies; one block with frozen faces); h_adj0 (optional, 9 global grids):
+ * the adjoint state at the start = d misfit / d initial state */
 int b2r_el_gradient_ex(b2r_el_shot* s, const float* h_obs, int nsteps, int ckpt, int flags,
                        double* misfit, float* h_glam, float* h_gmu, float* h_gb, float* h_adj0);
 /* Gradients of nshots shots (sizes may differ), shot k with observations
