@@ -386,9 +386,16 @@ __device__ __forceinline__ void gb_unpack(uint32_t w, uint4& a, uint4& b)
     b = make_uint4((w >> 4) & m, (w >> 5) & m, (w >> 6) & m, (w >> 7) & m);
 }
 
+// The two outputs of a full adder, each one v_bitop3 (truth tables 0x96 and
+// 0xE8, both symmetric in the three inputs).  Written as C expressions the
+// compiler shares a ^ b between them and emits three instructions for two.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
 __device__ __forceinline__ uint32_t maj3(uint32_t a, uint32_t b, uint32_t c)
 {
-    return (a & b) | (c & (a ^ b));
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0xE8);
 }
 
 // bits of the cells x .. x+31 inside [lo, hi]
@@ -601,7 +608,7 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
 #pragma unroll
             for (int j = 0; j < RT; ++j) {
                 const uint32_t ga = gen[j][t - 1][0], self = gen[j][t - 1][1], gc = gen[j][t - 1][2];
-                s0[j] = ga ^ self ^ gc;
+                s0[j] = xor3(ga, self, gc);
                 s1[j] = maj3(ga, self, gc);
                 sb[tid + j * NTH + 1] = make_uint2(s0[j], s1[j]);
             }
@@ -617,8 +624,8 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
                 const uint32_t r0 = __builtin_amdgcn_alignbit(Rg.x, s0[j], 1),
                                r1 = __builtin_amdgcn_alignbit(Rg.y, s1[j], 1);
                 // XZ = l + S + rr, four bit planes (0..9)
-                const uint32_t x0b = l0 ^ s0[j] ^ r0, k1 = maj3(l0, s0[j], r0);
-                const uint32_t u = l1 ^ s1[j] ^ r1, v = maj3(l1, s1[j], r1);
+                const uint32_t x0b = xor3(l0, s0[j], r0), k1 = maj3(l0, s0[j], r0);
+                const uint32_t u = xor3(l1, s1[j], r1), v = maj3(l1, s1[j], r1);
                 const uint32_t x1b = u ^ k1, w = u & k1;
                 X[j] = make_uint4(x0b, x1b, v ^ w, v & w);
                 xb[pos + NQ] = X[j];
@@ -630,13 +637,13 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
                 const uint32_t self = gen[j][t - 1][1];
                 const uint4 U = xb[pos], Dn = xb[pos + 2 * NQ];
                 // box = U + X + Dn (each <= 9), five bit planes
-                const uint32_t b0 = U.x ^ X[j].x ^ Dn.x, c1 = maj3(U.x, X[j].x, Dn.x);
-                const uint32_t e1 = U.y ^ X[j].y ^ Dn.y, c2a = maj3(U.y, X[j].y, Dn.y);
+                const uint32_t b0 = xor3(U.x, X[j].x, Dn.x), c1 = maj3(U.x, X[j].x, Dn.x);
+                const uint32_t e1 = xor3(U.y, X[j].y, Dn.y), c2a = maj3(U.y, X[j].y, Dn.y);
                 const uint32_t b1 = e1 ^ c1, c2b = e1 & c1;
-                const uint32_t e2 = U.z ^ X[j].z ^ Dn.z, c3a = maj3(U.z, X[j].z, Dn.z);
-                const uint32_t b2 = e2 ^ c2a ^ c2b, c3b = maj3(e2, c2a, c2b);
-                const uint32_t e3 = U.w ^ X[j].w ^ Dn.w, c4a = maj3(U.w, X[j].w, Dn.w);
-                const uint32_t b3 = e3 ^ c3a ^ c3b, c4b = maj3(e3, c3a, c3b);
+                const uint32_t e2 = xor3(U.z, X[j].z, Dn.z), c3a = maj3(U.z, X[j].z, Dn.z);
+                const uint32_t b2 = xor3(e2, c2a, c2b), c3b = maj3(e2, c2a, c2b);
+                const uint32_t e3 = xor3(U.w, X[j].w, Dn.w), c4a = maj3(U.w, X[j].w, Dn.w);
+                const uint32_t b3 = xor3(e3, c3a, c3b), c4b = maj3(e3, c3a, c3b);
                 const uint32_t b4 = c4a | c4b;
                 // rule: box == 5, or alive and 3 <= box <= 8
                 const uint32_t le8 = ~b4 & ~(b3 & (b2 | b1 | b0));
