@@ -31,9 +31,74 @@
 
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <vector>
+
+// Host worker threads, started on first use and kept until the process ends,
+// so a hybrid window pays no thread start-up: run(n, job) calls job(t, n) on
+// workers t = 0..n-1 and returns when all have finished.  Calls are serialised.
+class HostPool {
+public:
+    ~HostPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        go_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    void run(int n, const std::function<void(int, int)>& job)
+    {
+        std::lock_guard<std::mutex> one(call_);
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            while ((int)th_.size() < n) th_.emplace_back(&HostPool::worker, this, (int)th_.size());
+            job_ = &job;
+            n_ = n;
+            pending_ = (int)th_.size();
+            ++gen_;
+        }
+        go_.notify_all();
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+    }
+
+private:
+    void worker(int id)
+    {
+        long long seen = 0;
+        std::unique_lock<std::mutex> lk(m_);
+        for (;;) {
+            go_.wait(lk, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            const int n = n_;
+            const std::function<void(int, int)>* job = job_;
+            lk.unlock();
+            if (id < n) (*job)(id, n);
+            lk.lock();
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    std::mutex call_, m_;
+    std::condition_variable go_, done_;
+    std::vector<std::thread> th_;
+    const std::function<void(int, int)>* job_ = nullptr;
+    int n_ = 0, pending_ = 0;
+    long long gen_ = 0;
+    bool stop_ = false;
+};
+
+static HostPool& host_pool()
+{
+    static HostPool pool;
+    return pool;
+}
 
 // ---------------------------------------------------------------- host step
 template <int D, int MODE>
@@ -69,34 +134,36 @@ static inline float cpu_update(const float* c, long long sy, long long sz, const
 }
 
 // out = step(in) on the box lo..hi (inclusive, per axis) of a dense
-// Nx*Ny*Nz grid, x fastest; z planes shared out over the threads.
+// Nx*Ny*Nz grid, x fastest; the box's (z, y) rows shared out over the host
+// pool's workers in contiguous ranges, so a slab of a few planes still uses
+// every worker.
 template <int D, int MODE>
 static void cpu_box(const float* in, float* out, int Nx, int Ny, const int lo[3], const int hi[3],
                     const HeatK& hk, int nthreads)
 {
     const long long sy = Nx, sz = (long long)Nx * Ny;
-    const int nz = hi[2] - lo[2] + 1;
-    if (nz <= 0 || hi[0] < lo[0] || hi[1] < lo[1]) return;
+    const int nz = hi[2] - lo[2] + 1, ny = hi[1] - lo[1] + 1;
+    if (nz <= 0 || ny <= 0 || hi[0] < lo[0]) return;
+    const long long rows = (long long)nz * ny;
+    // the reference walks z downwards (heat-3D.cpp:55); the result does not
+    // depend on the order (the step reads in and writes out)
     auto work = [&](int t, int nt) {
-        // the reference walks z downwards (heat-3D.cpp:55); the result does
-        // not depend on the order (the step reads in and writes out)
-        for (int z = hi[2] - t; z >= lo[2]; z -= nt)
-            for (int y = lo[1]; y <= hi[1]; ++y) {
-                const long long r = z * sz + y * sy;
-                for (int x = lo[0]; x <= hi[0]; ++x)
-                    out[r + x] = cpu_update<D, MODE>(in + r + x, sy, sz, hk);
-            }
+        const long long r0 = rows * t / nt, r1 = rows * (t + 1) / nt;
+        for (long long r = r0; r < r1; ++r) {
+            const int z = hi[2] - (int)(r / ny), y = lo[1] + (int)(r % ny);
+            const long long o = z * sz + y * sy;
+            for (int x = lo[0]; x <= hi[0]; ++x)
+                out[o + x] = cpu_update<D, MODE>(in + o + x, sy, sz, hk);
+        }
     };
     int nt = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
     if (nt < 1) nt = 1;
-    if (nt > nz) nt = nz;
+    if (nt > rows) nt = (int)rows;
     if (nt == 1) {
         work(0, 1);
         return;
     }
-    std::vector<std::thread> pool;
-    for (int t = 0; t < nt; ++t) pool.emplace_back(work, t, nt);
-    for (auto& th : pool) th.join();
+    host_pool().run(nt, work);
 }
 
 static void cpu_box_any(const b2r_heat_params* p, const float* in, float* out, int Nx, int Ny,

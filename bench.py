@@ -644,6 +644,51 @@ def heat_sections(A, bw, rec=None):
     return also
 
 
+def hybrid_bench(A, nsteps=40):
+    """Hybrid CPU+GPU heat (the CPU owns the top cpu_planes z planes, joint
+    layers exchanged over PCIe every R steps) against the GPU-only run: heat
+    delta1 256^3, one block, frozen faces, R = 1 and 4, CPU shares of h and 4h
+    planes (h = R*D).  Wall time of whole runs (the hybrid run synchronises with
+    the host); the results are compared bit for bit."""
+    import numpy as np
+    env, k = (256, 256, 256), (0.1, 0.1, 0.1)
+    p = A.heat_params(1, A.HEAT_F32, k)
+    nthreads = CPUS_AVAILABLE or 0
+    out = {"workload": "heat delta1 256^3 F32, one block, frozen faces, %d steps" % nsteps,
+           "cpu_threads": nthreads, "runs": []}
+    for R in (1, 4):
+        with A.Context(env, (1, 1, 1), R, 1, A.BOUNDARY_FROZEN, 4) as c:
+            a, b = c.field(), c.field()
+            c.init_uniform(a, 0)
+            c.heat_run(a, b, p, 2 * R)  # warm-up, tunes nothing (the default shape)
+            c.init_uniform(a, 0)
+            c.sync()
+            t0 = time.perf_counter()
+            res, _ = c.heat_run(a, b, p, nsteps)
+            c.sync()
+            gpu_ms = (time.perf_counter() - t0) * 1e3
+            want = c.download(res)
+            for planes in (R, 4 * R):
+                c.init_uniform(a, 0)
+                c.heat_run_hybrid(a, b, p, 2 * R, planes, nthreads)  # warm-up
+                c.init_uniform(a, 0)
+                c.sync()
+                t0 = time.perf_counter()
+                res, _ = c.heat_run_hybrid(a, b, p, nsteps, planes, nthreads)
+                c.sync()
+                hyb_ms = (time.perf_counter() - t0) * 1e3
+                got = c.download(res)
+                out["runs"].append({
+                    "R": R, "cpu_planes": planes, "cpu_share": round(planes / env[2], 4),
+                    "gpu_only_ms_per_step": round(gpu_ms / nsteps, 4),
+                    "hybrid_ms_per_step": round(hyb_ms / nsteps, 4),
+                    "hybrid_over_gpu_only": round(hyb_ms / gpu_ms, 2),
+                    "bitwise_equal_to_gpu_only": bool(np.array_equal(got.view(np.uint32),
+                                                                     want.view(np.uint32)))})
+                log(json.dumps(out["runs"][-1]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -658,7 +703,7 @@ def main():
     ap.add_argument("--profile-kernels", action="store_true",
                     help="run the fixed kernel sequence used by the rocprofv3 counter runs")
     ap.add_argument("--only", choices=["elastic-gradient", "elastic-batch", "heat",
-                                       "elastic-full"],
+                                       "elastic-full", "hybrid"],
                     help="run one section alone and print its JSON (for profiling); "
                          "elastic-full: the 16-shot forward over 2000 steps and the 16-shot "
                          "256^3 gradient over 1500 steps (checkpoints every 50), batched "
@@ -690,6 +735,9 @@ def main():
                                       modes=((2, "batched_graph"), (0, "plain_loop")))
         print(json.dumps({"elastic_16_shots_2000_steps": fwd,
                           "elastic_gradient_16x256_1500_steps": grad}), flush=True)
+        return
+    if args.only == "hybrid":
+        print(json.dumps(hybrid_bench(A)), flush=True)
         return
     if args.only == "heat":
         bw = stream_bw(A)
@@ -745,6 +793,8 @@ def main():
         ref = cpu_heat_reference()
         if ref:
             also["heat_delta1_128_cpu_reference"], heat_ref_grid = ref
+        log("hybrid CPU+GPU heat")
+        also["heat_hybrid_256"] = hybrid_bench(A)
         log("elastic 16-shot batch")
         also["elastic_16_shots"] = elastic_batch_bench(A, W, 16, args.elastic_steps, rec)
         log(json.dumps(also["elastic_16_shots"]))
