@@ -36,22 +36,17 @@
 #include <functional>
 #include <mutex>
 #include <thread>
+#include <unistd.h>
 #include <vector>
 
 // Host worker threads, started on first use and kept until the process ends,
 // so a hybrid window pays no thread start-up: run(n, job) calls job(t, n) on
 // workers t = 0..n-1 and returns when all have finished.  Calls are serialised.
+// The pool is never destroyed (its idle workers end with the process), and a
+// process forked after using it starts a pool of its own (host_pool), since
+// the parent's workers do not exist in the child.
 class HostPool {
 public:
-    ~HostPool()
-    {
-        {
-            std::lock_guard<std::mutex> lk(m_);
-            stop_ = true;
-        }
-        go_.notify_all();
-        for (auto& t : th_) t.join();
-    }
     void run(int n, const std::function<void(int, int)>& job)
     {
         std::lock_guard<std::mutex> one(call_);
@@ -74,8 +69,7 @@ private:
         long long seen = 0;
         std::unique_lock<std::mutex> lk(m_);
         for (;;) {
-            go_.wait(lk, [&] { return stop_ || gen_ != seen; });
-            if (stop_) return;
+            go_.wait(lk, [&] { return gen_ != seen; });
             seen = gen_;
             const int n = n_;
             const std::function<void(int, int)>* job = job_;
@@ -91,13 +85,19 @@ private:
     const std::function<void(int, int)>* job_ = nullptr;
     int n_ = 0, pending_ = 0;
     long long gen_ = 0;
-    bool stop_ = false;
 };
 
 static HostPool& host_pool()
 {
-    static HostPool pool;
-    return pool;
+    static std::mutex mu;
+    static HostPool* pool = nullptr;
+    static pid_t owner = 0;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!pool || owner != getpid()) {
+        pool = new HostPool();  // a forked child leaves the parent's pool unused
+        owner = getpid();
+    }
+    return *pool;
 }
 
 // ---------------------------------------------------------------- host step

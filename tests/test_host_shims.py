@@ -88,6 +88,26 @@ def test_cpu_step_matches_oracle(D, mode):
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
+def test_cpu_step_thread_counts_and_fork():
+    # the host step's worker pool: any thread count (more threads than rows
+    # included) gives the one-thread result bit for bit, and a process forked
+    # after the pool started runs the step with workers of its own
+    rng = np.random.default_rng(3)
+    g = rng.uniform(0, 1, (7, 9, 21)).astype(np.float32)
+    p = A.heat_params(2, A.HEAT_F32, (0.05, 0.07, 0.03))
+    want = A.heat_cpu_step(p, g, 0, nthreads=1)
+    for nt in (2, 3, 8, 64, 0):
+        assert np.array_equal(A.heat_cpu_step(p, g, 0, nthreads=nt).view(np.uint32),
+                              want.view(np.uint32)), nt
+    pid = os.fork()
+    if pid == 0:
+        ok = np.array_equal(A.heat_cpu_step(p, g, 0, nthreads=4), want)
+        os._exit(0 if ok else 3)
+    _, status = os.waitpid(pid, 0)
+    assert os.WIFEXITED(status) and os.WEXITSTATUS(status) == 0
+    assert np.array_equal(A.heat_cpu_step(p, g, 0, nthreads=4), want)
+
+
 @pytest.mark.skipif(not O.ref_available(), reason="the reference's heat-3D.cpp is not built here")
 @pytest.mark.parametrize("b2r_i", [0, 1, 2])
 def test_cpu_step_matches_reference_step(b2r_i):
