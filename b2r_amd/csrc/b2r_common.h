@@ -201,6 +201,26 @@ __device__ __forceinline__ void sto(gfloat* __restrict__ p, uint32_t o, float v)
     *(gfloat*)((gbyte*)p + o) = v;
 }
 
+// Raw buffer access for marches: a resource over a uniform base pointer (no
+// range check: the caller keeps every offset inside the allocation), the
+// per-thread byte offset in a vector register and the uniform part (a plane
+// offset) in the instruction's scalar offset, so advancing a plane costs no
+// vector instruction.  0x00020000 is the gfx9 buffer resource word 3 for
+// 32-bit raw accesses.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, -1, 0x00020000);
+}
+__device__ __forceinline__ float bld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
+                                    float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+}
+
 // Element offset of local cell (x,y,z) of block b.
 __host__ __device__ inline long long cell_index(const KGeo& g, int b, int x, int y, int z)
 {

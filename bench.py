@@ -584,7 +584,7 @@ def profile_kernels(A):
             c.heat_run(a, b, prm, 2)
             t = c.autotune(0, a, b, prm, None)
             log(json.dumps({"profile": name, "kernel": "heat_step_kernel<%d, 0, %d, %d, %d>" % (
-                D, t["heat_ry"], t["heat_wy"], t["heat_wx"]), "heat_zc": t["heat_zc"],
+                D, t["heat_ry"], t["heat_wy"] or 4, t["heat_wx"] or 1), "heat_zc": t["heat_zc"],
                 "launches": 5}))
             c.time_steps(0, a, b, 5, prm)
     env = (256, 256, 256)
