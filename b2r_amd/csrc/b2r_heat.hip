@@ -291,13 +291,16 @@ static void launch_heat_wy(b2r_ctx* ctx, const float* in, float* out, const Heat
     }
 }
 
-// workgroup = 64*heat_wx x-lanes (1, 2 or 4 waves across) x heat_wy thread
-// rows (4 by default), each thread heat_ry rows; heat_wx*heat_wy <= 16
+// workgroup = 64*heat_wx x-lanes x heat_wy thread rows, each thread heat_ry
+// rows.  (heat_wy, heat_wx): (4, 1) by default, (4, 2), (4, 4), (8, 1), (8, 2),
+// (16, 1), and the 12-wave shapes (4, 3) and (2, 6), whose 192- and 384-cell
+// rows cover a 384-cell block row in 2 or 1 tiles
 template <int D, int MODE>
 static void launch_heat(b2r_ctx* ctx, const float* in, float* out, const HeatK& hk, int i)
 {
     const int wx = ctx->tune.heat_wx;
     switch (ctx->tune.heat_wy) {
+    case 2: launch_heat_wy<D, MODE, 2, 6>(ctx, in, out, hk, i); break;
     case 8:
         if (wx == 2) launch_heat_wy<D, MODE, 8, 2>(ctx, in, out, hk, i);
         else launch_heat_wy<D, MODE, 8, 1>(ctx, in, out, hk, i);
@@ -305,6 +308,7 @@ static void launch_heat(b2r_ctx* ctx, const float* in, float* out, const HeatK& 
     case 16: launch_heat_wy<D, MODE, 16, 1>(ctx, in, out, hk, i); break;
     default:
         if (wx == 4) launch_heat_wy<D, MODE, 4, 4>(ctx, in, out, hk, i);
+        else if (wx == 3) launch_heat_wy<D, MODE, 4, 3>(ctx, in, out, hk, i);
         else if (wx == 2) launch_heat_wy<D, MODE, 4, 2>(ctx, in, out, hk, i);
         else launch_heat_wy<D, MODE, 4, 1>(ctx, in, out, hk, i);
         break;

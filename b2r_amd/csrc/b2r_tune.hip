@@ -13,6 +13,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 extern "C" int b2r_time_steps(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
@@ -62,17 +63,21 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
                 cand.push_back(t);
             }
         } else {
+            std::vector<std::pair<int, int>> shapes;  // (heat_wy, heat_wx)
             for (int wy : wys)
-                for (int wx = 1; wx * wy <= 16; wx *= 2)
-                    for (int ry : rys)
-                        for (int zc : zcs_heat) {
-                            b2r_tuning t = base;
-                            t.heat_ry = ry;
-                            t.heat_wy = wy;
-                            t.heat_wx = wx;
-                            t.heat_zc = zc;
-                            cand.push_back(t);
-                        }
+                for (int wx = 1; wx * wy <= 16; wx *= 2) shapes.push_back({wy, wx});
+            shapes.push_back({4, 3});
+            shapes.push_back({2, 6});
+            for (const auto& sh : shapes)
+                for (int ry : rys)
+                    for (int zc : zcs_heat) {
+                        b2r_tuning t = base;
+                        t.heat_ry = ry;
+                        t.heat_wy = sh.first;
+                        t.heat_wx = sh.second;
+                        t.heat_zc = zc;
+                        cand.push_back(t);
+                    }
         }
     } else {
         for (int tile = 1; tile <= (kind == 2 ? 10 : 1); ++tile)
