@@ -8,12 +8,13 @@
 //
 // This kernel: one workgroup = 64*WX x-lanes x WY thread rows; each thread
 // owns RY consecutive rows of one x column and marches z through a chunk of
-// planes.  The z neighbours live in a register ring of 2D+2 planes per row
-// (the march unrolled by the ring length, so no value moves); the current
-// plane, with a D-wide x/y halo, is staged in LDS so the x and y neighbours
-// are LDS reads (two LDS planes alternate, one barrier per plane).  Loads of
-// the z-front are coalesced rows, issued one plane ahead as buffer loads whose
-// plane offset is a scalar.
+// planes.  The z neighbours live in registers: for D <= 2 a queue of 2D+1
+// values per row shifted by one plane per step, for D >= 3 a ring of 2D+2
+// planes (the march unrolled by the ring length, so no value moves, and the
+// loads are buffer loads whose plane offset is a scalar).  The current plane,
+// with a D-wide x/y halo, is staged in LDS so the x and y neighbours are LDS
+// reads (two LDS planes alternate, one barrier per plane).  Loads of the
+// z-front are coalesced rows, issued one plane ahead.
 //
 // Arithmetic (compiled without FP contraction, so CPU and GPU round alike):
 //   F32: u' = u + ((kx*Lx + ky*Ly) + kz*Lz)
@@ -65,8 +66,6 @@ __global__ __launch_bounds__(HEAT_TX * WX * WY) void heat_step_kernel(
     // zlo-D, cells addressed by 32-bit byte offsets from it (the launch keeps
     // zc + 2D planes below 4 GiB)
     const long long base = (long long)b * g.bstride + g.xoff + (long long)(zlo - D) * g.slice;
-    const __amdgpu_buffer_rsrc_t src = buf_rsrc(in + base);
-    const __amdgpu_buffer_rsrc_t dst = buf_rsrc(out + base);
     const uint32_t sz = 4u * (uint32_t)g.slice;
     const uint32_t pitch = 4u * (uint32_t)g.pitch;
 
@@ -117,118 +116,209 @@ __global__ __launch_bounds__(HEAT_TX * WX * WY) void heat_step_kernel(
         }
     }
 
-    // The z-queue is a ring of L = 2D+2 planes per row: at ring step r
-    // (plane z of the march, r = (z - zlo) mod L) slot (r + D + d) mod L holds
-    // plane z + d for d = -D..D, and slot (r + 2D + 1) mod L, the one plane
-    // z-D-1 left, receives plane z+D+1, loaded one plane ahead of its use.
-    // The march is unrolled by L, so every slot index is a compile-time
-    // constant: no value moves between registers (a shifted queue costs 2D
-    // moves per cell) and a load in flight is never copied, so it is waited
-    // for only where it is used.  L is even, so the LDS plane of step r is
-    // lds2[r & 1] across iterations as well.
-    //   The halo values of plane z are loaded HA+1 planes ahead into a ring of
-    // L slots per halo slot (plane z + d in slot (r + d) mod L).  For D >= 2,
-    // HA = D: a halo row is its neighbour tile's own row, which that tile
-    // loads D+1 planes ahead, so both requests reach L2 at about the same time
-    // instead of D planes apart (about 1 MB of plane data per XCD per plane at
-    // D=4, which the 4 MB L2 does not keep for 4 planes).  At D=1, HA = 0.
-    constexpr int L = 2 * D + 2;
-    constexpr int HA = D >= 2 ? D : 0;
-    static_assert(HA + 1 < L, "halo ring too short");
-    float q[RY][L], hq[NSLOT][L];
-    // planes zlo-D .. zlo+D into slots 0 .. 2D (ring step 0).  A cell
-    // outside the block reads the plane's first cell instead: no updated cell
-    // reads it (an updated cell's stencil lies in the block), so the loads
-    // are unconditional, as in the march.
-#pragma unroll
-    for (int p = 0; p <= 2 * D; ++p) {
-#pragma unroll
-        for (int j = 0; j < RY; ++j) q[j][p] = bld(src, yo[j], p * sz);
-    }
-#pragma unroll
-    for (int s = 0; s < NSLOT; ++s)
-#pragma unroll
-        for (int k = 0; k <= HA; ++k) hq[s][k] = bld(src, ho[s], (D + k) * sz);
-    // wait for all of them here, before the march: the waits at the loop's
-    // head serve the loop's entry as well, and without this one they would
-    // wait for the last prologue loads on every iteration (vmcnt(0),
-    // expcnt and lgkmcnt left open: 0x0F70 on gfx9)
-    __builtin_amdgcn_s_waitcnt(0x0F70);
-
-    // ring step r at plane z, whose byte offset from the base plane is pz
-    auto step = [&](auto rc, const int z, const uint32_t pz) {
-        constexpr int r = decltype(rc)::value;
-        constexpr int C = (r + D) % L;  // slot of plane z
-        float* const lds = lds2[r & 1];
-#pragma unroll
-        for (int j = 0; j < RY; ++j) lds[(wy * RY + j + D) * LX + lane + D] = q[j][C];
-#pragma unroll
-        for (int s = 0; s < NSLOT; ++s)
-            if (hok[s]) lds[hl[s]] = hq[s][r % L];
-        {
-            // halo plane z+1+HA and tile plane z+1+D.  The last plane, zhi,
-            // loads planes zhi+HA and zhi+D again instead (values unused),
-            // keeping the loads unconditional and inside the block (zhi+D <=
-            // S[2]-1).  The plane offset is the scalar offset of the buffer
-            // access, so advancing a plane adds nothing to the per-thread
-            // offsets.
-            const uint32_t adv = z < zhi ? sz : 0u;
-#pragma unroll
-            for (int s = 0; s < NSLOT; ++s) hq[s][(r + HA + 1) % L] = bld(src, ho[s], pz + HA * sz + adv);
-#pragma unroll
-            for (int j = 0; j < RY; ++j) q[j][(r + 2 * D + 1) % L] = bld(src, yo[j], pz + D * sz + adv);
+    if constexpr (D <= 2) {
+        // D = 1, 2: the z-queue of 2D+1 values per row shifts by one plane
+        // per step.  (The ring below issues a third fewer VALU instructions,
+        // which pays where the stencil's arithmetic is large; at D = 1 and 2
+        // the shifted queue measured 7 % and 1.5 % faster.)
+        const gfloat* __restrict__ src = G(in + base);
+        gfloat* __restrict__ dst = G(out + base);
+        constexpr int Q = 2 * D + 1;
+        float q[RY][Q];
+    #pragma unroll
+        for (int p = 0; p < 2 * D; ++p) {
+    #pragma unroll
+            for (int j = 0; j < RY; ++j)
+                q[j][p] = (xin && yin[j]) ? ldo(src, p * sz + yo[j]) : 0.0f;
         }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < RY; ++j) {
-            if (xupd && yupd[j]) {
-                const float* row = &lds[(wy * RY + j + D) * LX + lane + D];
-                const float uc = q[j][C];
-                float mx[D + 1], px[D + 1], my[D + 1], py[D + 1], mz[D + 1], pz_[D + 1];
-#pragma unroll
-                for (int k = 1; k <= D; ++k) {
-                    mx[k] = row[-k];
-                    px[k] = row[k];
-                    // rows of this thread are in registers (j, k are unrolled)
-                    my[k] = j - k >= 0 ? q[j - k >= 0 ? j - k : 0][C] : row[-k * LX];
-                    py[k] = j + k < RY ? q[j + k < RY ? j + k : 0][C] : row[k * LX];
-                    mz[k] = q[j][(r + D - k + L) % L];
-                    pz_[k] = q[j][(r + D + k) % L];
+
+        // The tile's values are loaded D planes ahead of the next plane (qn, for
+        // the z-queue).  For D >= 2 so are the halo values (a queue hq of HA+1
+        // planes): a halo row is its neighbour tile's own row, which that tile
+        // loads D planes ahead, so both requests reach L2 at about the same time
+        // instead of D planes apart (about 1 MB of plane data per XCD per plane at
+        // D=4, which the 4 MB L2 does not keep for 4 planes).  At D=1 the halo is
+        // loaded one plane ahead.
+        constexpr int HA = D >= 2 ? D : 0;
+        float hq[NSLOT][HA + 1], qn[RY];
+    #pragma unroll
+        for (int s = 0; s < NSLOT; ++s)
+    #pragma unroll
+            for (int k = 0; k <= HA; ++k) hq[s][k] = hok[s] ? ldo(src, (D + k) * sz + ho[s]) : 0.0f;
+    #pragma unroll
+        for (int j = 0; j < RY; ++j) qn[j] = (xin && yin[j]) ? ldo(src, 2 * D * sz + yo[j]) : 0.0f;
+        // one plane of the march; lds is lds2[0] or lds2[1] as a constant of each
+        // call, so the LDS addresses stay loop-invariant
+        auto plane = [&](const int z, const uint32_t pz, float* const lds) {
+    #pragma unroll
+            for (int j = 0; j < RY; ++j) q[j][2 * D] = qn[j];
+    #pragma unroll
+            for (int j = 0; j < RY; ++j) lds[(wy * RY + j + D) * LX + lane + D] = q[j][D];
+    #pragma unroll
+            for (int s = 0; s < NSLOT; ++s) {
+                if (hok[s]) lds[hl[s]] = hq[s][0];
+    #pragma unroll
+                for (int k = 0; k < HA; ++k) hq[s][k] = hq[s][k + 1];
+            }
+            if (z < zhi) {
+                // planes z+1+HA and z+1+D: z+1 <= zhi <= S[2]-1-D keeps both in the block
+                const uint32_t pn = pz + (D + 1) * sz;
+    #pragma unroll
+                for (int j = 0; j < RY; ++j)
+                    if (xin && yin[j]) qn[j] = ldo(src, pn + yo[j]);
+    #pragma unroll
+                for (int s = 0; s < NSLOT; ++s)
+                    if (hok[s]) hq[s][HA] = ldo(src, pz + (HA + 1) * sz + ho[s]);
+            }
+            __syncthreads();
+    #pragma unroll
+            for (int j = 0; j < RY; ++j) {
+                if (xupd && yupd[j]) {
+                    const float* row = &lds[(wy * RY + j + D) * LX + lane + D];
+                    const float uc = q[j][D];
+                    float mx[D + 1], px[D + 1], my[D + 1], py[D + 1], mz[D + 1], pz_[D + 1];
+    #pragma unroll
+                    for (int k = 1; k <= D; ++k) {
+                        mx[k] = row[-k];
+                        px[k] = row[k];
+                        // rows of this thread are in registers (j, k are unrolled)
+                        my[k] = j - k >= 0 ? q[j - k >= 0 ? j - k : 0][D] : row[-k * LX];
+                        py[k] = j + k < RY ? q[j + k < RY ? j + k : 0][D] : row[k * LX];
+                        mz[k] = q[j][D - k];
+                        pz_[k] = q[j][D + k];
+                    }
+                    sto(dst, pz + yo[j], heat_update<D, MODE>(mx, px, my, py, mz, pz_, uc, hk));
                 }
-                bst(dst, yo[j], pz, heat_update<D, MODE>(mx, px, my, py, mz, pz_, uc, hk));
+            }
+    #pragma unroll
+            for (int j = 0; j < RY; ++j)
+    #pragma unroll
+                for (int p = 0; p < 2 * D; ++p) q[j][p] = q[j][p + 1];
+        };
+        // planes in pairs (the buffer of plane z is (z - zlo) & 1), an odd one last
+        int z = zlo;
+        uint32_t pz = D * sz;  // byte offset of plane z
+        for (; z + 1 <= zhi; z += 2, pz += 2 * sz) {
+            plane(z, pz, lds2[0]);
+            plane(z + 1, pz + sz, lds2[1]);
+        }
+        if (z <= zhi) plane(z, pz, lds2[0]);
+    } else {
+        const __amdgpu_buffer_rsrc_t src = buf_rsrc(in + base);
+        const __amdgpu_buffer_rsrc_t dst = buf_rsrc(out + base);
+        // The z-queue is a ring of L = 2D+2 planes per row: at ring step r
+        // (plane z of the march, r = (z - zlo) mod L) slot (r + D + d) mod L holds
+        // plane z + d for d = -D..D, and slot (r + 2D + 1) mod L, the one plane
+        // z-D-1 left, receives plane z+D+1, loaded one plane ahead of its use.
+        // The march is unrolled by L, so every slot index is a compile-time
+        // constant: no value moves between registers (a shifted queue costs 2D
+        // moves per cell) and a load in flight is never copied, so it is waited
+        // for only where it is used.  L is even, so the LDS plane of step r is
+        // lds2[r & 1] across iterations as well.
+        //   The halo values of plane z are loaded HA+1 planes ahead into a ring of
+        // L slots per halo slot (plane z + d in slot (r + d) mod L).  For D >= 2,
+        // HA = D: a halo row is its neighbour tile's own row, which that tile
+        // loads D+1 planes ahead, so both requests reach L2 at about the same time
+        // instead of D planes apart (about 1 MB of plane data per XCD per plane at
+        // D=4, which the 4 MB L2 does not keep for 4 planes).  At D=1, HA = 0.
+        constexpr int L = 2 * D + 2;
+        constexpr int HA = D >= 2 ? D : 0;
+        static_assert(HA + 1 < L, "halo ring too short");
+        float q[RY][L], hq[NSLOT][L];
+        // planes zlo-D .. zlo+D into slots 0 .. 2D (ring step 0).  A cell
+        // outside the block reads the plane's first cell instead: no updated cell
+        // reads it (an updated cell's stencil lies in the block), so the loads
+        // are unconditional, as in the march.
+    #pragma unroll
+        for (int p = 0; p <= 2 * D; ++p) {
+    #pragma unroll
+            for (int j = 0; j < RY; ++j) q[j][p] = bld(src, yo[j], p * sz);
+        }
+    #pragma unroll
+        for (int s = 0; s < NSLOT; ++s)
+    #pragma unroll
+            for (int k = 0; k <= HA; ++k) hq[s][k] = bld(src, ho[s], (D + k) * sz);
+        // wait for all of them here, before the march: the waits at the loop's
+        // head serve the loop's entry as well, and without this one they would
+        // wait for the last prologue loads on every iteration (vmcnt(0),
+        // expcnt and lgkmcnt left open: 0x0F70 on gfx9)
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+
+        // ring step r at plane z, whose byte offset from the base plane is pz
+        auto step = [&](auto rc, const int z, const uint32_t pz) {
+            constexpr int r = decltype(rc)::value;
+            constexpr int C = (r + D) % L;  // slot of plane z
+            float* const lds = lds2[r & 1];
+    #pragma unroll
+            for (int j = 0; j < RY; ++j) lds[(wy * RY + j + D) * LX + lane + D] = q[j][C];
+    #pragma unroll
+            for (int s = 0; s < NSLOT; ++s)
+                if (hok[s]) lds[hl[s]] = hq[s][r % L];
+            {
+                // halo plane z+1+HA and tile plane z+1+D.  The last plane, zhi,
+                // loads planes zhi+HA and zhi+D again instead (values unused),
+                // keeping the loads unconditional and inside the block (zhi+D <=
+                // S[2]-1).  The plane offset is the scalar offset of the buffer
+                // access, so advancing a plane adds nothing to the per-thread
+                // offsets.
+                const uint32_t adv = z < zhi ? sz : 0u;
+    #pragma unroll
+                for (int s = 0; s < NSLOT; ++s) hq[s][(r + HA + 1) % L] = bld(src, ho[s], pz + HA * sz + adv);
+    #pragma unroll
+                for (int j = 0; j < RY; ++j) q[j][(r + 2 * D + 1) % L] = bld(src, yo[j], pz + D * sz + adv);
+            }
+            __syncthreads();
+    #pragma unroll
+            for (int j = 0; j < RY; ++j) {
+                if (xupd && yupd[j]) {
+                    const float* row = &lds[(wy * RY + j + D) * LX + lane + D];
+                    const float uc = q[j][C];
+                    float mx[D + 1], px[D + 1], my[D + 1], py[D + 1], mz[D + 1], pz_[D + 1];
+    #pragma unroll
+                    for (int k = 1; k <= D; ++k) {
+                        mx[k] = row[-k];
+                        px[k] = row[k];
+                        // rows of this thread are in registers (j, k are unrolled)
+                        my[k] = j - k >= 0 ? q[j - k >= 0 ? j - k : 0][C] : row[-k * LX];
+                        py[k] = j + k < RY ? q[j + k < RY ? j + k : 0][C] : row[k * LX];
+                        mz[k] = q[j][(r + D - k + L) % L];
+                        pz_[k] = q[j][(r + D + k) % L];
+                    }
+                    bst(dst, yo[j], pz, heat_update<D, MODE>(mx, px, my, py, mz, pz_, uc, hk));
+                }
+            }
+        };
+        // L planes per iteration; the kernel ends after plane zhi.  Each step
+        // leaves the kernel itself, so no path that skips the later steps of an
+        // iteration reaches the loop's back edge: the wait analysis, which merges
+        // paths, then sees every step's loads issued before the next iteration.
+        int z = zlo;
+        uint32_t pz = D * sz;  // byte offset of plane z
+    #define B2R_HEAT_STEP(R)                                        \
+        step(std::integral_constant<int, R>{}, z, pz);              \
+        if (++z > zhi) return;                                      \
+        pz += sz;
+        for (;;) {
+            B2R_HEAT_STEP(0)
+            B2R_HEAT_STEP(1)
+            B2R_HEAT_STEP(2)
+            B2R_HEAT_STEP(3)
+            if constexpr (L > 4) {
+                B2R_HEAT_STEP(4)
+                B2R_HEAT_STEP(5)
+            }
+            if constexpr (L > 6) {
+                B2R_HEAT_STEP(6)
+                B2R_HEAT_STEP(7)
+            }
+            if constexpr (L > 8) {
+                B2R_HEAT_STEP(8)
+                B2R_HEAT_STEP(9)
             }
         }
-    };
-    // L planes per iteration; the kernel ends after plane zhi.  Each step
-    // leaves the kernel itself, so no path that skips the later steps of an
-    // iteration reaches the loop's back edge: the wait analysis, which merges
-    // paths, then sees every step's loads issued before the next iteration.
-    int z = zlo;
-    uint32_t pz = D * sz;  // byte offset of plane z
-#define B2R_HEAT_STEP(R)                                        \
-    step(std::integral_constant<int, R>{}, z, pz);              \
-    if (++z > zhi) return;                                      \
-    pz += sz;
-    for (;;) {
-        B2R_HEAT_STEP(0)
-        B2R_HEAT_STEP(1)
-        B2R_HEAT_STEP(2)
-        B2R_HEAT_STEP(3)
-        if constexpr (L > 4) {
-            B2R_HEAT_STEP(4)
-            B2R_HEAT_STEP(5)
-        }
-        if constexpr (L > 6) {
-            B2R_HEAT_STEP(6)
-            B2R_HEAT_STEP(7)
-        }
-        if constexpr (L > 8) {
-            B2R_HEAT_STEP(8)
-            B2R_HEAT_STEP(9)
-        }
+    #undef B2R_HEAT_STEP
+        static_assert(L == 4 || L == 6 || L == 8 || L == 10, "ring of 2D+2 planes, D = 1..4");
     }
-#undef B2R_HEAT_STEP
-    static_assert(L == 4 || L == 6 || L == 8 || L == 10, "ring of 2D+2 planes, D = 1..4");
 }
 
 int b2r_heat_check(b2r_ctx* ctx, const b2r_heat_params* p)
