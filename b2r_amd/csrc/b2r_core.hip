@@ -227,9 +227,9 @@ extern "C" int b2r_set_tuning(b2r_ctx* ctx, const b2r_tuning* t)
     B2R_CHECK(t->heat_zc >= 0 && t->gol_zc >= 0 && t->gol_step_zc >= 0 && t->el_zc >= 0 &&
                   t->el_step_zc >= 0, B2R_E_ARG,
               "negative z chunk");
-    B2R_CHECK(t->heat_wy == 0 || t->heat_wy == 2 || t->heat_wy == 4 || t->heat_wy == 8 ||
-                  t->heat_wy == 16,
-              B2R_E_ARG, "heat_wy must be 0, 2, 4, 8 or 16");
+    B2R_CHECK(t->heat_wy == 0 || t->heat_wy == 2 || t->heat_wy == 3 || t->heat_wy == 4 ||
+                  t->heat_wy == 8 || t->heat_wy == 16,
+              B2R_E_ARG, "heat_wy must be 0, 2, 3, 4, 8 or 16");
     B2R_CHECK(t->gol_tile >= 0 && t->gol_tile <= 10, B2R_E_ARG, "gol_tile must be 0..10");
     B2R_CHECK(t->el_kernel >= 0 && t->el_kernel <= 2, B2R_E_ARG, "el_kernel must be 0, 1 or 2");
     B2R_CHECK(t->el_grad_mb >= 0, B2R_E_ARG, "el_grad_mb must be >= 0");
@@ -241,9 +241,9 @@ extern "C" int b2r_set_tuning(b2r_ctx* ctx, const b2r_tuning* t)
         const int wy = t->heat_wy > 0 ? t->heat_wy : 4, wx = t->heat_wx > 0 ? t->heat_wx : 1;
         const bool ok = (wy == 4 && (wx == 1 || wx == 2 || wx == 3 || wx == 4)) ||
                         (wy == 8 && (wx == 1 || wx == 2)) || (wy == 16 && wx == 1) ||
-                        (wy == 2 && wx == 6);
+                        (wy == 2 && wx == 6) || (wy == 3 && wx == 3);
         B2R_CHECK(ok && t->heat_wx >= 0, B2R_E_ARG,
-                  "(heat_wy, heat_wx) must be (4, 1|2|3|4), (8, 1|2), (16, 1) or (2, 6)");
+                  "(heat_wy, heat_wx) must be (4, 1|2|3|4), (8, 1|2), (16, 1), (2, 6) or (3, 3)");
     }
     ctx->tune = *t;
     return B2R_OK;

@@ -383,14 +383,15 @@ static void launch_heat_wy(b2r_ctx* ctx, const float* in, float* out, const Heat
 
 // workgroup = 64*heat_wx x-lanes x heat_wy thread rows, each thread heat_ry
 // rows.  (heat_wy, heat_wx): (4, 1) by default, (4, 2), (4, 4), (8, 1), (8, 2),
-// (16, 1), and the 12-wave shapes (4, 3) and (2, 6), whose 192- and 384-cell
-// rows cover a 384-cell block row in 2 or 1 tiles
+// (16, 1), the 12-wave shapes (4, 3) and (2, 6), whose 192- and 384-cell rows
+// cover a 384-cell block row in 2 or 1 tiles, and the 9-wave (3, 3)
 template <int D, int MODE>
 static void launch_heat(b2r_ctx* ctx, const float* in, float* out, const HeatK& hk, int i)
 {
     const int wx = ctx->tune.heat_wx;
     switch (ctx->tune.heat_wy) {
     case 2: launch_heat_wy<D, MODE, 2, 6>(ctx, in, out, hk, i); break;
+    case 3: launch_heat_wy<D, MODE, 3, 3>(ctx, in, out, hk, i); break;
     case 8:
         if (wx == 2) launch_heat_wy<D, MODE, 8, 2>(ctx, in, out, hk, i);
         else launch_heat_wy<D, MODE, 8, 1>(ctx, in, out, hk, i);

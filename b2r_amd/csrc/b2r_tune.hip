@@ -52,7 +52,7 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
     const int zcs[] = {0, 8, 16, 32, 64, 128, 256};
     // heat also tries chunk lengths between the powers of two, whose workgroup
     // counts can fill the CUs' slots in whole rounds where no power of two does
-    const int zcs_heat[] = {0, 8, 16, 24, 32, 48, 64, 96, 128, 192, 256};
+    const int zcs_heat[] = {0, 8, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384};
     if (kind == 0 || kind == 3) {
         const int rys[] = {1, 2, 4, 8};
         const int wys[] = {4, 8, 16};
@@ -68,6 +68,7 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
                 for (int wx = 1; wx * wy <= 16; wx *= 2) shapes.push_back({wy, wx});
             shapes.push_back({4, 3});
             shapes.push_back({2, 6});
+            shapes.push_back({3, 3});
             for (const auto& sh : shapes)
                 for (int ry : rys)
                     for (int zc : zcs_heat) {

@@ -98,14 +98,16 @@ typedef struct b2r_tuning {
     int gol_zc;       /* z planes marched per fused Game of Life workgroup          */
     int gol_step_zc;  /* z planes marched per single-step Game of Life workgroup    */
     int el_zc;        /* z planes marched per elastic workgroup                     */
-    int heat_wy;      /* waves (thread rows) per heat step workgroup: 2, 4, 8, 16 */
+    int heat_wy;      /* waves (thread rows) per heat step workgroup: 2, 3, 4, 8,
+                         16                                                    */
     int gol_tile;     /* fused Game of Life kernel and tile (cells x rows incl. halo
                          rows): 1-3 byte-lane 128 x 32, 128 x 48, 256 x 56; 4-8
                          bit-sliced 256 x 32, 256 x 64, 256 x 96, 256 x 40,
                          256 x 72; 9, 10 bit-sliced 256 x 72, 256 x 136 with
                          two rows per thread; 0 = 5                             */
     int heat_wx;      /* waves across x per heat step workgroup; (heat_wy,
-                         heat_wx) is (4, 1|2|3|4), (8, 1|2), (16, 1) or (2, 6)
+                         heat_wx) is (4, 1|2|3|4), (8, 1|2), (16, 1), (2, 6) or
+                         (3, 3)
                          (heat_wy 0 counts as 4, heat_wx 0 as 1)               */
     int el_kernel;    /* batched elastic step: 1 one fused stress + velocity launch
                          from a second copy of the state (ping-pong), 2 separate

@@ -400,7 +400,7 @@ def test_gol_packed_first_and_last_launch(periodic, R, gens):
 
 
 @pytest.mark.parametrize("wy,wx", [(4, 1), (4, 2), (4, 4), (8, 1), (8, 2), (16, 1), (4, 3),
-                                   (2, 6)])
+                                   (2, 6), (3, 3)])
 @pytest.mark.parametrize("ry", [1, 2, 4, 8])
 @pytest.mark.parametrize("zc", [0, 5])
 def test_heat_tuning_variants(wy, wx, ry, zc):
