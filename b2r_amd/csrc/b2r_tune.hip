@@ -52,7 +52,7 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
     const int zcs[] = {0, 8, 16, 32, 64, 128, 256};
     // heat also tries chunk lengths between the powers of two, whose workgroup
     // counts can fill the CUs' slots in whole rounds where no power of two does
-    const int zcs_heat[] = {0, 8, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384};
+    const int zcs_heat[] = {0, 8, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 768};
     if (kind == 0 || kind == 3) {
         const int rys[] = {1, 2, 4, 8};
         const int wys[] = {4, 8, 16};
