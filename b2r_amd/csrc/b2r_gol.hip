@@ -165,12 +165,14 @@ __global__ __launch_bounds__(256) void gol_step_kernel(const uint8_t* __restrict
 // ---------------------------------------------------------------------------
 // Tile shapes: gol_tile 1, 2, 3 select this kernel with TXQ output quads (16
 // cells) per tile row and NR rows including the R halo rows per side
-// (GF_SHAPES); gol_tile 4-10 select the bit-sliced kernel below with NW
-// output words (32 cells) per row, NR rows and RT rows per thread
-// (GB_SHAPES); 0 (automatic) is the bit-sliced 8 x 64 shape.
+// (GF_SHAPES); gol_tile 4-12 select the bit-sliced kernel below with NW
+// output words (32 cells) per row, NR rows, RT rows per thread and XD = 1
+// for waves of whole rows (GB_SHAPES); 0 (automatic) is the bit-sliced 8 x 64
+// shape.
 constexpr int GF_SHAPES[3][2] = {{8, 32}, {8, 48}, {16, 56}};
-constexpr int GB_SHAPES[7][3] = {{8, 32, 1}, {8, 64, 1}, {8, 96, 1}, {8, 40, 1},
-                                 {8, 72, 1}, {8, 72, 2}, {8, 136, 2}};
+constexpr int GB_SHAPES[9][4] = {{8, 32, 1, 0}, {8, 64, 1, 0}, {8, 96, 1, 0}, {8, 40, 1, 0},
+                                 {8, 72, 1, 0}, {8, 72, 2, 0}, {8, 136, 2, 0}, {8, 72, 1, 1},
+                                 {8, 48, 1, 1}};
 constexpr int GF_PD = 2;  // input planes in flight per thread
 
 __device__ __forceinline__ uint4 q_add(uint4 a, uint4 b)
@@ -328,7 +330,7 @@ __global__ __launch_bounds__((TXQ + 2) * NR) void gol_fused_kernel(const uint8_t
 }
 
 // ---------------------------------------------------------------------------
-// Bit-sliced form of the fused kernel (gol_tile 0, 4..8): R generations per launch
+// Bit-sliced form of the fused kernel (gol_tile 0, 4..12): R generations per launch
 // as above, but a thread owns a 32-cell word of a row, bit i = cell x+i,
 // packed from the byte cells when a plane is loaded and unpacked when
 // generation R is stored.  Per generation and plane:
@@ -420,12 +422,25 @@ struct PGeo {
     int pw;                // words per packed row
 };
 
-template <int R, int NW, int NR, int IO = 0, int RT = 1>
-__global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint8_t* __restrict__ in,
-                                                                      uint8_t* __restrict__ out,
-                                                                      KGeo g, int ntx, int nty,
-                                                                      int nzc, int zc, int pull,
-                                                                      PGeo pg)
+// threads of a bit-sliced workgroup: one per word and RT rows, or with XD
+// whole waves of 64 / (NW + 2) rows each
+__host__ __device__ constexpr int gb_threads(int NW, int NR, int RT, int XD)
+{
+    return XD ? NR / (64 / (NW + 2)) * 64 : (NW + 2) * NR / RT;
+}
+
+// XD = 1: every wave holds whole rows (64 / NQ rows, the lanes past them
+// idle), so a word's x neighbours are the next lanes: the z-sums reach them by
+// wave-wide DPP lane shifts instead of a trip through LDS and a barrier, and
+// the x-sums alternate between two LDS buffers, so one barrier per
+// generation and plane remains instead of two.  A row's first and last words
+// are halo words; the shift brings them another row's word (or nothing at the
+// wave's ends), which only touches bits that no output cell reads within R <=
+// 4 generations, as with the LDS path's spare words.
+template <int R, int NW, int NR, int IO = 0, int RT = 1, int XD = 0>
+__global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
+    const uint8_t* __restrict__ in, uint8_t* __restrict__ out, KGeo g, int ntx, int nty, int nzc,
+    int zc, int pull, PGeo pg)
 {
     constexpr bool PKI = (IO & 1) != 0, PKO = (IO & 2) != 0;
     constexpr int NQ = NW + 2;  // words per row: plus one per side
@@ -434,12 +449,26 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
     // a thread owns RT rows of the tile, NR/RT apart: twice the work between
     // the two barriers of a generation at RT = 2, for half as many waves
     static_assert(NR % RT == 0, "rows per thread must divide the tile rows");
-    constexpr int NTH = NT / RT;       // threads
-    __shared__ uint2 sb[NT + 2];       // z-sums of a plane, a spare word before and after
-    __shared__ uint4 xb[NT + 2 * NQ];  // x-sums, a spare row above and below
+    constexpr int RW = 64 / NQ;  // rows per wave with XD
+    static_assert(!XD || (RT == 1 && NR % RW == 0), "XD: one row per thread, whole waves");
+    constexpr int NTH = NT / RT;  // (word, row) slots per row set
+    __shared__ uint2 sb[XD ? 1 : NT + 2];        // z-sums of a plane, a spare word before and after
+    __shared__ uint4 xb[XD ? 2 : 1][NT + 2 * NQ];  // x-sums, a spare row above and below
 
     const int tid = threadIdx.x;
-    const int q = tid % NQ;
+    // this thread's word q of its row r0 (of row set 0); lanes past a wave's
+    // rows are idle: they load nothing, store nothing and write no x-sums
+    int q, r0;
+    bool act = true;
+    if (XD) {
+        const int wv = tid >> 6, l = tid & 63;
+        act = l < RW * NQ;
+        q = act ? l % NQ : 0;
+        r0 = wv * RW + (act ? l / NQ : RW - 1);
+    } else {
+        q = tid % NQ;
+        r0 = tid / NQ;
+    }
     // workgroup order: x tile, then the block's x coordinate, then y tile, z
     // chunk and the block's y, z coordinates, so the tiles of x-neighbouring
     // blocks over the same rows run together on one XCD and a pulled halo word
@@ -478,9 +507,9 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
     const uint8_t* __restrict__ colown[RT];
 #pragma unroll
     for (int j = 0; j < RT; ++j) {
-        y[j] = y0 - R + tid / NQ + j * (NR / RT);
-        ld_ok[j] = PKI ? y[j] < g.S[1] && (x - h) / 32 + 1 < pg.pw
-                      : y[j] < g.S[1] && x + 31 >= 0 && x < g.S[0];
+        y[j] = y0 - R + r0 + j * (NR / RT);
+        ld_ok[j] = act && (PKI ? y[j] < g.S[1] && (x - h) / 32 + 1 < pg.pw
+                               : y[j] < g.S[1] && x + 31 >= 0 && x < g.S[0]);
         // every thread loads (no branch): threads outside the block read the
         // tile's first word of the last row, in bounds and aligned, and use zero
         col[j] = src + (ld_ok[j] ? (long long)y[j] * pitch + xbyte(x)
@@ -548,7 +577,7 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
         ma[j] = q == 0 ? 0u : ~0u;
         mb[j] = b_ld ? ~0u : 0u;
         const int r = y[j] - (y0 - R);
-        out_ok[j] = q >= 1 && q <= NQ - 2 && r >= R && r < R + TY && y[j] < h + g.B[1] &&
+        out_ok[j] = act && q >= 1 && q <= NQ - 2 && r >= R && r < R + TY && y[j] < h + g.B[1] &&
                     x < h + g.B[0];
     }
 
@@ -567,6 +596,7 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
     }
 
     uint32_t gen[RT][R][3];  // generation t at planes (newest-2, newest-1, newest)
+    int xp = 0;              // XD: the x-sum buffer of the next generation step
     const int zend = zc1 + R;
     // Odd chunks march down, even chunks up, so two neighbouring chunks reach
     // their shared halo planes at the same time and the second read of them
@@ -610,14 +640,26 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
                 const uint32_t ga = gen[j][t - 1][0], self = gen[j][t - 1][1], gc = gen[j][t - 1][2];
                 s0[j] = xor3(ga, self, gc);
                 s1[j] = maj3(ga, self, gc);
-                sb[tid + j * NTH + 1] = make_uint2(s0[j], s1[j]);
+                if (!XD) sb[tid + j * NTH + 1] = make_uint2(s0[j], s1[j]);
             }
-            __syncthreads();
+            if (!XD) __syncthreads();
+            uint4* const xbp = xb[XD ? xp : 0];
+            xp ^= 1;
             uint4 X[RT];
 #pragma unroll
             for (int j = 0; j < RT; ++j) {
-                const int pos = tid + j * NTH;
-                const uint2 L = sb[pos], Rg = sb[pos + 2];
+                const int pos = r0 * NQ + q + j * NTH;
+                uint2 L, Rg;
+                if (XD) {
+                    // words q-1 and q+1 of the row are lanes l-1 and l+1
+                    L = make_uint2(__builtin_amdgcn_update_dpp(0u, s0[j], 0x138, 0xf, 0xf, false),
+                                   __builtin_amdgcn_update_dpp(0u, s1[j], 0x138, 0xf, 0xf, false));
+                    Rg = make_uint2(__builtin_amdgcn_update_dpp(0u, s0[j], 0x130, 0xf, 0xf, false),
+                                    __builtin_amdgcn_update_dpp(0u, s1[j], 0x130, 0xf, 0xf, false));
+                } else {
+                    L = sb[pos];
+                    Rg = sb[pos + 2];
+                }
                 // x neighbours of the z-sums: bit i of l is cell x+i-1, of rr cell x+i+1
                 const uint32_t l0 = __builtin_amdgcn_alignbit(s0[j], L.x, 31),
                                l1 = __builtin_amdgcn_alignbit(s1[j], L.y, 31);
@@ -628,14 +670,14 @@ __global__ __launch_bounds__((NW + 2) * NR / RT) void gol_bits_kernel(const uint
                 const uint32_t u = xor3(l1, s1[j], r1), v = maj3(l1, s1[j], r1);
                 const uint32_t x1b = u ^ k1, w = u & k1;
                 X[j] = make_uint4(x0b, x1b, v ^ w, v & w);
-                xb[pos + NQ] = X[j];
+                if (act) xbp[pos + NQ] = X[j];
             }
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < RT; ++j) {
-                const int pos = tid + j * NTH;
+                const int pos = r0 * NQ + q + j * NTH;
                 const uint32_t self = gen[j][t - 1][1];
-                const uint4 U = xb[pos], Dn = xb[pos + 2 * NQ];
+                const uint4 U = xbp[pos], Dn = xbp[pos + 2 * NQ];
                 // box = U + X + Dn (each <= 9), five bit planes
                 const uint32_t b0 = xor3(U.x, X[j].x, Dn.x), c1 = maj3(U.x, X[j].x, Dn.x);
                 const uint32_t e1 = xor3(U.y, X[j].y, Dn.y), c2a = maj3(U.y, X[j].y, Dn.y);
@@ -782,7 +824,7 @@ static PGeo gol_pgeo(const KGeo& g)
     return pg;
 }
 
-template <int R, int NW, int NR, int RT, int IO>
+template <int R, int NW, int NR, int RT, int XD, int IO>
 static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
 {
     constexpr int TY = NR - 2 * R;
@@ -801,7 +843,7 @@ static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
     if (zc > g.B[2]) zc = g.B[2];
     const int nzc = ceil_div(g.B[2], zc);
     dim3 grid(ntx * nty * nzc * g.nblk);
-    gol_bits_kernel<R, NW, NR, IO, RT><<<grid, (NW + 2) * NR / RT, 0, ctx->stream>>>(
+    gol_bits_kernel<R, NW, NR, IO, RT, XD><<<grid, gb_threads(NW, NR, RT, XD), 0, ctx->stream>>>(
         (const uint8_t*)in, (uint8_t*)out, g, ntx, nty, nzc, zc, pull,
         IO ? gol_pgeo(g) : PGeo{});
     B2R_LAUNCH_CHECK();
@@ -811,8 +853,9 @@ static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
 template <int R, int IO>
 static int gol_bits_launch_r(b2r_ctx* ctx, const void* in, void* out)
 {
-#define GB_LAUNCH(i) \
-    gol_bits_launch_t<R, GB_SHAPES[i][0], GB_SHAPES[i][1], GB_SHAPES[i][2], IO>(ctx, in, out)
+#define GB_LAUNCH(i)                                                                        \
+    gol_bits_launch_t<R, GB_SHAPES[i][0], GB_SHAPES[i][1], GB_SHAPES[i][2], GB_SHAPES[i][3], IO>( \
+        ctx, in, out)
     switch (ctx->tune.gol_tile) {
     case 4: return GB_LAUNCH(0);
     case 6: return GB_LAUNCH(2);
@@ -820,6 +863,9 @@ static int gol_bits_launch_r(b2r_ctx* ctx, const void* in, void* out)
     case 8: return GB_LAUNCH(4);
     case 9: return GB_LAUNCH(5);
     case 10: return GB_LAUNCH(6);
+    case 11: return GB_LAUNCH(7);
+    case 12: return GB_LAUNCH(8);
+
     default: return GB_LAUNCH(1);
     }
 #undef GB_LAUNCH

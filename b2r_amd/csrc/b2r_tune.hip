@@ -81,7 +81,7 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
                     }
         }
     } else {
-        for (int tile = 1; tile <= (kind == 2 ? 10 : 1); ++tile)
+        for (int tile = 1; tile <= (kind == 2 ? 12 : 1); ++tile)
             for (int zc : zcs) {
                 b2r_tuning t = base;
                 if (kind == 1) {

@@ -142,10 +142,11 @@ def gol_bench(A, R, steps, warmup, fused=1, zc=0, record=None, exchange=False):
 
 # (output quads of 16 cells per tile row, rows incl. halo) of the gol_tile values
 GOL_TILES = {1: (8, 32), 2: (8, 48), 3: (16, 56)}
-# (output words of 32 cells per row, rows incl. halo) of the bit-sliced kernel
-# and rows per thread
-GOL_BIT_TILES = {0: (8, 64, 1), 4: (8, 32, 1), 5: (8, 64, 1), 6: (8, 96, 1), 7: (8, 40, 1),
-                 8: (8, 72, 1), 9: (8, 72, 2), 10: (8, 136, 2)}
+# (output words of 32 cells per row, rows incl. halo, rows per thread, waves of
+# whole rows) of the bit-sliced kernel
+GOL_BIT_TILES = {0: (8, 64, 1, 0), 4: (8, 32, 1, 0), 5: (8, 64, 1, 0), 6: (8, 96, 1, 0),
+                 7: (8, 40, 1, 0), 8: (8, 72, 1, 0), 9: (8, 72, 2, 0), 10: (8, 136, 2, 0),
+                 11: (8, 72, 1, 1), 12: (8, 48, 1, 1)}
 
 
 def gol_kernel_name(R, tile, packed=True):
@@ -153,8 +154,8 @@ def gol_kernel_name(R, tile, packed=True):
     first launch of a run packs as it loads, IO 2, the last unpacks as it
     stores, IO 1), IO 0 on byte fields."""
     if tile in GOL_BIT_TILES:
-        nw, nr, rt = GOL_BIT_TILES[tile]
-        return "gol_bits_kernel<%d, %d, %d, %d, %d>" % (R, nw, nr, 3 if packed else 0, rt)
+        nw, nr, rt, xd = GOL_BIT_TILES[tile]
+        return "gol_bits_kernel<%d, %d, %d, %d, %d, %d>" % (R, nw, nr, 3 if packed else 0, rt, xd)
     return "gol_fused_kernel<%d, %d, %d>" % ((R,) + GOL_TILES[tile])
 
 

@@ -104,7 +104,9 @@ typedef struct b2r_tuning {
                          rows): 1-3 byte-lane 128 x 32, 128 x 48, 256 x 56; 4-8
                          bit-sliced 256 x 32, 256 x 64, 256 x 96, 256 x 40,
                          256 x 72; 9, 10 bit-sliced 256 x 72, 256 x 136 with
-                         two rows per thread; 0 = 5                             */
+                         two rows per thread; 11, 12 bit-sliced 256 x 72,
+                         256 x 48 with waves of whole rows (x neighbours by
+                         lane shifts); 0 = 5                                   */
     int heat_wx;      /* waves across x per heat step workgroup; (heat_wy,
                          heat_wx) is (4, 1|2|3|4), (8, 1|2), (16, 1), (2, 6) or
                          (3, 3)
@@ -276,7 +278,7 @@ int b2r_gol_step(b2r_ctx* ctx, const void* d_in, void* d_out, int b2r_i);
  * alternating the two fields (the loop of gol-3D.cu:138-145 plus the driver's
  * exchange, b2r.cpp:179-221), or with fused = 1 (periodic, or every face
  * frozen; R <= 4) one R-generation launch per window.  Where the bit-sliced
- * kernel runs (B_x % 32 == 0, gol_tile 0 or 4-10) no exchange runs: it reads
+ * kernel runs (B_x % 32 == 0, gol_tile 0 or 4-12) no exchange runs: it reads
  * its halo from the neighbouring blocks, and unless gol_bytes is set the
  * windows run on the context's two bit-packed fields (allocated on the first
  * such call), nsteps mod R generations at the end as one more, shorter

@@ -280,7 +280,7 @@ def test_gol_full_size_R_invariance_and_population():
     env = (512, 512, 512)
     outs = {}
     for R in (1, 2, 4):
-        for fused, tile in ((0, 0), (1, 1), (1, 5)):
+        for fused, tile in ((0, 0), (1, 1), (1, 5), (1, 11)):
             say("gol 512^3 R", R, "fused", fused, "tile", tile)
             with A.Context(env, (2, 2, 2), R, 1, 1, 1) as c:
                 c.set_tuning(gol_tile=tile)
@@ -294,7 +294,7 @@ def test_gol_full_size_R_invariance_and_population():
         assert np.array_equal(o, want), key
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12])
 @pytest.mark.parametrize("zc", [4, 9, 64])
 def test_gol_fused_chunking(zc, tile):
     # z-chunk lengths that do not divide B, and a single chunk, for every tile shape
@@ -310,7 +310,7 @@ def test_gol_fused_chunking(zc, tile):
 
 
 @pytest.mark.parametrize("gol_bytes", [0, 1])
-@pytest.mark.parametrize("tile", [6, 7, 8, 9, 10, 3])
+@pytest.mark.parametrize("tile", [6, 7, 8, 9, 10, 11, 12, 3])
 @pytest.mark.parametrize("R", [1, 2, 4])
 @pytest.mark.parametrize("periodic", [False, True])
 @pytest.mark.parametrize("env,blocks", [((64, 48, 40), (2, 3, 2)), ((64, 30, 28), (1, 2, 2)),
@@ -343,8 +343,9 @@ def test_gol_bits_halo_from_neighbours(env, blocks, periodic, R, tile, gol_bytes
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("tile", [8, 11])
 @pytest.mark.parametrize("periodic", [False, True])
-def test_gol_packed_runs_in_sequence(periodic):
+def test_gol_packed_runs_in_sequence(periodic, tile):
     # Two runs on the same fields and the fields of another context in
     # between: each run packs its input again (the packed pair is the
     # context's), whole windows only, then windows plus single steps, and the
@@ -353,10 +354,10 @@ def test_gol_packed_runs_in_sequence(periodic):
     u = O.bernoulli((env[2], env[1], env[0]), 12, 0.3)
     with A.Context(env, blocks, R, 1, 1 if periodic else 0, 1) as c, \
             A.Context((64, 32, 32), (1, 1, 1), 2, 1, 1, 1) as c2:
-        c.set_tuning(gol_tile=8, gol_zc=0, gol_bytes=0)
+        c.set_tuning(gol_tile=tile, gol_zc=0, gol_bytes=0)
         a, b = c.field(), c.field()
         c.upload(a, u)
-        say("gol packed run 1, periodic", periodic)
+        say("gol packed run 1, periodic", periodic, "tile", tile)
         res, _ = c.gol_run(a, b, 8, fused=1)
         x, y = c2.field(), c2.field()
         c2.init_bernoulli(x, 3, int(0.3 * 16777216))
@@ -376,18 +377,19 @@ def test_gol_packed_runs_in_sequence(periodic):
     assert all(np.array_equal(p, q) for p, q in zip(before, after))
 
 
+@pytest.mark.parametrize("tile", [8, 11])
 @pytest.mark.parametrize("periodic", [False, True])
 @pytest.mark.parametrize("R,gens", [(4, 1), (4, 3), (4, 4), (4, 5), (2, 3), (1, 1), (1, 2)])
-def test_gol_packed_first_and_last_launch(periodic, R, gens):
+def test_gol_packed_first_and_last_launch(periodic, R, gens, tile):
     # Runs of one and two launches on packed fields: the last launch unpacks
     # as it stores; with no frozen face the first packs as it loads (a single
     # launch then goes from byte field to byte field), with frozen faces a
     # pack pass fills the packed frame first.
     env, blocks = (128, 40, 36), (2, 2, 1)
     u = O.bernoulli((env[2], env[1], env[0]), 21, 0.3)
-    say("gol packed first/last launch, R", R, "gens", gens, "periodic", periodic)
+    say("gol packed first/last launch, R", R, "gens", gens, "periodic", periodic, "tile", tile)
     with A.Context(env, blocks, R, 1, 1 if periodic else 0, 1) as c:
-        c.set_tuning(gol_tile=8, gol_zc=0, gol_bytes=0)
+        c.set_tuning(gol_tile=tile, gol_zc=0, gol_bytes=0)
         a, b = c.field(), c.field()
         c.upload(a, u)
         res, _ = c.gol_run(a, b, gens, fused=1)
@@ -418,7 +420,7 @@ def test_heat_tuning_variants(wy, wx, ry, zc):
     assert bits_equal(got, want)
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12])
 @pytest.mark.parametrize("R", [1, 2, 3, 4])
 @pytest.mark.parametrize("periodic", [False, True])
 def test_gol_fused_tile_shapes(tile, R, periodic):
