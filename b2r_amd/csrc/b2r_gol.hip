@@ -652,10 +652,12 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
                 uint2 L, Rg;
                 if (XD) {
                     // words q-1 and q+1 of the row are lanes l-1 and l+1
-                    L = make_uint2(__builtin_amdgcn_update_dpp(0u, s0[j], 0x138, 0xf, 0xf, false),
-                                   __builtin_amdgcn_update_dpp(0u, s1[j], 0x138, 0xf, 0xf, false));
-                    Rg = make_uint2(__builtin_amdgcn_update_dpp(0u, s0[j], 0x130, 0xf, 0xf, false),
-                                    __builtin_amdgcn_update_dpp(0u, s1[j], 0x130, 0xf, 0xf, false));
+                    // (bound_ctrl: the wave's first and last lanes, halo
+                    // words, read 0, so no old value is set up)
+                    L = make_uint2(__builtin_amdgcn_mov_dpp(s0[j], 0x138, 0xf, 0xf, true),
+                                   __builtin_amdgcn_mov_dpp(s1[j], 0x138, 0xf, 0xf, true));
+                    Rg = make_uint2(__builtin_amdgcn_mov_dpp(s0[j], 0x130, 0xf, 0xf, true),
+                                    __builtin_amdgcn_mov_dpp(s1[j], 0x130, 0xf, 0xf, true));
                 } else {
                     L = sb[pos];
                     Rg = sb[pos + 2];
