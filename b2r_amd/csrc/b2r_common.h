@@ -96,6 +96,9 @@ int b2r_gol_fused_launch(b2r_ctx* ctx, const void* in, void* out, int packed);
 int b2r_gol_fused_reads_halo(const b2r_ctx* ctx);
 // 1: fused Game of Life runs keep their fields bit-packed between launches
 int b2r_gol_fused_packed(const b2r_ctx* ctx);
+// 1: gol_tile `tile` runs as itself on this context's geometry (the
+// whole-row tiles 13-15 need a grid periodic in x over 512 cells)
+int b2r_gol_tile_ok(const b2r_ctx* ctx, int tile);
 // The context's two packed fields (allocated on first use), both packed from
 // the byte field `from`, halo included (from = NULL: allocated only, contents
 // unspecified).

@@ -230,7 +230,7 @@ extern "C" int b2r_set_tuning(b2r_ctx* ctx, const b2r_tuning* t)
     B2R_CHECK(t->heat_wy == 0 || t->heat_wy == 2 || t->heat_wy == 3 || t->heat_wy == 4 ||
                   t->heat_wy == 8 || t->heat_wy == 16,
               B2R_E_ARG, "heat_wy must be 0, 2, 3, 4, 8 or 16");
-    B2R_CHECK(t->gol_tile >= 0 && t->gol_tile <= 12, B2R_E_ARG, "gol_tile must be 0..12");
+    B2R_CHECK(t->gol_tile >= 0 && t->gol_tile <= 15, B2R_E_ARG, "gol_tile must be 0..15");
     B2R_CHECK(t->el_kernel >= 0 && t->el_kernel <= 2, B2R_E_ARG, "el_kernel must be 0, 1 or 2");
     B2R_CHECK(t->el_grad_mb >= 0, B2R_E_ARG, "el_grad_mb must be >= 0");
     B2R_CHECK(t->gol_bytes == 0 || t->gol_bytes == 1, B2R_E_ARG, "gol_bytes must be 0 or 1");

@@ -165,14 +165,14 @@ __global__ __launch_bounds__(256) void gol_step_kernel(const uint8_t* __restrict
 // ---------------------------------------------------------------------------
 // Tile shapes: gol_tile 1, 2, 3 select this kernel with TXQ output quads (16
 // cells) per tile row and NR rows including the R halo rows per side
-// (GF_SHAPES); gol_tile 4-12 select the bit-sliced kernel below with NW
+// (GF_SHAPES); gol_tile 4-15 select the bit-sliced kernel below with NW
 // output words (32 cells) per row, NR rows, RT rows per thread and XD = 1
-// for waves of whole rows (GB_SHAPES); 0 (automatic) is the bit-sliced 8 x 64
-// shape.
+// for waves of whole block rows, XD = 2 for waves of whole periodic grid rows
+// (GB_SHAPES); 0 (automatic) is the bit-sliced 8 x 64 shape.
 constexpr int GF_SHAPES[3][2] = {{8, 32}, {8, 48}, {16, 56}};
-constexpr int GB_SHAPES[9][4] = {{8, 32, 1, 0}, {8, 64, 1, 0}, {8, 96, 1, 0}, {8, 40, 1, 0},
-                                 {8, 72, 1, 0}, {8, 72, 2, 0}, {8, 136, 2, 0}, {8, 72, 1, 1},
-                                 {8, 48, 1, 1}};
+constexpr int GB_SHAPES[12][4] = {{8, 32, 1, 0},  {8, 64, 1, 0},  {8, 96, 1, 0},  {8, 40, 1, 0},
+                                  {8, 72, 1, 0},  {8, 72, 2, 0},  {8, 136, 2, 0}, {8, 72, 1, 1},
+                                  {8, 48, 1, 1},  {16, 72, 2, 2}, {16, 40, 1, 2}, {16, 64, 1, 2}};
 constexpr int GF_PD = 2;  // input planes in flight per thread
 
 __device__ __forceinline__ uint4 q_add(uint4 a, uint4 b)
@@ -330,7 +330,7 @@ __global__ __launch_bounds__((TXQ + 2) * NR) void gol_fused_kernel(const uint8_t
 }
 
 // ---------------------------------------------------------------------------
-// Bit-sliced form of the fused kernel (gol_tile 0, 4..12): R generations per launch
+// Bit-sliced form of the fused kernel (gol_tile 0, 4..15): R generations per launch
 // as above, but a thread owns a 32-cell word of a row, bit i = cell x+i,
 // packed from the byte cells when a plane is loaded and unpacked when
 // generation R is stored.  Per generation and plane:
@@ -400,6 +400,60 @@ __device__ __forceinline__ uint32_t maj3(uint32_t a, uint32_t b, uint32_t c)
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0xE8);
 }
 
+// The x-sum of a word's z-sums (s0, s1: two bit planes, 0..3) and those of
+// its x neighbours (L: word x-32, Rg: word x+32): bit i of l is cell x+i-1,
+// of rr cell x+i+1; XZ = l + S + rr in four bit planes (0..9).
+__device__ __forceinline__ uint4 gb_xsum(uint32_t s0, uint32_t s1, uint2 L, uint2 Rg)
+{
+    const uint32_t l0 = __builtin_amdgcn_alignbit(s0, L.x, 31),
+                   l1 = __builtin_amdgcn_alignbit(s1, L.y, 31);
+    const uint32_t r0 = __builtin_amdgcn_alignbit(Rg.x, s0, 1),
+                   r1 = __builtin_amdgcn_alignbit(Rg.y, s1, 1);
+    const uint32_t x0b = xor3(l0, s0, r0), k1 = maj3(l0, s0, r0);
+    const uint32_t u = xor3(l1, s1, r1), v = maj3(l1, s1, r1);
+    const uint32_t x1b = u ^ k1, w = u & k1;
+    return make_uint4(x0b, x1b, v ^ w, v & w);
+}
+
+// The rule on box = U + X + Dn (x-sums of rows y-1, y, y+1, each <= 9): alive
+// next iff box == 5, or alive and 3 <= box <= 8.  The sum leaves bits b0..b3
+// and the two carries out of bit 3, c4a and c4b (box >= 16 iff either is
+// set; box < 32, so never both).  Below 16, with b3 set box is 8..15 and the
+// cell lives on iff it is alive and box == 8 (b2 b1 b0 = 0); with b3 clear
+// box is 0..7 and the cell is alive next iff box == 5, or it is alive and
+// box >= 3 (checked over all inputs 0..9 and both cell states).
+__device__ __forceinline__ uint32_t gb_rule(uint4 U, uint4 X, uint4 Dn, uint32_t self)
+{
+    const uint32_t b0 = xor3(U.x, X.x, Dn.x), c1 = maj3(U.x, X.x, Dn.x);
+    const uint32_t e1 = xor3(U.y, X.y, Dn.y), c2a = maj3(U.y, X.y, Dn.y);
+    const uint32_t b1 = e1 ^ c1, c2b = e1 & c1;
+    const uint32_t e2 = xor3(U.z, X.z, Dn.z), c3a = maj3(U.z, X.z, Dn.z);
+    const uint32_t b2 = xor3(e2, c2a, c2b), c3b = maj3(e2, c2a, c2b);
+    const uint32_t e3 = xor3(U.w, X.w, Dn.w), c4a = maj3(U.w, X.w, Dn.w);
+    const uint32_t b3 = xor3(e3, c3a, c3b), c4b = maj3(e3, c3a, c3b);
+    const uint32_t lo = b2 | b1 | b0;
+    const uint32_t ge3 = b2 | (b1 & b0);
+    const uint32_t eq5 = b2 & ~b1 & b0;
+    const uint32_t m = eq5 | (self & ge3);
+    const uint32_t z = b3 & self & ~lo;
+    const uint32_t tt = z | (~b3 & m);
+    return tt & ~c4a & ~c4b;
+}
+
+// Whole periodic grid rows of 16 words, one per 16-lane DPP row: the z-sums
+// of words q-1 and q+1 of the row are those of lanes (q-1) mod 16 and
+// (q+1) mod 16 of the DPP row (row_ror:1, row_ror:15).
+__device__ __forceinline__ uint2 gb_row_prev(uint32_t s0, uint32_t s1)
+{
+    return make_uint2(__builtin_amdgcn_mov_dpp(s0, 0x121, 0xf, 0xf, true),
+                      __builtin_amdgcn_mov_dpp(s1, 0x121, 0xf, 0xf, true));
+}
+__device__ __forceinline__ uint2 gb_row_next(uint32_t s0, uint32_t s1)
+{
+    return make_uint2(__builtin_amdgcn_mov_dpp(s0, 0x12F, 0xf, 0xf, true),
+                      __builtin_amdgcn_mov_dpp(s1, 0x12F, 0xf, 0xf, true));
+}
+
 // bits of the cells x .. x+31 inside [lo, hi]
 __device__ __forceinline__ uint32_t word_mask(int x, int lo, int hi)
 {
@@ -423,10 +477,10 @@ struct PGeo {
 };
 
 // threads of a bit-sliced workgroup: one per word and RT rows, or with XD
-// whole waves of 64 / (NW + 2) rows each
+// whole waves of 64 / (NW + 2) rows each (XD = 2: 64 / NW rows, no halo words)
 __host__ __device__ constexpr int gb_threads(int NW, int NR, int RT, int XD)
 {
-    return XD ? NR / (64 / (NW + 2)) * 64 : (NW + 2) * NR / RT;
+    return XD ? NR / RT / (64 / (XD == 2 ? NW : NW + 2)) * 64 : (NW + 2) * NR / RT;
 }
 
 // XD = 1: every wave holds whole rows (64 / NQ rows, the lanes past them
@@ -437,20 +491,29 @@ __host__ __device__ constexpr int gb_threads(int NW, int NR, int RT, int XD)
 // are halo words; the shift brings them another row's word (or nothing at the
 // wave's ends), which only touches bits that no output cell reads within R <=
 // 4 generations, as with the LDS path's spare words.
+// XD = 2 (gol_tile 13-15): the grid is periodic in x over 16 words (512
+// cells, the x blocks side by side), and every wave holds 4 whole grid rows,
+// one per 16-lane DPP row, lane q of a row holding word q of the grid row
+// (block q / (B_x/32)).  A word's x neighbours, the periodic wrap included,
+// come from a rotation within the DPP row (row_ror), so no lane holds a halo
+// word: all 64 lanes own output, against 48 of 64 with XD = 1.
 template <int R, int NW, int NR, int IO = 0, int RT = 1, int XD = 0>
 __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
     const uint8_t* __restrict__ in, uint8_t* __restrict__ out, KGeo g, int ntx, int nty, int nzc,
     int zc, int pull, PGeo pg)
 {
     constexpr bool PKI = (IO & 1) != 0, PKO = (IO & 2) != 0;
-    constexpr int NQ = NW + 2;  // words per row: plus one per side
+    constexpr bool WR = XD == 2;          // whole periodic grid rows
+    constexpr int NQ = WR ? NW : NW + 2;  // words per row: plus one per side
     constexpr int NT = NQ * NR;
     constexpr int TY = NR - 2 * R;
     // a thread owns RT rows of the tile, NR/RT apart: twice the work between
     // the two barriers of a generation at RT = 2, for half as many waves
     static_assert(NR % RT == 0, "rows per thread must divide the tile rows");
     constexpr int RW = 64 / NQ;  // rows per wave with XD
-    static_assert(!XD || (RT == 1 && NR % RW == 0), "XD: one row per thread, whole waves");
+    static_assert(!XD || ((WR || RT == 1) && (NR / RT) % RW == 0),
+                  "XD: whole waves (XD = 1: one row per thread)");
+    static_assert(!WR || NW == 16, "XD = 2: a grid row is one 16-lane DPP row");
     constexpr int NTH = NT / RT;  // (word, row) slots per row set
     __shared__ uint2 sb[XD ? 1 : NT + 2];        // z-sums of a plane, a spare word before and after
     __shared__ uint4 xb[XD ? 2 : 1][NT + 2 * NQ];  // x-sums, a spare row above and below
@@ -460,7 +523,10 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
     // rows are idle: they load nothing, store nothing and write no x-sums
     int q, r0;
     bool act = true;
-    if (XD) {
+    if (WR) {
+        q = tid & 15;
+        r0 = tid >> 4;
+    } else if (XD) {
         const int wv = tid >> 6, l = tid & 63;
         act = l < RW * NQ;
         q = act ? l % NQ : 0;
@@ -475,17 +541,26 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
     // is found in the L2 their owner filled (counting y tiles and z chunks
     // across blocks as well, so that y and z neighbours across a block face
     // also run together, was measured slower: 4589 against 4727 GLUP/s at R=4)
+    // (XD = 2: one tile spans the x blocks, the lane's word picks its block)
     int tt = xcd_tile(blockIdx.x, gridDim.x);
-    const int txi = tt % ntx;
-    tt /= ntx;
-    const int bxi = tt % g.nb[0];
-    tt /= g.nb[0];
+    int txi = 0, bxi;
+    if (WR) {
+        bxi = q / (g.B[0] >> 5);
+    } else {
+        txi = tt % ntx;
+        tt /= ntx;
+        bxi = tt % g.nb[0];
+        tt /= g.nb[0];
+    }
     const int tyi = tt % nty;
     tt /= nty;
     const int zci = tt % nzc;
     const int b = tt / nzc * g.nb[0] + bxi;
+    // update boxes along y and z are those of every block of the tile's y, z
+    // coordinates; along x (periodic with XD = 2) all blocks' boxes agree
+    const int bu = WR ? tt / nzc * g.nb[0] : b;
     const int h = g.h;
-    const int x0 = h + txi * 32 * NW;  // first output cell
+    const int x0 = h + txi * 32 * NW;  // first output cell (XD = 2: of the lane's block)
     const int y0 = h + tyi * TY;
     const int zc0 = h + zci * zc;
     const int zc1 = min(zc0 + zc, h + g.B[2]);
@@ -500,7 +575,7 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
     uint8_t* __restrict__ dst = out + boff_o;
 
     // this thread's words: x = x .. x+31, rows y[j]
-    const int x = x0 + 32 * (q - 1);
+    const int x = WR ? x0 + 32 * (q - bxi * (g.B[0] >> 5)) : x0 + 32 * (q - 1);
     int y[RT];
     bool ld_ok[RT];
     const uint8_t* __restrict__ col[RT];
@@ -568,16 +643,18 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
     uint32_t ldm[RT], ma[RT], mb[RT];
     int oa[RT], ob[RT];
     bool out_ok[RT];
+    // (XD = 2 has no halo words: every word loads both halves)
+    const bool hw0 = !WR && q == 0, hw1 = !WR && q == NQ - 1;
 #pragma unroll
     for (int j = 0; j < RT; ++j) {
         ldm[j] = ld_ok[j] ? ~0u : 0u;
-        oa[j] = q == 0 && ld_ok[j] ? 16 : 0;  // the clamped word reads its first half
-        const bool b_ld = ld_ok[j] && q != NQ - 1 && hi_in;
+        oa[j] = hw0 && ld_ok[j] ? 16 : 0;  // the clamped word reads its first half
+        const bool b_ld = ld_ok[j] && !hw1 && hi_in;
         ob[j] = b_ld ? 16 : oa[j];
-        ma[j] = q == 0 ? 0u : ~0u;
+        ma[j] = hw0 ? 0u : ~0u;
         mb[j] = b_ld ? ~0u : 0u;
         const int r = y[j] - (y0 - R);
-        out_ok[j] = act && q >= 1 && q <= NQ - 2 && r >= R && r < R + TY && y[j] < h + g.B[1] &&
+        out_ok[j] = act && !hw0 && !hw1 && r >= R && r < R + TY && y[j] < h + g.B[1] &&
                     x < h + g.B[0];
     }
 
@@ -586,10 +663,12 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
     uint32_t xm[R];  // bits inside generation t's box along x
 #pragma unroll
     for (int t = 1; t <= R; ++t) {
-        bx[t - 1] = update_box(g, b, 1, t - 1);
+        bx[t - 1] = update_box(g, bu, 1, t - 1);
         const int gr = R - t;
         const Box& B_ = bx[t - 1];
-        clip[t - 1] = x0 - 32 < B_.lo[0] || x0 + 32 * (NW + 1) - 1 > B_.hi[0] ||
+        // (XD = 2: the tile's words are the blocks' owned words, inside every
+        // box along the periodic x)
+        clip[t - 1] = (!WR && (x0 - 32 < B_.lo[0] || x0 + 32 * (NW + 1) - 1 > B_.hi[0])) ||
                       y0 - gr < B_.lo[1] || y0 + TY + gr - 1 > B_.hi[1] ||
                       zc0 - gr < B_.lo[2] || zc1 + gr - 1 > B_.hi[2];
         xm[t - 1] = word_mask(x, B_.lo[0], B_.hi[0]);
@@ -629,6 +708,36 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
                 lb = *reinterpret_cast<const uint4*>(pp + ob[j]);
             }
         }
+        // step t (plane p) of row j from its new value: the cell keeps its
+        // value outside the generation's box, then the value joins generation
+        // t's planes or, for generation R, is stored
+        auto finish = [&](int t, int j, int p, uint32_t nw, uint32_t self) {
+            if (clip[t - 1]) {
+                const Box& B_ = bx[t - 1];
+                const bool in_yz =
+                    y[j] >= B_.lo[1] && y[j] <= B_.hi[1] && p >= B_.lo[2] && p <= B_.hi[2];
+                const uint32_t m = in_yz ? xm[t - 1] : 0u;
+                nw = (nw & m) | (self & ~m);
+            }
+            if (t < R) {
+                gen[j][t][0] = gen[j][t][1];
+                gen[j][t][1] = gen[j][t][2];
+                gen[j][t][2] = nw;
+            } else if (out_ok[j]) {
+                if (PKO) {
+                    *reinterpret_cast<uint32_t*>(dst + (long long)p * slice_o +
+                                                 (long long)y[j] * pitch_o +
+                                                 4ll * ((x - h) / 32 + 1)) = nw;
+                } else {
+                    uint4 lo, hi;
+                    gb_unpack(nw, lo, hi);
+                    uint4* o = reinterpret_cast<uint4*>(dst + (long long)p * slice_o +
+                                                        (long long)y[j] * pitch_o + x);
+                    o[0] = lo;
+                    if (hi_in) o[1] = hi;
+                }
+            }
+        };
 #pragma unroll
         for (int t = 1; t <= R; ++t) {
             // uniform: generation t-1 not yet 3 planes, or a plane past the chunk
@@ -650,7 +759,10 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
             for (int j = 0; j < RT; ++j) {
                 const int pos = r0 * NQ + q + j * NTH;
                 uint2 L, Rg;
-                if (XD) {
+                if (WR) {
+                    L = gb_row_prev(s0[j], s1[j]);
+                    Rg = gb_row_next(s0[j], s1[j]);
+                } else if (XD) {
                     // words q-1 and q+1 of the row are lanes l-1 and l+1
                     // (bound_ctrl: the wave's first and last lanes, halo
                     // words, read 0, so no old value is set up)
@@ -662,16 +774,7 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
                     L = sb[pos];
                     Rg = sb[pos + 2];
                 }
-                // x neighbours of the z-sums: bit i of l is cell x+i-1, of rr cell x+i+1
-                const uint32_t l0 = __builtin_amdgcn_alignbit(s0[j], L.x, 31),
-                               l1 = __builtin_amdgcn_alignbit(s1[j], L.y, 31);
-                const uint32_t r0 = __builtin_amdgcn_alignbit(Rg.x, s0[j], 1),
-                               r1 = __builtin_amdgcn_alignbit(Rg.y, s1[j], 1);
-                // XZ = l + S + rr, four bit planes (0..9)
-                const uint32_t x0b = xor3(l0, s0[j], r0), k1 = maj3(l0, s0[j], r0);
-                const uint32_t u = xor3(l1, s1[j], r1), v = maj3(l1, s1[j], r1);
-                const uint32_t x1b = u ^ k1, w = u & k1;
-                X[j] = make_uint4(x0b, x1b, v ^ w, v & w);
+                X[j] = gb_xsum(s0[j], s1[j], L, Rg);
                 if (act) xbp[pos + NQ] = X[j];
             }
             __syncthreads();
@@ -679,46 +782,7 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
             for (int j = 0; j < RT; ++j) {
                 const int pos = r0 * NQ + q + j * NTH;
                 const uint32_t self = gen[j][t - 1][1];
-                const uint4 U = xbp[pos], Dn = xbp[pos + 2 * NQ];
-                // box = U + X + Dn (each <= 9), five bit planes
-                const uint32_t b0 = xor3(U.x, X[j].x, Dn.x), c1 = maj3(U.x, X[j].x, Dn.x);
-                const uint32_t e1 = xor3(U.y, X[j].y, Dn.y), c2a = maj3(U.y, X[j].y, Dn.y);
-                const uint32_t b1 = e1 ^ c1, c2b = e1 & c1;
-                const uint32_t e2 = xor3(U.z, X[j].z, Dn.z), c3a = maj3(U.z, X[j].z, Dn.z);
-                const uint32_t b2 = xor3(e2, c2a, c2b), c3b = maj3(e2, c2a, c2b);
-                const uint32_t e3 = xor3(U.w, X[j].w, Dn.w), c4a = maj3(U.w, X[j].w, Dn.w);
-                const uint32_t b3 = xor3(e3, c3a, c3b), c4b = maj3(e3, c3a, c3b);
-                const uint32_t b4 = c4a | c4b;
-                // rule: box == 5, or alive and 3 <= box <= 8
-                const uint32_t le8 = ~b4 & ~(b3 & (b2 | b1 | b0));
-                const uint32_t ge3 = b4 | b3 | b2 | (b1 & b0);
-                const uint32_t eq5 = ~b4 & ~b3 & b2 & ~b1 & b0;
-                uint32_t nw = eq5 | (self & le8 & ge3);
-                if (clip[t - 1]) {
-                    const Box& B_ = bx[t - 1];
-                    const bool in_yz = y[j] >= B_.lo[1] && y[j] <= B_.hi[1] && p >= B_.lo[2] &&
-                                       p <= B_.hi[2];
-                    const uint32_t m = in_yz ? xm[t - 1] : 0u;
-                    nw = (nw & m) | (self & ~m);
-                }
-                if (t < R) {
-                    gen[j][t][0] = gen[j][t][1];
-                    gen[j][t][1] = gen[j][t][2];
-                    gen[j][t][2] = nw;
-                } else if (out_ok[j]) {
-                    if (PKO) {
-                        *reinterpret_cast<uint32_t*>(dst + (long long)p * slice_o +
-                                                     (long long)y[j] * pitch_o +
-                                                     4ll * ((x - h) / 32 + 1)) = nw;
-                    } else {
-                        uint4 lo, hi;
-                        gb_unpack(nw, lo, hi);
-                        uint4* o = reinterpret_cast<uint4*>(dst + (long long)p * slice_o +
-                                                            (long long)y[j] * pitch_o + x);
-                        o[0] = lo;
-                        if (hi_in) o[1] = hi;
-                    }
-                }
+                finish(t, j, p, gb_rule(xbp[pos], X[j], xbp[pos + 2 * NQ], self), self);
             }
         }
     };
@@ -778,6 +842,19 @@ __global__ __launch_bounds__(256) void gol_pack_kernel(const uint8_t* __restrict
 // above) when the block's owned x range is a whole number of 32-cell words.
 static int gol_bits_pull(const KGeo& g) { return g.B[0] % 32 == 0; }
 
+// The whole-row tiles (gol_tile 13-15, XD = 2) need a grid periodic in x over
+// 16 words: B_x a multiple of 32 and nb_x * B_x = 512.  Elsewhere those tiles
+// run as tile 11.
+static bool gol_whole_rows(const KGeo& g)
+{
+    return (g.periodic & 1) && g.B[0] % 32 == 0 && g.nb[0] * g.B[0] == 512;
+}
+
+int b2r_gol_tile_ok(const b2r_ctx* ctx, int tile)
+{
+    return tile < 13 || gol_whole_rows(ctx->k);
+}
+
 int b2r_gol_fused_reads_halo(const b2r_ctx* ctx)
 {
     const int t = ctx->tune.gol_tile;
@@ -833,9 +910,11 @@ static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
     const KGeo& g = ctx->k;
     // with gol_exchange the run exchanges first and the kernel reads its own halo
     const int pull = gol_bits_pull(g) && !ctx->tune.gol_exchange;
-    const int ntx = ceil_div(g.B[0], 32 * NW);
+    // XD = 2: one tile spans the x blocks of its rows
+    const int ntx = XD == 2 ? 1 : ceil_div(g.B[0], 32 * NW);
     const int nty = ceil_div(g.B[1], TY);
-    long long tiles = (long long)ntx * nty * g.nblk;
+    const int nbt = XD == 2 ? g.nblk / g.nb[0] : g.nblk;  // blocks (sets) per tile grid
+    long long tiles = (long long)ntx * nty * nbt;
     int zc = ctx->tune.gol_zc;
     if (zc <= 0) {
         int nzc = (int)((2048 + tiles - 1) / tiles);
@@ -844,7 +923,7 @@ static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
     }
     if (zc > g.B[2]) zc = g.B[2];
     const int nzc = ceil_div(g.B[2], zc);
-    dim3 grid(ntx * nty * nzc * g.nblk);
+    dim3 grid(ntx * nty * nzc * nbt);
     gol_bits_kernel<R, NW, NR, IO, RT, XD><<<grid, gb_threads(NW, NR, RT, XD), 0, ctx->stream>>>(
         (const uint8_t*)in, (uint8_t*)out, g, ntx, nty, nzc, zc, pull,
         IO ? gol_pgeo(g) : PGeo{});
@@ -867,7 +946,9 @@ static int gol_bits_launch_r(b2r_ctx* ctx, const void* in, void* out)
     case 10: return GB_LAUNCH(6);
     case 11: return GB_LAUNCH(7);
     case 12: return GB_LAUNCH(8);
-
+    case 13: return gol_whole_rows(ctx->k) ? GB_LAUNCH(9) : GB_LAUNCH(7);
+    case 14: return gol_whole_rows(ctx->k) ? GB_LAUNCH(10) : GB_LAUNCH(7);
+    case 15: return gol_whole_rows(ctx->k) ? GB_LAUNCH(11) : GB_LAUNCH(7);
     default: return GB_LAUNCH(1);
     }
 #undef GB_LAUNCH

@@ -81,8 +81,9 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
                     }
         }
     } else {
-        for (int tile = 1; tile <= (kind == 2 ? 12 : 1); ++tile)
+        for (int tile = 1; tile <= (kind == 2 ? 15 : 1); ++tile)
             for (int zc : zcs) {
+                if (kind == 2 && !b2r_gol_tile_ok(ctx, tile)) continue;
                 b2r_tuning t = base;
                 if (kind == 1) {
                     t.gol_step_zc = zc;

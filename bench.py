@@ -143,10 +143,12 @@ def gol_bench(A, R, steps, warmup, fused=1, zc=0, record=None, exchange=False):
 # (output quads of 16 cells per tile row, rows incl. halo) of the gol_tile values
 GOL_TILES = {1: (8, 32), 2: (8, 48), 3: (16, 56)}
 # (output words of 32 cells per row, rows incl. halo, rows per thread, waves of
-# whole rows) of the bit-sliced kernel
+# whole block rows 1 or of whole periodic 512-cell grid rows 2) of the
+# bit-sliced kernel
 GOL_BIT_TILES = {0: (8, 64, 1, 0), 4: (8, 32, 1, 0), 5: (8, 64, 1, 0), 6: (8, 96, 1, 0),
                  7: (8, 40, 1, 0), 8: (8, 72, 1, 0), 9: (8, 72, 2, 0), 10: (8, 136, 2, 0),
-                 11: (8, 72, 1, 1), 12: (8, 48, 1, 1)}
+                 11: (8, 72, 1, 1), 12: (8, 48, 1, 1), 13: (16, 72, 2, 2), 14: (16, 40, 1, 2),
+                 15: (16, 64, 1, 2)}
 
 
 def gol_kernel_name(R, tile, packed=True):

@@ -106,7 +106,11 @@ typedef struct b2r_tuning {
                          256 x 72; 9, 10 bit-sliced 256 x 72, 256 x 136 with
                          two rows per thread; 11, 12 bit-sliced 256 x 72,
                          256 x 48 with waves of whole rows (x neighbours by
-                         lane shifts); 0 = 5                                   */
+                         lane shifts); 13-15 bit-sliced whole grid rows of
+                         512 cells x 72 (two rows per thread), x 40, x 64,
+                         where the grid is periodic in x over 512 cells (x
+                         neighbours by rotations within 16 lanes; elsewhere
+                         they run as 11); 0 = 5                                */
     int heat_wx;      /* waves across x per heat step workgroup; (heat_wy,
                          heat_wx) is (4, 1|2|3|4), (8, 1|2), (16, 1), (2, 6) or
                          (3, 3)
@@ -278,7 +282,7 @@ int b2r_gol_step(b2r_ctx* ctx, const void* d_in, void* d_out, int b2r_i);
  * alternating the two fields (the loop of gol-3D.cu:138-145 plus the driver's
  * exchange, b2r.cpp:179-221), or with fused = 1 (periodic, or every face
  * frozen; R <= 4) one R-generation launch per window.  Where the bit-sliced
- * kernel runs (B_x % 32 == 0, gol_tile 0 or 4-12) no exchange runs: it reads
+ * kernel runs (B_x % 32 == 0, gol_tile 0 or 4-15) no exchange runs: it reads
  * its halo from the neighbouring blocks, and unless gol_bytes is set the
  * windows run on the context's two bit-packed fields (allocated on the first
  * such call), nsteps mod R generations at the end as one more, shorter

@@ -1,0 +1,76 @@
+"""The bit-sliced Game of Life arithmetic of gol_bits_kernel (b2r_gol.hip),
+restated bit for bit on the CPU: full adders as xor3 / majority, the x-sum of
+three 2-bit z-sums and the rule on the carry-save y-sum, checked against the
+count and the rule they stand for (gol-3D.cu:93-99: born at n == 5, survives
+2 <= n <= 7, with box = n + self) over every input they can receive.  The GPU
+parity tests check the kernel itself; these pin the formulation, so that a
+wrong truth table or a dropped carry fails here with the input that breaks it."""
+import itertools
+
+
+def xor3(a, b, c):
+    return a ^ b ^ c
+
+
+def maj3(a, b, c):
+    return (a & b) | (a & c) | (b & c)
+
+
+def bit(v, i):
+    return (v >> i) & 1
+
+
+def xsum(l, s, r):
+    """gb_xsum: three 2-bit numbers (bit planes 0, 1) -> 4-bit sum."""
+    l0, l1, s0, s1, r0, r1 = bit(l, 0), bit(l, 1), bit(s, 0), bit(s, 1), bit(r, 0), bit(r, 1)
+    x0b, k1 = xor3(l0, s0, r0), maj3(l0, s0, r0)
+    u, v = xor3(l1, s1, r1), maj3(l1, s1, r1)
+    x1b, w = u ^ k1, u & k1
+    return x0b | (x1b << 1) | ((v ^ w) << 2) | ((v & w) << 3)
+
+
+def rule(U, X, D, self):
+    """gb_rule on one cell: U, X, D are 4-bit x-sums of rows y-1, y, y+1."""
+    b0, c1 = xor3(bit(U, 0), bit(X, 0), bit(D, 0)), maj3(bit(U, 0), bit(X, 0), bit(D, 0))
+    e1, c2a = xor3(bit(U, 1), bit(X, 1), bit(D, 1)), maj3(bit(U, 1), bit(X, 1), bit(D, 1))
+    b1, c2b = e1 ^ c1, e1 & c1
+    e2, c3a = xor3(bit(U, 2), bit(X, 2), bit(D, 2)), maj3(bit(U, 2), bit(X, 2), bit(D, 2))
+    b2, c3b = xor3(e2, c2a, c2b), maj3(e2, c2a, c2b)
+    e3, c4a = xor3(bit(U, 3), bit(X, 3), bit(D, 3)), maj3(bit(U, 3), bit(X, 3), bit(D, 3))
+    b3, c4b = xor3(e3, c3a, c3b), maj3(e3, c3a, c3b)
+    lo = b2 | b1 | b0
+    ge3 = b2 | (b1 & b0)
+    eq5 = b2 & (1 - b1) & b0
+    m = eq5 | (self & ge3)
+    z = b3 & self & (1 - lo)
+    tt = z | ((1 - b3) & m)
+    return tt & (1 - c4a) & (1 - c4b)
+
+
+def test_xsum_of_three_zsums():
+    for l, s, r in itertools.product(range(4), repeat=3):
+        assert xsum(l, s, r) == l + s + r, (l, s, r)
+
+
+def test_rule_on_every_box_sum():
+    # x-sums are 0..9 (three z-sums of 0..3 each)
+    for U, X, D, self in itertools.product(range(10), range(10), range(10), (0, 1)):
+        box = U + X + D
+        n = box - self
+        want = int((not self and n == 5) or (self and 2 <= n <= 7))
+        assert rule(U, X, D, self) == want, (U, X, D, self)
+
+
+def test_rule_on_every_neighbourhood_through_the_adders():
+    # all 2^27 boxes are too many; every count of each z column, through the
+    # x-sum, covers every reachable (U, X, D) with its multiplicity structure
+    zs = range(4)
+    for zl, zc, zr in itertools.product(zs, repeat=3):
+        X = xsum(zl, zc, zr)
+        for U, D in itertools.product(range(10), repeat=2):
+            for self in (0, 1):
+                if self and zc == 0:
+                    continue  # the cell itself lies in the middle z column
+                box = U + X + D
+                want = int(box == 5 or (self and 3 <= box <= 8))
+                assert rule(U, X, D, self) == want

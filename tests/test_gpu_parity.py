@@ -280,7 +280,7 @@ def test_gol_full_size_R_invariance_and_population():
     env = (512, 512, 512)
     outs = {}
     for R in (1, 2, 4):
-        for fused, tile in ((0, 0), (1, 1), (1, 5), (1, 11)):
+        for fused, tile in ((0, 0), (1, 1), (1, 5), (1, 11), (1, 13), (1, 14)):
             say("gol 512^3 R", R, "fused", fused, "tile", tile)
             with A.Context(env, (2, 2, 2), R, 1, 1, 1) as c:
                 c.set_tuning(gol_tile=tile)
@@ -343,14 +343,63 @@ def test_gol_bits_halo_from_neighbours(env, blocks, periodic, R, tile, gol_bytes
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("tile", [8, 11])
+WHOLE_ROWS = [((512, 40, 24), (2, 1, 1)), ((512, 36, 28), (1, 2, 2)),
+              ((512, 24, 20), (16, 1, 1)), ((512, 48, 24), (4, 2, 1))]
+
+
+@pytest.mark.parametrize("gol_bytes,zc", [(0, 0), (0, 7), (1, 0)])
+@pytest.mark.parametrize("tile", [13, 14, 15])
+@pytest.mark.parametrize("R", [1, 2, 3, 4])
+@pytest.mark.parametrize("env,blocks", WHOLE_ROWS)
+def test_gol_whole_rows(env, blocks, R, tile, gol_bytes, zc):
+    # the whole-row tiles: a grid periodic in x over 512 cells, 1 to 16 blocks
+    # across it (a lane's word then lies in block q / (B_x / 32)), partial y
+    # tiles, z chunks that do not divide B_z, and 3R + 1 generations (the last
+    # as a one-generation launch); packed and byte fields
+    gens = 3 * R + 1
+    say("gol whole rows", env, blocks, "R", R, "tile", tile, "bytes", gol_bytes, "zc", zc)
+    with A.Context(env, blocks, R, 1, 1, 1) as c:
+        c.set_tuning(gol_tile=tile, gol_zc=zc, gol_bytes=gol_bytes)
+        a, b = c.field(), c.field()
+        c.init_bernoulli(a, 17, int(0.3 * 16777216))
+        res, _ = c.gol_run(a, b, gens, fused=1)
+        got = c.download(res)
+    u = O.bernoulli((env[2], env[1], env[0]), 17, 0.3)
+    assert np.array_equal(got, O.gol(u, gens, True))
+
+
+@pytest.mark.parametrize("tile", [13, 14])
+@pytest.mark.parametrize("env,blocks,periodic", [((512, 40, 24), (2, 1, 1), False),
+                                                 ((256, 40, 24), (2, 1, 1), True),
+                                                 ((544, 40, 24), (1, 1, 1), True)])
+def test_gol_whole_rows_elsewhere(env, blocks, periodic, tile):
+    # where the grid is not periodic in x over 512 cells the whole-row tiles
+    # run as tile 11
+    R, gens = 4, 9
+    say("gol whole rows fallback", env, blocks, "periodic", periodic, "tile", tile)
+    with A.Context(env, blocks, R, 1, 1 if periodic else 0, 1) as c:
+        c.set_tuning(gol_tile=tile, gol_zc=0, gol_bytes=0)
+        a, b = c.field(), c.field()
+        c.init_bernoulli(a, 19, int(0.3 * 16777216))
+        res, _ = c.gol_run(a, b, gens, fused=1)
+        got = c.download(res)
+    u = O.bernoulli((env[2], env[1], env[0]), 19, 0.3)
+    if periodic:
+        want = O.gol(u, gens, True)
+    else:
+        want = O.unframe(O.gol(O.frame(u, 1), gens, False), 1)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("tile", [8, 11, 14])
 @pytest.mark.parametrize("periodic", [False, True])
 def test_gol_packed_runs_in_sequence(periodic, tile):
     # Two runs on the same fields and the fields of another context in
     # between: each run packs its input again (the packed pair is the
     # context's), whole windows only, then windows plus single steps, and the
     # timing helper's packed launches leave the byte fields untouched.
-    env, blocks, R = (128, 40, 36), (2, 2, 1), 4
+    # (tile 14 on a 512-cell periodic x extent: whole grid rows)
+    env, blocks, R = ((512, 40, 36) if tile >= 13 else (128, 40, 36)), (2, 2, 1), 4
     u = O.bernoulli((env[2], env[1], env[0]), 12, 0.3)
     with A.Context(env, blocks, R, 1, 1 if periodic else 0, 1) as c, \
             A.Context((64, 32, 32), (1, 1, 1), 2, 1, 1, 1) as c2:
@@ -377,7 +426,7 @@ def test_gol_packed_runs_in_sequence(periodic, tile):
     assert all(np.array_equal(p, q) for p, q in zip(before, after))
 
 
-@pytest.mark.parametrize("tile", [8, 11])
+@pytest.mark.parametrize("tile", [8, 11, 14])
 @pytest.mark.parametrize("periodic", [False, True])
 @pytest.mark.parametrize("R,gens", [(4, 1), (4, 3), (4, 4), (4, 5), (2, 3), (1, 1), (1, 2)])
 def test_gol_packed_first_and_last_launch(periodic, R, gens, tile):
@@ -385,7 +434,7 @@ def test_gol_packed_first_and_last_launch(periodic, R, gens, tile):
     # as it stores; with no frozen face the first packs as it loads (a single
     # launch then goes from byte field to byte field), with frozen faces a
     # pack pass fills the packed frame first.
-    env, blocks = (128, 40, 36), (2, 2, 1)
+    env, blocks = ((512, 40, 36) if tile >= 13 else (128, 40, 36)), (2, 2, 1)
     u = O.bernoulli((env[2], env[1], env[0]), 21, 0.3)
     say("gol packed first/last launch, R", R, "gens", gens, "periodic", periodic, "tile", tile)
     with A.Context(env, blocks, R, 1, 1 if periodic else 0, 1) as c:
