@@ -543,9 +543,11 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
     // also run together, was measured slower: 4589 against 4727 GLUP/s at R=4)
     // (XD = 2: one tile spans the x blocks, the lane's word picks its block)
     int tt = xcd_tile(blockIdx.x, gridDim.x);
+    // (XD = 2: B_x / 32 = 16 / nb_x words per block row, a power of two)
+    const int lwb = WR ? __builtin_ctz((unsigned)(g.B[0] >> 5)) : 0;
     int txi = 0, bxi;
     if (WR) {
-        bxi = q / (g.B[0] >> 5);
+        bxi = q >> lwb;
     } else {
         txi = tt % ntx;
         tt /= ntx;
@@ -575,7 +577,7 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
     uint8_t* __restrict__ dst = out + boff_o;
 
     // this thread's words: x = x .. x+31, rows y[j]
-    const int x = WR ? x0 + 32 * (q - bxi * (g.B[0] >> 5)) : x0 + 32 * (q - 1);
+    const int x = WR ? x0 + 32 * (q & ((1 << lwb) - 1)) : x0 + 32 * (q - 1);
     int y[RT];
     bool ld_ok[RT];
     const uint8_t* __restrict__ col[RT];
@@ -597,7 +599,27 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
     // (colown) at a frozen z face
     long long zoff[2] = {0, 0};
     bool znb[2] = {false, false};
-    if (pull) {
+    if (WR && pull) {
+        // a lane's block is the tile's first block bu plus its x index, and so
+        // are its y and z neighbours (neighbours across y and z keep x): the
+        // neighbour indices are those of bu, uniform over the workgroup
+        const int ny[2] = {neighbour_block(g, bu, 1, 0), neighbour_block(g, bu, 1, 1)};
+#pragma unroll
+        for (int j = 0; j < RT; ++j) {
+            if (!ld_ok[j]) continue;
+            const int oy = y[j] < h ? -1 : (y[j] >= h + g.B[1] ? 1 : 0);
+            const int nb = ny[oy > 0];
+            if (oy != 0 && nb >= 0)
+                col[j] = in + (long long)(nb + bxi) * bstr + xo +
+                         (long long)(y[j] - oy * g.B[1]) * pitch + xbyte(x);
+        }
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int nb = neighbour_block(g, bu, 2, side);
+            znb[side] = nb >= 0;
+            zoff[side] = (long long)(nb - bu) * bstr + (side ? -1ll : 1ll) * g.B[2] * slice;
+        }
+    } else if (pull) {
 #pragma unroll
         for (int j = 0; j < RT; ++j) {
             if (!ld_ok[j]) continue;

@@ -123,6 +123,25 @@ def test_2d_gol_matches_oracle(env, blocks, R, periodic, fused):
     assert np.any(want) and np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("tile", [13, 14])
+@pytest.mark.parametrize("R", [2, 4])
+@pytest.mark.parametrize("blocks", [(2, 2, 1), (1, 1, 1)])
+def test_2d_gol_whole_rows(blocks, R, tile):
+    # the whole-row tiles on a 2D grid periodic in x over 512 cells: the z
+    # faces stay frozen, so every generation clips its update box along z
+    env = (512, 48, 1)
+    say("2d gol whole rows", blocks, R, tile)
+    with A.Context(env, blocks, R, 1, 1, 1) as c:
+        c.set_tuning(gol_tile=tile, gol_zc=0, gol_bytes=0)
+        a, b = c.field(), c.field()
+        c.init_bernoulli(a, 5, int(0.4 * 16777216))
+        res, _ = c.gol_run(a, b, 2 * R + 1, fused=1)
+        got = c.download(res)
+    u = O.bernoulli((1, env[1], env[0]), 5, 0.4)
+    want = O.gol(np.pad(u, ((1, 1), (0, 0), (0, 0))), 2 * R + 1, 3)[1:-1]
+    assert np.any(want) and np.array_equal(got, want)
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_2d_elastic_forward_matches_oracle(mode):
     # one plane with a zero z frame: every z derivative is zero there, the
