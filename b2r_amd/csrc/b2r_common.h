@@ -99,6 +99,9 @@ int b2r_gol_fused_packed(const b2r_ctx* ctx);
 // 1: gol_tile `tile` runs as itself on this context's geometry (the
 // whole-row tiles 13-15 need a grid periodic in x over 512 cells)
 int b2r_gol_tile_ok(const b2r_ctx* ctx, int tile);
+// the z chunk a fused Game of Life launch takes with the context's tuning
+// (gol_zc, or the automatic chunk where that is 0)
+int b2r_gol_fused_chunk(const b2r_ctx* ctx);
 // The context's two packed fields (allocated on first use), both packed from
 // the byte field `from`, halo included (from = NULL: allocated only, contents
 // unspecified).

@@ -877,6 +877,7 @@ int b2r_gol_tile_ok(const b2r_ctx* ctx, int tile)
     return tile < 13 || gol_whole_rows(ctx->k);
 }
 
+
 int b2r_gol_fused_reads_halo(const b2r_ctx* ctx)
 {
     const int t = ctx->tune.gol_tile;
@@ -888,6 +889,21 @@ static bool gol_fused_ok(const b2r_ctx* ctx, int R)
     return R >= 1 && R <= 4 && closed_faces(ctx->k);
 }
 
+// The z chunk of a fused launch with `tiles` tiles per plane set: the tuned
+// gol_zc, or where that is 0 about four workgroup waves over 256 CUs (two
+// resident per CU), at least 4R planes; at most the block's depth.
+static int gol_chunk(const b2r_ctx* ctx, long long tiles, int R)
+{
+    const KGeo& g = ctx->k;
+    int zc = ctx->tune.gol_zc;
+    if (zc <= 0) {
+        int nzc = (int)((2048 + tiles - 1) / tiles);
+        zc = ceil_div(g.B[2], nzc < 1 ? 1 : nzc);
+        if (zc < 4 * R) zc = 4 * R;
+    }
+    return zc > g.B[2] ? g.B[2] : zc;
+}
+
 template <int R, int TXQ, int NR>
 static int gol_fused_launch_t(b2r_ctx* ctx, const void* in, void* out)
 {
@@ -895,21 +911,35 @@ static int gol_fused_launch_t(b2r_ctx* ctx, const void* in, void* out)
     const KGeo& g = ctx->k;
     const int ntx = ceil_div(g.B[0], 16 * TXQ);
     const int nty = ceil_div(g.B[1], TY);
-    long long tiles = (long long)ntx * nty * g.nblk;
-    int zc = ctx->tune.gol_zc;
-    if (zc <= 0) {
-        // about four workgroup waves over 256 CUs (two resident per CU)
-        int nzc = (int)((2048 + tiles - 1) / tiles);
-        zc = ceil_div(g.B[2], nzc < 1 ? 1 : nzc);
-        if (zc < 4 * R) zc = 4 * R;
-    }
-    if (zc > g.B[2]) zc = g.B[2];
+    const int zc = gol_chunk(ctx, (long long)ntx * nty * g.nblk, R);
     const int nzc = ceil_div(g.B[2], zc);
     dim3 grid(ntx * nty * nzc * g.nblk);
     gol_fused_kernel<R, TXQ, NR><<<grid, (TXQ + 2) * NR, 0, ctx->stream>>>(
         (const uint8_t*)in, (uint8_t*)out, g, ntx, nty, nzc, zc);
     B2R_LAUNCH_CHECK();
     return B2R_OK;
+}
+
+int b2r_gol_fused_chunk(const b2r_ctx* ctx)
+{
+    const KGeo& g = ctx->k;
+    const int R = ctx->g.R;
+    int tile = ctx->tune.gol_tile;
+    if (tile >= 13 && !gol_whole_rows(g)) tile = 11;
+    int nw, nr, wr = 0;
+    if (tile >= 1 && tile <= 3) {
+        nw = GF_SHAPES[tile - 1][0] / 2;  // 16-cell quads -> 32-cell words
+        nr = GF_SHAPES[tile - 1][1];
+    } else {
+        static const int idx[16] = {1, 1, 1, 1, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
+        const int* s = GB_SHAPES[idx[tile]];
+        nw = s[0];
+        nr = s[1];
+        wr = s[3] == 2;
+    }
+    const int ntx = wr ? 1 : ceil_div(g.B[0], 32 * nw);
+    const int nty = ceil_div(g.B[1], nr - 2 * R);
+    return gol_chunk(ctx, (long long)ntx * nty * (wr ? g.nblk / g.nb[0] : g.nblk), R);
 }
 
 // words per packed row of a block: B_x/32 owned words and one per side
@@ -936,14 +966,7 @@ static int gol_bits_launch_t(b2r_ctx* ctx, const void* in, void* out)
     const int ntx = XD == 2 ? 1 : ceil_div(g.B[0], 32 * NW);
     const int nty = ceil_div(g.B[1], TY);
     const int nbt = XD == 2 ? g.nblk / g.nb[0] : g.nblk;  // blocks (sets) per tile grid
-    long long tiles = (long long)ntx * nty * nbt;
-    int zc = ctx->tune.gol_zc;
-    if (zc <= 0) {
-        int nzc = (int)((2048 + tiles - 1) / tiles);
-        zc = ceil_div(g.B[2], nzc < 1 ? 1 : nzc);
-        if (zc < 4 * R) zc = 4 * R;
-    }
-    if (zc > g.B[2]) zc = g.B[2];
+    const int zc = gol_chunk(ctx, (long long)ntx * nty * nbt, R);
     const int nzc = ceil_div(g.B[2], zc);
     dim3 grid(ntx * nty * nzc * nbt);
     gol_bits_kernel<R, NW, NR, IO, RT, XD><<<grid, gb_threads(NW, NR, RT, XD), 0, ctx->stream>>>(

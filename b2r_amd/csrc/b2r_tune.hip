@@ -122,6 +122,9 @@ extern "C" int b2r_autotune(b2r_ctx* ctx, int kind, void* d_a, void* d_b,
         }
     }
     ctx->tune = best;
+    // a fused Game of Life choice keeps its z chunk explicit (the automatic
+    // chunk it stood for on this geometry)
+    if (kind == 2 && best.gol_zc == 0) ctx->tune.gol_zc = best.gol_zc = b2r_gol_fused_chunk(ctx);
     if (out) *out = best;
     if (record_path && *record_path) {
         FILE* f = fopen(record_path, "a");

@@ -151,6 +151,15 @@ GOL_BIT_TILES = {0: (8, 64, 1, 0), 4: (8, 32, 1, 0), 5: (8, 64, 1, 0), 6: (8, 96
                  15: (16, 64, 1, 2)}
 
 
+# (gol_tile, gol_zc) profiled beside the tuner's choice at 512^3 on 2x2x2 blocks,
+# and (heat_ry, heat_wy, heat_wx, heat_zc) beside it for heat delta1 768^3 and
+# delta4 768^3 on 2x2x2 blocks
+GOL_PROFILED = ((14, 16), (13, 8), (11, 16))
+HEAT_PROFILED = {1: ((8, 4, 4, 256), (8, 3, 3, 384)), 4: ((8, 3, 3, 384),)}
+# (el_step_zc, el_zc) of the elastic 256^3 profile
+EL_PROFILED = (128, 32)
+
+
 def gol_kernel_name(R, tile, packed=True):
     """The template of the run's middle launches: IO 3 packed to packed (the
     first launch of a run packs as it loads, IO 2, the last unpacks as it
@@ -549,16 +558,18 @@ def elastic_checks(A, W):
 
 def profile_kernels(A):
     """A short fixed sequence for rocprofv3 counter runs, each kernel at the
-    launch shape the bench runs it with: a copy kernel of known size (1 GiB
+    launch shapes the bench runs it with: a copy kernel of known size (1 GiB
     read + 1 GiB written per launch, the scale for FETCH_SIZE / WRITE_SIZE),
     then, after the same autotuning the bench does, 5 launches each of the
-    fused Game of Life kernel (512^3, 2x2x2, periodic, R = 1, 2, 4, on the
-    bit-packed fields the runs use), the heat delta1 step kernel (768^3), the
-    heat delta4 step kernel (768^3 on 2x2x2 blocks) and the elastic step
-    (256^3, the kernel and z chunk b2r_el_autotune picks, then the other
-    kernel).  The tuner's own launches come first, so the last 5 dispatches
-    of each kernel are the profiled ones; each choice is logged as a JSON line
-    on stderr with the kernel's name."""
+    tuner's choice and of the other shapes it picks between from run to run
+    (GOL_PROFILED, HEAT_PROFILED) for the fused Game of Life kernel (512^3,
+    2x2x2, periodic, R = 1, 2, 4, on the bit-packed fields the runs use), the
+    heat delta1 step kernel (768^3) and the heat delta4 step kernel (768^3 on
+    2x2x2 blocks), then 5 elastic steps (256^3) with the fused step kernel and
+    with the stage kernels, at the z chunks EL_PROFILED.  The tuner's own
+    launches come first, so the last 5 dispatches of each kernel template and
+    grid are the profiled ones; each launch set is logged as a JSON line on
+    stderr with the kernel's name."""
     from b2r_amd import workloads as W
     nbytes = 1 << 30
     with A.Context((16, 16, 16), (1, 1, 1), 1, 1, 0, 4) as c:
@@ -571,13 +582,18 @@ def profile_kernels(A):
             c.init_bernoulli(a, 1, int(0.3 * 16777216))
             c.boundary_updating_direct([a])
             t = c.autotune(2, a, b, None, None)
-            c.init_bernoulli(a, 1, int(0.3 * 16777216))
-            packed = t["gol_tile"] in GOL_BIT_TILES and not t["gol_bytes"]
-            log(json.dumps({"profile": "gol_R%d" % R,
-                            "kernel": gol_kernel_name(R, t["gol_tile"], packed),
-                            "gol_tile": t["gol_tile"], "gol_zc": t["gol_zc"],
-                            "packed": packed, "launches": 5}))
-            c.time_steps(2, a, b, 5)
+            choice = (t["gol_tile"], t["gol_zc"])
+            # the tuner's choice, then the shapes it picks between from run to
+            # run (within a few per cent of each other), so that the counters
+            # cover the bench's choice whichever it is
+            for tile, zc in [choice] + [x for x in GOL_PROFILED if x != choice]:
+                c.set_tuning(gol_tile=tile, gol_zc=zc)
+                c.init_bernoulli(a, 1, int(0.3 * 16777216))
+                packed = tile in GOL_BIT_TILES and not t["gol_bytes"]
+                log(json.dumps({"profile": "gol_R%d" % R, "kernel": gol_kernel_name(R, tile, packed),
+                                "gol_tile": tile, "gol_zc": zc, "chosen": (tile, zc) == choice,
+                                "packed": packed, "launches": 5}))
+                c.time_steps(2, a, b, 5)
     for name, env, blocks, D, k in (("heat_D1_768", (768, 768, 768), (1, 1, 1), 1, 0.1),
                                     ("heat_D4_768_2x2x2", (768, 768, 768), (2, 2, 2), 4, 0.05)):
         prm = A.heat_params(D, A.HEAT_F32, (k, k, k))
@@ -586,10 +602,14 @@ def profile_kernels(A):
             c.init_uniform(a, 0)
             c.heat_run(a, b, prm, 2)
             t = c.autotune(0, a, b, prm, None)
-            log(json.dumps({"profile": name, "kernel": "heat_step_kernel<%d, 0, %d, %d, %d>" % (
-                D, t["heat_ry"], t["heat_wy"] or 4, t["heat_wx"] or 1), "heat_zc": t["heat_zc"],
-                "launches": 5}))
-            c.time_steps(0, a, b, 5, prm)
+            choice = (t["heat_ry"], t["heat_wy"] or 4, t["heat_wx"] or 1, t["heat_zc"])
+            # the tuner's choice, then the shapes it picks between from run to run
+            for ry, wy, wx, zc in [choice] + [x for x in HEAT_PROFILED[D] if x != choice]:
+                c.set_tuning(heat_ry=ry, heat_wy=wy, heat_wx=wx, heat_zc=zc)
+                log(json.dumps({"profile": name, "kernel": "heat_step_kernel<%d, 0, %d, %d, %d>" % (
+                    D, ry, wy, wx), "heat_ry": ry, "heat_wy": wy, "heat_wx": wx, "heat_zc": zc,
+                    "chosen": (ry, wy, wx, zc) == choice, "launches": 5}))
+                c.time_steps(0, a, b, 5, prm)
     env = (256, 256, 256)
     vp, vs, rho = W.layered_medium(env, 0)
     with A.Context(env, (1, 1, 1), 1, 4, 0, 4) as c:
@@ -597,12 +617,16 @@ def profile_kernels(A):
         s.set_medium(*W.lame(vp, vs, rho))
         t = s.autotune(None)
         first = t["el_kernel"]
+        # both kernels at fixed z chunks (the tuner's picks vary between runs
+        # by a few per cent), so every counter pass profiles the same launches
+        c.set_tuning(el_step_zc=EL_PROFILED[0], el_zc=EL_PROFILED[1])
         for kern in (first, 3 - first):
             c.set_tuning(el_kernel=kern)
             log(json.dumps({"profile": "elastic_256", "kernel": "el_step_kernel" if kern == 1
                             else "el_stress_kernel + el_velocity_kernel<7>",
-                            "el_zc": t["el_zc"], "el_step_zc": t["el_step_zc"],
-                            "chosen": kern == first, "steps": 5}))
+                            "el_step_zc": EL_PROFILED[0], "el_zc": EL_PROFILED[1],
+                            "tuner": {"el_kernel": first, "el_step_zc": t["el_step_zc"],
+                                      "el_zc": t["el_zc"]}, "steps": 5}))
             s.forward(5, 1)
         s.close()
     print(json.dumps({"profile_kernels": "done"}), flush=True)
@@ -770,7 +794,7 @@ def main():
     # counters of the same R, tile shape and field form (the profile sequence
     # runs the autotuner's choices); null where they were taken at another one
     form = "packed" if best["packed"] else "bytes"
-    tpre = f"gol_fused_R{best['R']}_tile{best['tile']}_{form}"
+    tpre = f"gol_fused_R{best['R']}_tile{best['tile']}_zc{best['zc']}_{form}"
     traffic = float(tr[tpre + "_bytes_per_launch"]) if tpre + "_bytes_per_launch" in tr else None
     valu = (float(tr[tpre + "_valu_insts_per_launch"])
             if tpre + "_valu_insts_per_launch" in tr else None)
