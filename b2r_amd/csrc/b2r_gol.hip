@@ -335,13 +335,14 @@ __global__ __launch_bounds__((TXQ + 2) * NR) void gol_fused_kernel(const uint8_t
 // packed from the byte cells when a plane is loaded and unpacked when
 // generation R is stored.  Per generation and plane:
 //   S   = z-sum of its three planes of generation t-1, two bit planes (0..3);
-//   XZ  = S(x-1) + S + S(x+1) through an LDS plane, the x neighbours shifted
-//         in across word boundaries, four bit planes (0..9);
-//   box = XZ(y-1) + XZ + XZ(y+1) through a second LDS plane, five bit planes;
+//   XZ  = S(x-1) + S + S(x+1), the x neighbours' z-sums (through an LDS
+//         plane, or DPP lane moves with XD) shifted in across word
+//         boundaries, four bit planes (0..9);
+//   box = XZ(y-1) + XZ + XZ(y+1) through an LDS plane;
 //   the rule: alive next iff box == 5, or alive and 3 <= box <= 8.
 // Every adder is built from full adders of two 3-input logic operations
-// (xor3, majority); the compiled generation takes about 70 instructions per 32
-// cells against about 150 for the byte-lane form.
+// (xor3, majority); with XD a generation compiles to about 40 VALU
+// instructions per 32 cells, against about 150 for the byte-lane form.
 // With pull set (B_x a multiple of 32), a word outside the block's owned
 // cells is loaded from the neighbouring block that owns it, so no exchange is
 // needed before the launch (its X phase alone moved 1-4 bytes per row, one

@@ -192,6 +192,23 @@ def test_gol_fused_with_pack_unpack_exchange(R, periodic):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("tile", [13, 14])
+@pytest.mark.parametrize("R", [1, 2, 4])
+def test_gol_whole_rows_with_pack_unpack_exchange(R, tile):
+    # the whole-row tiles after the pack/unpack exchange: y and z halos read
+    # from the block's own (exchanged) halo, x neighbours from the grid row
+    env = (512, 40, 48)
+    say("gol exchange whole rows", R, tile)
+    with A.Context(env, (2, 2, 2), R, 1, 1, 1) as c:
+        c.set_tuning(gol_exchange=1, gol_tile=tile, gol_zc=0)
+        a, b = c.field(), c.field()
+        c.init_bernoulli(a, 2, int(0.3 * 16777216))
+        res, _ = c.gol_run(a, b, 3 * R + 1, fused=1)
+        got = c.download(res)
+    u = O.bernoulli((env[2], env[1], env[0]), 2, 0.3)
+    assert np.array_equal(got, O.gol(u, 3 * R + 1, True))
+
+
 def test_el_gradient_plan_groups():
     # the groups of the batched gradient follow the memory budget
     env = (24, 20, 16)
