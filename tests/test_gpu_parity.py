@@ -344,7 +344,8 @@ def test_gol_bits_halo_from_neighbours(env, blocks, periodic, R, tile, gol_bytes
 
 
 WHOLE_ROWS = [((512, 40, 24), (2, 1, 1)), ((512, 36, 28), (1, 2, 2)),
-              ((512, 24, 20), (16, 1, 1)), ((512, 48, 24), (4, 2, 1))]
+              ((512, 24, 20), (16, 1, 1)), ((512, 48, 24), (4, 2, 1)),
+              ((512, 20, 36), (8, 1, 2))]
 
 
 @pytest.mark.parametrize("gol_bytes,zc", [(0, 0), (0, 7), (1, 0)])
