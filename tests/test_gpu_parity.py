@@ -490,3 +490,49 @@ def test_gol_fused_tile_shapes(tile, R, periodic):
     else:
         want = O.unframe(O.gol(O.frame(u, 1), gens, False), 1)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("env,blocks,R", [((512, 40, 24), (2, 1, 1), 2), ((96, 40, 24), (1, 1, 1), 4)])
+def test_gol_autotune_keeps_an_explicit_choice(env, blocks, R, tmp_path):
+    # the fused Game of Life tuner keeps a tile that runs as itself on the
+    # geometry (whole-row tiles only on a 512-cell periodic x extent) with an
+    # explicit z chunk, records it, and the run with it stays bit-exact
+    rec = str(tmp_path / "rec.txt")
+    say("gol autotune", env, blocks, R)
+    with A.Context(env, blocks, R, 1, 1, 1) as c:
+        a, b = c.field(), c.field()
+        c.init_bernoulli(a, 23, int(0.3 * 16777216))
+        t = c.autotune(2, a, b, None, rec)
+        assert t == {k: v for k, v in c.get_tuning().items()}
+        assert 1 <= t["gol_tile"] <= 15 and t["gol_zc"] > 0
+        if env[0] != 512:
+            assert t["gol_tile"] < 13
+        c.init_bernoulli(a, 23, int(0.3 * 16777216))
+        res, _ = c.gol_run(a, b, 2 * R + 1, fused=1)
+        got = c.download(res)
+    line = open(rec).read().strip().splitlines()[-1]
+    assert "kind=gol_fused" in line and f"best=gol_tile:{t['gol_tile']},gol_zc:{t['gol_zc']}" in line
+    u = O.bernoulli((env[2], env[1], env[0]), 23, 0.3)
+    assert np.array_equal(got, O.gol(u, 2 * R + 1, True))
+
+
+def test_heat_autotune_keeps_a_valid_choice(tmp_path):
+    # the heat step tuner's choice is one of the built shapes, recorded, and
+    # the run with it equals the oracle bit for bit
+    env, D, k = (70, 40, 36), 4, (0.021, 0.033, 0.017)
+    rec = str(tmp_path / "rec.txt")
+    p = A.heat_params(D, 0, k)
+    with A.Context(env, (1, 2, 1), 1, D, 0, 4) as c:
+        a, b = c.field(), c.field()
+        c.init_uniform(a, 9)
+        t = c.autotune(0, a, b, p, rec)
+        assert (t["heat_wy"], t["heat_wx"]) in {(4, 1), (4, 2), (4, 3), (4, 4), (8, 1), (8, 2),
+                                                (16, 1), (2, 6), (3, 3), (0, 0)}
+        assert t["heat_ry"] in (0, 1, 2, 4, 8)
+        c.init_uniform(a, 9)
+        res, _ = c.heat_run(a, b, p, 5)
+        got = c.download(res)
+    assert "kind=heat_step" in open(rec).read()
+    g = O.frame(O.uniform((36, 40, 70), 9), D)
+    want = O.unframe(O.heat_f32(g, D, 0, k, 0, (1, 1, 1), 5, False), D)
+    assert bits_equal(got, want)
