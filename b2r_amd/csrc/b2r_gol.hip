@@ -541,10 +541,10 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
     // blocks over the same rows run together on one XCD and a pulled halo word
     // is found in the L2 their owner filled (counting y tiles and z chunks
     // across blocks as well, so that y and z neighbours across a block face
-    // also run together, was measured slower: 4589 against 4727 GLUP/s at R=4)
-    // (XD = 2: one tile spans the x blocks, the lane's word picks its block)
+    // also run together, was measured slower: 4589 against 4727 GLUP/s at R=4).
+    // With XD = 2 one tile spans the x blocks and a lane's word q picks its
+    // block, q / (B_x/32), where B_x/32 = 16/nb_x is a power of two.
     int tt = xcd_tile(blockIdx.x, gridDim.x);
-    // (XD = 2: B_x / 32 = 16 / nb_x words per block row, a power of two)
     const int lwb = WR ? __builtin_ctz((unsigned)(g.B[0] >> 5)) : 0;
     int txi = 0, bxi;
     if (WR) {
