@@ -12,8 +12,8 @@
 // promotion of heat-3D.cpp:57-60), compiled like the device code without FP
 // contraction, so a host step equals a device step bit for bit.  It updates the
 // region of the reference's plain step, [D(i+1), N-1-D(i+1)] on every axis
-// (the reference's CPU delta2-4 use the bounds and taps of delta1, SURVEY
-// defect D3, which is not reproduced).
+// (the reference's CPU delta2-4 use the bounds and taps of delta1, which is not
+// reproduced).
 //
 // Hybrid run: the host cores own the top cpu_planes owned z planes of the
 // global grid (a slab over the whole x-y extent), the GPU the planes below,

@@ -14,7 +14,7 @@
 // a block's halo is left out, as the next exchange would overwrite it there).
 //
 // The elastic step is two radius-2 stages, so the library runs it with a
-// halo of 4R (SURVEY defect D6: the reference's R*B2R_D is too shallow); when
+// halo of 4R (the reference's halo of R*B2R_D is too shallow for it); when
 // b2r_sync_init made the context with another radius, eqwp_cuda_init makes it
 // again with radius 4 and points d_vx .. d_sigma_zz (b2r.h:60-64) at the
 // fields, which the driver hands back to boundary_updating (b2r.cpp:197-206).

@@ -239,8 +239,8 @@ double gol_3D_gpu_main(unsigned char* h_in, int b2r_R, char* ef, int Ngx, int Ng
 // heat-3D.cpp:136-184: R inner steps of delta-b2r_D on the host cores
 // (b2r_heat_cpu_step, the reference expression), ping-pong, and the owned z
 // slab [R*D, Ngz-R*D) of the last result copied back into h_in when it is the
-// second buffer.  The reference copies with the macro B2R_D in place of
-// b2r_D (SURVEY defect D4); this uses b2r_D.  Returns wall seconds.
+// second buffer.  The reference copies with the macro B2R_D in place of its
+// b2r_D argument; this uses b2r_D.  Returns wall seconds.
 float heat_3D_cpu_main(CELL_DT* h_in, int b2r_R, int b2r_D, int Ngx, int Ngy, int Ngz)
 {
     auto t0 = std::chrono::steady_clock::now();

@@ -50,7 +50,8 @@ extern "C" {
 /* Heat arithmetic modes.
  * B2R_HEAT_F32: u' = u + ((kx*Lx + ky*Ly) + kz*Lz) in FP32, L_a the radius-D
  *   second difference evaluated outermost tap first (heat-3D.cu:314-318 order).
- * B2R_HEAT_REF: the reference expression with its FP64 promotion (D10):
+ * B2R_HEAT_REF: the reference expression with its FP64 promotion (its double
+ *   constants promote the sum):
  *   u' = (float)((double)u + adt*((Lx/hx2 + Ly/hy2) + Lz/hz2)), L_a in FP32
  *   exactly as heat-3D.cpp:57-60 (D=1) and heat-3D.cu:314-318, 409-413, 513-517. */
 #define B2R_HEAT_F32 0
@@ -256,8 +257,8 @@ int b2r_heat_gradient(b2r_ctx* ctx, const void* d_u0, const void* d_target,
  * Ngx*Ngy*Ngz host grid (x fastest): out = step(in) on [D(b2r_i+1),
  * N-1-D(b2r_i+1)] per axis, the arithmetic of the device kernels term for term
  * (either mode), so a host step equals a device step bit for bit.  Replaces
- * cpu_fdm_heat_diffuse1..4 (heat-3D.cpp:48-134; their delta2-4 bounds and taps,
- * SURVEY defect D3, are not reproduced); nthreads <= 0 takes every hardware
+ * cpu_fdm_heat_diffuse1..4 (heat-3D.cpp:48-134; their delta2-4 use the bounds
+ * and taps of delta1, which is not reproduced); nthreads <= 0 takes every hardware
  * thread.  No device, no context. */
 int b2r_heat_cpu_step(const b2r_heat_params* p, const float* h_in, float* h_out, int Ngx,
                       int Ngy, int Ngz, int b2r_i, int nthreads);

@@ -284,7 +284,8 @@ void orc_heat_ref1_block_step(const float* h1, float* h2, int Ngx, int Ngy, int 
 }
 
 /* heat_3D_cpu_main's R-step ping-pong (heat-3D.cpp:136-169) without its
- * copy-back (D4): runs R steps in place on h_in using scratch. */
+ * copy-back (which uses the config's B2R_D for the step's D): runs R steps in
+ * place on h_in using scratch. */
 void orc_heat_ref1_block_run(float* h_in, float* scratch, int R, int Ngx, int Ngy, int Ngz,
                              double adt, double dx2, double dy2, double dz2)
 {

@@ -81,6 +81,12 @@ def test_upload_download_roundtrip_and_errors():
             c.heat_step(f, c.field(), A.heat_params(2), 0)  # D != radius
         with pytest.raises(A.B2RError):
             c.gol_step(f, c.field(), 0)                     # wrong cell type
+        for bad in ({"gol_tile": 16}, {"gol_tile": -1}, {"heat_wy": 5}, {"heat_wy": 3, "heat_wx": 2},
+                    {"heat_ry": 3}, {"gol_zc": -1}, {"el_kernel": 3}):
+            with pytest.raises(A.B2RError):
+                c.set_tuning(**bad)                         # no such launch shape
+        c.set_tuning(gol_tile=15, heat_wy=3, heat_wx=3)     # the largest built ones
+        assert c.get_tuning()["gol_tile"] == 15
 
 
 # ------------------------------------------------------------------ exchange

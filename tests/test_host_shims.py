@@ -136,7 +136,7 @@ def test_heat_3D_cpu_main_matches_reference(R):
     # heat-3D.cpp:136-184 at D = 1: the R-step ping-pong over the reference's
     # own step.  With R even the result is h_in itself in both; with R odd the
     # reference copies back the z slab [R*B2R_D, Ngz - R*B2R_D) with the
-    # config's B2R_D (2) in place of the step's D (defect D4), the shim the
+    # config's B2R_D (2) in place of the step's D (a defect), the shim the
     # slab [R*D, Ngz - R*D): checked against the reference's steps with that copy
     rng = np.random.default_rng(R)
     blk = rng.uniform(0, 1, (16, 13, 15)).astype(np.float32)
