@@ -356,12 +356,6 @@ __global__ __launch_bounds__((TXQ + 2) * NR) void gol_fused_kernel(const uint8_t
 // loads and stores its second half only when that half is in the row.  Cells
 // hold 0 or 1.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t dswap(uint32_t x, int d, uint32_t m)
-{
-    const uint32_t t = ((x >> d) ^ x) & m;
-    return x ^ t ^ (t << d);
-}
-
 // 32 byte cells (a: cells 0..15, b: 16..31) -> one word, bit i = cell i (bit
 // 0 of each byte).  Byte j of dword k is cell 4k+j: a byte dot product of the
 // masked dword with the weights 1, 2, 4, 8 (16, 32, 64, 128 for odd k, added
@@ -383,15 +377,19 @@ __device__ __forceinline__ uint32_t gb_pack(uint4 a, uint4 b)
     return lo | (hi << 16);
 }
 
+// One word -> 32 byte cells, the inverse of gb_pack.  Dword k holds cells
+// 4k..4k+3: their nibble of the word times 0x204081 (bits 0, 7, 14, 21) puts
+// nibble bit i at bits i, i+7, i+14, i+21, 16 distinct bits for the 4 nibble
+// bits, so nothing carries, and bit 8i holds nibble bit i; the byte mask keeps
+// bits 0, 8, 16, 24.
+__device__ __forceinline__ uint32_t gb_spread4(uint32_t w, int k)
+{
+    return __builtin_amdgcn_ubfe(w, 4 * k, 4) * 0x204081u & 0x01010101u;
+}
 __device__ __forceinline__ void gb_unpack(uint32_t w, uint4& a, uint4& b)
 {
-    w = dswap(w, 1, 0x22222222u);
-    w = dswap(w, 3, 0x0A0A0A0Au);
-    w = dswap(w, 6, 0x00CC00CCu);
-    w = dswap(w, 12, 0x0000F0F0u);
-    const uint32_t m = 0x01010101u;
-    a = make_uint4(w & m, (w >> 1) & m, (w >> 2) & m, (w >> 3) & m);
-    b = make_uint4((w >> 4) & m, (w >> 5) & m, (w >> 6) & m, (w >> 7) & m);
+    a = make_uint4(gb_spread4(w, 0), gb_spread4(w, 1), gb_spread4(w, 2), gb_spread4(w, 3));
+    b = make_uint4(gb_spread4(w, 4), gb_spread4(w, 5), gb_spread4(w, 6), gb_spread4(w, 7));
 }
 
 // The two outputs of a full adder, each one v_bitop3 (truth tables 0x96 and

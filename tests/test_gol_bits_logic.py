@@ -2,7 +2,9 @@
 restated bit for bit on the CPU: full adders as xor3 / majority, the x-sum of
 three 2-bit z-sums and the rule on the carry-save y-sum, checked against the
 count and the rule they stand for (gol-3D.cu:93-99: born at n == 5, survives
-2 <= n <= 7, with box = n + self) over every input they can receive.  The GPU
+2 <= n <= 7, with box = n + self) over every input they can receive, and the
+integer arithmetic of the byte <-> bit packing (gb_pack's byte dot products,
+gb_unpack's nibble spread).  The GPU
 parity tests check the kernel itself; these pin the formulation, so that a
 wrong truth table or a dropped carry fails here with the input that breaks it."""
 import itertools
@@ -74,3 +76,25 @@ def test_rule_on_every_neighbourhood_through_the_adders():
                 box = U + X + D
                 want = int(box == 5 or (self and 3 <= box <= 8))
                 assert rule(U, X, D, self) == want
+
+
+def test_unpack_nibble_spread():
+    # gb_spread4: nibble * 0x204081 & 0x01010101 puts nibble bit i in byte i
+    for n in range(16):
+        v = (n * 0x204081) & 0x01010101
+        assert v == sum(((n >> i) & 1) << (8 * i) for i in range(4)), n
+
+
+def test_pack_byte_dot_products():
+    # gb_pack's dot products: bytes (bit 0 each) of an even and an odd dword
+    # against weights 1, 2, 4, 8 and 16, 32, 64, 128 give cells 0..7 as bits
+    def udot4(a, b, c):
+        return c + sum(((a >> 8 * i) & 255) * ((b >> 8 * i) & 255) for i in range(4))
+    m = 0x01010101
+    for bits in range(256):
+        cells = [(bits >> i) & 1 for i in range(8)]
+        # odd byte values: bit 0 is the cell, the mask drops the rest
+        e = sum((c | 6) << (8 * i) for i, c in enumerate(cells[:4]))
+        o = sum((c | 2) << (8 * i) for i, c in enumerate(cells[4:]))
+        got = udot4(o & m, 0x80402010, udot4(e & m, 0x08040201, 0))
+        assert got == bits
