@@ -486,7 +486,8 @@ extern "C" int b2r_heat_run(b2r_ctx* ctx, void* d_a, void* d_b, const b2r_heat_p
 // columns sit in a register queue; the x/y neighbours of the plane being
 // computed come from one shared LDS plane (with a D-wide unused border),
 // reused level after level.  The valid region shrinks by D per level, so the
-// inner (64-2RD) x (32-2RD) columns of level R are written.  Cells outside a
+// inner (64-2RD) x (32-2RD) columns of level R are written (R*D <= 8: at least
+// 48 x 16 of the 64 x 32).  Cells outside a
 // level's update box (frozen global faces) keep their previous value; halo
 // cells of the output are never written.  Same arithmetic as the step kernel,
 // so the results are bit-identical.
@@ -604,7 +605,7 @@ __global__ __launch_bounds__(256) void heat_fused_kernel(const float* __restrict
 static bool heat_fused_ok(const b2r_ctx* ctx, int D)
 {
     const int R = ctx->g.R;
-    return R * D <= 4 && closed_faces(ctx->k);
+    return R * D <= 8 && closed_faces(ctx->k);
 }
 
 template <int D, int MODE, int R>
@@ -641,12 +642,20 @@ static int heat_fused_mode(b2r_ctx* ctx, const float* in, float* out, const Heat
     case 1 * 8 + 2: return heat_fused_t<1, MODE, 2>(ctx, in, out, hk);
     case 1 * 8 + 3: return heat_fused_t<1, MODE, 3>(ctx, in, out, hk);
     case 1 * 8 + 4: return heat_fused_t<1, MODE, 4>(ctx, in, out, hk);
+    case 1 * 8 + 5: return heat_fused_t<1, MODE, 5>(ctx, in, out, hk);
+    case 1 * 8 + 6: return heat_fused_t<1, MODE, 6>(ctx, in, out, hk);
+    case 1 * 8 + 7: return heat_fused_t<1, MODE, 7>(ctx, in, out, hk);
+    case 1 * 8 + 8: return heat_fused_t<1, MODE, 8>(ctx, in, out, hk);
     case 2 * 8 + 1: return heat_fused_t<2, MODE, 1>(ctx, in, out, hk);
     case 2 * 8 + 2: return heat_fused_t<2, MODE, 2>(ctx, in, out, hk);
+    case 2 * 8 + 3: return heat_fused_t<2, MODE, 3>(ctx, in, out, hk);
+    case 2 * 8 + 4: return heat_fused_t<2, MODE, 4>(ctx, in, out, hk);
     case 3 * 8 + 1: return heat_fused_t<3, MODE, 1>(ctx, in, out, hk);
+    case 3 * 8 + 2: return heat_fused_t<3, MODE, 2>(ctx, in, out, hk);
     case 4 * 8 + 1: return heat_fused_t<4, MODE, 1>(ctx, in, out, hk);
+    case 4 * 8 + 2: return heat_fused_t<4, MODE, 2>(ctx, in, out, hk);
     }
-    b2r_set_error("fused heat needs R*D <= 4 (R=%d, D=%d)", R, D);
+    b2r_set_error("fused heat needs R*D <= 8 (R=%d, D=%d)", R, D);
     return B2R_E_UNSUPPORTED;
 }
 

@@ -652,7 +652,7 @@ def heat_sections(A, bw, rec=None):
     log(json.dumps(h1f))
     amp, (ex, ey, ez) = blob_tables(env)
     h4runs = []
-    for R4, f4 in ((1, 0), (1, 1), (2, 0)):
+    for R4, f4 in ((1, 0), (1, 1), (2, 0), (2, 1)):
         log(f"heat delta4 768^3 blobs 2x2x2, 500 steps, R={R4} fused={f4}")
         h4 = heat_bench(A, env, (2, 2, 2), R4, 4, (0.05, 0.05, 0.05), 500,
                         lambda c, f: c.init_blobs(f, amp, ex, ey, ez), fused=f4, record=rec)

@@ -237,7 +237,8 @@ int b2r_heat_step(b2r_ctx* ctx, const void* d_in, void* d_out,
  * R inner steps.  The result is in d_a if *result_in_b == 0, else in d_b.
  * d_a and d_b must hold identical halos (the frozen frame) on entry.
  * fused != 0 runs the R inner steps of each exchange period in one launch
- * (temporal blocking in LDS) when supported.  *ms receives the device time. */
+ * (temporal blocking in LDS) where R*D <= 8 and every face is frozen or
+ * periodic; elsewhere it runs them step by step.  *ms receives the device time. */
 int b2r_heat_run(b2r_ctx* ctx, void* d_a, void* d_b, const b2r_heat_params* p,
                  int nsteps, int fused, int* result_in_b, float* ms);
 /* Hand-written discrete adjoint of the heat time loop; no reference

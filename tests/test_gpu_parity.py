@@ -150,6 +150,11 @@ HEAT = [
     ((40, 32, 48), (1, 1, 2), 1, 4, True, 5),
     ((130, 66, 70), (2, 1, 1), 2, 1, False, 6),
     ((96, 80, 72), (2, 2, 2), 2, 4, False, 4),
+    # deep windows, R*D up to 8 (the fused kernel's inner 48 x 16 columns)
+    ((40, 40, 40), (1, 1, 1), 6, 1, False, 13),
+    ((40, 36, 32), (1, 1, 1), 8, 1, True, 17),
+    ((48, 40, 44), (2, 1, 1), 4, 2, True, 9),
+    ((54, 40, 48), (1, 2, 2), 3, 2, False, 7),
 ]
 
 
