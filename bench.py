@@ -316,6 +316,9 @@ def elastic_gradient_bench(A, W, nshots=8, nsteps=300, ckpt=50, record=None,
             out[name + "_shot_glups"] = round(nshots * cells * nsteps / (ms * 1e-3) / 1e9, 3)
             out[name + "_ms_per_shot_step"] = round(ms / (nshots * nsteps), 4)
             log(f"  gradient {name}: {ms:.1f} ms")
+        if "plain_loop" not in res:  # the batched path alone (a profiling run)
+            out["misfit"] = [round(m, 6) for m in res["batched_graph"][0]]
+            return out
         out["speedup_batched_vs_plain"] = round(out["plain_loop_ms"] / out["batched_graph_ms"], 3)
         mb, gb_ = res["batched_graph"]
         mp, gp = res["plain_loop"]
@@ -729,12 +732,13 @@ def main():
                     help="append the autotuner's choices to this file")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="run the fixed kernel sequence used by the rocprofv3 counter runs")
-    ap.add_argument("--only", choices=["elastic-gradient", "elastic-batch", "heat",
-                                       "elastic-full", "hybrid"],
+    ap.add_argument("--only", choices=["elastic-gradient", "elastic-gradient-batched",
+                                       "elastic-batch", "heat", "elastic-full", "hybrid"],
                     help="run one section alone and print its JSON (for profiling); "
-                         "elastic-full: the 16-shot forward over 2000 steps and the 16-shot "
-                         "256^3 gradient over 1500 steps (checkpoints every 50), batched "
-                         "against the plain per-shot loop")
+                         "elastic-gradient-batched: the batched gradient alone, 4 shots of "
+                         "256^3; elastic-full: the 16-shot forward over 2000 steps and the "
+                         "16-shot 256^3 gradient over 1500 steps (checkpoints every 50), "
+                         "batched against the plain per-shot loop")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the halo "
@@ -749,6 +753,10 @@ def main():
         return
     if args.only == "elastic-gradient":
         print(json.dumps(elastic_gradient_bench(A, W, 8, args.gradient_steps)), flush=True)
+        return
+    if args.only == "elastic-gradient-batched":
+        print(json.dumps(elastic_gradient_bench(A, W, 4, args.gradient_steps,
+                                                modes=((2, "batched_graph"),))), flush=True)
         return
     if args.only == "elastic-batch":
         print(json.dumps(elastic_batch_bench(A, W, 16, args.elastic_steps)), flush=True)
