@@ -804,12 +804,18 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
                 if (act) xbp[pos + NQ] = X[j];
             }
             __syncthreads();
+            // XD: a wave past the step's barrier (y-sum and rule) issues at a
+            // lower priority than one still on its way to the next barrier
+            // (z-sum, x-sum, LDS write), so the barriers release sooner
+            // (3-5 % per R=4 launch)
+            if (XD) __builtin_amdgcn_s_setprio(0);
 #pragma unroll
             for (int j = 0; j < RT; ++j) {
                 const int pos = r0 * NQ + q + j * NTH;
                 const uint32_t self = gen[j][t - 1][1];
                 finish(t, j, p, gb_rule(xbp[pos], X[j], xbp[pos + 2 * NQ], self), self);
             }
+            if (XD) __builtin_amdgcn_s_setprio(1);
         }
     };
     uint4 sl[GF_PD][RT][2] = {};
