@@ -10,7 +10,7 @@ import pytest
 
 import _elastic as E
 import _oracle as O
-from _util import block_view, expected_halo_ids, expected_layout, id_field, Lay
+from _util import block_view, expected_halo_ids, expected_layout, id_field
 
 pytestmark = pytest.mark.gpu
 

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import _oracle as O
-from _util import Lay, expected_layout, id_field, rel_l2
+from _util import block_view, expected_layout, id_field, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -225,7 +225,6 @@ def test_heat_step_single_inner_region():
         c.fill(b, -7.0)
         c.heat_step(a, b, A.heat_params(D), 1)
         lay = c.layout
-        from _util import block_view
         blk_in = block_view(c.raw(a), lay, 0)
         blk = block_view(c.raw(b), lay, 0)
         S = lay.S[0]
