@@ -847,6 +847,9 @@ int b2r_direct_exchange(b2r_ctx* ctx, void* const* fields, int nfields)
         FieldList fl;
         for (int i = 0; i < n; ++i) fl.p[i] = fields[base + i];
         for (int d = 0; d < (g.two_d ? 2 : 3); ++d) {
+            // one block along d and not periodic: no block has a neighbour
+            // there, and every thread of the launch would skip its copy
+            if (g.nb[d] == 1 && !((g.periodic >> d) & 1)) continue;
             int V;
             long long rows;
             if (d == 0) {
