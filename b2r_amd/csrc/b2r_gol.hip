@@ -666,7 +666,7 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
     // second half of a row's last word may lie past the padded row end: such a
     // half reloads the word's other half (no extra traffic) and is masked off
     const bool hi_in = x + 32 <= g.pitch - g.xoff;  // second half inside the byte row
-    uint32_t ldm[RT], ma[RT], mb[RT];
+    uint32_t ldm[RT], ma[RT], mb[RT], pm[RT];
     int oa[RT], ob[RT];
     bool out_ok[RT];
     // (XD = 2 has no halo words: every word loads both halves)
@@ -679,6 +679,9 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
         ob[j] = b_ld ? 16 : oa[j];
         ma[j] = hw0 ? 0u : ~0u;
         mb[j] = b_ld ? ~0u : 0u;
+        // gb_pack forms bits 0-15 from the first 16 bytes and bits 16-31 from
+        // the second, so masking the packed word equals masking the bytes
+        pm[j] = ldm[j] & ((ma[j] & 0xFFFFu) | (mb[j] & 0xFFFF0000u));
         const int r = y[j] - (y0 - R);
         out_ok[j] = act && !hw0 && !hw1 && r >= R && r < R + TY && y[j] < h + g.B[1] &&
                     x < h + g.B[0];
@@ -721,9 +724,7 @@ __global__ __launch_bounds__(gb_threads(NW, NR, RT, XD)) void gol_bits_kernel(
             if (PKI)
                 gen[j][0][2] = la.x & ldm[j];
             else
-                gen[j][0][2] = gb_pack(make_uint4(la.x & ma[j], la.y & ma[j], la.z & ma[j], la.w & ma[j]),
-                                       make_uint4(lb.x & mb[j], lb.y & mb[j], lb.z & mb[j], lb.w & mb[j])) &
-                               ldm[j];
+                gen[j][0][2] = gb_pack(la, lb) & pm[j];
             // the load GF_PD planes ahead (the last ones are unused)
             const int zz = zp(min(k + GF_PD, nin - 1));
             const uint8_t* pp = colp(j, zz) + (long long)zz * slice;

@@ -98,3 +98,36 @@ def test_pack_byte_dot_products():
         o = sum((c | 2) << (8 * i) for i, c in enumerate(cells[4:]))
         got = udot4(o & m, 0x80402010, udot4(e & m, 0x08040201, 0))
         assert got == bits
+
+
+def gb_pack(a, b):
+    """gb_pack on 8 dwords (a: cells 0..15, b: 16..31): byte dot products of
+    even/odd dword pairs, then the two byte permutes and the shift-or."""
+    def udot4(x, w, c):
+        return c + sum(((x >> 8 * i) & 255) * ((w >> 8 * i) & 255) for i in range(4))
+    m = 0x01010101
+
+    def two(e, o):
+        return udot4(o & m, 0x80402010, udot4(e & m, 0x08040201, 0))
+    n = [two(a[0], a[1]), two(a[2], a[3]), two(b[0], b[1]), two(b[2], b[3])]
+    lo = (n[0] & 255) | ((n[1] & 255) << 8)
+    hi = (n[2] & 255) | ((n[3] & 255) << 8)
+    return (lo | (hi << 16)) & 0xFFFFFFFF
+
+
+def test_pack_mask_on_the_word_equals_mask_on_the_bytes():
+    # the kernel masks the packed word (bits 0-15 from the first 16 bytes,
+    # 16-31 from the second) instead of the 8 input dwords
+    import random
+    rng = random.Random(7)
+    for _ in range(2000):
+        a = [rng.getrandbits(32) for _ in range(4)]
+        b = [rng.getrandbits(32) for _ in range(4)]
+        for ma, mb in itertools.product((0, 0xFFFFFFFF), repeat=2):
+            want = gb_pack([x & ma for x in a], [x & mb for x in b])
+            got = gb_pack(a, b) & ((ma & 0xFFFF) | (mb & 0xFFFF0000))
+            assert got == want
+    # and on cells: bit i of the word is cell i
+    cells = [rng.getrandbits(1) for _ in range(32)]
+    dw = [sum(cells[4 * k + j] << (8 * j) for j in range(4)) for k in range(8)]
+    assert gb_pack(dw[:4], dw[4:]) == sum(c << i for i, c in enumerate(cells))
