@@ -212,7 +212,8 @@ int b2r_boundary_updating(b2r_ctx* ctx, int time_step, void* d_field);
 int b2r_boundary_updating_noalloc(b2r_ctx* ctx, int time_step, void* d_field);
 /* Same result with each halo cell copied straight from its source block (no
  * message buffers): the device-direct path of boundary_updating_direct
- * (b2r_halo_sync.cu:389-431).  Several fields in one launch per direction. */
+ * (b2r_halo_sync.cu:389-431).  Several fields in one launch per direction;
+ * no launch along an axis with one block and frozen faces (no neighbours). */
 int b2r_boundary_updating_direct(b2r_ctx* ctx, int time_step, void* const* d_fields, int nfields);
 /* Message-level building blocks: pack (sub_matrix_pack_{x,y,z},
  * b2r_halo_sync.cu:195-282) every block's two faces along dim (0,1,2) into
